@@ -1,0 +1,189 @@
+/* zsmpm: MI355X (gfx950) MLS-MPM substep on zpc's sparse block grid, C ABI.
+ *
+ * The entry points fill zpc's empty `rocm` execution space
+ * (include/zensim/types/Property.h:46,51,86; resource/Resource.cpp:156-158)
+ * for the operators of one explicit MLS-MPM substep. Each takes a policy
+ * (device + stream + scratch, the counterpart of
+ * CudaExecutionPolicy, cuda/execution/ExecutionPolicy.cuh:362-396) and
+ * non-owning descriptors of device buffers in zpc's container layouts:
+ *   table     HashTable<i32,3,int>: keys[T] (vec<i32,3>), indices[T], status[T],
+ *             _cnt, _activeKeys[T]            (container/HashTable.hpp:40-58,200-205)
+ *   grid      Grids<f32,3,4>{m:1, v:3, rhs:3}: element (ch, cell) of block b at
+ *             (b*7 + ch)*64 + cell, cell = (x<<4)|(y<<2)|z
+ *                                             (container/TileVector.hpp:108,397;
+ *                                              geometry/Structure.hpp:849-860)
+ *   particles Particles<f32,3>: x vec3, v vec3, C vec9, F vec9 (column-major),
+ *             m, logJp, one array per attribute (geometry/Structurefree.hpp:23-290)
+ *   buckets   IndexBuckets counts[B+1], offsets[B+1], indices[N]
+ *                                             (container/IndexBuckets.hpp:11-66)
+ * Naming follows zpc's py_interop `op__backend` exports
+ * (py_interop/cuda/ExecutionPolicy.cpp:8-35); results are the int32 codes of
+ * execution/AsyncRuntimeAbi.hpp:28-34, extended. Nothing throws across the
+ * ABI. Calls are asynchronous on the policy's stream; device-side faults
+ * (full table, missing block, ...) are reported by zsmpm_policy_sync.
+ */
+#ifndef ZSMPM_H
+#define ZSMPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZSMPM_ABI_MAJOR 1
+#define ZSMPM_ABI_MINOR 0
+
+enum zsmpm_result {
+  ZSMPM_OK = 0,
+  ZSMPM_ERROR = -1,
+  ZSMPM_INCOMPATIBLE_MAJOR = -2,
+  ZSMPM_INSUFFICIENT_SIZE = -3,
+  ZSMPM_UNSUPPORTED_OPERATION = -4,
+  ZSMPM_INVALID_ARGUMENT = -5,
+  ZSMPM_DEVICE_ERROR = -6,
+  ZSMPM_TABLE_FULL = -7,       /* a probe walked the whole table / more keys than slots */
+  ZSMPM_KEY_RANGE = -8,        /* a block key component outside [-2^20, 2^20) */
+  ZSMPM_STENCIL_OUTSIDE = -9,  /* a particle stencil outside its bucket's 2x2x2 blocks */
+  ZSMPM_MISSING_BLOCK = -10,   /* a needed block is not in the table (not activated/enlarged) */
+  ZSMPM_CAPACITY = -11         /* more blocks/particles than the buffers or reservation hold */
+};
+
+/* Versioned, size-prefixed struct header (execution/AsyncRuntimeAbi.hpp:12-27). */
+typedef struct zsmpm_abi_header {
+  uint32_t size;  /* sizeof the enclosing struct */
+  uint16_t major; /* ZSMPM_ABI_MAJOR */
+  uint16_t minor;
+} zsmpm_abi_header;
+
+typedef struct zsmpm_table {
+  zsmpm_abi_header hdr;
+  int32_t table_size; /* _tableSize, a power of two (evaluateTableSize, HashTable.hpp:88-91) */
+  int32_t* keys;      /* [3*T] */
+  int32_t* indices;   /* [T] */
+  int32_t* status;    /* [T] */
+  int32_t* cnt;       /* [1] */
+  int32_t* active_keys; /* [3*T] */
+} zsmpm_table;
+
+typedef struct zsmpm_particles {
+  zsmpm_abi_header hdr;
+  int64_t n;
+  float* x;     /* [3n] */
+  float* v;     /* [3n] */
+  float* C;     /* [9n] column-major */
+  float* F;     /* [9n] column-major */
+  float* m;     /* [n] */
+  float* logJp; /* [n], NULL for models without plasticity */
+} zsmpm_particles;
+
+typedef struct zsmpm_grid {
+  zsmpm_abi_header hdr;
+  float dx;
+  int32_t capacity_blocks; /* blocks allocated in data */
+  float* data;             /* [capacity_blocks * 7 * 64] */
+} zsmpm_grid;
+
+typedef struct zsmpm_buckets {
+  zsmpm_abi_header hdr;
+  int32_t capacity_blocks; /* counts/offsets hold capacity_blocks + 1 entries */
+  int32_t* counts;
+  int32_t* offsets;
+  int32_t* indices; /* [n] */
+} zsmpm_buckets;
+
+enum zsmpm_model_kind { ZSMPM_FIXED_COROTATED = 0, ZSMPM_DRUCKER_PRAGER = 1 };
+
+/* FixedCorotatedConfig / DruckerPragerConfig (physics/ConstitutiveModel.hpp:736-754). */
+typedef struct zsmpm_model {
+  zsmpm_abi_header hdr;
+  int32_t kind;
+  float volume;
+  float E, nu;
+  float cohesion, beta, yield_surface;
+  int32_t volume_correction;
+  float svd_fp64_gap; /* >0: FP64 SVD when min sigma gap < gap * sigma_1 */
+} zsmpm_model;
+
+enum zsmpm_collider_type { ZSMPM_STICKY = 0, ZSMPM_SLIP = 1, ZSMPM_SEPARATE = 2 };
+
+/* Collider<AnalyticLevelSet<Plane>> with an identity transform (geometry/Collider.h:80-107). */
+typedef struct zsmpm_plane_collider {
+  int32_t type;
+  float origin[3];
+  float normal[3];
+} zsmpm_plane_collider;
+
+#define ZSMPM_MAX_COLLIDERS 8
+
+typedef struct zsmpm_policy_s* zsmpm_policy;
+
+/* ABI version of the loaded library. */
+int32_t zsmpm_abi_version(zsmpm_abi_header* out);
+
+/* Policy for device `devid` on `stream` (NULL: the null stream)
+ * (CudaExecutionPolicy, cuda/execution/ExecutionPolicy.cuh:362-396;
+ *  py_interop/cuda/ExecutionPolicy.cpp:8-12 policy__device). */
+int32_t zsmpm_policy__rocm(int32_t devid, void* stream, zsmpm_policy* out);
+/* py_interop/cuda/ExecutionPolicy.cpp:13-15 del_policy__device. */
+int32_t zsmpm_del_policy__rocm(zsmpm_policy pol);
+/* Preallocate the policy's scratch for up to max_particles and a table of
+ * table_size slots (get_temporary_memory_source, omp/execution/ExecutionPolicy.hpp:26). */
+int32_t zsmpm_policy_reserve__rocm(zsmpm_policy pol, int64_t max_particles, int32_t table_size);
+/* Wait for the stream; returns the first device-side error raised since the
+ * last sync (cuda/execution/ExecutionPolicy.cuh:441-451 checks). */
+int32_t zsmpm_policy_sync(zsmpm_policy pol);
+/* Diagnostic counters after a sync: [0] new keys of the last insertion,
+ * [2] particle runs of the last bucket sort, [3] FP64 SVDs, [4] closure fallbacks,
+ * [5] max |v|^2 bits of the last grid update. out must hold 16 entries. */
+int32_t zsmpm_policy_counters(zsmpm_policy pol, uint32_t* out);
+
+/* CleanSparsity over range(tableSize), simulation/sparsity/SparsityOp.hpp:41-55. */
+int32_t zsmpm_table_clear__rocm(zsmpm_policy pol, const zsmpm_table* table);
+/* ComputeSparsity over range(n), SparsityOp.hpp:57-86: insert floor-div(floor(x/dx*... )
+ * block keys; new keys are numbered in order of first occurrence (seq semantics). */
+int32_t zsmpm_table_activate__rocm(zsmpm_policy pol, const zsmpm_table* table, const float* x, int64_t n,
+                                   float dx, int32_t block_len, int32_t offset, float displacement);
+/* EnlargeSparsity over range(table.size()), SparsityOp.hpp:88-112. */
+int32_t zsmpm_table_enlarge__rocm(zsmpm_policy pol, const zsmpm_table* table, const int32_t lo[3],
+                                  const int32_t hi[3]);
+/* HashTableView::query, container/HashTable.hpp:447-456 (modular probe wrap). */
+int32_t zsmpm_table_query__rocm(zsmpm_policy pol, const zsmpm_table* table, const int32_t* keys, int64_t n,
+                                int32_t* out);
+/* index_buckets_for_particles: SpatiallyCount + exclusive_scan + SpatiallyDistribute,
+ * simulation/particle/Query.tpp:30-56, SparsityOp.hpp:114-195; stable by particle index. */
+int32_t zsmpm_bucket_sort__rocm(zsmpm_policy pol, const zsmpm_table* table, const float* x, int64_t n,
+                                float dx, int32_t block_len, int32_t offset, float displacement,
+                                const zsmpm_buckets* buckets);
+/* CleanGridBlocks over Collapse{table.size(), 64}, simulation/grid/GridOp.hpp:54-69. */
+int32_t zsmpm_grid_clear__rocm(zsmpm_policy pol, const zsmpm_grid* grid, const zsmpm_table* table);
+/* P2GTransfer (APIC, quadratic) over range(n), simulation/transfer/P2G.hpp:45-127.
+ * `buckets` is the particle bucket sort by (blockLen 4, offset -2, displacement 0.5)
+ * keys; NULL computes it into policy scratch. Writes back logJp only. */
+int32_t zsmpm_p2g__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
+                        const zsmpm_table* table, const zsmpm_buckets* buckets, const zsmpm_grid* grid,
+                        float dt);
+/* ComputeGridBlockVelocity + ApplyBoundaryConditionOnGridBlocks per collider,
+ * GridOp.hpp:71-164. fold_rhs != 0 first adds the rhs channels to momentum
+ * (the explicit driver's convention). max_vel_sqr: device float (atomic max,
+ * caller-initialised like zpc's maxVel) or NULL. */
+int32_t zsmpm_grid_update__rocm(zsmpm_policy pol, const zsmpm_grid* grid, const zsmpm_table* table, float dt,
+                                const float extf[3], const zsmpm_plane_collider* colliders, int32_t ncolliders,
+                                int32_t fold_rhs, float* max_vel_sqr);
+/* G2PTransfer (APIC) over range(n), simulation/transfer/G2P.hpp:43-84. */
+int32_t zsmpm_g2p__rocm(zsmpm_policy pol, const zsmpm_particles* particles, const zsmpm_table* table,
+                        const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt);
+/* One explicit substep in the reconstructed driver's stage order: table clear,
+ * activation (blockLen 4, offset -2, displacement 0.5), enlargement {0,0,0}..{2,2,2},
+ * bucket sort, grid clear, P2G, rhs fold + grid velocity + colliders, G2P.
+ * table, buckets and grid hold that substep's structures on return. */
+int32_t zsmpm_substep__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
+                            const zsmpm_table* table, const zsmpm_buckets* buckets, const zsmpm_grid* grid,
+                            float dt, const float extf[3], const zsmpm_plane_collider* colliders,
+                            int32_t ncolliders);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZSMPM_H */

@@ -1,0 +1,278 @@
+// zsmpm oracle C API: TEST INFRASTRUCTURE ONLY (ctypes entry points over
+// zsmpm_oracle.hpp for tests/, smoke() and the cpu_baseline leg of bench.py).
+// Sequential semantics; see zsmpm_oracle.hpp for the reference citations.
+#include <cstdint>
+#include <cstring>
+#include <memory>
+
+#include "zsmpm_oracle.hpp"
+
+using namespace zso;
+
+extern "C" {
+
+/// Scene/step parameters shared by the f32 and f64 entry points. The f32
+/// entry points round each field to float once, the same rounding the
+/// product's C ABI applies to its float arguments.
+typedef struct {
+  double dt, dx;
+  double extf[3];
+  int32_t mat_kind;
+  double volume, E, nu, cohesion, beta, yield_surface;
+  int32_t volume_correction;
+  int32_t table_size;
+  int32_t ncols;
+  double col_origin[8][3];
+  double col_normal[8][3];
+  int32_t col_type[8];
+} zso_params;
+}
+
+namespace {
+
+template <typename T>
+StepParams<T> to_params(const zso_params* p, std::vector<PlaneCollider<T>>& cols) {
+  StepParams<T> sp;
+  sp.dt = (T)p->dt;
+  sp.dx = (T)p->dx;
+  for (int d = 0; d < 3; ++d) sp.extf[d] = (T)p->extf[d];
+  sp.mat.kind = p->mat_kind;
+  sp.mat.volume = (T)p->volume;
+  sp.mat.E = (T)p->E;
+  sp.mat.nu = (T)p->nu;
+  sp.mat.cohesion = (T)p->cohesion;
+  sp.mat.beta = (T)p->beta;
+  sp.mat.yieldSurface = (T)p->yield_surface;
+  sp.mat.volumeCorrection = p->volume_correction != 0;
+  sp.tableSize = p->table_size;
+  cols.resize(p->ncols);
+  for (int c = 0; c < p->ncols; ++c) {
+    for (int d = 0; d < 3; ++d) {
+      cols[c].origin[d] = (T)p->col_origin[c][d];
+      cols[c].normal[d] = (T)p->col_normal[c][d];
+    }
+    cols[c].type = p->col_type[c];
+  }
+  sp.cols = cols.data();
+  sp.ncols = p->ncols;
+  return sp;
+}
+
+template <typename T>
+struct Ctx {
+  StepOutputs<T> out;
+  std::vector<PlaneCollider<T>> cols;
+};
+
+template <typename T>
+ParticlesView<T> view(int64_t n, T* x, T* v, T* C, T* F, T* m, T* logJp) {
+  ParticlesView<T> ps;
+  ps.n = n;
+  ps.x = x;
+  ps.v = v;
+  ps.C = C;
+  ps.F = F;
+  ps.m = m;
+  ps.logJp = logJp;
+  return ps;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- math pieces
+int32_t zso_hash(int32_t k0, int32_t k1, int32_t k2) { return do_hash(k0, k1, k2); }
+int32_t zso_home_slot(int32_t k0, int32_t k1, int32_t k2, int32_t T) { return home_slot(k0, k1, k2, T); }
+int64_t zso_evaluate_table_size(int64_t n) { return evaluate_table_size(n); }
+
+void zso_svd_f32(int64_t n, const float* F, float* U, float* S, float* V) {
+  for (int64_t i = 0; i < n; ++i) svd_colmajor(F + 9 * i, U + 9 * i, S + 3 * i, V + 9 * i);
+}
+void zso_svd_f64(int64_t n, const double* F, double* U, double* S, double* V) {
+  for (int64_t i = 0; i < n; ++i) svd_colmajor(F + 9 * i, U + 9 * i, S + 3 * i, V + 9 * i);
+}
+void zso_lame_f32(float E, float nu, float* mu, float* lam) { lame(E, nu, *mu, *lam); }
+void zso_lame_f64(double E, double nu, double* mu, double* lam) { lame(E, nu, *mu, *lam); }
+
+void zso_stress_fc_f32(int64_t n, float vol, float mu, float lam, const float* F, float* PF) {
+  for (int64_t i = 0; i < n; ++i) stress_fixedcorotated(vol, mu, lam, F + 9 * i, PF + 9 * i);
+}
+void zso_stress_fc_f64(int64_t n, double vol, double mu, double lam, const double* F, double* PF) {
+  for (int64_t i = 0; i < n; ++i) stress_fixedcorotated(vol, mu, lam, F + 9 * i, PF + 9 * i);
+}
+void zso_stress_sand_f32(int64_t n, float vol, float mu, float lam, float coh, float beta, float ys,
+                         int32_t volcorr, float* logJp, const float* F, float* PF, float* Fproj,
+                         int32_t* which) {
+  for (int64_t i = 0; i < n; ++i) {
+    int w = stress_sand(vol, mu, lam, coh, beta, ys, volcorr != 0, logJp[i], F + 9 * i, PF + 9 * i,
+                        Fproj ? Fproj + 9 * i : nullptr);
+    if (which) which[i] = w;
+  }
+}
+void zso_stress_sand_f64(int64_t n, double vol, double mu, double lam, double coh, double beta,
+                         double ys, int32_t volcorr, double* logJp, const double* F, double* PF,
+                         double* Fproj, int32_t* which) {
+  for (int64_t i = 0; i < n; ++i) {
+    int w = stress_sand(vol, mu, lam, coh, beta, ys, volcorr != 0, logJp[i], F + 9 * i, PF + 9 * i,
+                        Fproj ? Fproj + 9 * i : nullptr);
+    if (which) which[i] = w;
+  }
+}
+
+/// LocalArena for n positions: corner (3), weights (27 in ndrange order),
+/// diffs (27 x 3).
+void zso_arena_f32(int64_t n, float dx, const float* x, int32_t* corner, float* w, float* diff) {
+  for (int64_t p = 0; p < n; ++p) {
+    Arena<float> ar;
+    ar.init(dx, x + 3 * p);
+    for (int d = 0; d < 3; ++d) corner[3 * p + d] = ar.corner[d];
+    int e = 0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k, ++e) {
+          w[27 * p + e] = ar.weight(i, j, k);
+          ar.diff(i, j, k, diff + 81 * p + 3 * e);
+        }
+  }
+}
+
+void zso_block_keys_f32(int64_t n, const float* x, float dx, int32_t blockLen, int32_t offset,
+                        float disp, int32_t* keys) {
+  float dxinv = (float)1.0 / dx;
+  for (int64_t p = 0; p < n; ++p) block_key(x + 3 * p, dxinv, blockLen, offset, disp, keys + 3 * p);
+}
+
+// ---------------------------------------------------------------- contexts
+void* zso_ctx_new_f32() { return new Ctx<float>(); }
+void* zso_ctx_new_f64() { return new Ctx<double>(); }
+void zso_ctx_free_f32(void* c) { delete static_cast<Ctx<float>*>(c); }
+void zso_ctx_free_f64(void* c) { delete static_cast<Ctx<double>*>(c); }
+
+/// sizes: [T, cnt, n_indices, n_grid_floats, quirk_hit]
+void zso_ctx_sizes_f32(void* c, int64_t* s) {
+  auto* ctx = static_cast<Ctx<float>*>(c);
+  s[0] = ctx->out.table.T;
+  s[1] = ctx->out.table.cnt;
+  s[2] = (int64_t)ctx->out.indices.size();
+  s[3] = (int64_t)ctx->out.grid.size();
+  s[4] = ctx->out.table.query_wrap_quirk_hit ? 1 : 0;
+}
+void zso_ctx_sizes_f64(void* c, int64_t* s) {
+  auto* ctx = static_cast<Ctx<double>*>(c);
+  s[0] = ctx->out.table.T;
+  s[1] = ctx->out.table.cnt;
+  s[2] = (int64_t)ctx->out.indices.size();
+  s[3] = (int64_t)ctx->out.grid.size();
+  s[4] = ctx->out.table.query_wrap_quirk_hit ? 1 : 0;
+}
+
+/// Copy out the table (keys 3T, indices T, status T, activeKeys 3*cnt).
+void zso_ctx_table_f32(void* c, int32_t* keys, int32_t* indices, int32_t* status, int32_t* active) {
+  auto& t = static_cast<Ctx<float>*>(c)->out.table;
+  std::memcpy(keys, t.keys.data(), sizeof(int32_t) * 3 * (size_t)t.T);
+  std::memcpy(indices, t.indices.data(), sizeof(int32_t) * (size_t)t.T);
+  std::memcpy(status, t.status.data(), sizeof(int32_t) * (size_t)t.T);
+  std::memcpy(active, t.active.data(), sizeof(int32_t) * 3 * (size_t)t.cnt);
+}
+/// Copy out the bucket sort (counts, offsets: cnt + 1; indices: n).
+void zso_ctx_sort_f32(void* c, int32_t* counts, int32_t* offsets, int32_t* indices) {
+  auto& o = static_cast<Ctx<float>*>(c)->out;
+  std::memcpy(counts, o.counts.data(), sizeof(int32_t) * o.counts.size());
+  std::memcpy(offsets, o.offsets.data(), sizeof(int32_t) * o.offsets.size());
+  std::memcpy(indices, o.indices.data(), sizeof(int32_t) * o.indices.size());
+}
+/// Copy out the grids: after P2G (m, mv, rhs) and after update + boundary.
+float zso_ctx_grids_f32(void* c, float* grid_p2g, float* grid_v) {
+  auto& o = static_cast<Ctx<float>*>(c)->out;
+  if (grid_p2g) std::memcpy(grid_p2g, o.grid.data(), sizeof(float) * o.grid.size());
+  if (grid_v) std::memcpy(grid_v, o.gridv.data(), sizeof(float) * o.gridv.size());
+  return o.maxVel;
+}
+double zso_ctx_grids_f64(void* c, double* grid_p2g, double* grid_v) {
+  auto& o = static_cast<Ctx<double>*>(c)->out;
+  if (grid_p2g) std::memcpy(grid_p2g, o.grid.data(), sizeof(double) * o.grid.size());
+  if (grid_v) std::memcpy(grid_v, o.gridv.data(), sizeof(double) * o.gridv.size());
+  return o.maxVel;
+}
+
+// ---------------------------------------------------------------- stages
+/// CleanSparsity + ComputeSparsity into the context's table of size T.
+void zso_activate_f32(void* c, int32_t T, int64_t n, const float* x, float dx, int32_t blockLen,
+                      int32_t offset, float disp) {
+  auto& t = static_cast<Ctx<float>*>(c)->out.table;
+  if (t.T != T) t.resize(T);
+  t.clear();
+  compute_sparsity(t, x, n, dx, blockLen, offset, disp);
+}
+void zso_enlarge_f32(void* c, const int32_t* lo, const int32_t* hi) {
+  enlarge_sparsity(static_cast<Ctx<float>*>(c)->out.table, lo, hi);
+}
+void zso_bucket_sort_f32(void* c, int64_t n, const float* x, float dx, int32_t blockLen,
+                         int32_t offset, float disp) {
+  auto& o = static_cast<Ctx<float>*>(c)->out;
+  o.counts.assign(o.table.cnt + 1, 0);
+  o.offsets.assign(o.table.cnt + 1, 0);
+  o.indices.assign(n, 0);
+  bucket_sort(o.table, x, n, dx, blockLen, offset, disp, o.counts.data(), o.offsets.data(),
+              o.indices.data());
+}
+/// Clean the grid of table.cnt blocks and run P2G; the grid stays in the context.
+void zso_p2g_f32(void* c, const zso_params* prm, int64_t n, float* x, float* v, float* C, float* F,
+                 float* m, float* logJp) {
+  auto* ctx = static_cast<Ctx<float>*>(c);
+  auto sp = to_params<float>(prm, ctx->cols);
+  auto& o = ctx->out;
+  o.grid.assign((size_t)o.table.cnt * kChannels * 64, 0.f);
+  p2g(sp.mat, sp.dt, sp.dx, view<float>(n, x, v, C, F, m, logJp), o.table, o.grid.data());
+}
+/// FP64 P2G of double particle arrays on the f32 context's table (so the
+/// grid layout matches); grid_out holds table.cnt * 7 * 64 doubles.
+void zso_p2g_f64_on(void* c32, const zso_params* prm, int64_t n, double* x, double* v, double* C, double* F,
+                    double* m, double* logJp, double* grid_out) {
+  auto* ctx = static_cast<Ctx<float>*>(c32);
+  std::vector<PlaneCollider<double>> cols;
+  auto sp = to_params<double>(prm, cols);
+  const int64_t nb = ctx->out.table.cnt;
+  for (int64_t i = 0; i < nb * kChannels * 64; ++i) grid_out[i] = 0.0;
+  p2g(sp.mat, sp.dt, sp.dx, view<double>(n, x, v, C, F, m, logJp), ctx->out.table, grid_out);
+}
+
+/// Fold, grid velocity and boundary on a copy of the P2G grid; returns maxVel.
+float zso_grid_update_f32(void* c, const zso_params* prm) {
+  auto* ctx = static_cast<Ctx<float>*>(c);
+  auto sp = to_params<float>(prm, ctx->cols);
+  auto& o = ctx->out;
+  o.gridv = o.grid;
+  const int64_t nb = o.table.cnt;
+  fold_rhs(o.gridv.data(), nb);
+  float mv = 0;
+  grid_velocity(o.gridv.data(), nb, sp.dt, sp.extf, mv);
+  grid_boundary(o.gridv.data(), o.table, nb, sp.dx, sp.cols, sp.ncols);
+  o.maxVel = mv;
+  return mv;
+}
+void zso_g2p_f32(void* c, const zso_params* prm, int64_t n, float* x, float* v, float* C, float* F) {
+  auto* ctx = static_cast<Ctx<float>*>(c);
+  auto sp = to_params<float>(prm, ctx->cols);
+  g2p(sp.dt, sp.dx, view<float>(n, x, v, C, F, nullptr, nullptr), ctx->out.table, ctx->out.gridv.data());
+}
+
+/// Run `steps` substeps in place; the context keeps the last substep's
+/// table, sort and grids.
+void zso_run_f32(void* c, const zso_params* prm, int32_t steps, int64_t n, float* x, float* v,
+                 float* C, float* F, float* m, float* logJp) {
+  auto* ctx = static_cast<Ctx<float>*>(c);
+  auto sp = to_params<float>(prm, ctx->cols);
+  for (int s = 0; s < steps; ++s)
+    substep(sp, view<float>(n, x, v, C, F, m, logJp), ctx->out, s == steps - 1);
+}
+void zso_run_f64(void* c, const zso_params* prm, int32_t steps, int64_t n, double* x, double* v,
+                 double* C, double* F, double* m, double* logJp) {
+  auto* ctx = static_cast<Ctx<double>*>(c);
+  auto sp = to_params<double>(prm, ctx->cols);
+  for (int s = 0; s < steps; ++s)
+    substep(sp, view<double>(n, x, v, C, F, m, logJp), ctx->out, s == steps - 1);
+}
+
+}  // extern "C"

@@ -1,0 +1,485 @@
+// C ABI of the zsmpm library (declared in include/zsmpm.h): policies with
+// their scratch workspace, descriptor validation and the stage entry points.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "../../include/zsmpm.h"
+#include "zsmpm_internal.hpp"
+
+using namespace zsmpm;
+
+struct zsmpm_policy_s {
+  int devid = 0;
+  hipStream_t stream = nullptr;
+  Workspace ws{};
+  std::vector<void*> allocs;
+  int64_t reserved_n = 0;
+  int32_t reserved_T = 0;
+  // internal buckets for calls that pass none
+  int32_t* own_counts = nullptr;
+  int32_t* own_offsets = nullptr;
+  int32_t* own_indices = nullptr;
+  int32_t own_cap_blocks = 0;
+};
+
+namespace {
+
+bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+
+template <typename D>
+int32_t check_header(const D* d) {
+  if (!d) return ZSMPM_INVALID_ARGUMENT;
+  if (d->hdr.major != ZSMPM_ABI_MAJOR) return ZSMPM_INCOMPATIBLE_MAJOR;
+  if (d->hdr.size < sizeof(D)) return ZSMPM_INSUFFICIENT_SIZE;
+  return ZSMPM_OK;
+}
+
+int32_t from_hip(hipError_t e) {
+  if (e == hipSuccess) return ZSMPM_OK;
+  if (e == hipErrorInvalidValue) return ZSMPM_INVALID_ARGUMENT;
+  if (e == hipErrorOutOfMemory) return ZSMPM_CAPACITY;
+  return ZSMPM_DEVICE_ERROR;
+}
+
+int32_t from_err_bits(uint32_t b) {
+  if (b & kErrTableFull) return ZSMPM_TABLE_FULL;
+  if (b & kErrKeyRange) return ZSMPM_KEY_RANGE;
+  if (b & kErrArena) return ZSMPM_STENCIL_OUTSIDE;
+  if (b & kErrMissingBlock) return ZSMPM_MISSING_BLOCK;
+  if (b & kErrCapacity) return ZSMPM_CAPACITY;
+  return ZSMPM_OK;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+template <typename T>
+hipError_t alloc(zsmpm_policy_s* p, T** out, size_t count) {
+  void* ptr = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(&ptr, count * sizeof(T));
+  if (e != hipSuccess) return e;
+  p->allocs.push_back(ptr);
+  *out = static_cast<T*>(ptr);
+  return hipSuccess;
+}
+
+void free_all(zsmpm_policy_s* p) {
+  for (void* a : p->allocs) (void)hipFree(a);
+  p->allocs.clear();
+  p->ws = Workspace{};
+  p->own_counts = p->own_offsets = p->own_indices = nullptr;
+  p->own_cap_blocks = 0;
+  p->reserved_n = 0;
+  p->reserved_T = 0;
+}
+
+int32_t reserve(zsmpm_policy_s* p, int64_t n, int32_t T) {
+  if (n < 0 || T < 0) return ZSMPM_INVALID_ARGUMENT;
+  if (T > 0 && !is_pow2(T)) return ZSMPM_INVALID_ARGUMENT;
+  if (n <= p->reserved_n && T <= p->reserved_T && !p->allocs.empty()) return ZSMPM_OK;
+  if (n > (int64_t)0x7fffffff) return ZSMPM_CAPACITY;
+  const int64_t nn = n > p->reserved_n ? n : p->reserved_n;
+  const int32_t TT = T > p->reserved_T ? T : p->reserved_T;
+  DeviceGuard g(p->devid);
+  (void)hipStreamSynchronize(p->stream);
+  free_all(p);
+  Workspace& w = p->ws;
+  w.n_cap = (uint32_t)(nn > 0 ? nn : 1);
+  w.ts_cap = (uint32_t)(TT > 0 ? TT : 1024);
+  w.list_cap = w.ts_cap;
+  hipError_t e = hipSuccess;
+#define ZA(ptr, cnt) \
+  if (e == hipSuccess) e = alloc(p, &(ptr), (size_t)(cnt))
+  ZA(w.skey, w.ts_cap);
+  ZA(w.sorder, w.ts_cap);
+  ZA(w.la_key, w.list_cap);
+  ZA(w.la_val, w.list_cap);
+  ZA(w.lb_key, w.list_cap);
+  ZA(w.lb_val, w.list_cap);
+  ZA(w.bkt, w.n_cap);
+  ZA(w.run_start, w.n_cap);
+  ZA(w.rk_a, w.n_cap);
+  ZA(w.rv_a, w.n_cap);
+  ZA(w.rk_b, w.n_cap);
+  ZA(w.rv_b, w.n_cap);
+  ZA(w.run_len, w.n_cap);
+  ZA(w.run_dest, w.n_cap);
+  ZA(w.tile_heads, w.n_cap / 4096 + 2);
+  ZA(w.tile_base, w.n_cap / 4096 + 2);
+  const uint32_t big = w.n_cap > w.ts_cap ? w.n_cap : w.ts_cap;
+  w.rs_words = radix_sort_scratch_words(big);
+  ZA(w.rs_scratch, w.rs_words);
+  w.scan_words = 16384 + 64;
+  ZA(w.scan_tmp, w.scan_words);
+  ZA(w.counters, kCtrCount);
+  ZA(w.err, 4);
+  p->own_cap_blocks = (int32_t)w.ts_cap;
+  ZA(p->own_counts, (size_t)p->own_cap_blocks + 1);
+  ZA(p->own_offsets, (size_t)p->own_cap_blocks + 1);
+  ZA(p->own_indices, w.n_cap);
+#undef ZA
+  if (e == hipSuccess) e = hipMemsetAsync(w.counters, 0, sizeof(uint32_t) * kCtrCount, p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(w.err, 0, sizeof(uint32_t) * 4, p->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+  if (e != hipSuccess) {
+    free_all(p);
+    return from_hip(e);
+  }
+  p->reserved_n = nn;
+  p->reserved_T = TT;
+  return ZSMPM_OK;
+}
+
+int32_t view_table(const zsmpm_table* d, TableView& t) {
+  int32_t r = check_header(d);
+  if (r) return r;
+  if (!is_pow2(d->table_size) || !d->keys || !d->indices || !d->status || !d->cnt || !d->active_keys)
+    return ZSMPM_INVALID_ARGUMENT;
+  t.T = d->table_size;
+  t.keys = d->keys;
+  t.indices = d->indices;
+  t.status = d->status;
+  t.cnt = d->cnt;
+  t.active = d->active_keys;
+  return ZSMPM_OK;
+}
+
+int32_t view_particles(const zsmpm_particles* d, ParticlesView& ps, bool need_state) {
+  int32_t r = check_header(d);
+  if (r) return r;
+  if (d->n < 0 || !d->x) return ZSMPM_INVALID_ARGUMENT;
+  if (need_state && (!d->v || !d->C || !d->F || !d->m)) return ZSMPM_INVALID_ARGUMENT;
+  ps.n = d->n;
+  ps.x = d->x;
+  ps.v = d->v;
+  ps.C = d->C;
+  ps.F = d->F;
+  ps.m = d->m;
+  ps.logJp = d->logJp;
+  return ZSMPM_OK;
+}
+
+int32_t view_grid(const zsmpm_grid* d) {
+  int32_t r = check_header(d);
+  if (r) return r;
+  if (!d->data || d->capacity_blocks <= 0 || !(d->dx > 0.f)) return ZSMPM_INVALID_ARGUMENT;
+  return ZSMPM_OK;
+}
+
+int32_t view_buckets(const zsmpm_buckets* d, SortView& sv) {
+  int32_t r = check_header(d);
+  if (r) return r;
+  if (!d->counts || !d->offsets || !d->indices || d->capacity_blocks <= 0) return ZSMPM_INVALID_ARGUMENT;
+  sv.counts = d->counts;
+  sv.offsets = d->offsets;
+  sv.indices = d->indices;
+  sv.cap_blocks = d->capacity_blocks;
+  return ZSMPM_OK;
+}
+
+int32_t make_model(const zsmpm_model* d, Model& m) {
+  int32_t r = check_header(d);
+  if (r) return r;
+  if (d->kind != ZSMPM_FIXED_COROTATED && d->kind != ZSMPM_DRUCKER_PRAGER) return ZSMPM_UNSUPPORTED_OPERATION;
+  m.kind = d->kind;
+  m.volume = d->volume;
+  // lame_parameters (physics/ConstitutiveModel.hpp:34-38); mu is formed in double
+  const float onep = 1 + d->nu;
+  m.mu = (float)(0.5 * (double)d->E / (double)onep);
+  m.lambda = d->E * d->nu / ((1 + d->nu) * (1 - 2 * d->nu));
+  m.cohesion = d->cohesion;
+  m.beta = d->beta;
+  m.yield_surface = d->yield_surface;
+  m.volume_correction = d->volume_correction;
+  m.fp64_gap = d->svd_fp64_gap;
+  return ZSMPM_OK;
+}
+
+int32_t make_colliders(const zsmpm_plane_collider* c, int32_t n, Colliders& cols) {
+  if (n < 0 || n > kMaxColliders || (n > 0 && !c)) return ZSMPM_INVALID_ARGUMENT;
+  cols.n = n;
+  for (int i = 0; i < n; ++i) {
+    if (c[i].type < 0 || c[i].type > 2) return ZSMPM_INVALID_ARGUMENT;
+    cols.c[i].type = c[i].type;
+    for (int d = 0; d < 3; ++d) {
+      cols.c[i].origin[d] = c[i].origin[d];
+      cols.c[i].normal[d] = c[i].normal[d];
+    }
+  }
+  return ZSMPM_OK;
+}
+
+int32_t ensure(zsmpm_policy_s* p, int64_t n, int32_t T) {
+  if (n <= p->reserved_n && T <= p->reserved_T && !p->allocs.empty()) return ZSMPM_OK;
+  return reserve(p, n > p->reserved_n ? n : p->reserved_n, T > p->reserved_T ? T : p->reserved_T);
+}
+
+SortView own_buckets(zsmpm_policy_s* p) {
+  SortView sv;
+  sv.counts = p->own_counts;
+  sv.offsets = p->own_offsets;
+  sv.indices = p->own_indices;
+  sv.cap_blocks = p->own_cap_blocks;
+  return sv;
+}
+
+__global__ void set_maxvel_kernel(uint32_t* counters) { counters[kCtrMaxVel] = 0u; }
+__global__ void copy_maxvel_kernel(const uint32_t* counters, float* out) {
+  atomicMax(reinterpret_cast<uint32_t*>(out), counters[kCtrMaxVel]);
+}
+
+#define ZS_TRY(expr)              \
+  do {                            \
+    int32_t _r = (expr);          \
+    if (_r != ZSMPM_OK) return _r; \
+  } while (0)
+#define ZS_HIP(expr) ZS_TRY(from_hip(expr))
+
+const int32_t kStdKey[3] = {0, 0, 0};
+
+int32_t run_buckets(zsmpm_policy_s* p, const TableView& t, const ParticlesView& ps, float dx, SortView& sv) {
+  return from_hip(bucket_sort(t, p->ws, ps.x, ps.n, dx, 4, -2, 0.5f, sv, p->stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t zsmpm_abi_version(zsmpm_abi_header* out) {
+  if (!out) return ZSMPM_INVALID_ARGUMENT;
+  out->size = sizeof(zsmpm_abi_header);
+  out->major = ZSMPM_ABI_MAJOR;
+  out->minor = ZSMPM_ABI_MINOR;
+  return ZSMPM_OK;
+}
+
+int32_t zsmpm_policy__rocm(int32_t devid, void* stream, zsmpm_policy* out) {
+  if (!out) return ZSMPM_INVALID_ARGUMENT;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || devid < 0 || devid >= ndev) return ZSMPM_INVALID_ARGUMENT;
+  {
+    DeviceGuard g(devid);
+    if (check_launch_limits(devid) != hipSuccess) return ZSMPM_UNSUPPORTED_OPERATION;
+  }
+  zsmpm_policy_s* p = new (std::nothrow) zsmpm_policy_s();
+  if (!p) return ZSMPM_ERROR;
+  p->devid = devid;
+  p->stream = static_cast<hipStream_t>(stream);
+  *out = p;
+  return ZSMPM_OK;
+}
+
+int32_t zsmpm_del_policy__rocm(zsmpm_policy pol) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
+  {
+    DeviceGuard g(pol->devid);
+    (void)hipStreamSynchronize(pol->stream);
+    free_all(pol);
+  }
+  delete pol;
+  return ZSMPM_OK;
+}
+
+int32_t zsmpm_policy_reserve__rocm(zsmpm_policy pol, int64_t max_particles, int32_t table_size) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
+  return reserve(pol, max_particles, table_size);
+}
+
+int32_t zsmpm_policy_sync(zsmpm_policy pol) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
+  DeviceGuard g(pol->devid);
+  ZS_HIP(hipStreamSynchronize(pol->stream));
+  if (!pol->ws.err) return ZSMPM_OK;
+  uint32_t bits = 0;
+  ZS_HIP(hipMemcpy(&bits, pol->ws.err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (bits) ZS_HIP(hipMemset(pol->ws.err, 0, sizeof(uint32_t)));
+  return from_err_bits(bits);
+}
+
+int32_t zsmpm_policy_counters(zsmpm_policy pol, uint32_t* out) {
+  if (!pol || !out) return ZSMPM_INVALID_ARGUMENT;
+  memset(out, 0, sizeof(uint32_t) * kCtrCount);
+  if (!pol->ws.counters) return ZSMPM_OK;
+  DeviceGuard g(pol->devid);
+  ZS_HIP(hipStreamSynchronize(pol->stream));
+  ZS_HIP(hipMemcpy(out, pol->ws.counters, sizeof(uint32_t) * kCtrCount, hipMemcpyDeviceToHost));
+  return ZSMPM_OK;
+}
+
+int32_t zsmpm_table_clear__rocm(zsmpm_policy pol, const zsmpm_table* table) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
+  TableView t;
+  ZS_TRY(view_table(table, t));
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, pol->reserved_n, t.T));
+  return from_hip(table_clear(t, pol->ws, pol->stream));
+}
+
+int32_t zsmpm_table_activate__rocm(zsmpm_policy pol, const zsmpm_table* table, const float* x, int64_t n,
+                                   float dx, int32_t block_len, int32_t offset, float displacement) {
+  if (!pol || n < 0 || (n > 0 && !x) || block_len <= 0 || !(dx > 0.f)) return ZSMPM_INVALID_ARGUMENT;
+  TableView t;
+  ZS_TRY(view_table(table, t));
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, n, t.T));
+  return from_hip(table_activate(t, pol->ws, x, n, dx, block_len, offset, displacement, pol->stream));
+}
+
+int32_t zsmpm_table_enlarge__rocm(zsmpm_policy pol, const zsmpm_table* table, const int32_t lo[3],
+                                  const int32_t hi[3]) {
+  if (!pol || !lo || !hi) return ZSMPM_INVALID_ARGUMENT;
+  TableView t;
+  ZS_TRY(view_table(table, t));
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, pol->reserved_n, t.T));
+  return from_hip(table_enlarge(t, pol->ws, lo, hi, t.T, pol->stream));
+}
+
+int32_t zsmpm_table_query__rocm(zsmpm_policy pol, const zsmpm_table* table, const int32_t* keys, int64_t n,
+                                int32_t* out) {
+  if (!pol || n < 0 || (n > 0 && (!keys || !out))) return ZSMPM_INVALID_ARGUMENT;
+  TableView t;
+  ZS_TRY(view_table(table, t));
+  DeviceGuard g(pol->devid);
+  return from_hip(table_query(t, keys, n, out, pol->stream));
+}
+
+int32_t zsmpm_bucket_sort__rocm(zsmpm_policy pol, const zsmpm_table* table, const float* x, int64_t n, float dx,
+                                int32_t block_len, int32_t offset, float displacement,
+                                const zsmpm_buckets* buckets) {
+  if (!pol || n < 0 || (n > 0 && !x) || block_len <= 0 || !(dx > 0.f)) return ZSMPM_INVALID_ARGUMENT;
+  TableView t;
+  SortView sv;
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(view_buckets(buckets, sv));
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, n, t.T));
+  return from_hip(bucket_sort(t, pol->ws, x, n, dx, block_len, offset, displacement, sv, pol->stream));
+}
+
+int32_t zsmpm_grid_clear__rocm(zsmpm_policy pol, const zsmpm_grid* grid, const zsmpm_table* table) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
+  TableView t;
+  ZS_TRY(view_grid(grid));
+  ZS_TRY(view_table(table, t));
+  DeviceGuard g(pol->devid);
+  return from_hip(zsmpm::grid_clear(grid->data, t.cnt, grid->capacity_blocks, pol->stream));
+}
+
+int32_t zsmpm_p2g__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
+                        const zsmpm_table* table, const zsmpm_buckets* buckets, const zsmpm_grid* grid,
+                        float dt) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
+  Model m;
+  ParticlesView ps;
+  TableView t;
+  ZS_TRY(make_model(model, m));
+  ZS_TRY(view_particles(particles, ps, true));
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(view_grid(grid));
+  if (m.kind == ZSMPM_DRUCKER_PRAGER && !ps.logJp) return ZSMPM_INVALID_ARGUMENT;
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, ps.n, t.T));
+  SortView sv;
+  if (buckets) {
+    ZS_TRY(view_buckets(buckets, sv));
+  } else {
+    sv = own_buckets(pol);
+    ZS_TRY(run_buckets(pol, t, ps, grid->dx, sv));
+  }
+  return from_hip(zsmpm::p2g(m, ps, t, sv, grid->data, grid->capacity_blocks, dt, grid->dx, false, pol->ws, pol->stream));
+}
+
+int32_t zsmpm_grid_update__rocm(zsmpm_policy pol, const zsmpm_grid* grid, const zsmpm_table* table, float dt,
+                                const float extf[3], const zsmpm_plane_collider* colliders, int32_t ncolliders,
+                                int32_t fold_rhs, float* max_vel_sqr) {
+  if (!pol || !extf) return ZSMPM_INVALID_ARGUMENT;
+  TableView t;
+  Colliders cols;
+  ZS_TRY(view_grid(grid));
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(make_colliders(colliders, ncolliders, cols));
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, pol->reserved_n, t.T));
+  set_maxvel_kernel<<<1, 1, 0, pol->stream>>>(pol->ws.counters);
+  ZS_HIP(zsmpm::grid_update(grid->data, grid->capacity_blocks, t, dt, extf, cols, grid->dx, fold_rhs, pol->ws, pol->stream));
+  if (max_vel_sqr) copy_maxvel_kernel<<<1, 1, 0, pol->stream>>>(pol->ws.counters, max_vel_sqr);
+  return from_hip(hipGetLastError());
+}
+
+int32_t zsmpm_g2p__rocm(zsmpm_policy pol, const zsmpm_particles* particles, const zsmpm_table* table,
+                        const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
+  ParticlesView ps;
+  TableView t;
+  ZS_TRY(view_particles(particles, ps, true));
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(view_grid(grid));
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, ps.n, t.T));
+  SortView sv;
+  if (buckets) {
+    ZS_TRY(view_buckets(buckets, sv));
+  } else {
+    sv = own_buckets(pol);
+    ZS_TRY(run_buckets(pol, t, ps, grid->dx, sv));
+  }
+  return from_hip(zsmpm::g2p(ps, t, sv, grid->data, grid->capacity_blocks, dt, grid->dx, pol->ws, pol->stream));
+}
+
+int32_t zsmpm_substep__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
+                            const zsmpm_table* table, const zsmpm_buckets* buckets, const zsmpm_grid* grid,
+                            float dt, const float extf[3], const zsmpm_plane_collider* colliders,
+                            int32_t ncolliders) {
+  if (!pol || !extf) return ZSMPM_INVALID_ARGUMENT;
+  Model m;
+  ParticlesView ps;
+  TableView t;
+  Colliders cols;
+  SortView sv;
+  ZS_TRY(make_model(model, m));
+  ZS_TRY(view_particles(particles, ps, true));
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(view_grid(grid));
+  ZS_TRY(make_colliders(colliders, ncolliders, cols));
+  if (m.kind == ZSMPM_DRUCKER_PRAGER && !ps.logJp) return ZSMPM_INVALID_ARGUMENT;
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, ps.n, t.T));
+  if (buckets) {
+    ZS_TRY(view_buckets(buckets, sv));
+  } else {
+    sv = own_buckets(pol);
+  }
+  const float dx = grid->dx;
+  hipStream_t s = pol->stream;
+  Workspace& ws = pol->ws;
+  const int32_t lo[3] = {0, 0, 0}, hi[3] = {2, 2, 2};
+  ZS_HIP(table_clear(t, ws, s));
+  ZS_HIP(table_activate(t, ws, ps.x, ps.n, dx, 4, -2, 0.5f, s));
+  ZS_HIP(table_enlarge(t, ws, lo, hi, t.T / 8 > 0 ? t.T / 8 : 1, s));
+  ZS_HIP(bucket_sort(t, ws, ps.x, ps.n, dx, 4, -2, 0.5f, sv, s));
+  ZS_HIP(zsmpm::grid_clear(grid->data, t.cnt, grid->capacity_blocks, s));
+  // P2G folds the force into the momentum channels (the driver's
+  // v = (mv + rhs)/m convention), so the update must not add channels 4..6
+  ZS_HIP(zsmpm::p2g(m, ps, t, sv, grid->data, grid->capacity_blocks, dt, dx, true, ws, s));
+  set_maxvel_kernel<<<1, 1, 0, s>>>(ws.counters);
+  ZS_HIP(zsmpm::grid_update(grid->data, grid->capacity_blocks, t, dt, extf, cols, dx, 0, ws, s));
+  ZS_HIP(zsmpm::g2p(ps, t, sv, grid->data, grid->capacity_blocks, dt, dx, ws, s));
+  (void)kStdKey;
+  return ZSMPM_OK;
+}
+
+}  // extern "C"

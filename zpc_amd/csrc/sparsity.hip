@@ -1,0 +1,567 @@
+// Block activation, neighbour enlargement and the particle -> block bucket
+// sort on zpc's HashTable<i32,3,int> layout, for gfx950.
+//
+// zpc inserts keys with per-slot spin locks and an atomic block counter
+// (container/HashTable.hpp:356-398), so under its parallel policies the block
+// numbers and slot layout depend on timing. Here they are deterministic and
+// equal to zpc's sequential semantics:
+//   * numbering: a key gets the next block number in order of its first
+//     occurrence (particle index for activation, (block, dx, dy, dz) loop
+//     order for enlargement) -- found with a dedup hash holding the minimum
+//     order per key, then a radix sort of the new keys by that order;
+//   * slots: keys are placed by priority linear probing on the reference
+//     probe sequence (home = (h % T + T) % T, step 127) with priority = block
+//     number; the result equals inserting serially in block-number order
+//     (Shun & Blelloch, "Phase-Concurrent Hash Tables for Determinism",
+//     SPAA 2014), i.e. the layout zpc's seq_exec path produces.
+// The bucket sort is stable by particle index (zpc's seq semantics of
+// SpatiallyCount/SpatiallyDistribute): particles are cut into runs of equal
+// bucket, runs are radix-sorted by bucket (stable) and every particle lands at
+// its run's destination plus its offset in the run.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "zsmpm_device.hpp"
+#include "zsmpm_internal.hpp"
+
+namespace zsmpm {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kTileItems = 16;
+constexpr int kTile = kThreads * kTileItems;  // 4096 particles per bucket-sort tile
+
+inline int groups_for(int64_t n, int threads) {
+  int64_t g = (n + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > 65535 * 8) g = 65535 * 8;
+  return (int)g;
+}
+
+// --------------------------------------------------------------- clearing
+__global__ void clear_table_kernel(TableView t, uint64_t* skey, uint32_t* sorder, uint32_t ts,
+                                   uint32_t* counters) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < t.T; e += stride) {
+    t.keys[3 * e] = kKeySentinel;
+    t.keys[3 * e + 1] = kKeySentinel;
+    t.keys[3 * e + 2] = kKeySentinel;
+    t.indices[e] = -1;
+    t.status[e] = -1;
+  }
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ts; e += stride) {
+    skey[e] = kEmpty64;
+    sorder[e] = 0xffffffffu;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *t.cnt = 0;
+    counters[kCtrList] = 0;
+  }
+}
+
+__global__ void clear_scratch_kernel(uint64_t* skey, uint32_t* sorder, uint32_t ts,
+                                     const int32_t* cnt, uint32_t* counters) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ts; e += stride) {
+    skey[e] = kEmpty64;
+    sorder[e] = 0xffffffffu;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    counters[kCtrList] = 0;
+    counters[kCtrCnt0] = (uint32_t)*cnt;
+  }
+}
+
+// ----------------------------------------------------------- dedup inserts
+__device__ __forceinline__ void dedup_insert(uint64_t pk, uint32_t order, uint64_t* skey,
+                                             uint32_t* sorder, uint32_t ts, uint32_t* err) {
+  uint32_t s = (uint32_t)mix64(pk) & (ts - 1u);
+  for (uint32_t probe = 0; probe < ts; ++probe) {
+    const uint64_t prev = atomicCAS((unsigned long long*)&skey[s], (unsigned long long)kEmpty64,
+                                    (unsigned long long)pk);
+    if (prev == kEmpty64 || prev == pk) {
+      atomicMin(&sorder[s], order);
+      return;
+    }
+    s = (s + 1u) & (ts - 1u);
+  }
+  atomicOr(err, kErrTableFull);
+}
+
+// ComputeSparsity candidates: one per particle, order = particle index. Lanes
+// that repeat the previous lane's key skip the insert (the run head carries
+// the smallest index of the run).
+__global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restrict__ x, int64_t n,
+                                                            float dxinv, int blockLen, int offset,
+                                                            float disp, uint64_t* skey, uint32_t* sorder,
+                                                            uint32_t ts, uint32_t* err) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    const int64_t p = base + threadIdx.x;
+    const bool valid = p < n;
+    uint64_t pk = kEmpty64;
+    if (valid) {
+      int32_t k0, k1, k2;
+      block_key(x[3 * p], x[3 * p + 1], x[3 * p + 2], dxinv, blockLen, offset, disp, k0, k1, k2);
+      if (key_packable(k0, k1, k2)) {
+        pk = pack_key(k0, k1, k2);
+      } else {
+        atomicOr(err, kErrKeyRange);
+      }
+    }
+    const uint64_t prev = __shfl_up(pk, 1, kWave);
+    const bool head = valid && pk != kEmpty64 && (lane_id() == 0 || prev != pk);
+    if (head) dedup_insert(pk, (uint32_t)p, skey, sorder, ts, err);
+  }
+}
+
+// EnlargeSparsity candidates: key_i + (dx, dy, dz) for i < n0 in loop order.
+__global__ __launch_bounds__(kThreads) void enlarge_kernel(const int32_t* __restrict__ active,
+                                                           const uint32_t* counters, int32_t lo0, int32_t lo1,
+                                                           int32_t lo2, int32_t e0, int32_t e1, int32_t e2,
+                                                           uint64_t* skey, uint32_t* sorder, uint32_t ts,
+                                                           uint32_t* err) {
+  const int64_t n0 = counters[kCtrCnt0];
+  const int64_t nd = (int64_t)e0 * e1 * e2;
+  const int64_t total = n0 * nd;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < total; c += stride) {
+    const int64_t i = c / nd;
+    const int32_t j = (int32_t)(c - i * nd);
+    const int32_t dz = j % e2, dy = (j / e2) % e1, dx = j / (e1 * e2);
+    const int32_t k0 = active[3 * i] + lo0 + dx, k1 = active[3 * i + 1] + lo1 + dy,
+                  k2 = active[3 * i + 2] + lo2 + dz;
+    if (!key_packable(k0, k1, k2)) {
+      atomicOr(err, kErrKeyRange);
+      continue;
+    }
+    dedup_insert(pack_key(k0, k1, k2), (uint32_t)c, skey, sorder, ts, err);
+  }
+}
+
+// Gather the dedup entries whose key is not in the table yet: (order, slot).
+__global__ __launch_bounds__(kThreads) void collect_new_kernel(const uint64_t* __restrict__ skey,
+                                                               const uint32_t* __restrict__ sorder,
+                                                               uint32_t ts, TableView t, uint32_t* counters,
+                                                               uint32_t* lkey, uint32_t* lval,
+                                                               uint32_t list_cap, uint32_t* err) {
+  const bool table_empty = counters[kCtrCnt0] == 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < ts; base += stride) {
+    const int64_t s = base + threadIdx.x;
+    bool take = false;
+    if (s < ts) {
+      const uint64_t pk = skey[s];
+      if (pk != kEmpty64) {
+        take = true;
+        if (!table_empty) {
+          int32_t k0, k1, k2;
+          unpack_key(pk, k0, k1, k2);
+          take = table_find(t, k0, k1, k2) < 0;
+        }
+      }
+    }
+    const uint64_t m = ballot(take);
+    if (m == 0) continue;
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t wbase = 0;
+    if (lane_id() == leader) wbase = atomicAdd(&counters[kCtrList], (uint32_t)__popcll(m));
+    wbase = shfl(wbase, leader);
+    if (take) {
+      const uint32_t slot = wbase + (uint32_t)__popcll(m & lanemask_lt());
+      if (slot < list_cap) {
+        lkey[slot] = sorder[s];
+        lval[slot] = (uint32_t)s;
+      } else {
+        atomicOr(err, kErrCapacity);
+      }
+    }
+  }
+}
+
+// Number the new keys cnt0 + rank, record _activeKeys and place them by
+// priority linear probing on indices[] (lower block number wins a slot).
+__global__ __launch_bounds__(kThreads) void assign_kernel(const uint32_t* __restrict__ sorted_slot,
+                                                          const uint64_t* __restrict__ skey,
+                                                          TableView t, const uint32_t* counters,
+                                                          uint32_t list_cap, uint32_t* err) {
+  const uint32_t nl = min(counters[kCtrList], list_cap);
+  const uint32_t cnt0 = counters[kCtrCnt0];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nl; i += stride) {
+    const uint64_t pk = skey[sorted_slot[i]];
+    int32_t k0, k1, k2;
+    unpack_key(pk, k0, k1, k2);
+    const uint32_t rank = cnt0 + (uint32_t)i;
+    if (rank >= (uint32_t)t.T) {
+      atomicOr(err, kErrTableFull);
+      continue;
+    }
+    t.active[3 * (int64_t)rank] = k0;
+    t.active[3 * (int64_t)rank + 1] = k1;
+    t.active[3 * (int64_t)rank + 2] = k2;
+    // carried element: starts as this key; a displaced lower-priority entry
+    // continues from the slot after the one it lost
+    uint32_t cur = rank;
+    int32_t s = zpc_home(k0, k1, k2, t.T);
+    int64_t steps = 0;
+    const int64_t limit = 2ll * t.T + 64;
+    while (true) {
+      const uint32_t old = atomicMin((uint32_t*)&t.indices[s], cur);
+      if (old == 0xffffffffu) break;
+      if (old > cur) cur = old;
+      s = probe_next(s, t.T);
+      if (++steps > limit) {
+        atomicOr(err, kErrTableFull);
+        break;
+      }
+    }
+  }
+}
+
+// Write keys[] of every newly placed entry: block r probes from its home
+// until it finds the slot holding r.
+__global__ __launch_bounds__(kThreads) void fill_keys_kernel(TableView t, const uint32_t* counters,
+                                                             uint32_t list_cap, uint32_t* err) {
+  const uint32_t nl = min(counters[kCtrList], list_cap);
+  const uint32_t cnt0 = counters[kCtrCnt0];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nl; i += stride) {
+    const int32_t r = (int32_t)(cnt0 + (uint32_t)i);
+    if (r >= t.T) continue;
+    const int32_t k0 = t.active[3 * (int64_t)r], k1 = t.active[3 * (int64_t)r + 1],
+                  k2 = t.active[3 * (int64_t)r + 2];
+    int32_t s = zpc_home(k0, k1, k2, t.T);
+    int32_t probe = 0;
+    for (; probe < t.T; ++probe) {
+      if (t.indices[s] == r) break;
+      s = probe_next(s, t.T);
+    }
+    if (probe == t.T) {
+      atomicOr(err, kErrTableFull);
+      continue;
+    }
+    t.keys[3 * (int64_t)s] = k0;
+    t.keys[3 * (int64_t)s + 1] = k1;
+    t.keys[3 * (int64_t)s + 2] = k2;
+  }
+}
+
+__global__ void bump_cnt_kernel(TableView t, const uint32_t* counters, uint32_t list_cap) {
+  const uint32_t nl = min(counters[kCtrList], list_cap);
+  const int64_t c = (int64_t)counters[kCtrCnt0] + nl;
+  *t.cnt = (int32_t)(c < t.T ? c : t.T);
+}
+
+__global__ void query_kernel(TableView t, const int32_t* __restrict__ keys, int64_t n, int32_t* out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = table_find(t, keys[3 * i], keys[3 * i + 1], keys[3 * i + 2]);
+}
+
+// ------------------------------------------------------------ bucket sort
+__global__ void zero_u32_kernel(uint32_t* p, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
+}
+
+// Bucket of every particle (the block number its key maps to), per-bucket
+// counts and the number of run heads in each tile. Lanes query the table only
+// at run heads (and lane 0), the rest take the head's bucket.
+__global__ __launch_bounds__(kThreads) void bucket_kernel(const float* __restrict__ x, int64_t n, float dxinv,
+                                                          int blockLen, int offset, float disp, TableView t,
+                                                          uint32_t* __restrict__ bkt, uint32_t* counts,
+                                                          int32_t cap_blocks, uint32_t* err) {
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  const int lane = lane_id();
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (int r = 0; r < kTileItems; ++r) {
+      const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
+      const bool valid = p < n;
+      int32_t k0 = 0, k1 = 0, k2 = 0;
+      if (valid) block_key(x[3 * p], x[3 * p + 1], x[3 * p + 2], dxinv, blockLen, offset, disp, k0, k1, k2);
+      int32_t q0 = __shfl_up(k0, 1, kWave), q1 = __shfl_up(k1, 1, kWave), q2 = __shfl_up(k2, 1, kWave);
+      if (lane == 0 && valid && p > 0)
+        block_key(x[3 * (p - 1)], x[3 * (p - 1) + 1], x[3 * (p - 1) + 2], dxinv, blockLen, offset, disp, q0,
+                  q1, q2);
+      const bool head = valid && (p == 0 || q0 != k0 || q1 != k1 || q2 != k2);
+      const bool lookup = valid && (head || lane == 0);
+      int32_t b = -1;
+      if (lookup) {
+        b = table_find(t, k0, k1, k2);
+        if (b < 0) atomicOr(err, kErrMissingBlock);
+        if (b >= cap_blocks) {
+          atomicOr(err, kErrCapacity);
+          b = -1;
+        }
+      }
+      const uint64_t lm = ballot(lookup);
+      const uint64_t le = lanemask_lt() | (1ull << lane);
+      const int src = 63 - __clzll(lm & le);
+      b = shfl(b, src < 0 ? 0 : src);
+      if (valid) bkt[p] = (uint32_t)(b < 0 ? 0 : b);
+      // per-bucket counts: one atomic per (lookup segment) in the wave
+      const uint64_t vm = ballot(valid);
+      if (lookup && b >= 0) {
+        const uint64_t above = lm & ~le;
+        const int next = above ? (__ffsll((long long)above) - 1) : 64;
+        const uint64_t seg = (next >= 64 ? ~0ull : ((1ull << next) - 1ull)) & ~((1ull << lane) - 1ull);
+        atomicAdd(&counts[b], (uint32_t)__popcll(seg & vm));
+      }
+    }
+  }
+}
+
+// Run heads per tile, counted on the bucket ids themselves (the same test
+// run_emit and particle_scatter apply), so the three passes agree on the runs
+// even where a key found no bucket.
+__global__ __launch_bounds__(kThreads) void run_heads_kernel(const uint32_t* __restrict__ bkt, int64_t n,
+                                                             uint32_t* tile_heads) {
+  __shared__ uint32_t wsum[kThreads / kWave];
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t p0 = tile * kTile + (int64_t)threadIdx.x * kTileItems;
+    uint32_t prev = (p0 > 0 && p0 - 1 < n) ? bkt[p0 - 1] : 0xffffffffu;
+    uint32_t heads = 0;
+#pragma unroll
+    for (int i = 0; i < kTileItems; ++i) {
+      const int64_t p = p0 + i;
+      if (p < n) {
+        const uint32_t b = bkt[p];
+        heads += (p == 0 || b != prev) ? 1u : 0u;
+        prev = b;
+      }
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) heads += __shfl_xor(heads, o, kWave);
+    if (lane_id() == 0) wsum[threadIdx.x / kWave] = heads;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int i = 0; i < kThreads / kWave; ++i) t += wsum[i];
+      tile_heads[tile] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// Per tile, blocked layout (thread t owns kTileItems consecutive particles):
+// exclusive count of run heads before each particle. Returns the global run
+// id of the run containing particle p (valid only for p < n).
+__device__ __forceinline__ void tile_run_ids(const uint32_t* __restrict__ bkt, int64_t n, int64_t tile,
+                                             uint32_t tile_base, uint32_t (&rid)[kTileItems],
+                                             bool (&headf)[kTileItems], uint32_t* lds) {
+  const int64_t p0 = tile * kTile + (int64_t)threadIdx.x * kTileItems;
+  uint32_t prev = (p0 > 0 && p0 - 1 < n) ? bkt[p0 - 1] : 0xffffffffu;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int i = 0; i < kTileItems; ++i) {
+    const int64_t p = p0 + i;
+    const uint32_t b = p < n ? bkt[p] : prev;
+    headf[i] = p < n && (p == 0 || b != prev);
+    cnt += headf[i] ? 1u : 0u;
+    prev = b;
+  }
+  // workgroup exclusive scan of cnt
+  constexpr int kWaves = kThreads / kWave;
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+  uint32_t inc = cnt;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t t = __shfl_up(inc, o, kWave);
+    if (lane >= o) inc += t;
+  }
+  if (lane == kWave - 1) lds[wave] = inc;
+  __syncthreads();
+  uint32_t wo = 0;
+  for (int w = 0; w < wave; ++w) wo += lds[w];
+  __syncthreads();
+  uint32_t run = tile_base + wo + inc - cnt;  // heads strictly before this thread's chunk
+#pragma unroll
+  for (int i = 0; i < kTileItems; ++i) {
+    run += headf[i] ? 1u : 0u;
+    rid[i] = run - 1u;  // id of the run containing p
+  }
+  (void)kWaves;
+}
+
+__global__ __launch_bounds__(kThreads) void run_emit_kernel(const uint32_t* __restrict__ bkt, int64_t n,
+                                                            const uint32_t* __restrict__ tile_base,
+                                                            uint32_t* run_start, uint32_t* rkey, uint32_t* rval) {
+  __shared__ uint32_t lds[kThreads / kWave];
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint32_t rid[kTileItems];
+    bool headf[kTileItems];
+    tile_run_ids(bkt, n, tile, tile_base[tile], rid, headf, lds);
+    const int64_t p0 = tile * kTile + (int64_t)threadIdx.x * kTileItems;
+#pragma unroll
+    for (int i = 0; i < kTileItems; ++i) {
+      if (headf[i] && rid[i] < (uint32_t)n) {
+        const int64_t p = p0 + i;
+        run_start[rid[i]] = (uint32_t)p;
+        rkey[rid[i]] = bkt[p];
+        rval[rid[i]] = rid[i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Lengths of the runs in sorted order.
+__global__ void run_len_kernel(const uint32_t* __restrict__ sorted_ids, const uint32_t* __restrict__ run_start,
+                               const uint32_t* counters, int64_t n, uint32_t* run_len) {
+  const uint32_t R = min(counters[kCtrRuns], (uint32_t)n);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < R; i += stride) {
+    const uint32_t id = sorted_ids[i];
+    if (id >= R) {  // cannot happen when the run passes agree; never index with it
+      run_len[i] = 0;
+      continue;
+    }
+    const uint32_t end = (id + 1 < R) ? run_start[id + 1] : (uint32_t)n;
+    const uint32_t st = run_start[id];
+    run_len[i] = (end >= st && end <= (uint32_t)n) ? end - st : 0u;
+  }
+}
+
+__global__ void run_dest_kernel(const uint32_t* __restrict__ sorted_ids, const uint32_t* __restrict__ dest_sorted,
+                                const uint32_t* counters, int64_t n, uint32_t* run_dest) {
+  const uint32_t R = min(counters[kCtrRuns], (uint32_t)n);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < R; i += stride) {
+    const uint32_t id = sorted_ids[i];
+    if (id < R) run_dest[id] = dest_sorted[i];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void particle_scatter_kernel(const uint32_t* __restrict__ bkt, int64_t n,
+                                                                    const uint32_t* __restrict__ tile_base,
+                                                                    const uint32_t* __restrict__ run_start,
+                                                                    const uint32_t* __restrict__ run_dest,
+                                                                    int32_t* __restrict__ indices) {
+  __shared__ uint32_t lds[kThreads / kWave];
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint32_t rid[kTileItems];
+    bool headf[kTileItems];
+    tile_run_ids(bkt, n, tile, tile_base[tile], rid, headf, lds);
+    const int64_t p0 = tile * kTile + (int64_t)threadIdx.x * kTileItems;
+#pragma unroll
+    for (int i = 0; i < kTileItems; ++i) {
+      const int64_t p = p0 + i;
+      if (p < n && rid[i] < (uint32_t)n) {
+        const uint64_t dst = (uint64_t)run_dest[rid[i]] + ((uint32_t)p - run_start[rid[i]]);
+        if (dst < (uint64_t)n) indices[dst] = (int32_t)p;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void set_counter_kernel(uint32_t* counters, int slot, const uint32_t* src) {
+  counters[slot] = *src;
+}
+
+int bits_for(uint64_t maxval) {
+  int b = 0;
+  while (b < 32 && (maxval >> b) != 0) ++b;
+  return b;
+}
+
+// dedup entries -> ranked, placed and keyed table entries (shared tail of
+// activation and enlargement)
+hipError_t finish_insertion(const TableView& t, Workspace& ws, uint64_t order_bound, hipStream_t s) {
+  const int g = groups_for(ws.ts_cap, kThreads);
+  collect_new_kernel<<<g, kThreads, 0, s>>>(ws.skey, ws.sorder, ws.ts_cap, t, ws.counters, ws.la_key,
+                                            ws.la_val, ws.list_cap, ws.err);
+  bool alt = false;
+  hipError_t e = radix_sort_pairs(ws.la_key, ws.la_val, ws.lb_key, ws.lb_val, ws.counters + kCtrList,
+                                  ws.list_cap, bits_for(order_bound), ws.rs_scratch, s, &alt);
+  if (e != hipSuccess) return e;
+  const uint32_t* sorted_slot = alt ? ws.lb_val : ws.la_val;
+  const int gl = groups_for(ws.list_cap, kThreads);
+  assign_kernel<<<gl, kThreads, 0, s>>>(sorted_slot, ws.skey, t, ws.counters, ws.list_cap, ws.err);
+  fill_keys_kernel<<<gl, kThreads, 0, s>>>(t, ws.counters, ws.list_cap, ws.err);
+  bump_cnt_kernel<<<1, 1, 0, s>>>(t, ws.counters, ws.list_cap);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t table_clear(const TableView& t, Workspace& ws, hipStream_t s) {
+  const int64_t m = (int64_t)t.T > (int64_t)ws.ts_cap ? t.T : ws.ts_cap;
+  clear_table_kernel<<<groups_for(m, kThreads), kThreads, 0, s>>>(t, ws.skey, ws.sorder, ws.ts_cap,
+                                                                   ws.counters);
+  return hipGetLastError();
+}
+
+hipError_t table_activate(const TableView& t, Workspace& ws, const float* x, int64_t n, float dx,
+                          int blockLen, int offset, float disp, hipStream_t s) {
+  if (blockLen <= 0) return hipErrorInvalidValue;
+  clear_scratch_kernel<<<groups_for(ws.ts_cap, kThreads), kThreads, 0, s>>>(ws.skey, ws.sorder, ws.ts_cap,
+                                                                             t.cnt, ws.counters);
+  const float dxinv = (float)1.0 / dx;
+  if (n > 0)
+    activate_kernel<<<groups_for(n, kThreads), kThreads, 0, s>>>(x, n, dxinv, blockLen, offset, disp,
+                                                                  ws.skey, ws.sorder, ws.ts_cap, ws.err);
+  return finish_insertion(t, ws, (uint64_t)(n > 0 ? n : 1), s);
+}
+
+hipError_t table_enlarge(const TableView& t, Workspace& ws, const int32_t lo[3], const int32_t hi[3],
+                         int32_t n0_cap, hipStream_t s) {
+  const int32_t e0 = hi[0] - lo[0], e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
+  if (e0 <= 0 || e1 <= 0 || e2 <= 0) return hipSuccess;
+  clear_scratch_kernel<<<groups_for(ws.ts_cap, kThreads), kThreads, 0, s>>>(ws.skey, ws.sorder, ws.ts_cap,
+                                                                             t.cnt, ws.counters);
+  const int64_t total_cap = (int64_t)n0_cap * e0 * e1 * e2;
+  enlarge_kernel<<<groups_for(total_cap, kThreads), kThreads, 0, s>>>(
+      t.active, ws.counters, lo[0], lo[1], lo[2], e0, e1, e2, ws.skey, ws.sorder, ws.ts_cap, ws.err);
+  return finish_insertion(t, ws, (uint64_t)(total_cap > 0 ? total_cap : 1), s);
+}
+
+hipError_t table_query(const TableView& t, const int32_t* keys, int64_t n, int32_t* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  query_kernel<<<groups_for(n, kThreads), kThreads, 0, s>>>(t, keys, n, out);
+  return hipGetLastError();
+}
+
+hipError_t bucket_sort(const TableView& t, Workspace& ws, const float* x, int64_t n, float dx, int blockLen,
+                       int offset, float disp, const SortView& out, hipStream_t s) {
+  if (n > (int64_t)ws.n_cap) return hipErrorInvalidValue;
+  const int64_t nb1 = (int64_t)out.cap_blocks + 1;
+  uint32_t* counts = (uint32_t*)out.counts;
+  zero_u32_kernel<<<groups_for(nb1, kThreads), kThreads, 0, s>>>(counts, nb1);
+  if (n <= 0) {
+    return exclusive_scan_u32(counts, (uint32_t*)out.offsets, nullptr, (uint32_t)nb1, ws.scan_tmp, nullptr, s);
+  }
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  const int gt = (int)(ntiles < 65536 ? ntiles : 65536);
+  const float dxinv = (float)1.0 / dx;
+  bucket_kernel<<<gt, kThreads, 0, s>>>(x, n, dxinv, blockLen, offset, disp, t, ws.bkt, counts, out.cap_blocks,
+                                        ws.err);
+  run_heads_kernel<<<gt, kThreads, 0, s>>>(ws.bkt, n, ws.tile_heads);
+  hipError_t e = exclusive_scan_u32(ws.tile_heads, ws.tile_base, nullptr, (uint32_t)ntiles, ws.scan_tmp,
+                                    ws.counters + kCtrRuns, s);
+  if (e != hipSuccess) return e;
+  run_emit_kernel<<<gt, kThreads, 0, s>>>(ws.bkt, n, ws.tile_base, ws.run_start, ws.rk_a, ws.rv_a);
+  bool alt = false;
+  e = radix_sort_pairs(ws.rk_a, ws.rv_a, ws.rk_b, ws.rv_b, ws.counters + kCtrRuns, (uint32_t)n,
+                       bits_for((uint64_t)out.cap_blocks), ws.rs_scratch, s, &alt);
+  if (e != hipSuccess) return e;
+  const uint32_t* sorted_ids = alt ? ws.rv_b : ws.rv_a;
+  uint32_t* spare = alt ? ws.rk_a : ws.rk_b;  // free buffer for the sorted dest scan
+  run_len_kernel<<<groups_for(n, kThreads), kThreads, 0, s>>>(sorted_ids, ws.run_start, ws.counters, n,
+                                                               ws.run_len);
+  e = exclusive_scan_u32(ws.run_len, spare, ws.counters + kCtrRuns, (uint32_t)n, ws.scan_tmp, nullptr, s);
+  if (e != hipSuccess) return e;
+  run_dest_kernel<<<groups_for(n, kThreads), kThreads, 0, s>>>(sorted_ids, spare, ws.counters, n, ws.run_dest);
+  e = exclusive_scan_u32(counts, (uint32_t*)out.offsets, nullptr, (uint32_t)nb1, ws.scan_tmp, nullptr, s);
+  if (e != hipSuccess) return e;
+  particle_scatter_kernel<<<gt, kThreads, 0, s>>>(ws.bkt, n, ws.tile_base, ws.run_start, ws.run_dest,
+                                                  out.indices);
+  return hipGetLastError();
+}
+
+}  // namespace zsmpm

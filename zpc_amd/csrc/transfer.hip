@@ -1,0 +1,677 @@
+// P2G, grid update and G2P for the gfx950 MLS-MPM substep.
+//
+// Work is organised per particle bucket: the 4^3 block a particle's key maps
+// to (SparsityOp.hpp:72-77 with blockLen 4, offset -2, displacement 0.5). A
+// particle with key cell c has its quadratic stencil base at c+1 (c or c+2
+// only at rounding ties), so the stencils of a bucket whose key block starts
+// at cell 4b lie in nodes [4b, 4b+8)^3, all inside the 2x2x2 blocks
+// b + {0,1}^3 that EnlargeSparsity{0..2} activates.
+//
+// P2G. On gfx950 LDS float atomics (ds_add_f32) retire about one lane every
+// three clocks, roughly 30x slower than a plain LDS read-modify-write, so the
+// scatter avoids them:
+//   A  every lane takes one particle (full occupancy): stress (3x3 SVD and
+//      the constitutive model), then an affine record -- mass, B and dx*A with
+//      node value(o) = W(o) (B + dx A o) -- kept in LDS, and a base-cell id
+//      (one of the 64 main cells [4b+1, 4b+5)^3, or "overflow");
+//   B  a counting sort of the chunk by base cell (LDS integer atomics);
+//   C  rounds: in round r lane l of a wave takes the r-th particle of main
+//      cell l, so the 64 lanes of one instruction address 64 different nodes
+//      and scatter with plain float4 read-modify-writes into the wave's own
+//      6^3 node arena; overflow particles scatter with global atomics;
+//   then the four wave arenas are summed and flushed with one global float
+//   atomic per touched node and channel (zpc's CUDA P2G issues 189 global
+//   atomics per particle, cuda/simulation/transfer/P2G.hpp:98-115).
+// With kFold the force term is added to the momentum channels at P2G (the
+// explicit driver's v = (mv + rhs)/m convention) and the grid carries
+// {m, mv + rhs}; otherwise the rhs channels 4..6 are kept as zpc lays them out.
+//
+// G2P stages the 8^3 node velocities of a bucket in LDS (float4 per node) and
+// gathers from there.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "zsmpm_device.hpp"
+#include "zsmpm_internal.hpp"
+
+namespace zsmpm {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kWave;
+constexpr int kMainCells = 64;    // base cells [4b+1, 4b+5)^3
+constexpr int kArena6 = 216;      // nodes [4b+1, 4b+7)^3
+constexpr int kArena8 = 512;      // nodes [4b, 4b+8)^3
+
+inline int groups_for(int64_t n, int threads) {
+  int64_t g = (n + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > 65535 * 8) g = 65535 * 8;
+  return (int)g;
+}
+
+__device__ __forceinline__ int64_t grid_at(int32_t b, int ch, int cell) {
+  return ((int64_t)b * kChannels + ch) * kBlockCells + cell;
+}
+
+// node r in [0, 8)^3 relative to 4b -> (neighbour block of b + {0,1}^3, cell)
+__device__ __forceinline__ void node_home(int r0, int r1, int r2, int& nb, int& cell) {
+  nb = ((r0 >> 2) << 2) | ((r1 >> 2) << 1) | (r2 >> 2);
+  cell = ((r0 & 3) << 4) | ((r1 & 3) << 2) | (r2 & 3);
+}
+
+__global__ void grid_clear_kernel(float* grid, const int32_t* nblocks, int32_t cap) {
+  const int32_t nb = min(*nblocks, cap);
+  const int64_t total = (int64_t)nb * kChannels * kBlockCells / 4;
+  float4* g4 = reinterpret_cast<float4*>(grid);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += stride)
+    g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// active block count, clipped to what the grid and bucket arrays hold
+__device__ __forceinline__ int32_t bounded_blocks(const TableView& t, const SortView& sv, int32_t grid_cap,
+                                                  uint32_t* err) {
+  const int32_t nb = *t.cnt;
+  const int32_t cap = min(grid_cap, sv.cap_blocks);
+  if (nb > cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, kErrCapacity);
+    return cap;
+  }
+  return nb;
+}
+
+// neighbour blocks key + (d>>2&1, d>>1&1, d&1) of bucket b
+__device__ __forceinline__ void load_neighbours(const TableView& t, int32_t b, int32_t* nbr, uint32_t* err) {
+  if (threadIdx.x < 8) {
+    const int d = threadIdx.x;
+    const int32_t k0 = t.active[3 * (int64_t)b] + ((d >> 2) & 1);
+    const int32_t k1 = t.active[3 * (int64_t)b + 1] + ((d >> 1) & 1);
+    const int32_t k2 = t.active[3 * (int64_t)b + 2] + (d & 1);
+    const int32_t r = table_find(t, k0, k1, k2);
+    nbr[d] = r;
+    if (r < 0) atomicOr(err, kErrMissingBlock);
+  }
+}
+
+// quadratic B-spline weights from the local position l in [0.5, 1.5)
+// (quadratic_bspline_weights, math/curve/InterpolationKernel.hpp:95-130)
+__device__ __forceinline__ void bspline_w(float l, float (&w)[3]) {
+#pragma clang fp contract(off)
+  const float d0 = l - (float)(int32_t)floorf(l - 0.5f);
+  w[0] = 0.5f * (1.5f - d0) * (1.5f - d0);
+  const float d1 = d0 - 1.0f;
+  w[1] = 0.75f - d1 * d1;
+  const float zz = 0.5f + d1;
+  w[2] = 0.5f * zz * zz;
+}
+
+// Record layout (float4 units): q0 = (m, B0, B1, B2), q1 = (l0, l1, l2, A8),
+// q2 = A0..A3, q3 = A4..A7 with A column-major (A[d + 3e]) already times dx;
+// the unfolded layout appends the force part q4 = (Bf0..2, Af8), q5, q6.
+template <bool kFold>
+struct Rec {
+  static constexpr int kQuads = kFold ? 4 : 7;
+  // particles staged per pass; keeps the workgroup's LDS under 64 KB
+  static constexpr int kChunk = kFold ? 512 : 256;
+};
+
+template <int kKind, bool kFold>
+__global__ __launch_bounds__(kThreads) void p2g_kernel(Model m, ParticlesView ps, TableView t, SortView sv,
+                                                       float* __restrict__ grid, int32_t grid_cap, float dt,
+                                                       float dx, uint32_t* err, uint32_t* counters) {
+  constexpr int kQ = Rec<kFold>::kQuads;
+  constexpr int kChunk = Rec<kFold>::kChunk;
+  constexpr int kNV = kFold ? 1 : 2;  // float4 per arena node
+  __shared__ float4 arena[kWaves][kArena6 * kNV];
+  __shared__ float4 rec[kChunk * kQ];
+  __shared__ uint16_t list[kChunk];
+  __shared__ uint8_t cellof[kChunk];
+  __shared__ int32_t ccount[kMainCells + 1], cstart[kMainCells + 1], cfill[kMainCells + 1];
+  __shared__ int32_t nbr[8];
+  __shared__ int32_t maxc;
+
+  const int32_t nb = bounded_blocks(t, sv, grid_cap, err);
+  const float dx_inv = (float)1.0 / dx;
+  const float D_inv = 4.f * dx_inv * dx_inv;
+  const float scale = -dt * D_inv;
+  const int wave = threadIdx.x / kWave, lane = lane_id();
+  uint32_t fp64_used = 0;
+
+  for (int32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    const int32_t cnt = sv.counts[b];
+    const int32_t off = sv.offsets[b];
+    if (cnt <= 0 || off < 0 || (int64_t)off + cnt > ps.n) continue;
+    for (int i = threadIdx.x; i < kWaves * kArena6 * kNV; i += kThreads)
+      (&arena[0][0])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    load_neighbours(t, b, nbr, err);
+    const int32_t o0 = t.active[3 * (int64_t)b] * 4, o1 = t.active[3 * (int64_t)b + 1] * 4,
+                  o2 = t.active[3 * (int64_t)b + 2] * 4;
+
+    for (int32_t c0 = 0; c0 < cnt; c0 += kChunk) {
+      const int32_t nch = min(kChunk, cnt - c0);
+      if (threadIdx.x <= kMainCells) {
+        ccount[threadIdx.x] = 0;
+        cfill[threadIdx.x] = 0;
+      }
+      __syncthreads();
+      // ---- A: stress + affine record per particle
+      for (int32_t i = threadIdx.x; i < nch; i += kThreads) {
+        const int64_t p = sv.indices[off + c0 + i];
+        if (p < 0 || p >= ps.n) {
+          atomicOr(err, kErrCapacity);
+          cellof[i] = 255;
+          list[i] = 0;
+          continue;
+        }
+        const float x0 = ps.x[3 * p], x1 = ps.x[3 * p + 1], x2 = ps.x[3 * p + 2];
+        const float mass = ps.m[p];
+        float F[9], C[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = ps.F[9 * p + k];
+        float contrib[9];
+        bool f64 = false;
+        if (kKind == 0) {
+          stress_fixedcorotated(m, F, contrib, f64);
+        } else {
+          float lj = ps.logJp[p];
+          stress_sand(m, F, lj, contrib, nullptr, f64);
+          ps.logJp[p] = lj;
+        }
+        fp64_used += f64 ? 1u : 0u;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) C[k] = ps.C[9 * p + k];
+        const float v0 = ps.v[3 * p], v1 = ps.v[3 * p + 1], v2 = ps.v[3 * p + 2];
+        // base node and local position in grid units (LocalArena::init, Utils.hpp:51-70)
+        float l[3];
+        int32_t base[3];
+        {
+#pragma clang fp contract(off)
+          const float xs[3] = {x0, x1, x2};
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const float X = xs[d] / dx;
+            base[d] = (int32_t)floorf(X - 0.5f);
+            l[d] = X - (float)base[d];
+          }
+        }
+        const int r0 = base[0] - o0, r1 = base[1] - o1, r2 = base[2] - o2;  // in [0, 6)
+        int cell;
+        if (r0 >= 1 && r0 <= 4 && r1 >= 1 && r1 <= 4 && r2 >= 1 && r2 <= 4) {
+          cell = ((r0 - 1) << 4) | ((r1 - 1) << 2) | (r2 - 1);
+        } else if (r0 >= 0 && r0 <= 5 && r1 >= 0 && r1 <= 5 && r2 >= 0 && r2 <= 5) {
+          cell = kMainCells;  // overflow: scattered with global atomics
+        } else {
+          atomicOr(err, kErrArena);
+          cell = 255;
+        }
+        // affine record: momentum m v + m C (o - l) dx, force contrib' (o - l) dx
+        float A[9], Bv[3];
+        const float mv[3] = {mass * v0, mass * v1, mass * v2};
+        float Af[9], Bf[3];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          const float cf = contrib[k] * scale;
+          if (kFold) {
+            A[k] = (mass * C[k] + cf) * dx;
+          } else {
+            A[k] = mass * C[k] * dx;
+            Af[k] = cf * dx;
+          }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          Bv[d] = mv[d] - (A[d] * l[0] + A[d + 3] * l[1] + A[d + 6] * l[2]);
+          if (!kFold) Bf[d] = -(Af[d] * l[0] + Af[d + 3] * l[1] + Af[d + 6] * l[2]);
+        }
+        float4* q = rec + i * kQ;
+        q[0] = make_float4(mass, Bv[0], Bv[1], Bv[2]);
+        q[1] = make_float4(l[0], l[1], l[2], A[8]);
+        q[2] = make_float4(A[0], A[1], A[2], A[3]);
+        q[3] = make_float4(A[4], A[5], A[6], A[7]);
+        if (!kFold) {
+          q[4 % kQ] = make_float4(Bf[0], Bf[1], Bf[2], Af[8]);
+          q[5 % kQ] = make_float4(Af[0], Af[1], Af[2], Af[3]);
+          q[6 % kQ] = make_float4(Af[4], Af[5], Af[6], Af[7]);
+        }
+        cellof[i] = (uint8_t)cell;
+        if (cell <= kMainCells) atomicAdd(&ccount[cell], 1);
+        list[i] = (uint16_t)((r0 << 6) | (r1 << 3) | r2);  // base offsets; the list is rebuilt in B
+      }
+      __syncthreads();
+      // ---- B: counting sort of the chunk by base cell
+      if (threadIdx.x < kWave) {
+        const int32_t c = (threadIdx.x < kMainCells) ? ccount[threadIdx.x] : 0;
+        int32_t inc = c;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+          const int32_t u = __shfl_up(inc, o, kWave);
+          if (lane >= o) inc += u;
+        }
+        cstart[threadIdx.x] = inc - c;
+        int32_t mx = c;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, kWave));
+        if (threadIdx.x == kWave - 1) {
+          cstart[kMainCells] = inc;  // overflow particles follow the main cells
+          maxc = mx;
+        }
+      }
+      // each thread keeps the base offsets and cells of its own particles
+      // across the rebuild of list[]
+      uint16_t keep[kChunk / kThreads];
+      uint8_t keepc[kChunk / kThreads];
+#pragma unroll
+      for (int k = 0; k < kChunk / kThreads; ++k) {
+        const int32_t i = threadIdx.x + k * kThreads;
+        keep[k] = i < nch ? list[i] : (uint16_t)0;
+        keepc[k] = i < nch ? cellof[i] : (uint8_t)255;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kChunk / kThreads; ++k) {
+        const int32_t i = threadIdx.x + k * kThreads;
+        if (i < nch && keepc[k] <= kMainCells) {
+          const int32_t slot = atomicAdd(&cfill[keepc[k]], 1);
+          list[cstart[keepc[k]] + slot] = (uint16_t)i;
+        }
+      }
+      __syncthreads();
+      // ---- C: rounds, lane = main cell
+      {
+        const int32_t lcount = ccount[lane];
+        const int32_t lstart = cstart[lane];
+        const int lr0 = lane >> 4, lr1 = (lane >> 2) & 3, lr2 = lane & 3;  // base - (4b+1)
+        float4* ar = arena[wave];
+        const int32_t rounds = maxc;
+        for (int32_t r = wave; r < rounds; r += kWaves) {
+          if (r < lcount) {
+            const int32_t i = list[lstart + r];
+            const float4* q = rec + i * kQ;
+            const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+            const float Am[9] = {q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w, q1.w};
+            float Af[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, Bf[3] = {0.f, 0.f, 0.f};
+            if (!kFold) {
+              const float4 qf0 = q[4 % kQ], qf1 = q[5 % kQ], qf2 = q[6 % kQ];
+              Bf[0] = qf0.x;
+              Bf[1] = qf0.y;
+              Bf[2] = qf0.z;
+              Af[0] = qf1.x;
+              Af[1] = qf1.y;
+              Af[2] = qf1.z;
+              Af[3] = qf1.w;
+              Af[4] = qf2.x;
+              Af[5] = qf2.y;
+              Af[6] = qf2.z;
+              Af[7] = qf2.w;
+              Af[8] = qf0.w;
+            }
+            float w0[3], w1[3], w2[3];
+            bspline_w(q1.x, w0);
+            bspline_w(q1.y, w1);
+            bspline_w(q1.z, w2);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+              for (int bb = 0; bb < 3; ++bb) {
+                const float w01 = w0[a] * w1[bb];
+                const float fa = (float)a, fb = (float)bb;
+                const float s0 = q0.y + Am[0] * fa + Am[3] * fb;
+                const float s1 = q0.z + Am[1] * fa + Am[4] * fb;
+                const float s2 = q0.w + Am[2] * fa + Am[5] * fb;
+                float f0 = 0.f, f1 = 0.f, f2 = 0.f;
+                if (!kFold) {
+                  f0 = Bf[0] + Af[0] * fa + Af[3] * fb;
+                  f1 = Bf[1] + Af[1] * fa + Af[4] * fb;
+                  f2 = Bf[2] + Af[2] * fa + Af[5] * fb;
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                  const float W = w01 * w2[c];
+                  const float fc = (float)c;
+                  const int n = (lr0 + a) * 36 + (lr1 + bb) * 6 + (lr2 + c);
+                  // Lanes of one step touch distinct nodes, but a node a lane
+                  // updates in this step may be one another lane updated in an
+                  // earlier step. The LDS unit serves a wave's accesses in issue
+                  // order; this barrier keeps the compiler from hoisting this
+                  // step's read above earlier steps' writes.
+                  asm volatile("" ::: "memory");
+                  float4 acc = ar[n * kNV];
+                  acc.x += q0.x * W;
+                  acc.y += W * (s0 + Am[6] * fc);
+                  acc.z += W * (s1 + Am[7] * fc);
+                  acc.w += W * (s2 + Am[8] * fc);
+                  ar[n * kNV] = acc;
+                  if (!kFold) {
+                    float4 accf = ar[n * kNV + 1];
+                    accf.x += W * (f0 + Af[6] * fc);
+                    accf.y += W * (f1 + Af[7] * fc);
+                    accf.z += W * (f2 + Af[8] * fc);
+                    ar[n * kNV + 1] = accf;
+                  }
+                }
+              }
+          }
+        }
+      }
+      // ---- overflow particles: global atomics straight into the grid
+#pragma unroll
+      for (int k = 0; k < kChunk / kThreads; ++k) {
+        const int32_t i = threadIdx.x + k * kThreads;
+        if (i >= nch || keepc[k] != kMainCells) continue;
+        const int r0 = (keep[k] >> 6) & 7, r1 = (keep[k] >> 3) & 7, r2 = keep[k] & 7;
+        const float4* q = rec + i * kQ;
+        const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+        const float Am[9] = {q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w, q1.w};
+        float Af[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, Bf[3] = {0.f, 0.f, 0.f};
+        if (!kFold) {
+          const float4 qf0 = q[4 % kQ], qf1 = q[5 % kQ], qf2 = q[6 % kQ];
+          Bf[0] = qf0.x;
+          Bf[1] = qf0.y;
+          Bf[2] = qf0.z;
+          Af[0] = qf1.x;
+          Af[1] = qf1.y;
+          Af[2] = qf1.z;
+          Af[3] = qf1.w;
+          Af[4] = qf2.x;
+          Af[5] = qf2.y;
+          Af[6] = qf2.z;
+          Af[7] = qf2.w;
+          Af[8] = qf0.w;
+        }
+        float w0[3], w1[3], w2[3];
+        bspline_w(q1.x, w0);
+        bspline_w(q1.y, w1);
+        bspline_w(q1.z, w2);
+        for (int a = 0; a < 3; ++a)
+          for (int bb = 0; bb < 3; ++bb)
+            for (int c = 0; c < 3; ++c) {
+              const float W = w0[a] * w1[bb] * w2[c];
+              int nbi, cell;
+              node_home(r0 + a, r1 + bb, r2 + c, nbi, cell);
+              const int32_t gb = nbr[nbi];
+              if (gb < 0 || gb >= grid_cap) continue;
+              float* gp = grid + grid_at(gb, 0, cell);
+              const float fa = (float)a, fb = (float)bb, fc = (float)c;
+              unsafeAtomicAdd(gp, q0.x * W);
+              unsafeAtomicAdd(gp + 1 * kBlockCells, W * (q0.y + Am[0] * fa + Am[3] * fb + Am[6] * fc));
+              unsafeAtomicAdd(gp + 2 * kBlockCells, W * (q0.z + Am[1] * fa + Am[4] * fb + Am[7] * fc));
+              unsafeAtomicAdd(gp + 3 * kBlockCells, W * (q0.w + Am[2] * fa + Am[5] * fb + Am[8] * fc));
+              if (!kFold) {
+                unsafeAtomicAdd(gp + 4 * kBlockCells, W * (Bf[0] + Af[0] * fa + Af[3] * fb + Af[6] * fc));
+                unsafeAtomicAdd(gp + 5 * kBlockCells, W * (Bf[1] + Af[1] * fa + Af[4] * fb + Af[7] * fc));
+                unsafeAtomicAdd(gp + 6 * kBlockCells, W * (Bf[2] + Af[2] * fa + Af[5] * fb + Af[8] * fc));
+              }
+            }
+      }
+      __syncthreads();
+    }
+    // ---- flush: sum the wave arenas, one global atomic per touched node/channel
+    for (int n = threadIdx.x; n < kArena6; n += kThreads) {
+      float4 s = arena[0][n * kNV];
+#pragma unroll
+      for (int w = 1; w < kWaves; ++w) {
+        const float4 u = arena[w][n * kNV];
+        s.x += u.x;
+        s.y += u.y;
+        s.z += u.z;
+        s.w += u.w;
+      }
+      if (s.x != 0.f) {
+        int nbi, cell;
+        node_home(n / 36 + 1, (n / 6) % 6 + 1, n % 6 + 1, nbi, cell);
+        const int32_t gb = nbr[nbi];
+        if (gb >= 0 && gb < grid_cap) {
+          float* gp = grid + grid_at(gb, 0, cell);
+          unsafeAtomicAdd(gp, s.x);
+          unsafeAtomicAdd(gp + 1 * kBlockCells, s.y);
+          unsafeAtomicAdd(gp + 2 * kBlockCells, s.z);
+          unsafeAtomicAdd(gp + 3 * kBlockCells, s.w);
+          if (!kFold) {
+            float4 f = arena[0][n * kNV + 1];
+#pragma unroll
+            for (int w = 1; w < kWaves; ++w) {
+              const float4 u = arena[w][n * kNV + 1];
+              f.x += u.x;
+              f.y += u.y;
+              f.z += u.z;
+            }
+            unsafeAtomicAdd(gp + 4 * kBlockCells, f.x);
+            unsafeAtomicAdd(gp + 5 * kBlockCells, f.y);
+            unsafeAtomicAdd(gp + 6 * kBlockCells, f.z);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (fp64_used) atomicAdd(&counters[kCtrFp64], fp64_used);
+}
+
+// v = (mv [+ rhs]) / m + extf dt where m != 0 (GridOp.hpp:71-110), max |v|^2,
+// then the colliders in order on nodes with m > 0 (GridOp.hpp:112-164,
+// Collider.h:80-107). fold adds the rhs channels first (the explicit
+// driver's convention when P2G kept them separate).
+__global__ __launch_bounds__(kThreads) void grid_update_kernel(float* __restrict__ grid, int32_t grid_cap,
+                                                               TableView t, float dt, float g0, float g1,
+                                                               float g2, Colliders cols, float dx, int fold,
+                                                               uint32_t* counters, uint32_t* err) {
+  int32_t nb = *t.cnt;
+  if (nb > grid_cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, kErrCapacity);
+    nb = grid_cap;
+  }
+  const int64_t nodes = (int64_t)nb * kBlockCells;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  float vmax = 0.f;
+  const float e0 = g0 * dt, e1 = g1 * dt, e2 = g2 * dt;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nodes; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    if (i < nodes) {
+      const int32_t b = (int32_t)(i >> 6);
+      const int cell = (int)(i & 63);
+      float* gp = grid + grid_at(b, 0, cell);
+      const float mass = gp[0];
+      if (mass != 0.f) {
+        const float inv = 1.f / mass;
+        float mv0 = gp[1 * kBlockCells], mv1 = gp[2 * kBlockCells], mv2 = gp[3 * kBlockCells];
+        if (fold) {
+          mv0 = mv0 + gp[4 * kBlockCells];
+          mv1 = mv1 + gp[5 * kBlockCells];
+          mv2 = mv2 + gp[6 * kBlockCells];
+        }
+        float v[3] = {mv0 * inv + e0, mv1 * inv + e1, mv2 * inv + e2};
+        vmax = fmaxf(vmax, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (mass > 0.f && cols.n > 0) {
+          const float pos[3] = {((float)t.active[3 * (int64_t)b] * 4.f + (float)((cell >> 4) & 3)) * dx,
+                                ((float)t.active[3 * (int64_t)b + 1] * 4.f + (float)((cell >> 2) & 3)) * dx,
+                                ((float)t.active[3 * (int64_t)b + 2] * 4.f + (float)(cell & 3)) * dx};
+          for (int c = 0; c < cols.n; ++c) {
+            const PlaneCollider& pc = cols.c[c];
+            const float phi = pc.normal[0] * (pos[0] - pc.origin[0]) + pc.normal[1] * (pos[1] - pc.origin[1]) +
+                              pc.normal[2] * (pos[2] - pc.origin[2]);
+            if (phi < 0.f) {
+              if (pc.type == 0) {
+                v[0] = v[1] = v[2] = 0.f;
+              } else {
+                const float proj = pc.normal[0] * v[0] + pc.normal[1] * v[1] + pc.normal[2] * v[2];
+                if ((pc.type == 2 && proj < 0.f) || pc.type == 1) {
+                  v[0] -= proj * pc.normal[0];
+                  v[1] -= proj * pc.normal[1];
+                  v[2] -= proj * pc.normal[2];
+                }
+              }
+            }
+          }
+        }
+        gp[1 * kBlockCells] = v[0];
+        gp[2 * kBlockCells] = v[1];
+        gp[3 * kBlockCells] = v[2];
+      }
+    }
+  }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, kWave));
+  if (lane_id() == 0 && vmax > 0.f) atomicMax(&counters[kCtrMaxVel], __float_as_uint(vmax));
+}
+
+__global__ __launch_bounds__(kThreads) void g2p_kernel(ParticlesView ps, TableView t, SortView sv,
+                                                       const float* __restrict__ grid, int32_t grid_cap, float dt,
+                                                       float dx, uint32_t* err) {
+  __shared__ float4 vel[kArena8];
+  __shared__ int32_t nbr[8];
+  const int32_t nb = bounded_blocks(t, sv, grid_cap, err);
+  const float dx_inv = (float)1 / dx;
+  const float D_inv = 4.f * dx_inv * dx_inv;
+  for (int32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    const int32_t cnt = sv.counts[b];
+    const int32_t off = sv.offsets[b];
+    if (cnt <= 0 || off < 0 || (int64_t)off + cnt > ps.n) continue;
+    load_neighbours(t, b, nbr, err);
+    const int32_t o0 = t.active[3 * (int64_t)b] * 4, o1 = t.active[3 * (int64_t)b + 1] * 4,
+                  o2 = t.active[3 * (int64_t)b + 2] * 4;
+    __syncthreads();
+    // stage node velocities: slot s -> (neighbour block s>>6, cell s&63), stored at (r0*8 + r1)*8 + r2
+    for (int s = threadIdx.x; s < kArena8; s += kThreads) {
+      const int nbi = s >> 6, cell = s & 63;
+      const int r0 = ((nbi >> 2) & 1) * 4 + ((cell >> 4) & 3);
+      const int r1 = ((nbi >> 1) & 1) * 4 + ((cell >> 2) & 3);
+      const int r2 = (nbi & 1) * 4 + (cell & 3);
+      const int32_t gb = nbr[nbi];
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (gb >= 0 && gb < grid_cap) {
+        const float* gp = grid + grid_at(gb, 1, cell);
+        v = make_float4(gp[0], gp[kBlockCells], gp[2 * kBlockCells], 0.f);
+      }
+      vel[(r0 * 8 + r1) * 8 + r2] = v;
+    }
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < cnt; i += kThreads) {
+      const int64_t p = sv.indices[off + i];
+      if (p < 0 || p >= ps.n) {
+        atomicOr(err, kErrCapacity);
+        continue;
+      }
+      const float x[3] = {ps.x[3 * p], ps.x[3 * p + 1], ps.x[3 * p + 2]};
+      Arena ar;
+      make_arena(x[0], x[1], x[2], dx, ar);
+      const int lb0 = ar.base[0] - o0, lb1 = ar.base[1] - o1, lb2 = ar.base[2] - o2;
+      if (lb0 < 0 || lb0 > 5 || lb1 < 0 || lb1 > 5 || lb2 < 0 || lb2 > 5) {
+        atomicOr(err, kErrArena);
+        continue;
+      }
+      float v[3] = {0.f, 0.f, 0.f};
+      float C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float xa = (float)a * dx - ar.lp[0];
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) {
+          const float xb = (float)bb * dx - ar.lp[1];
+          const float wab = ar.w[0][a] * ar.w[1][bb];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const float xc = (float)c * dx - ar.lp[2];
+            const float W = wab * ar.w[2][c];
+            const float4 vi = vel[((lb0 + a) * 8 + (lb1 + bb)) * 8 + (lb2 + c)];
+            v[0] += vi.x * W;
+            v[1] += vi.y * W;
+            v[2] += vi.z * W;
+            const float w0 = W * vi.x * D_inv, w1 = W * vi.y * D_inv, w2 = W * vi.z * D_inv;
+            C[0] += w0 * xa;
+            C[1] += w1 * xa;
+            C[2] += w2 * xa;
+            C[3] += w0 * xb;
+            C[4] += w1 * xb;
+            C[5] += w2 * xb;
+            C[6] += w0 * xc;
+            C[7] += w1 * xc;
+            C[8] += w2 * xc;
+          }
+        }
+      }
+      float F[9], Fn[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) F[k] = ps.F[9 * p + k];
+      // F <- (I + dt C) F, column-major (G2P.hpp:75-78)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          const float a0 = C[r] * dt + (r == 0 ? 1.f : 0.f);
+          const float a1 = C[r + 3] * dt + (r == 1 ? 1.f : 0.f);
+          const float a2 = C[r + 6] * dt + (r == 2 ? 1.f : 0.f);
+          Fn[r + 3 * j] = a0 * F[3 * j] + a1 * F[3 * j + 1] + a2 * F[3 * j + 2];
+        }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        ps.x[3 * p + d] = x[d] + v[d] * dt;
+        ps.v[3 * p + d] = v[d];
+      }
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        ps.C[9 * p + k] = C[k];
+        ps.F[9 * p + k] = Fn[k];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+hipError_t check_launch_limits(int device) {
+  int max_lds = 0;
+  hipError_t e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+  if (e != hipSuccess) return e;
+  const void* kernels[] = {
+      reinterpret_cast<const void*>(&p2g_kernel<0, true>), reinterpret_cast<const void*>(&p2g_kernel<0, false>),
+      reinterpret_cast<const void*>(&p2g_kernel<1, true>), reinterpret_cast<const void*>(&p2g_kernel<1, false>),
+      reinterpret_cast<const void*>(&g2p_kernel)};
+  for (const void* k : kernels) {
+    hipFuncAttributes attr;
+    e = hipFuncGetAttributes(&attr, k);
+    if (e != hipSuccess) return e;
+    if ((int)attr.sharedSizeBytes > max_lds) return hipErrorInvalidConfiguration;
+  }
+  return hipSuccess;
+}
+
+hipError_t grid_clear(float* grid, const int32_t* nblocks_dev, int32_t cap_blocks, hipStream_t s) {
+  const int64_t total = (int64_t)cap_blocks * kChannels * kBlockCells / 4;
+  grid_clear_kernel<<<groups_for(total, kThreads), kThreads, 0, s>>>(grid, nblocks_dev, cap_blocks);
+  return hipGetLastError();
+}
+
+hipError_t p2g(const Model& m, const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
+               int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s) {
+  const int g = persistent_groups();
+  if (m.kind == 0) {
+    if (fold)
+      p2g_kernel<0, true><<<g, kThreads, 0, s>>>(m, ps, t, sv, grid, grid_cap, dt, dx, ws.err, ws.counters);
+    else
+      p2g_kernel<0, false><<<g, kThreads, 0, s>>>(m, ps, t, sv, grid, grid_cap, dt, dx, ws.err, ws.counters);
+  } else {
+    if (fold)
+      p2g_kernel<1, true><<<g, kThreads, 0, s>>>(m, ps, t, sv, grid, grid_cap, dt, dx, ws.err, ws.counters);
+    else
+      p2g_kernel<1, false><<<g, kThreads, 0, s>>>(m, ps, t, sv, grid, grid_cap, dt, dx, ws.err, ws.counters);
+  }
+  return hipGetLastError();
+}
+
+hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float dt, const float extf[3],
+                       const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s) {
+  grid_update_kernel<<<persistent_groups(), kThreads, 0, s>>>(grid, grid_cap, t, dt, extf[0], extf[1], extf[2],
+                                                               cols, dx, fold_rhs, ws.counters, ws.err);
+  return hipGetLastError();
+}
+
+hipError_t g2p(const ParticlesView& ps, const TableView& t, const SortView& sv, const float* grid,
+               int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s) {
+  g2p_kernel<<<persistent_groups(), kThreads, 0, s>>>(ps, t, sv, grid, grid_cap, dt, dx, ws.err);
+  return hipGetLastError();
+}
+
+}  // namespace zsmpm

@@ -1,0 +1,147 @@
+// Host-side declarations shared by the zsmpm HIP translation units: non-owning
+// device views in zpc's container layouts, the scratch workspace a policy
+// owns, and the stage launchers the C ABI strings together.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "zsmpm_device.hpp"
+
+namespace zsmpm {
+
+/// HashTable<i32,3,int> arrays (container/HashTable.hpp:40-58, 200-205).
+struct TableView {
+  int32_t T = 0;
+  int32_t* keys = nullptr;     // [3T]
+  int32_t* indices = nullptr;  // [T]
+  int32_t* status = nullptr;   // [T]
+  int32_t* cnt = nullptr;      // [1]
+  int32_t* active = nullptr;   // [3T] _activeKeys
+};
+
+/// Particles<f32,3> attributes (one array per attribute, C and F column-major).
+struct ParticlesView {
+  int64_t n = 0;
+  float* x = nullptr;
+  float* v = nullptr;
+  float* C = nullptr;
+  float* F = nullptr;
+  float* m = nullptr;
+  float* logJp = nullptr;
+};
+
+/// Bucket-sort outputs (IndexBuckets, container/IndexBuckets.hpp:11-66).
+struct SortView {
+  int32_t* counts = nullptr;   // [B + 1]
+  int32_t* offsets = nullptr;  // [B + 1]
+  int32_t* indices = nullptr;  // [N]
+  int32_t cap_blocks = 0;      // capacity of counts/offsets minus one
+};
+
+struct PlaneCollider {
+  int32_t type;  // 0 sticky, 1 slip, 2 separate
+  float origin[3];
+  float normal[3];
+};
+constexpr int kMaxColliders = 8;
+struct Colliders {
+  int32_t n = 0;
+  PlaneCollider c[kMaxColliders];
+};
+
+// counters[] slots in the workspace
+enum : int {
+  kCtrList = 0,     // unique new keys found by a dedup pass
+  kCtrCnt0 = 1,     // table size before the current insertion pass
+  kCtrRuns = 2,     // particle runs in the bucket sort
+  kCtrFp64 = 3,     // particles that took the FP64 SVD path
+  kCtrFallback = 4, // particles that took the SVD fallback of the learned closure
+  kCtrMaxVel = 5,   // max |v|^2 as float bits
+  kCtrCount = 16
+};
+
+/// Scratch device memory owned by a policy (the get_temporary_memory_source
+/// counterpart). Sized by reserve(); kernels read live counts from `counters`.
+struct Workspace {
+  uint32_t n_cap = 0;       // particles
+  uint32_t ts_cap = 0;      // dedup table slots (power of two)
+  uint64_t* skey = nullptr;     // [ts_cap] packed keys, ~0 = empty
+  uint32_t* sorder = nullptr;   // [ts_cap] first-occurrence order (min)
+  uint32_t* la_key = nullptr;   // [list_cap] sort buffers for unique keys
+  uint32_t* la_val = nullptr;
+  uint32_t* lb_key = nullptr;
+  uint32_t* lb_val = nullptr;
+  uint32_t list_cap = 0;
+  uint32_t* bkt = nullptr;      // [n_cap] bucket of each particle
+  uint32_t* run_start = nullptr;  // [n_cap]
+  uint32_t* rk_a = nullptr;     // [n_cap] run sort keys/vals (ping-pong)
+  uint32_t* rv_a = nullptr;
+  uint32_t* rk_b = nullptr;
+  uint32_t* rv_b = nullptr;
+  uint32_t* run_len = nullptr;  // [n_cap]
+  uint32_t* run_dest = nullptr; // [n_cap]
+  uint32_t* tile_heads = nullptr;  // [n_cap / 4096 + 1]
+  uint32_t* tile_base = nullptr;
+  uint32_t* rs_scratch = nullptr;
+  size_t rs_words = 0;
+  uint32_t* scan_tmp = nullptr;  // tile sums for scans
+  size_t scan_words = 0;
+  uint32_t* counters = nullptr;  // [kCtrCount]
+  uint32_t* err = nullptr;       // [1] error bits
+};
+
+// Next slot of zpc's probe sequence, (s + 127) % T (HashTable.hpp:362), for
+// any table size T >= 1 and s in [0, T).
+__device__ __forceinline__ int32_t probe_next(int32_t s, int32_t T) {
+  return (int32_t)(((uint32_t)s + (uint32_t)kProbeStep) % (uint32_t)T);
+}
+
+// HashTableView::query (HashTable.hpp:447-456) with the modular wrap; at most
+// T probes.
+__device__ __forceinline__ int32_t table_find(const TableView& t, int32_t k0, int32_t k1, int32_t k2) {
+  int32_t s = zpc_home(k0, k1, k2, t.T);
+  for (int32_t probe = 0; probe < t.T; ++probe) {
+    const int32_t* kp = t.keys + 3 * (int64_t)s;
+    if (kp[0] == k0 && kp[1] == k1 && kp[2] == k2) return t.indices[s];
+    if (t.indices[s] == -1) return -1;
+    s = probe_next(s, t.T);
+  }
+  return -1;
+}
+
+// ---------------------------------------------------------------- primitives
+uint32_t scan_tiles_for(uint32_t n_cap);
+uint32_t radix_tiles_for(uint32_t n_cap);
+size_t radix_sort_scratch_words(uint32_t n_cap);
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_cap,
+                              uint32_t* tile_sums, uint32_t* total_out, hipStream_t s);
+hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                            const uint32_t* n_dev, uint32_t n_cap, int bits, uint32_t* scratch,
+                            hipStream_t s, bool* result_in_alt);
+
+// ---------------------------------------------------------------- sparsity
+hipError_t table_clear(const TableView& t, Workspace& ws, hipStream_t s);
+hipError_t table_activate(const TableView& t, Workspace& ws, const float* x, int64_t n, float dx,
+                          int blockLen, int offset, float disp, hipStream_t s);
+hipError_t table_enlarge(const TableView& t, Workspace& ws, const int32_t lo[3], const int32_t hi[3],
+                         int32_t n0_cap, hipStream_t s);
+hipError_t table_query(const TableView& t, const int32_t* keys, int64_t n, int32_t* out, hipStream_t s);
+hipError_t bucket_sort(const TableView& t, Workspace& ws, const float* x, int64_t n, float dx,
+                       int blockLen, int offset, float disp, const SortView& out, hipStream_t s);
+
+// ---------------------------------------------------------------- transfer
+// every transfer kernel's static LDS fits the device's per-workgroup limit
+hipError_t check_launch_limits(int device);
+hipError_t grid_clear(float* grid, const int32_t* nblocks_dev, int32_t cap_blocks, hipStream_t s);
+hipError_t p2g(const Model& m, const ParticlesView& ps, const TableView& t, const SortView& sv,
+               float* grid, int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s);
+hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float dt, const float extf[3],
+                       const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s);
+hipError_t g2p(const ParticlesView& ps, const TableView& t, const SortView& sv, const float* grid,
+               int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s);
+
+// grid-stride launch width for kernels that loop over a device-resident count
+inline int persistent_groups() { return 4096; }
+
+}  // namespace zsmpm

@@ -1,0 +1,340 @@
+"""ctypes bindings of the zsmpm C ABI (include/zsmpm.h) and device containers.
+
+Device buffers are torch tensors on the HIP device (torch is only the
+allocator and stream provider here); every computation runs in the hand-written
+gfx950 kernels of libzsmpm_rocm.so. The containers keep zpc's layouts:
+HashTable<i32,3,int>, Grids<f32,3,4>{m,v,rhs}, Particles<f32,3> attributes and
+IndexBuckets. Loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import scenes as S
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libzsmpm_rocm.so")
+
+ABI_MAJOR = 1
+ABI_MINOR = 0
+
+RESULTS = {
+    0: "OK", -1: "ERROR", -2: "INCOMPATIBLE_MAJOR", -3: "INSUFFICIENT_SIZE", -4: "UNSUPPORTED_OPERATION",
+    -5: "INVALID_ARGUMENT", -6: "DEVICE_ERROR", -7: "TABLE_FULL", -8: "KEY_RANGE", -9: "STENCIL_OUTSIDE",
+    -10: "MISSING_BLOCK", -11: "CAPACITY",
+}
+
+
+class ZsmpmError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what}: {RESULTS.get(code, code)} ({code})")
+        self.code = code
+
+
+class Header(C.Structure):
+    _fields_ = [("size", C.c_uint32), ("major", C.c_uint16), ("minor", C.c_uint16)]
+
+
+def _hdr(struct_type):
+    return Header(C.sizeof(struct_type), ABI_MAJOR, ABI_MINOR)
+
+
+class TableDesc(C.Structure):
+    _fields_ = [("hdr", Header), ("table_size", C.c_int32), ("keys", C.c_void_p), ("indices", C.c_void_p),
+                ("status", C.c_void_p), ("cnt", C.c_void_p), ("active_keys", C.c_void_p)]
+
+
+class ParticlesDesc(C.Structure):
+    _fields_ = [("hdr", Header), ("n", C.c_int64), ("x", C.c_void_p), ("v", C.c_void_p), ("C", C.c_void_p),
+                ("F", C.c_void_p), ("m", C.c_void_p), ("logJp", C.c_void_p)]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("hdr", Header), ("dx", C.c_float), ("capacity_blocks", C.c_int32), ("data", C.c_void_p)]
+
+
+class BucketsDesc(C.Structure):
+    _fields_ = [("hdr", Header), ("capacity_blocks", C.c_int32), ("counts", C.c_void_p),
+                ("offsets", C.c_void_p), ("indices", C.c_void_p)]
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("hdr", Header), ("kind", C.c_int32), ("volume", C.c_float), ("E", C.c_float), ("nu", C.c_float),
+                ("cohesion", C.c_float), ("beta", C.c_float), ("yield_surface", C.c_float),
+                ("volume_correction", C.c_int32), ("svd_fp64_gap", C.c_float)]
+
+
+class PlaneCollider(C.Structure):
+    _fields_ = [("type", C.c_int32), ("origin", C.c_float * 3), ("normal", C.c_float * 3)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libzsmpm_rocm.so (built by zpc_amd.build) and declare its ABI."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m zpc_amd.build` "
+                           "(or __graft_entry__.build()) before using the HIP path")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float
+    P = C.POINTER
+    sig = {
+        "zsmpm_abi_version": [P(Header)],
+        "zsmpm_policy__rocm": [i32, vp, P(vp)],
+        "zsmpm_del_policy__rocm": [vp],
+        "zsmpm_policy_reserve__rocm": [vp, i64, i32],
+        "zsmpm_policy_sync": [vp],
+        "zsmpm_policy_counters": [vp, P(C.c_uint32)],
+        "zsmpm_table_clear__rocm": [vp, P(TableDesc)],
+        "zsmpm_table_activate__rocm": [vp, P(TableDesc), vp, i64, f32, i32, i32, f32],
+        "zsmpm_table_enlarge__rocm": [vp, P(TableDesc), P(C.c_int32 * 3), P(C.c_int32 * 3)],
+        "zsmpm_table_query__rocm": [vp, P(TableDesc), vp, i64, vp],
+        "zsmpm_bucket_sort__rocm": [vp, P(TableDesc), vp, i64, f32, i32, i32, f32, P(BucketsDesc)],
+        "zsmpm_grid_clear__rocm": [vp, P(GridDesc), P(TableDesc)],
+        "zsmpm_p2g__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(TableDesc), P(BucketsDesc), P(GridDesc), f32],
+        "zsmpm_grid_update__rocm": [vp, P(GridDesc), P(TableDesc), f32, P(C.c_float * 3), P(PlaneCollider), i32,
+                                    i32, vp],
+        "zsmpm_g2p__rocm": [vp, P(ParticlesDesc), P(TableDesc), P(BucketsDesc), P(GridDesc), f32],
+        "zsmpm_substep__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(TableDesc), P(BucketsDesc), P(GridDesc), f32,
+                                P(C.c_float * 3), P(PlaneCollider), i32],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = i32
+    _lib = L
+    return L
+
+
+def check(code: int, what: str) -> None:
+    if code != 0:
+        raise ZsmpmError(code, what)
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("the zsmpm HIP path needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _ptr(t) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+class Policy:
+    """zsmpm_policy: device, stream and scratch (CudaExecutionPolicy's counterpart)."""
+
+    def __init__(self, device: int = 0, stream=None):
+        torch = _torch()
+        self.L = lib()
+        self.device = device
+        self.stream = stream if stream is not None else torch.cuda.current_stream(device)
+        h = C.c_void_p()
+        check(self.L.zsmpm_policy__rocm(device, C.c_void_p(self.stream.cuda_stream), C.byref(h)), "policy")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.zsmpm_del_policy__rocm(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reserve(self, n: int, table_size: int):
+        check(self.L.zsmpm_policy_reserve__rocm(self.h, n, table_size), "reserve")
+
+    def sync(self):
+        check(self.L.zsmpm_policy_sync(self.h), "device")
+
+    def counters(self) -> np.ndarray:
+        out = (C.c_uint32 * 16)()
+        check(self.L.zsmpm_policy_counters(self.h, out), "counters")
+        return np.frombuffer(out, dtype=np.uint32).copy()
+
+
+class DeviceTable:
+    """HashTable<i32,3,int> buffers (container/HashTable.hpp:40-58, 200-205)."""
+
+    def __init__(self, table_size: int, device="cuda"):
+        torch = _torch()
+        T = int(table_size)
+        self.T = T
+        self.keys = torch.full((T, 3), 2 ** 31 - 1, dtype=torch.int32, device=device)
+        self.indices = torch.full((T,), -1, dtype=torch.int32, device=device)
+        self.status = torch.full((T,), -1, dtype=torch.int32, device=device)
+        self.cnt = torch.zeros(1, dtype=torch.int32, device=device)
+        self.active = torch.zeros((T, 3), dtype=torch.int32, device=device)
+        self.desc = TableDesc(_hdr(TableDesc), T, _ptr(self.keys), _ptr(self.indices), _ptr(self.status),
+                              _ptr(self.cnt), _ptr(self.active))
+
+    def host(self):
+        cnt = int(self.cnt.item())
+        return {"T": self.T, "cnt": cnt, "keys": self.keys.cpu().numpy(), "indices": self.indices.cpu().numpy(),
+                "status": self.status.cpu().numpy(), "active": self.active[:cnt].cpu().numpy()}
+
+
+class DeviceParticles:
+    """Particles<f32,3> attributes on the device."""
+
+    def __init__(self, x, v, Cm, F, m, logJp=None, device="cuda"):
+        torch = _torch()
+
+        def dev(a, shape):
+            return torch.as_tensor(np.ascontiguousarray(a, np.float32)).reshape(shape).to(device).contiguous()
+
+        n = int(np.asarray(x).reshape(-1, 3).shape[0])
+        self.n = n
+        self.x = dev(x, (n, 3))
+        self.v = dev(v, (n, 3))
+        self.C = dev(Cm, (n, 9))
+        self.F = dev(F, (n, 9))
+        self.m = dev(m, (n,))
+        self.logJp = None if logJp is None else dev(logJp, (n,))
+        self.desc = ParticlesDesc(_hdr(ParticlesDesc), n, _ptr(self.x), _ptr(self.v), _ptr(self.C), _ptr(self.F),
+                                  _ptr(self.m), _ptr(self.logJp))
+
+    @staticmethod
+    def from_scene(sc: S.Scene, device="cuda"):
+        return DeviceParticles(sc.x, sc.v, sc.C, sc.F, sc.m, sc.logJp, device)
+
+    def host(self):
+        out = {k: getattr(self, k).cpu().numpy() for k in ("x", "v", "C", "F", "m")}
+        out["logJp"] = None if self.logJp is None else self.logJp.cpu().numpy()
+        return out
+
+
+class DeviceGrid:
+    """Grids<f32,3,4>{m:1, v:3, rhs:3}: (capacity, 7, 64) floats."""
+
+    def __init__(self, dx: float, capacity_blocks: int, device="cuda"):
+        torch = _torch()
+        self.dx = float(dx)
+        self.cap = int(capacity_blocks)
+        self.data = torch.zeros((self.cap, 7, 64), dtype=torch.float32, device=device)
+        self.desc = GridDesc(_hdr(GridDesc), self.dx, self.cap, _ptr(self.data))
+
+
+class DeviceBuckets:
+    """IndexBuckets counts/offsets (capacity + 1) and indices (n)."""
+
+    def __init__(self, capacity_blocks: int, n: int, device="cuda"):
+        torch = _torch()
+        self.cap = int(capacity_blocks)
+        self.counts = torch.zeros(self.cap + 1, dtype=torch.int32, device=device)
+        self.offsets = torch.zeros(self.cap + 1, dtype=torch.int32, device=device)
+        self.indices = torch.zeros(max(int(n), 1), dtype=torch.int32, device=device)
+        self.desc = BucketsDesc(_hdr(BucketsDesc), self.cap, _ptr(self.counts), _ptr(self.offsets),
+                                _ptr(self.indices))
+
+    def host(self, nblocks: int, n: int):
+        return (self.counts[: nblocks + 1].cpu().numpy(), self.offsets[: nblocks + 1].cpu().numpy(),
+                self.indices[:n].cpu().numpy())
+
+
+def model_desc(mat: S.Material, svd_fp64_gap: float = 0.0) -> ModelDesc:
+    return ModelDesc(_hdr(ModelDesc), int(mat.kind), float(mat.volume), float(mat.E), float(mat.nu),
+                     float(mat.cohesion), float(mat.beta), float(mat.yield_surface), int(mat.volume_correction),
+                     float(svd_fp64_gap))
+
+
+def colliders_array(cols: Sequence[S.Collider]):
+    arr = (PlaneCollider * max(len(cols), 1))()
+    for i, c in enumerate(cols):
+        arr[i].type = int(c.kind)
+        for d in range(3):
+            arr[i].origin[d] = float(c.origin[d])
+            arr[i].normal[d] = float(c.normal[d])
+    return arr
+
+
+class Simulator:
+    """Drives the C-ABI substep for one scene: the explicit driver zpc itself
+    lacks, in the stage order its operators imply (table clear, activation,
+    enlargement, bucket sort, grid clear, P2G, grid update, G2P). Device state
+    stays resident."""
+
+    def __init__(self, sc: S.Scene, device: int = 0, grid_capacity: Optional[int] = None,
+                 svd_fp64_gap: float = 0.0):
+        self.sc = sc
+        self.pol = Policy(device)
+        T = int(sc.table_size)
+        self.table = DeviceTable(T)
+        cap = int(grid_capacity) if grid_capacity else max(T // 16, 64)
+        self.grid = DeviceGrid(sc.dx, cap)
+        self.parts = DeviceParticles.from_scene(sc)
+        self.buckets = DeviceBuckets(cap, sc.n)
+        self.model = model_desc(sc.material, svd_fp64_gap)
+        self.cols = colliders_array(sc.colliders)
+        self.ncols = len(sc.colliders)
+        self.extf = (C.c_float * 3)(*[float(g) for g in sc.gravity])
+        self.pol.reserve(sc.n, T)
+        self.L = lib()
+
+    def substep(self, dt: Optional[float] = None):
+        dt = float(self.sc.dt if dt is None else dt)
+        check(self.L.zsmpm_substep__rocm(self.pol.h, C.byref(self.model), C.byref(self.parts.desc),
+                                         C.byref(self.table.desc), C.byref(self.buckets.desc),
+                                         C.byref(self.grid.desc), dt, C.byref(self.extf), self.cols, self.ncols),
+              "substep")
+
+    # stage-level operators (zpc functor equivalents)
+    def clear_table(self):
+        check(self.L.zsmpm_table_clear__rocm(self.pol.h, C.byref(self.table.desc)), "table_clear")
+
+    def activate(self, x=None, n=None, block_len=4, offset=-2, disp=0.5):
+        x = self.parts.x if x is None else x
+        n = self.parts.n if n is None else n
+        check(self.L.zsmpm_table_activate__rocm(self.pol.h, C.byref(self.table.desc), x.data_ptr(), n,
+                                                self.grid.dx, block_len, offset, disp), "activate")
+
+    def enlarge(self, lo=(0, 0, 0), hi=(2, 2, 2)):
+        lo_ = (C.c_int32 * 3)(*lo)
+        hi_ = (C.c_int32 * 3)(*hi)
+        check(self.L.zsmpm_table_enlarge__rocm(self.pol.h, C.byref(self.table.desc), C.byref(lo_), C.byref(hi_)),
+              "enlarge")
+
+    def bucket_sort(self, block_len=4, offset=-2, disp=0.5):
+        check(self.L.zsmpm_bucket_sort__rocm(self.pol.h, C.byref(self.table.desc), self.parts.x.data_ptr(),
+                                             self.parts.n, self.grid.dx, block_len, offset, disp,
+                                             C.byref(self.buckets.desc)), "bucket_sort")
+
+    def clear_grid(self):
+        check(self.L.zsmpm_grid_clear__rocm(self.pol.h, C.byref(self.grid.desc), C.byref(self.table.desc)),
+              "grid_clear")
+
+    def p2g(self, dt=None):
+        dt = float(self.sc.dt if dt is None else dt)
+        check(self.L.zsmpm_p2g__rocm(self.pol.h, C.byref(self.model), C.byref(self.parts.desc),
+                                     C.byref(self.table.desc), C.byref(self.buckets.desc), C.byref(self.grid.desc),
+                                     dt), "p2g")
+
+    def grid_update(self, dt=None, fold=1, max_vel=None):
+        dt = float(self.sc.dt if dt is None else dt)
+        check(self.L.zsmpm_grid_update__rocm(self.pol.h, C.byref(self.grid.desc), C.byref(self.table.desc), dt,
+                                             C.byref(self.extf), self.cols, self.ncols, fold,
+                                             None if max_vel is None else max_vel.data_ptr()), "grid_update")
+
+    def g2p(self, dt=None):
+        dt = float(self.sc.dt if dt is None else dt)
+        check(self.L.zsmpm_g2p__rocm(self.pol.h, C.byref(self.parts.desc), C.byref(self.table.desc),
+                                     C.byref(self.buckets.desc), C.byref(self.grid.desc), dt), "g2p")
+
+    def sync(self):
+        self.pol.sync()
+
+    def nblocks(self) -> int:
+        return int(self.table.cnt.item())
