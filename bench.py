@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Throughput of the explicit MLS-MPM substep on one MI355X.
+
+Workload: the Drucker-Prager sand column of 128 x 256 x 64 cells at dx = 1/512,
+8 particles per cell (16,777,216 particles), seeded jitter, E = 3.5e5, nu = 0.3,
+friction 30 deg, dt = 5e-5, sticky box walls, gravity -9.8 along y. One step is
+one substep (table clear, activation, enlargement, bucket sort, grid clear,
+P2G, grid update, G2P) on state resident in HBM, through
+zsmpm_substep_sorted__rocm: G2P also writes the particles back in bucket order
+(zpc's Vector::reorder), so every substep reads them block by block
+(--in-place: zsmpm_substep__rocm, particles never move in memory).
+
+Prints one JSON line: whole-job particle-substeps/s, the roofline of the
+dominant kernel (algorithmic bytes per launch over its mean duration, from HIP
+events recorded on the policy's stream inside the timed region), the same for
+the whole substep, and the host-core baseline (the C++/OpenMP restatement of
+the reference CPU path, timed on a bounded sample of the same workload).
+
+    python bench.py [--steps K] [--warmup W] [--dims 128,256,64] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+METRIC = "particle-substeps/s @16M particles + % HBM peak; surrogate vs plain"
+
+
+def stage_bytes(stage: str, n: int, blocks: int, plastic: bool, table_size: int) -> float:
+    """Algorithmic HBM bytes of one launch of a stage: every byte the algorithm
+    must move once (particle state, permutation, grid channels), nothing for
+    re-reads, hash probes or LDS."""
+    nodes = 64 * blocks
+    lj = 4 if plastic else 0
+    if stage == "p2g":
+        # read x, v, C, F, m (+ logJp), the bucket permutation; write logJp;
+        # flush m and the force-folded momentum (4 channels) once per node
+        return n * (12 + 12 + 36 + 36 + 4 + lj + 4 + lj) + nodes * 16
+    if stage == "g2p":
+        # read x, F and the permutation; write x, v, C, F; gather v once per node
+        return n * (12 + 36 + 4 + 12 + 12 + 36 + 36) + nodes * 12
+    if stage == "grid_update":
+        return nodes * (16 + 12)  # read m + momentum, write velocity
+    if stage == "grid_clear":
+        return nodes * 28
+    if stage == "activate":
+        return n * 12
+    if stage == "bucket_sort":
+        return n * (12 + 4)  # read x, write the permutation
+    if stage == "table_clear":
+        return table_size * 20
+    if stage == "enlarge":
+        return blocks * 12 * 8
+    return 0.0
+
+
+def substep_bytes(n: int, blocks: int, plastic: bool) -> float:
+    """B_alg per substep: particle state read + written once (196 B, or 204 B
+    with logJp), 16 B of sort traffic, 108 B per active grid node."""
+    state = 204 if plastic else 196
+    return n * (state + 16) + 108 * 64 * blocks
+
+
+def cpu_threads() -> int:
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, n - 1)  # hardware_concurrency - 1, zpc's OpenMP default
+
+
+def cpu_baseline(sc, warmup: int, steps: int) -> dict:
+    """Time the OpenMP restatement on the host cores (a bounded sample)."""
+    from oracle import oracle as O
+
+    ps = O.HostParticles(sc.x, sc.v, sc.C, sc.F, sc.m, sc.logJp)
+    threads = cpu_threads()
+    secs, used, _ = O.omp_run(O.make_params(sc), ps, warmup, steps, threads)
+    return {
+        "value": sc.n * steps / secs,
+        "unit": "particle-substeps/s",
+        "cores": used,
+        "kind": "port",
+        "sample": f"{steps} timed substeps after {warmup} warm-up of the same {sc.n}-particle column "
+                  f"(C++/OpenMP restatement of the reference CPU path, {used} threads, {secs:.1f} s)",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--dims", default="128,256,64", help="sand column cells (x,y,z)")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
+    ap.add_argument("--in-place", action="store_true",
+                    help="keep the particles in their original order (no reorder after G2P)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the substep runs on a single MI355X here; see the sharding notes in the design document)",
+              file=sys.stderr)
+        return 2
+    if args.steps <= 0 or args.warmup < 0:
+        print("--steps must be positive and --warmup non-negative", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from zpc_amd import rocm
+    from zpc_amd import scenes as S
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        return 1
+
+    dims = tuple(int(v) for v in args.dims.split(","))
+    sc = S.sand_scene(dims=dims)
+    plastic = sc.logJp is not None
+    print(f"# scene {sc.name}: {sc.n} particles, table {sc.table_size}", file=sys.stderr, flush=True)
+
+    sim = rocm.Simulator(sc, reorder=not args.in_place)
+    for _ in range(args.warmup):
+        sim.substep()
+    sim.sync()
+    b0 = sim.nblocks()
+    print(f"# warm-up done ({args.warmup} substeps), {b0} active blocks", file=sys.stderr, flush=True)
+
+    sim.pol.stage_times(reset=True)
+    sim.pol.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.substep()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sim.sync()
+    sim.pol.profile(False)
+    stages = sim.pol.stage_times(reset=True)
+    b1 = sim.nblocks()
+    blocks = 0.5 * (b0 + b1)
+
+    wall = t1 - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = sc.n * args.steps / wall
+
+    per_launch = {k: (ms / calls if calls else 0.0) for k, (ms, calls) in stages.items()}
+    dom = max(per_launch, key=lambda k: stages[k][0])
+    dom_bytes = stage_bytes(dom, sc.n, blocks, plastic, sc.table_size)
+    dom_gbs = dom_bytes / (per_launch[dom] * 1e-3) / 1e9
+    sub_bytes = substep_bytes(sc.n, blocks, plastic)
+    sub_gbs = sub_bytes / (ms_per_step * 1e-3) / 1e9
+    print("# stage ms/launch: " + ", ".join(f"{k} {v:.3f}" for k, v in per_launch.items()),
+          file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded jittered sand column (splitmix64), initial state F=I, C=0, v=0",
+        "config": {
+            "workload": f"Drucker-Prager sand column {dims[0]}x{dims[1]}x{dims[2]} cells, 8 ppc, dx=1/512, "
+                        f"{sc.n} particles, dt=5e-5, sticky walls, plain SVD return mapping",
+            "particles": sc.n,
+            "active_blocks": blocks,
+            "table_size": sc.table_size,
+            "path": "plain",
+            "particle_order": "original (in place)" if args.in_place else
+                              "bucket order, reordered inside G2P every substep",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": dom_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": dom_gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": dom_bytes,
+            "ms_per_launch": per_launch[dom],
+        },
+        "substep_roofline": {
+            "bytes_per_substep": sub_bytes,
+            "achieved": sub_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": sub_gbs / HBM_PEAK_GBS,
+        },
+        "stage_ms_per_launch": per_launch,
+        "cpu_baseline": None,
+    }
+    del sim
+    torch.cuda.synchronize()
+    if not args.no_cpu:
+        print(f"# cpu baseline: {args.cpu_warmup} + {args.cpu_steps} substeps on {cpu_threads()} threads",
+              file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_warmup, args.cpu_steps)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -75,6 +75,7 @@ typedef struct zsmpm_particles {
   float* F;     /* [9n] column-major */
   float* m;     /* [n] */
   float* logJp; /* [n], NULL for models without plasticity */
+  int32_t* id;  /* [n] particle ids carried by reordering, or NULL */
 } zsmpm_particles;
 
 typedef struct zsmpm_grid {
@@ -138,6 +139,26 @@ int32_t zsmpm_policy_sync(zsmpm_policy pol);
  * [5] max |v|^2 bits of the last grid update. out must hold 16 entries. */
 int32_t zsmpm_policy_counters(zsmpm_policy pol, uint32_t* out);
 
+/* Stages timed when profiling is on. */
+enum zsmpm_stage {
+  ZSMPM_STAGE_TABLE_CLEAR = 0,
+  ZSMPM_STAGE_ACTIVATE = 1,
+  ZSMPM_STAGE_ENLARGE = 2,
+  ZSMPM_STAGE_BUCKET_SORT = 3,
+  ZSMPM_STAGE_GRID_CLEAR = 4,
+  ZSMPM_STAGE_P2G = 5,
+  ZSMPM_STAGE_GRID_UPDATE = 6,
+  ZSMPM_STAGE_G2P = 7,
+  ZSMPM_STAGE_COUNT = 8
+};
+/* policy.profile(bool) (execution/ExecutionPolicy.hpp:105-122; per-launch stream
+ * timers cuda/Cuda.h:161-178): when enabled, every stage of the calls below is
+ * bracketed by HIP events on the policy's stream. */
+int32_t zsmpm_policy_profile(zsmpm_policy pol, int32_t enable);
+/* Synchronises, then returns the accumulated milliseconds and call counts of
+ * the first n stages (zsmpm_stage order); reset != 0 zeroes the totals. */
+int32_t zsmpm_policy_stage_times(zsmpm_policy pol, double* ms, uint64_t* calls, int32_t n, int32_t reset);
+
 /* CleanSparsity over range(tableSize), simulation/sparsity/SparsityOp.hpp:41-55. */
 int32_t zsmpm_table_clear__rocm(zsmpm_policy pol, const zsmpm_table* table);
 /* ComputeSparsity over range(n), SparsityOp.hpp:57-86: insert floor-div(floor(x/dx*... )
@@ -181,6 +202,23 @@ int32_t zsmpm_substep__rocm(zsmpm_policy pol, const zsmpm_model* model, const zs
                             const zsmpm_table* table, const zsmpm_buckets* buckets, const zsmpm_grid* grid,
                             float dt, const float extf[3], const zsmpm_plane_collider* colliders,
                             int32_t ncolliders);
+
+/* Vector::reorder(pol, buckets.indices, false_c) on every attribute
+ * (container/Vector.hpp:617-646, gather form): out.attr[i] = in.attr[indices[i]],
+ * out->id[i] = in->id ? in->id[indices[i]] : indices[i] when out->id is set.
+ * in and out must not overlap; both hold in->n particles. */
+int32_t zsmpm_reorder__rocm(zsmpm_policy pol, const zsmpm_buckets* buckets, const zsmpm_particles* in,
+                            const zsmpm_particles* out);
+/* zsmpm_substep__rocm followed by the driver-side reorder of the particles by
+ * the substep's own bucket sort (Vector::reorder, container/Vector.hpp:617-646),
+ * fused into G2P: the updated particles are written to `out` in bucket order,
+ * so the next substep reads them block by block. P2G updates in->logJp in
+ * place; the rest of `in` is read only. in and out must not overlap. */
+int32_t zsmpm_substep_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* in,
+                                   const zsmpm_particles* out, const zsmpm_table* table,
+                                   const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt,
+                                   const float extf[3], const zsmpm_plane_collider* colliders,
+                                   int32_t ncolliders);
 
 #ifdef __cplusplus
 }

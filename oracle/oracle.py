@@ -99,6 +99,11 @@ def lib():
                                   _f32p, _f32p, _f32p, C.c_void_p]
         L.zso_run_f64.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int32, C.c_int64, _f64p, _f64p,
                                   _f64p, _f64p, _f64p, C.c_void_p]
+        L.zso_run_sorted_f32.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int32, C.c_int64, _f32p, _f32p,
+                                         _f32p, _f32p, _f32p, C.c_void_p, _i32p]
+        L.zso_run_sorted_f64.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int32, C.c_int64, _f64p, _f64p,
+                                         _f64p, _f64p, _f64p, C.c_void_p, _i32p]
+        L.zso_reorder_f32.argtypes = [C.c_int64, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_void_p, _i32p]
         _lib = L
     return _lib
 
@@ -313,7 +318,13 @@ class Context:
     def g2p(self, params, ps):
         self.L.zso_g2p_f32(self.h, C.byref(params), ps.n, ps.x, ps.v, ps.C, ps.F)
 
-    def run(self, params, ps, steps):
+    def run(self, params, ps, steps, reorder=False):
+        """Run `steps` substeps in place. With reorder, the particles (and
+        ps.ids) are gathered into bucket order after each substep's G2P."""
+        if reorder:
+            fn = getattr(self.L, f"zso_run_sorted_{self.sfx}")
+            fn(self.h, C.byref(params), steps, ps.n, ps.x, ps.v, ps.C, ps.F, ps.m, _ptr(ps.logJp), ps.ids)
+            return
         fn = getattr(self.L, f"zso_run_{self.sfx}")
         fn(self.h, C.byref(params), steps, ps.n, ps.x, ps.v, ps.C, ps.F, ps.m, _ptr(ps.logJp))
 
@@ -330,9 +341,26 @@ class HostParticles:
         self.F = np.ascontiguousarray(F, dt).reshape(-1).copy()
         self.m = np.ascontiguousarray(m, dt).reshape(-1).copy()
         self.logJp = None if logJp is None else np.ascontiguousarray(logJp, dt).reshape(-1).copy()
+        self.ids = np.arange(self.n, dtype=np.int32)
 
     def astype(self, dtype):
-        return HostParticles(self.x, self.v, self.C, self.F, self.m, self.logJp, dtype)
+        out = HostParticles(self.x, self.v, self.C, self.F, self.m, self.logJp, dtype)
+        out.ids = self.ids.copy()
+        return out
+
+    def reorder(self, order):
+        """Vector::reorder gather of every attribute (and ids) by `order`."""
+        order = np.ascontiguousarray(order, np.int32)
+        assert self.x.dtype == np.float32 and order.shape == (self.n,)
+        lib().zso_reorder_f32(self.n, order, self.x, self.v, self.C, self.F, self.m, _ptr(self.logJp), self.ids)
+
+    def by_id(self, name):
+        """Attribute `name` in particle-id order."""
+        a = getattr(self, name)
+        w = a.size // self.n
+        out = np.empty_like(a).reshape(self.n, w)
+        out[self.ids] = a.reshape(self.n, w)
+        return out.reshape(-1)
 
 
 def omp_run(params, ps: HostParticles, warmup: int, steps: int, threads: int = 0):

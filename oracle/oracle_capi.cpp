@@ -275,4 +275,27 @@ void zso_run_f64(void* c, const zso_params* prm, int32_t steps, int64_t n, doubl
     substep(sp, view<double>(n, x, v, C, F, m, logJp), ctx->out, s == steps - 1);
 }
 
+/// As zso_run_*, reordering the particles by each substep's bucket sort
+/// after G2P; ids (may be NULL) is permuted along.
+void zso_run_sorted_f32(void* c, const zso_params* prm, int32_t steps, int64_t n, float* x, float* v,
+                        float* C, float* F, float* m, float* logJp, int32_t* ids) {
+  auto* ctx = static_cast<Ctx<float>*>(c);
+  auto sp = to_params<float>(prm, ctx->cols);
+  for (int s = 0; s < steps; ++s)
+    substep(sp, view<float>(n, x, v, C, F, m, logJp), ctx->out, s == steps - 1, true, ids);
+}
+void zso_run_sorted_f64(void* c, const zso_params* prm, int32_t steps, int64_t n, double* x, double* v,
+                        double* C, double* F, double* m, double* logJp, int32_t* ids) {
+  auto* ctx = static_cast<Ctx<double>*>(c);
+  auto sp = to_params<double>(prm, ctx->cols);
+  for (int s = 0; s < steps; ++s)
+    substep(sp, view<double>(n, x, v, C, F, m, logJp), ctx->out, s == steps - 1, true, ids);
+}
+
+/// The Vector::reorder gather on every attribute with map[n].
+void zso_reorder_f32(int64_t n, const int32_t* map, float* x, float* v, float* C, float* F, float* m,
+                     float* logJp, int32_t* ids) {
+  reorder_particles(view<float>(n, x, v, C, F, m, logJp), map, ids);
+}
+
 }  // extern "C"

@@ -1061,12 +1061,40 @@ struct StepOutputs {
   T maxVel = 0;
 };
 
+/// Vector<T>::reorder(pol, map, false_c) applied to every attribute
+/// (container/Vector.hpp:617-646, the gather form ordered[i] = vs[map[i]]),
+/// with an optional particle-id array carried along.
+template <typename T>
+void reorder_particles(ParticlesView<T> ps, const i32* map, i32* ids) {
+  const i64 n = ps.n;
+  auto gather = [&](T* a, int w) {
+    if (!a) return;
+    std::vector<T> tmp((size_t)n * w);
+    for (i64 i = 0; i < n; ++i)
+      for (int k = 0; k < w; ++k) tmp[(size_t)i * w + k] = a[(i64)map[i] * w + k];
+    std::copy(tmp.begin(), tmp.end(), a);
+  };
+  gather(ps.x, 3);
+  gather(ps.v, 3);
+  gather(ps.C, 9);
+  gather(ps.F, 9);
+  gather(ps.m, 1);
+  gather(ps.logJp, 1);
+  if (ids) {
+    std::vector<i32> tmp(n);
+    for (i64 i = 0; i < n; ++i) tmp[i] = ids[map[i]];
+    std::copy(tmp.begin(), tmp.end(), ids);
+  }
+}
+
 /// One substep: clear table, activate, enlarge, bucket-sort, clear grid, P2G,
 /// fold, grid velocity, boundary, G2P (the stage list of the reconstructed
-/// explicit driver).
+/// explicit driver). With `reorder`, the driver then reorders every particle
+/// attribute by the substep's bucket sort (the Vector::reorder gather), so the
+/// particles are stored block by block for the next substep.
 template <typename T>
 void substep(const StepParams<T>& sp, ParticlesView<T> ps, StepOutputs<T>& out,
-             bool keep_grids = true) {
+             bool keep_grids = true, bool reorder = false, i32* ids = nullptr) {
   Table& t = out.table;
   if (t.T != sp.tableSize) t.resize(sp.tableSize);
   t.clear();
@@ -1090,6 +1118,7 @@ void substep(const StepParams<T>& sp, ParticlesView<T> ps, StepOutputs<T>& out,
   out.maxVel = maxVel;
   g2p(sp.dt, sp.dx, ps, t, grid.data());
   if (keep_grids) out.gridv = std::move(grid);
+  if (reorder) reorder_particles(ps, out.indices.data(), ids);
 }
 
 }  // namespace zso

@@ -253,6 +253,78 @@ def test_multistep_drift(scene, steps, request):
     del base
 
 
+def test_reorder_gather_exact(sand):
+    """zsmpm_reorder__rocm is Vector::reorder's gather on every attribute."""
+    rocm = _rocm()
+    sim = rocm.Simulator(sand, reorder=True)
+    sim.clear_table()
+    sim.activate()
+    sim.enlarge()
+    sim.bucket_sort()
+    sim.reorder_particles()
+    sim.sync()
+    _, _, idx = sim.buckets.host(sim.nblocks(), sand.n)
+    ps = _oracle_particles(sand)
+    ps.reorder(idx)
+    got = sim.parts.host()
+    for k in ("x", "v", "C", "F", "m", "logJp"):
+        assert np.array_equal(got[k].ravel(), getattr(ps, k)), k
+    assert np.array_equal(got["id"], idx)
+
+
+@pytest.mark.parametrize("scene", ["cube", "sand"])
+def test_sorted_substep_parity(scene, request):
+    """The substep with the particle reorder fused into G2P: same table and
+    sort as the oracle, particles in the oracle's bucket order (ids exact),
+    fields within tolerance."""
+    sc = request.getfixturevalue(scene)
+    rocm = _rocm()
+    sim = rocm.Simulator(sc, reorder=True)
+    sim.substep()
+    sim.sync()
+    prm = O.make_params(sc)
+    ps = _oracle_particles(sc)
+    ctx = O.Context()
+    ctx.run(prm, ps, 1, reorder=True)
+    _assert_table_equal(sim.table.host(), ctx.table())
+    counts, offsets, indices = ctx.sort()
+    gc, go, gi = sim.buckets.host(ctx.sizes()["cnt"], sc.n)
+    assert np.array_equal(gc, counts) and np.array_equal(go, offsets) and np.array_equal(gi, indices)
+    got = sim.parts.host()
+    assert np.array_equal(got["id"], ps.ids)
+    assert np.array_equal(got["m"], ps.m)
+    ps64 = _oracle_particles(sc).astype(np.float64)
+    O.Context(np.float64).run(prm, ps64, 1, reorder=True)
+    assert np.array_equal(ps64.ids, ps.ids)
+    for name, ref in (("x", ps.x), ("v", ps.v), ("C", ps.C), ("F", ps.F)):
+        check_field(name, got[name], ref, getattr(ps64, name), TOL_1STEP)
+    if ps.logJp is not None:
+        check_field("logJp", got["logJp"], ps.logJp, ps64.logJp, TOL_1STEP)
+
+
+@pytest.mark.parametrize("scene", ["cube", "sand"])
+def test_sorted_multistep_drift(scene):
+    """100 reordered substeps against the plain (never reordered) oracle run,
+    compared particle by particle through the carried ids."""
+    steps = 100
+    sc = S.cube_scene(cells=20) if scene == "cube" else S.sand_scene(dims=(16, 32, 16))
+    rocm = _rocm()
+    sim = rocm.Simulator(sc, reorder=True)
+    for _ in range(steps):
+        sim.substep()
+    sim.sync()
+    got = sim.parts.host_by_id()
+    ps = _oracle_particles(sc)
+    O.Context().run(O.make_params(sc), ps, steps)
+    for name in ("x", "v", "F"):
+        e = rel_err(got[name], getattr(ps, name))
+        assert e < TOL_100STEP, f"{name} after {steps}: {e:.3e}"
+    disp_ref = ps.x - sc.x.ravel()
+    disp_err = np.max(np.abs(got["x"].ravel() - ps.x)) / max(np.max(np.abs(disp_ref)), 1e-30)
+    assert disp_err < TOL_100STEP, f"displacement error {disp_err:.3e}"
+    assert np.array_equal(got["m"], ps.m)
+
+
 @pytest.mark.parametrize("T", [32, 64, 128, 256])
 def test_small_tables_exact(T):
     """Probe wrap for tables smaller than the probe step (127)."""

@@ -441,6 +441,42 @@ def test_fp64_twin_close_to_fp32():
         assert np.max(np.abs(ra - rb)) <= 1e-4 * np.abs(rb).max(), f
 
 
+def test_reorder_is_vector_gather():
+    """Vector::reorder(gather): ordered[i] = vs[map[i]] for every attribute."""
+    sc = S.perturb(S.sand_scene(dims=(4, 6, 4)), seed=8)
+    ps = O.HostParticles(sc.x, sc.v, sc.C, sc.F, sc.m, sc.logJp)
+    order = np.random.default_rng(4).permutation(ps.n).astype(np.int32)
+    before = {k: getattr(ps, k).reshape(ps.n, -1).copy() for k in ("x", "v", "C", "F", "m", "logJp")}
+    ps.reorder(order)
+    for k, a in before.items():
+        assert np.array_equal(getattr(ps, k).reshape(ps.n, -1), a[order]), k
+    assert np.array_equal(ps.ids, order)
+    ps.reorder(np.argsort(order).astype(np.int32))
+    assert np.array_equal(ps.ids, np.arange(ps.n))
+
+
+def test_sorted_substep_is_substep_then_bucket_gather():
+    """One sorted substep = the plain substep followed by the gather of every
+    attribute by that substep's bucket sort (bit for bit); later substeps only
+    change the summation order, so they stay within float rounding."""
+    sc = S.perturb(S.cube_scene(cells=10), seed=31)
+    a = O.HostParticles(sc.x, sc.v, sc.C, sc.F, sc.m, sc.logJp)
+    b = O.HostParticles(sc.x, sc.v, sc.C, sc.F, sc.m, sc.logJp)
+    prm = O.make_params(sc)
+    ca, cb = O.Context(), O.Context()
+    ca.run(prm, a, 1)
+    cb.run(prm, b, 1, reorder=True)
+    _, _, idx = ca.sort()
+    assert np.array_equal(b.ids, idx)
+    for f in ("x", "v", "C", "F", "m"):
+        assert np.array_equal(b.by_id(f), getattr(a, f)), f
+    ca.run(prm, a, 4)
+    cb.run(prm, b, 4, reorder=True)
+    for f in ("x", "v", "F"):
+        ra = getattr(a, f)
+        assert np.max(np.abs(b.by_id(f) - ra)) <= 1e-5 * np.abs(ra).max(), f
+
+
 # ------------------------------------------------------------------ the C ABI library
 
 def _declared_symbols():

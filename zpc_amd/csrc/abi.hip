@@ -24,6 +24,17 @@ struct zsmpm_policy_s {
   int32_t* own_offsets = nullptr;
   int32_t* own_indices = nullptr;
   int32_t own_cap_blocks = 0;
+  // stage timing (pol.profile(true)): event pairs recorded on the stream,
+  // read back and accumulated at the next synchronisation
+  bool profile = false;
+  struct Pending {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<hipEvent_t> ev_free;
+  std::vector<Pending> pending;
+  double stage_ms[ZSMPM_STAGE_COUNT] = {};
+  uint64_t stage_calls[ZSMPM_STAGE_COUNT] = {};
 };
 
 namespace {
@@ -74,6 +85,63 @@ hipError_t alloc(zsmpm_policy_s* p, T** out, size_t count) {
   p->allocs.push_back(ptr);
   *out = static_cast<T*>(ptr);
   return hipSuccess;
+}
+
+hipEvent_t take_event(zsmpm_policy_s* p) {
+  if (!p->ev_free.empty()) {
+    hipEvent_t e = p->ev_free.back();
+    p->ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one stage with events on the policy's stream when profiling is on
+// (the StreamExecutionTimer of cuda/Cuda.h:161-178, kept per stage).
+struct StageScope {
+  zsmpm_policy_s* p;
+  int stage;
+  hipEvent_t a = nullptr;
+  StageScope(zsmpm_policy_s* pol, int st) : p(pol), stage(st) {
+    if (!p->profile) return;
+    a = take_event(p);
+    if (a && hipEventRecord(a, p->stream) != hipSuccess) {
+      p->ev_free.push_back(a);
+      a = nullptr;
+    }
+  }
+  ~StageScope() {
+    if (!a) return;
+    hipEvent_t b = take_event(p);
+    if (!b || hipEventRecord(b, p->stream) != hipSuccess) {
+      p->ev_free.push_back(a);
+      if (b) p->ev_free.push_back(b);
+      return;
+    }
+    p->pending.push_back({stage, a, b});
+  }
+};
+
+// after the stream is idle: fold the recorded event pairs into the totals
+void harvest_events(zsmpm_policy_s* p) {
+  for (auto& q : p->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess) {
+      p->stage_ms[q.stage] += ms;
+      p->stage_calls[q.stage] += 1;
+    }
+    p->ev_free.push_back(q.a);
+    p->ev_free.push_back(q.b);
+  }
+  p->pending.clear();
+}
+
+void destroy_events(zsmpm_policy_s* p) {
+  harvest_events(p);
+  for (hipEvent_t e : p->ev_free) (void)hipEventDestroy(e);
+  p->ev_free.clear();
 }
 
 void free_all(zsmpm_policy_s* p) {
@@ -169,6 +237,28 @@ int32_t view_particles(const zsmpm_particles* d, ParticlesView& ps, bool need_st
   ps.F = d->F;
   ps.m = d->m;
   ps.logJp = d->logJp;
+  ps.id = d->id;
+  return ZSMPM_OK;
+}
+
+bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
+  if (!a || !b || na == 0 || nb == 0) return false;
+  const char* pa = static_cast<const char*>(a);
+  const char* pb = static_cast<const char*>(b);
+  return pa < pb + nb && pb < pa + na;
+}
+
+// a full particle set for reordered output, disjoint from `in`
+int32_t check_out_particles(const ParticlesView& in, const ParticlesView& out) {
+  if (out.n != in.n || !out.v || !out.C || !out.F || !out.m) return ZSMPM_INVALID_ARGUMENT;
+  if (in.logJp && !out.logJp) return ZSMPM_INVALID_ARGUMENT;
+  const size_t n = (size_t)in.n;
+  const void* ia[] = {in.x, in.v, in.C, in.F, in.m, in.logJp, in.id};
+  const size_t is[] = {12 * n, 12 * n, 36 * n, 36 * n, 4 * n, 4 * n, 4 * n};
+  const void* oa[] = {out.x, out.v, out.C, out.F, out.m, out.logJp, out.id};
+  for (int i = 0; i < 7; ++i)
+    for (int j = 0; j < 7; ++j)
+      if (overlaps(ia[i], is[i], oa[j], is[j])) return ZSMPM_INVALID_ARGUMENT;
   return ZSMPM_OK;
 }
 
@@ -248,10 +338,79 @@ __global__ void copy_maxvel_kernel(const uint32_t* counters, float* out) {
   } while (0)
 #define ZS_HIP(expr) ZS_TRY(from_hip(expr))
 
-const int32_t kStdKey[3] = {0, 0, 0};
-
 int32_t run_buckets(zsmpm_policy_s* p, const TableView& t, const ParticlesView& ps, float dx, SortView& sv) {
   return from_hip(bucket_sort(t, p->ws, ps.x, ps.n, dx, 4, -2, 0.5f, sv, p->stream));
+}
+
+// The explicit driver's substep (stage order of the reconstructed zpc
+// driver); with `particles_out`, G2P also performs the Vector::reorder gather.
+int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_particles* particles,
+                    const zsmpm_particles* particles_out, const zsmpm_table* table, const zsmpm_buckets* buckets,
+                    const zsmpm_grid* grid, float dt, const float extf[3], const zsmpm_plane_collider* colliders,
+                    int32_t ncolliders) {
+  if (!pol || !extf) return ZSMPM_INVALID_ARGUMENT;
+  Model m;
+  ParticlesView ps, po;
+  TableView t;
+  Colliders cols;
+  SortView sv;
+  ZS_TRY(make_model(model, m));
+  ZS_TRY(view_particles(particles, ps, true));
+  if (particles_out) {
+    ZS_TRY(view_particles(particles_out, po, true));
+    ZS_TRY(check_out_particles(ps, po));
+  }
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(view_grid(grid));
+  ZS_TRY(make_colliders(colliders, ncolliders, cols));
+  if (m.kind == ZSMPM_DRUCKER_PRAGER && !ps.logJp) return ZSMPM_INVALID_ARGUMENT;
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, ps.n, t.T));
+  if (buckets) {
+    ZS_TRY(view_buckets(buckets, sv));
+  } else {
+    sv = own_buckets(pol);
+  }
+  const float dx = grid->dx;
+  hipStream_t s = pol->stream;
+  Workspace& ws = pol->ws;
+  const int32_t lo[3] = {0, 0, 0}, hi[3] = {2, 2, 2};
+  {
+    StageScope sc(pol, ZSMPM_STAGE_TABLE_CLEAR);
+    ZS_HIP(table_clear(t, ws, s));
+  }
+  {
+    StageScope sc(pol, ZSMPM_STAGE_ACTIVATE);
+    ZS_HIP(table_activate(t, ws, ps.x, ps.n, dx, 4, -2, 0.5f, s));
+  }
+  {
+    StageScope sc(pol, ZSMPM_STAGE_ENLARGE);
+    ZS_HIP(table_enlarge(t, ws, lo, hi, t.T / 8 > 0 ? t.T / 8 : 1, s));
+  }
+  {
+    StageScope sc(pol, ZSMPM_STAGE_BUCKET_SORT);
+    ZS_HIP(bucket_sort(t, ws, ps.x, ps.n, dx, 4, -2, 0.5f, sv, s));
+  }
+  {
+    StageScope sc(pol, ZSMPM_STAGE_GRID_CLEAR);
+    ZS_HIP(zsmpm::grid_clear(grid->data, t.cnt, grid->capacity_blocks, s));
+  }
+  {
+    // P2G folds the force into the momentum channels (the driver's
+    // v = (mv + rhs)/m convention), so the update must not add channels 4..6
+    StageScope sc(pol, ZSMPM_STAGE_P2G);
+    ZS_HIP(zsmpm::p2g(m, ps, t, sv, grid->data, grid->capacity_blocks, dt, dx, true, ws, s));
+  }
+  {
+    StageScope sc(pol, ZSMPM_STAGE_GRID_UPDATE);
+    set_maxvel_kernel<<<1, 1, 0, s>>>(ws.counters);
+    ZS_HIP(zsmpm::grid_update(grid->data, grid->capacity_blocks, t, dt, extf, cols, dx, 0, ws, s));
+  }
+  {
+    StageScope sc(pol, ZSMPM_STAGE_G2P);
+    ZS_HIP(zsmpm::g2p(ps, particles_out ? &po : nullptr, t, sv, grid->data, grid->capacity_blocks, dt, dx, ws, s));
+  }
+  return ZSMPM_OK;
 }
 
 }  // namespace
@@ -287,6 +446,7 @@ int32_t zsmpm_del_policy__rocm(zsmpm_policy pol) {
   {
     DeviceGuard g(pol->devid);
     (void)hipStreamSynchronize(pol->stream);
+    destroy_events(pol);
     free_all(pol);
   }
   delete pol;
@@ -302,6 +462,7 @@ int32_t zsmpm_policy_sync(zsmpm_policy pol) {
   if (!pol) return ZSMPM_INVALID_ARGUMENT;
   DeviceGuard g(pol->devid);
   ZS_HIP(hipStreamSynchronize(pol->stream));
+  harvest_events(pol);
   if (!pol->ws.err) return ZSMPM_OK;
   uint32_t bits = 0;
   ZS_HIP(hipMemcpy(&bits, pol->ws.err, sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -319,12 +480,37 @@ int32_t zsmpm_policy_counters(zsmpm_policy pol, uint32_t* out) {
   return ZSMPM_OK;
 }
 
+int32_t zsmpm_policy_profile(zsmpm_policy pol, int32_t enable) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
+  pol->profile = enable != 0;
+  return ZSMPM_OK;
+}
+
+int32_t zsmpm_policy_stage_times(zsmpm_policy pol, double* ms, uint64_t* calls, int32_t n, int32_t reset) {
+  if (!pol || n < 0 || (n > 0 && (!ms || !calls))) return ZSMPM_INVALID_ARGUMENT;
+  DeviceGuard g(pol->devid);
+  ZS_HIP(hipStreamSynchronize(pol->stream));
+  harvest_events(pol);
+  for (int i = 0; i < n; ++i) {
+    ms[i] = i < ZSMPM_STAGE_COUNT ? pol->stage_ms[i] : 0.0;
+    calls[i] = i < ZSMPM_STAGE_COUNT ? pol->stage_calls[i] : 0;
+  }
+  if (reset) {
+    for (int i = 0; i < ZSMPM_STAGE_COUNT; ++i) {
+      pol->stage_ms[i] = 0.0;
+      pol->stage_calls[i] = 0;
+    }
+  }
+  return ZSMPM_OK;
+}
+
 int32_t zsmpm_table_clear__rocm(zsmpm_policy pol, const zsmpm_table* table) {
   if (!pol) return ZSMPM_INVALID_ARGUMENT;
   TableView t;
   ZS_TRY(view_table(table, t));
   DeviceGuard g(pol->devid);
   ZS_TRY(ensure(pol, pol->reserved_n, t.T));
+  StageScope sc(pol, ZSMPM_STAGE_TABLE_CLEAR);
   return from_hip(table_clear(t, pol->ws, pol->stream));
 }
 
@@ -335,6 +521,7 @@ int32_t zsmpm_table_activate__rocm(zsmpm_policy pol, const zsmpm_table* table, c
   ZS_TRY(view_table(table, t));
   DeviceGuard g(pol->devid);
   ZS_TRY(ensure(pol, n, t.T));
+  StageScope sc(pol, ZSMPM_STAGE_ACTIVATE);
   return from_hip(table_activate(t, pol->ws, x, n, dx, block_len, offset, displacement, pol->stream));
 }
 
@@ -345,6 +532,7 @@ int32_t zsmpm_table_enlarge__rocm(zsmpm_policy pol, const zsmpm_table* table, co
   ZS_TRY(view_table(table, t));
   DeviceGuard g(pol->devid);
   ZS_TRY(ensure(pol, pol->reserved_n, t.T));
+  StageScope sc(pol, ZSMPM_STAGE_ENLARGE);
   return from_hip(table_enlarge(t, pol->ws, lo, hi, t.T, pol->stream));
 }
 
@@ -367,6 +555,7 @@ int32_t zsmpm_bucket_sort__rocm(zsmpm_policy pol, const zsmpm_table* table, cons
   ZS_TRY(view_buckets(buckets, sv));
   DeviceGuard g(pol->devid);
   ZS_TRY(ensure(pol, n, t.T));
+  StageScope sc(pol, ZSMPM_STAGE_BUCKET_SORT);
   return from_hip(bucket_sort(t, pol->ws, x, n, dx, block_len, offset, displacement, sv, pol->stream));
 }
 
@@ -376,6 +565,7 @@ int32_t zsmpm_grid_clear__rocm(zsmpm_policy pol, const zsmpm_grid* grid, const z
   ZS_TRY(view_grid(grid));
   ZS_TRY(view_table(table, t));
   DeviceGuard g(pol->devid);
+  StageScope sc(pol, ZSMPM_STAGE_GRID_CLEAR);
   return from_hip(zsmpm::grid_clear(grid->data, t.cnt, grid->capacity_blocks, pol->stream));
 }
 
@@ -398,8 +588,10 @@ int32_t zsmpm_p2g__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_
     ZS_TRY(view_buckets(buckets, sv));
   } else {
     sv = own_buckets(pol);
+    StageScope sc(pol, ZSMPM_STAGE_BUCKET_SORT);
     ZS_TRY(run_buckets(pol, t, ps, grid->dx, sv));
   }
+  StageScope sc(pol, ZSMPM_STAGE_P2G);
   return from_hip(zsmpm::p2g(m, ps, t, sv, grid->data, grid->capacity_blocks, dt, grid->dx, false, pol->ws, pol->stream));
 }
 
@@ -414,6 +606,7 @@ int32_t zsmpm_grid_update__rocm(zsmpm_policy pol, const zsmpm_grid* grid, const 
   ZS_TRY(make_colliders(colliders, ncolliders, cols));
   DeviceGuard g(pol->devid);
   ZS_TRY(ensure(pol, pol->reserved_n, t.T));
+  StageScope sc(pol, ZSMPM_STAGE_GRID_UPDATE);
   set_maxvel_kernel<<<1, 1, 0, pol->stream>>>(pol->ws.counters);
   ZS_HIP(zsmpm::grid_update(grid->data, grid->capacity_blocks, t, dt, extf, cols, grid->dx, fold_rhs, pol->ws, pol->stream));
   if (max_vel_sqr) copy_maxvel_kernel<<<1, 1, 0, pol->stream>>>(pol->ws.counters, max_vel_sqr);
@@ -435,51 +628,42 @@ int32_t zsmpm_g2p__rocm(zsmpm_policy pol, const zsmpm_particles* particles, cons
     ZS_TRY(view_buckets(buckets, sv));
   } else {
     sv = own_buckets(pol);
+    StageScope sc(pol, ZSMPM_STAGE_BUCKET_SORT);
     ZS_TRY(run_buckets(pol, t, ps, grid->dx, sv));
   }
-  return from_hip(zsmpm::g2p(ps, t, sv, grid->data, grid->capacity_blocks, dt, grid->dx, pol->ws, pol->stream));
+  StageScope sc(pol, ZSMPM_STAGE_G2P);
+  return from_hip(
+      zsmpm::g2p(ps, nullptr, t, sv, grid->data, grid->capacity_blocks, dt, grid->dx, pol->ws, pol->stream));
 }
 
 int32_t zsmpm_substep__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
                             const zsmpm_table* table, const zsmpm_buckets* buckets, const zsmpm_grid* grid,
                             float dt, const float extf[3], const zsmpm_plane_collider* colliders,
                             int32_t ncolliders) {
-  if (!pol || !extf) return ZSMPM_INVALID_ARGUMENT;
-  Model m;
-  ParticlesView ps;
-  TableView t;
-  Colliders cols;
+  return run_substep(pol, model, particles, nullptr, table, buckets, grid, dt, extf, colliders, ncolliders);
+}
+
+int32_t zsmpm_substep_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* in,
+                                   const zsmpm_particles* out, const zsmpm_table* table,
+                                   const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt,
+                                   const float extf[3], const zsmpm_plane_collider* colliders,
+                                   int32_t ncolliders) {
+  if (!out) return ZSMPM_INVALID_ARGUMENT;
+  return run_substep(pol, model, in, out, table, buckets, grid, dt, extf, colliders, ncolliders);
+}
+
+int32_t zsmpm_reorder__rocm(zsmpm_policy pol, const zsmpm_buckets* buckets, const zsmpm_particles* in,
+                            const zsmpm_particles* out) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
   SortView sv;
-  ZS_TRY(make_model(model, m));
-  ZS_TRY(view_particles(particles, ps, true));
-  ZS_TRY(view_table(table, t));
-  ZS_TRY(view_grid(grid));
-  ZS_TRY(make_colliders(colliders, ncolliders, cols));
-  if (m.kind == ZSMPM_DRUCKER_PRAGER && !ps.logJp) return ZSMPM_INVALID_ARGUMENT;
+  ParticlesView pi, po;
+  ZS_TRY(view_buckets(buckets, sv));
+  ZS_TRY(view_particles(in, pi, true));
+  ZS_TRY(view_particles(out, po, true));
+  ZS_TRY(check_out_particles(pi, po));
   DeviceGuard g(pol->devid);
-  ZS_TRY(ensure(pol, ps.n, t.T));
-  if (buckets) {
-    ZS_TRY(view_buckets(buckets, sv));
-  } else {
-    sv = own_buckets(pol);
-  }
-  const float dx = grid->dx;
-  hipStream_t s = pol->stream;
-  Workspace& ws = pol->ws;
-  const int32_t lo[3] = {0, 0, 0}, hi[3] = {2, 2, 2};
-  ZS_HIP(table_clear(t, ws, s));
-  ZS_HIP(table_activate(t, ws, ps.x, ps.n, dx, 4, -2, 0.5f, s));
-  ZS_HIP(table_enlarge(t, ws, lo, hi, t.T / 8 > 0 ? t.T / 8 : 1, s));
-  ZS_HIP(bucket_sort(t, ws, ps.x, ps.n, dx, 4, -2, 0.5f, sv, s));
-  ZS_HIP(zsmpm::grid_clear(grid->data, t.cnt, grid->capacity_blocks, s));
-  // P2G folds the force into the momentum channels (the driver's
-  // v = (mv + rhs)/m convention), so the update must not add channels 4..6
-  ZS_HIP(zsmpm::p2g(m, ps, t, sv, grid->data, grid->capacity_blocks, dt, dx, true, ws, s));
-  set_maxvel_kernel<<<1, 1, 0, s>>>(ws.counters);
-  ZS_HIP(zsmpm::grid_update(grid->data, grid->capacity_blocks, t, dt, extf, cols, dx, 0, ws, s));
-  ZS_HIP(zsmpm::g2p(ps, t, sv, grid->data, grid->capacity_blocks, dt, dx, ws, s));
-  (void)kStdKey;
-  return ZSMPM_OK;
+  ZS_TRY(ensure(pol, pi.n, pol->reserved_T));
+  return from_hip(reorder_particles(pi, po, sv.indices, pol->ws, pol->stream));
 }
 
 }  // extern "C"

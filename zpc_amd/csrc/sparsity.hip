@@ -89,14 +89,15 @@ __device__ __forceinline__ void dedup_insert(uint64_t pk, uint32_t order, uint64
   atomicOr(err, kErrTableFull);
 }
 
-// ComputeSparsity candidates: one per particle, order = particle index. Lanes
-// that repeat the previous lane's key skip the insert (the run head carries
-// the smallest index of the run).
+// ComputeSparsity candidates: one per particle, order = particle index. Each
+// distinct key of a wave is inserted once, by its lowest lane (the smallest
+// particle index holding it), so hot keys see one atomic per wave.
 __global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restrict__ x, int64_t n,
                                                             float dxinv, int blockLen, int offset,
                                                             float disp, uint64_t* skey, uint32_t* sorder,
                                                             uint32_t ts, uint32_t* err) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int lane = lane_id();
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     const int64_t p = base + threadIdx.x;
     const bool valid = p < n;
@@ -110,8 +111,15 @@ __global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restr
         atomicOr(err, kErrKeyRange);
       }
     }
-    const uint64_t prev = __shfl_up(pk, 1, kWave);
-    const bool head = valid && pk != kEmpty64 && (lane_id() == 0 || prev != pk);
+    // peel off one distinct key per iteration: the lowest pending lane leads
+    bool pending = pk != kEmpty64;
+    bool head = false;
+    for (uint64_t m = ballot(pending); m != 0; m = ballot(pending)) {
+      const int leader = __ffsll((long long)m) - 1;
+      const uint64_t lk = shfl(pk, leader);
+      head = head || lane == leader;
+      pending = pending && pk != lk;
+    }
     if (head) dedup_insert(pk, (uint32_t)p, skey, sorder, ts, err);
   }
 }
@@ -140,18 +148,42 @@ __global__ __launch_bounds__(kThreads) void enlarge_kernel(const int32_t* __rest
   }
 }
 
+constexpr int kCollectGroups = 256;
+constexpr int kCollectBuf = 2048;
+
+__device__ __forceinline__ void list_put(uint32_t slot, uint32_t key, uint32_t val, uint32_t* lkey,
+                                         uint32_t* lval, uint32_t list_cap, uint32_t* err) {
+  if (slot < list_cap) {
+    lkey[slot] = key;
+    lval[slot] = val;
+  } else {
+    atomicOr(err, kErrCapacity);
+  }
+}
+
 // Gather the dedup entries whose key is not in the table yet: (order, slot).
+// Each workgroup scans one contiguous range, collects its entries in LDS and
+// reserves list space with a single global atomic (entries beyond the LDS
+// buffer go straight to the list). The list order is irrelevant: it is
+// sorted by order next.
 __global__ __launch_bounds__(kThreads) void collect_new_kernel(const uint64_t* __restrict__ skey,
                                                                const uint32_t* __restrict__ sorder,
                                                                uint32_t ts, TableView t, uint32_t* counters,
                                                                uint32_t* lkey, uint32_t* lval,
                                                                uint32_t list_cap, uint32_t* err) {
+  __shared__ uint32_t bkey[kCollectBuf], bval[kCollectBuf];
+  __shared__ uint32_t bcount, gbase;
   const bool table_empty = counters[kCtrCnt0] == 0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < ts; base += stride) {
+  const int64_t per = ((int64_t)ts + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = min((int64_t)ts, lo + per);
+  if (threadIdx.x == 0) bcount = 0;
+  __syncthreads();
+  const int lane = lane_id();
+  for (int64_t base = lo; base < hi; base += blockDim.x) {
     const int64_t s = base + threadIdx.x;
     bool take = false;
-    if (s < ts) {
+    if (s < hi) {
       const uint64_t pk = skey[s];
       if (pk != kEmpty64) {
         take = true;
@@ -166,18 +198,24 @@ __global__ __launch_bounds__(kThreads) void collect_new_kernel(const uint64_t* _
     if (m == 0) continue;
     const int leader = __ffsll((long long)m) - 1;
     uint32_t wbase = 0;
-    if (lane_id() == leader) wbase = atomicAdd(&counters[kCtrList], (uint32_t)__popcll(m));
+    if (lane == leader) wbase = atomicAdd(&bcount, (uint32_t)__popcll(m));
     wbase = shfl(wbase, leader);
     if (take) {
       const uint32_t slot = wbase + (uint32_t)__popcll(m & lanemask_lt());
-      if (slot < list_cap) {
-        lkey[slot] = sorder[s];
-        lval[slot] = (uint32_t)s;
+      if (slot < (uint32_t)kCollectBuf) {
+        bkey[slot] = sorder[s];
+        bval[slot] = (uint32_t)s;
       } else {
-        atomicOr(err, kErrCapacity);
+        list_put(atomicAdd(&counters[kCtrList], 1u), sorder[s], (uint32_t)s, lkey, lval, list_cap, err);
       }
     }
   }
+  __syncthreads();
+  const uint32_t nloc = min(bcount, (uint32_t)kCollectBuf);
+  if (threadIdx.x == 0) gbase = nloc ? atomicAdd(&counters[kCtrList], nloc) : 0u;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nloc; i += blockDim.x)
+    list_put(gbase + i, bkey[i], bval[i], lkey, lval, list_cap, err);
 }
 
 // Number the new keys cnt0 + rank, record _activeKeys and place them by
@@ -473,8 +511,7 @@ int bits_for(uint64_t maxval) {
 // dedup entries -> ranked, placed and keyed table entries (shared tail of
 // activation and enlargement)
 hipError_t finish_insertion(const TableView& t, Workspace& ws, uint64_t order_bound, hipStream_t s) {
-  const int g = groups_for(ws.ts_cap, kThreads);
-  collect_new_kernel<<<g, kThreads, 0, s>>>(ws.skey, ws.sorder, ws.ts_cap, t, ws.counters, ws.la_key,
+  collect_new_kernel<<<kCollectGroups, kThreads, 0, s>>>(ws.skey, ws.sorder, ws.ts_cap, t, ws.counters, ws.la_key,
                                             ws.la_val, ws.list_cap, ws.err);
   bool alt = false;
   hipError_t e = radix_sort_pairs(ws.la_key, ws.la_val, ws.lb_key, ws.lb_val, ws.counters + kCtrList,

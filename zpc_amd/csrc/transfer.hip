@@ -511,14 +511,26 @@ __global__ __launch_bounds__(kThreads) void grid_update_kernel(float* __restrict
       }
     }
   }
+  // max |v|^2: wave, then workgroup, then one atomic per workgroup -- and only
+  // when it raises the running maximum (same-address atomics serialise)
+  __shared__ float wmax[kWaves];
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, kWave));
-  if (lane_id() == 0 && vmax > 0.f) atomicMax(&counters[kCtrMaxVel], __float_as_uint(vmax));
+  if (lane_id() == 0) wmax[threadIdx.x / kWave] = vmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float g = wmax[0];
+    for (int w = 1; w < kWaves; ++w) g = fmaxf(g, wmax[w]);
+    const uint32_t bits = __float_as_uint(g);
+    if (g > 0.f && bits > __atomic_load_n(&counters[kCtrMaxVel], __ATOMIC_RELAXED))
+      atomicMax(&counters[kCtrMaxVel], bits);
+  }
 }
 
-__global__ __launch_bounds__(kThreads) void g2p_kernel(ParticlesView ps, TableView t, SortView sv,
-                                                       const float* __restrict__ grid, int32_t grid_cap, float dt,
-                                                       float dx, uint32_t* err) {
+template <bool kReorder>
+__global__ __launch_bounds__(kThreads) void g2p_kernel(ParticlesView ps, ParticlesView out, TableView t,
+                                                       SortView sv, const float* __restrict__ grid,
+                                                       int32_t grid_cap, float dt, float dx, uint32_t* err) {
   __shared__ float4 vel[kArena8];
   __shared__ int32_t nbr[8];
   const int32_t nb = bounded_blocks(t, sv, grid_cap, err);
@@ -553,12 +565,30 @@ __global__ __launch_bounds__(kThreads) void g2p_kernel(ParticlesView ps, TableVi
         atomicOr(err, kErrCapacity);
         continue;
       }
+      // destination: the particle's own slot, or its sorted position
+      const int64_t q = kReorder ? (int64_t)off + i : p;
+      const ParticlesView& dst = kReorder ? out : ps;
+      if (kReorder) {
+        dst.m[q] = ps.m[p];
+        if (ps.logJp) dst.logJp[q] = ps.logJp[p];
+        if (dst.id) dst.id[q] = ps.id ? ps.id[p] : (int32_t)p;
+      }
       const float x[3] = {ps.x[3 * p], ps.x[3 * p + 1], ps.x[3 * p + 2]};
       Arena ar;
       make_arena(x[0], x[1], x[2], dx, ar);
       const int lb0 = ar.base[0] - o0, lb1 = ar.base[1] - o1, lb2 = ar.base[2] - o2;
       if (lb0 < 0 || lb0 > 5 || lb1 < 0 || lb1 > 5 || lb2 < 0 || lb2 > 5) {
         atomicOr(err, kErrArena);
+        if (kReorder) {  // keep the moved state whole; the error is reported at sync
+          for (int d = 0; d < 3; ++d) {
+            dst.x[3 * q + d] = x[d];
+            dst.v[3 * q + d] = ps.v[3 * p + d];
+          }
+          for (int k = 0; k < 9; ++k) {
+            dst.C[9 * q + k] = ps.C[9 * p + k];
+            dst.F[9 * q + k] = ps.F[9 * p + k];
+          }
+        }
         continue;
       }
       float v[3] = {0.f, 0.f, 0.f};
@@ -606,16 +636,42 @@ __global__ __launch_bounds__(kThreads) void g2p_kernel(ParticlesView ps, TableVi
         }
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        ps.x[3 * p + d] = x[d] + v[d] * dt;
-        ps.v[3 * p + d] = v[d];
+        dst.x[3 * q + d] = x[d] + v[d] * dt;
+        dst.v[3 * q + d] = v[d];
       }
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
-        ps.C[9 * p + k] = C[k];
-        ps.F[9 * p + k] = Fn[k];
+        dst.C[9 * q + k] = C[k];
+        dst.F[9 * q + k] = Fn[k];
       }
     }
     __syncthreads();
+  }
+}
+
+// Vector::reorder gather (container/Vector.hpp:617-646) of every attribute
+__global__ __launch_bounds__(kThreads) void reorder_kernel(ParticlesView in, ParticlesView out,
+                                                           const int32_t* __restrict__ map, uint32_t* err) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < out.n; i += stride) {
+    const int64_t p = map[i];
+    if (p < 0 || p >= in.n) {
+      atomicOr(err, kErrCapacity);
+      continue;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      out.x[3 * i + d] = in.x[3 * p + d];
+      out.v[3 * i + d] = in.v[3 * p + d];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      out.C[9 * i + k] = in.C[9 * p + k];
+      out.F[9 * i + k] = in.F[9 * p + k];
+    }
+    out.m[i] = in.m[p];
+    if (in.logJp) out.logJp[i] = in.logJp[p];
+    if (out.id) out.id[i] = in.id ? in.id[p] : (int32_t)p;
   }
 }
 
@@ -628,7 +684,7 @@ hipError_t check_launch_limits(int device) {
   const void* kernels[] = {
       reinterpret_cast<const void*>(&p2g_kernel<0, true>), reinterpret_cast<const void*>(&p2g_kernel<0, false>),
       reinterpret_cast<const void*>(&p2g_kernel<1, true>), reinterpret_cast<const void*>(&p2g_kernel<1, false>),
-      reinterpret_cast<const void*>(&g2p_kernel)};
+      reinterpret_cast<const void*>(&g2p_kernel<false>), reinterpret_cast<const void*>(&g2p_kernel<true>)};
   for (const void* k : kernels) {
     hipFuncAttributes attr;
     e = hipFuncGetAttributes(&attr, k);
@@ -668,9 +724,19 @@ hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float 
   return hipGetLastError();
 }
 
-hipError_t g2p(const ParticlesView& ps, const TableView& t, const SortView& sv, const float* grid,
-               int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s) {
-  g2p_kernel<<<persistent_groups(), kThreads, 0, s>>>(ps, t, sv, grid, grid_cap, dt, dx, ws.err);
+hipError_t g2p(const ParticlesView& ps, const ParticlesView* out, const TableView& t, const SortView& sv,
+               const float* grid, int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s) {
+  if (out)
+    g2p_kernel<true><<<persistent_groups(), kThreads, 0, s>>>(ps, *out, t, sv, grid, grid_cap, dt, dx, ws.err);
+  else
+    g2p_kernel<false><<<persistent_groups(), kThreads, 0, s>>>(ps, ps, t, sv, grid, grid_cap, dt, dx, ws.err);
+  return hipGetLastError();
+}
+
+hipError_t reorder_particles(const ParticlesView& in, const ParticlesView& out, const int32_t* map,
+                             Workspace& ws, hipStream_t s) {
+  if (out.n <= 0) return hipSuccess;
+  reorder_kernel<<<groups_for(out.n, kThreads), kThreads, 0, s>>>(in, out, map, ws.err);
   return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@ struct ParticlesView {
   float* F = nullptr;
   float* m = nullptr;
   float* logJp = nullptr;
+  int32_t* id = nullptr;  // optional particle ids, carried by reordering
 };
 
 /// Bucket-sort outputs (IndexBuckets, container/IndexBuckets.hpp:11-66).
@@ -138,8 +139,13 @@ hipError_t p2g(const Model& m, const ParticlesView& ps, const TableView& t, cons
                float* grid, int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s);
 hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float dt, const float extf[3],
                        const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s);
-hipError_t g2p(const ParticlesView& ps, const TableView& t, const SortView& sv, const float* grid,
-               int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s);
+// out == nullptr updates ps in place; otherwise the updated particles (and
+// m, logJp, id) are written to out in bucket order (slot = sorted position)
+hipError_t g2p(const ParticlesView& ps, const ParticlesView* out, const TableView& t, const SortView& sv,
+               const float* grid, int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s);
+// out.attr[i] = in.attr[map[i]] for every attribute; out.id[i] = in.id ? in.id[map[i]] : map[i]
+hipError_t reorder_particles(const ParticlesView& in, const ParticlesView& out, const int32_t* map,
+                             Workspace& ws, hipStream_t s);
 
 // grid-stride launch width for kernels that loop over a device-resident count
 inline int persistent_groups() { return 4096; }

@@ -50,7 +50,7 @@ class TableDesc(C.Structure):
 
 class ParticlesDesc(C.Structure):
     _fields_ = [("hdr", Header), ("n", C.c_int64), ("x", C.c_void_p), ("v", C.c_void_p), ("C", C.c_void_p),
-                ("F", C.c_void_p), ("m", C.c_void_p), ("logJp", C.c_void_p)]
+                ("F", C.c_void_p), ("m", C.c_void_p), ("logJp", C.c_void_p), ("id", C.c_void_p)]
 
 
 class GridDesc(C.Structure):
@@ -74,6 +74,9 @@ class PlaneCollider(C.Structure):
 
 _lib = None
 
+# zsmpm_stage order
+STAGES = ("table_clear", "activate", "enlarge", "bucket_sort", "grid_clear", "p2g", "grid_update", "g2p")
+
 
 def lib():
     """Load libzsmpm_rocm.so (built by zpc_amd.build) and declare its ABI."""
@@ -93,6 +96,8 @@ def lib():
         "zsmpm_policy_reserve__rocm": [vp, i64, i32],
         "zsmpm_policy_sync": [vp],
         "zsmpm_policy_counters": [vp, P(C.c_uint32)],
+        "zsmpm_policy_profile": [vp, i32],
+        "zsmpm_policy_stage_times": [vp, P(C.c_double), P(C.c_uint64), i32, i32],
         "zsmpm_table_clear__rocm": [vp, P(TableDesc)],
         "zsmpm_table_activate__rocm": [vp, P(TableDesc), vp, i64, f32, i32, i32, f32],
         "zsmpm_table_enlarge__rocm": [vp, P(TableDesc), P(C.c_int32 * 3), P(C.c_int32 * 3)],
@@ -105,6 +110,9 @@ def lib():
         "zsmpm_g2p__rocm": [vp, P(ParticlesDesc), P(TableDesc), P(BucketsDesc), P(GridDesc), f32],
         "zsmpm_substep__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(TableDesc), P(BucketsDesc), P(GridDesc), f32,
                                 P(C.c_float * 3), P(PlaneCollider), i32],
+        "zsmpm_substep_sorted__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(ParticlesDesc), P(TableDesc),
+                                       P(BucketsDesc), P(GridDesc), f32, P(C.c_float * 3), P(PlaneCollider), i32],
+        "zsmpm_reorder__rocm": [vp, P(BucketsDesc), P(ParticlesDesc), P(ParticlesDesc)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -165,6 +173,18 @@ class Policy:
         check(self.L.zsmpm_policy_counters(self.h, out), "counters")
         return np.frombuffer(out, dtype=np.uint32).copy()
 
+    def profile(self, enable: bool = True):
+        """Time every stage with HIP events on the policy's stream."""
+        check(self.L.zsmpm_policy_profile(self.h, 1 if enable else 0), "profile")
+
+    def stage_times(self, reset: bool = False) -> dict:
+        """{stage: (total ms, calls)} accumulated since the last reset."""
+        n = len(STAGES)
+        ms = (C.c_double * n)()
+        calls = (C.c_uint64 * n)()
+        check(self.L.zsmpm_policy_stage_times(self.h, ms, calls, n, 1 if reset else 0), "stage_times")
+        return {name: (float(ms[i]), int(calls[i])) for i, name in enumerate(STAGES)}
+
 
 class DeviceTable:
     """HashTable<i32,3,int> buffers (container/HashTable.hpp:40-58, 200-205)."""
@@ -190,11 +210,11 @@ class DeviceTable:
 class DeviceParticles:
     """Particles<f32,3> attributes on the device."""
 
-    def __init__(self, x, v, Cm, F, m, logJp=None, device="cuda"):
+    def __init__(self, x, v, Cm, F, m, logJp=None, device="cuda", ids=None):
         torch = _torch()
 
-        def dev(a, shape):
-            return torch.as_tensor(np.ascontiguousarray(a, np.float32)).reshape(shape).to(device).contiguous()
+        def dev(a, shape, dtype=np.float32):
+            return torch.as_tensor(np.ascontiguousarray(a, dtype)).reshape(shape).to(device).contiguous()
 
         n = int(np.asarray(x).reshape(-1, 3).shape[0])
         self.n = n
@@ -204,16 +224,49 @@ class DeviceParticles:
         self.F = dev(F, (n, 9))
         self.m = dev(m, (n,))
         self.logJp = None if logJp is None else dev(logJp, (n,))
-        self.desc = ParticlesDesc(_hdr(ParticlesDesc), n, _ptr(self.x), _ptr(self.v), _ptr(self.C), _ptr(self.F),
-                                  _ptr(self.m), _ptr(self.logJp))
+        self.id = None if ids is None else dev(ids, (n,), np.int32)
+        self._describe()
+
+    def _describe(self):
+        self.desc = ParticlesDesc(_hdr(ParticlesDesc), self.n, _ptr(self.x), _ptr(self.v), _ptr(self.C),
+                                  _ptr(self.F), _ptr(self.m), _ptr(self.logJp), _ptr(self.id))
 
     @staticmethod
-    def from_scene(sc: S.Scene, device="cuda"):
-        return DeviceParticles(sc.x, sc.v, sc.C, sc.F, sc.m, sc.logJp, device)
+    def from_scene(sc: S.Scene, device="cuda", ids=False):
+        return DeviceParticles(sc.x, sc.v, sc.C, sc.F, sc.m, sc.logJp, device,
+                               np.arange(sc.n, dtype=np.int32) if ids else None)
+
+    def empty_like(self):
+        """Uninitialised buffers of the same shape (the reorder target)."""
+        torch = _torch()
+        out = DeviceParticles.__new__(DeviceParticles)
+        out.n = self.n
+        for k in ("x", "v", "C", "F", "m", "logJp", "id"):
+            a = getattr(self, k)
+            setattr(out, k, None if a is None else torch.empty_like(a))
+        out._describe()
+        return out
 
     def host(self):
         out = {k: getattr(self, k).cpu().numpy() for k in ("x", "v", "C", "F", "m")}
         out["logJp"] = None if self.logJp is None else self.logJp.cpu().numpy()
+        out["id"] = None if self.id is None else self.id.cpu().numpy()
+        return out
+
+    def host_by_id(self):
+        """host() with every attribute scattered back to particle-id order."""
+        h = self.host()
+        ids = h.pop("id")
+        if ids is None:
+            return h
+        out = {}
+        for k, a in h.items():
+            if a is None:
+                out[k] = None
+                continue
+            b = np.empty_like(a)
+            b[ids] = a
+            out[k] = b
         return out
 
 
@@ -268,14 +321,17 @@ class Simulator:
     stays resident."""
 
     def __init__(self, sc: S.Scene, device: int = 0, grid_capacity: Optional[int] = None,
-                 svd_fp64_gap: float = 0.0):
+                 svd_fp64_gap: float = 0.0, reorder: bool = False):
         self.sc = sc
         self.pol = Policy(device)
         T = int(sc.table_size)
         self.table = DeviceTable(T)
         cap = int(grid_capacity) if grid_capacity else max(T // 16, 64)
         self.grid = DeviceGrid(sc.dx, cap)
-        self.parts = DeviceParticles.from_scene(sc)
+        # reorder: particles are kept in bucket order (ping-pong buffers, ids carried)
+        self.reorder = bool(reorder)
+        self.parts = DeviceParticles.from_scene(sc, ids=self.reorder)
+        self.spare = self.parts.empty_like() if self.reorder else None
         self.buckets = DeviceBuckets(cap, sc.n)
         self.model = model_desc(sc.material, svd_fp64_gap)
         self.cols = colliders_array(sc.colliders)
@@ -286,6 +342,13 @@ class Simulator:
 
     def substep(self, dt: Optional[float] = None):
         dt = float(self.sc.dt if dt is None else dt)
+        if self.reorder:
+            check(self.L.zsmpm_substep_sorted__rocm(self.pol.h, C.byref(self.model), C.byref(self.parts.desc),
+                                                    C.byref(self.spare.desc), C.byref(self.table.desc),
+                                                    C.byref(self.buckets.desc), C.byref(self.grid.desc), dt,
+                                                    C.byref(self.extf), self.cols, self.ncols), "substep_sorted")
+            self.parts, self.spare = self.spare, self.parts
+            return
         check(self.L.zsmpm_substep__rocm(self.pol.h, C.byref(self.model), C.byref(self.parts.desc),
                                          C.byref(self.table.desc), C.byref(self.buckets.desc),
                                          C.byref(self.grid.desc), dt, C.byref(self.extf), self.cols, self.ncols),
@@ -332,6 +395,16 @@ class Simulator:
         dt = float(self.sc.dt if dt is None else dt)
         check(self.L.zsmpm_g2p__rocm(self.pol.h, C.byref(self.parts.desc), C.byref(self.table.desc),
                                      C.byref(self.buckets.desc), C.byref(self.grid.desc), dt), "g2p")
+
+    def reorder_particles(self):
+        """Vector::reorder of every attribute by the current bucket sort."""
+        if self.spare is None:
+            self.spare = self.parts.empty_like()
+        if self.parts.id is not None and self.spare.id is None:
+            raise RuntimeError("reorder target lacks ids")
+        check(self.L.zsmpm_reorder__rocm(self.pol.h, C.byref(self.buckets.desc), C.byref(self.parts.desc),
+                                         C.byref(self.spare.desc)), "reorder")
+        self.parts, self.spare = self.spare, self.parts
 
     def sync(self):
         self.pol.sync()
