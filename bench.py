@@ -66,6 +66,21 @@ def substep_bytes(n: int, blocks: int, plastic: bool) -> float:
     return n * (state + 16) + 108 * 64 * blocks
 
 
+def measured_traffic(kernel_stage: str):
+    """HBM bytes per launch of a stage's main kernel from a committed counter
+    summary (profiles/traffic_<stage>_sand16M.txt: a line
+    `bytes_per_launch <value>`), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"traffic_{kernel_stage}_sand16M.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 2 and parts[0] == "bytes_per_launch":
+            return float(parts[1])
+    return None
+
+
 def cpu_threads() -> int:
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit() and int(env) > 0:
@@ -105,6 +120,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--in-place", action="store_true",
                     help="keep the particles in their original order (no reorder after G2P)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="first run one device-to-device copy of this many bytes (the reference kernel "
+                         "that scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -129,6 +147,16 @@ def main() -> int:
     sc = S.sand_scene(dims=dims)
     plastic = sc.logJp is not None
     print(f"# scene {sc.name}: {sc.n} particles, table {sc.table_size}", file=sys.stderr, flush=True)
+
+    if args.copy_bytes > 0:
+        words = args.copy_bytes // 4
+        src = torch.ones(words, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# copy kernel: {4 * words} bytes read, {4 * words} bytes written", file=sys.stderr, flush=True)
+        del src, dst
 
     sim = rocm.Simulator(sc, reorder=not args.in_place)
     for _ in range(args.warmup):
@@ -194,7 +222,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": dom_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom) if dims == (128, 256, 64) else None,
             "bytes_per_launch": dom_bytes,
             "ms_per_launch": per_launch[dom],
         },

@@ -149,7 +149,8 @@ enum zsmpm_stage {
   ZSMPM_STAGE_P2G = 5,
   ZSMPM_STAGE_GRID_UPDATE = 6,
   ZSMPM_STAGE_G2P = 7,
-  ZSMPM_STAGE_COUNT = 8
+  ZSMPM_STAGE_STRESS = 8, /* the per-particle constitutive part of P2G */
+  ZSMPM_STAGE_COUNT = 9
 };
 /* policy.profile(bool) (execution/ExecutionPolicy.hpp:105-122; per-launch stream
  * timers cuda/Cuda.h:161-178): when enabled, every stage of the calls below is

@@ -194,6 +194,8 @@ int32_t reserve(zsmpm_policy_s* p, int64_t n, int32_t T) {
   ZA(w.scan_tmp, w.scan_words);
   ZA(w.counters, kCtrCount);
   ZA(w.err, 4);
+  ZA(w.contrib, (size_t)9 * w.n_cap);
+  ZA(w.nbr, (size_t)8 * w.ts_cap);
   p->own_cap_blocks = (int32_t)w.ts_cap;
   ZA(p->own_counts, (size_t)p->own_cap_blocks + 1);
   ZA(p->own_offsets, (size_t)p->own_cap_blocks + 1);
@@ -390,16 +392,21 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
   {
     StageScope sc(pol, ZSMPM_STAGE_BUCKET_SORT);
     ZS_HIP(bucket_sort(t, ws, ps.x, ps.n, dx, 4, -2, 0.5f, sv, s));
+    ZS_HIP(block_neighbours(t, ws, s));
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_GRID_CLEAR);
     ZS_HIP(zsmpm::grid_clear(grid->data, t.cnt, grid->capacity_blocks, s));
   }
   {
+    StageScope sc(pol, ZSMPM_STAGE_STRESS);
+    ZS_HIP(zsmpm::stress(m, ps, ws, s));
+  }
+  {
     // P2G folds the force into the momentum channels (the driver's
     // v = (mv + rhs)/m convention), so the update must not add channels 4..6
     StageScope sc(pol, ZSMPM_STAGE_P2G);
-    ZS_HIP(zsmpm::p2g(m, ps, t, sv, grid->data, grid->capacity_blocks, dt, dx, true, ws, s));
+    ZS_HIP(zsmpm::p2g_scatter(ps, t, sv, grid->data, grid->capacity_blocks, dt, dx, true, ws, s));
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_GRID_UPDATE);
@@ -591,8 +598,14 @@ int32_t zsmpm_p2g__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_
     StageScope sc(pol, ZSMPM_STAGE_BUCKET_SORT);
     ZS_TRY(run_buckets(pol, t, ps, grid->dx, sv));
   }
+  {
+    StageScope sc(pol, ZSMPM_STAGE_STRESS);
+    ZS_HIP(zsmpm::stress(m, ps, pol->ws, pol->stream));
+  }
   StageScope sc(pol, ZSMPM_STAGE_P2G);
-  return from_hip(zsmpm::p2g(m, ps, t, sv, grid->data, grid->capacity_blocks, dt, grid->dx, false, pol->ws, pol->stream));
+  ZS_HIP(block_neighbours(t, pol->ws, pol->stream));
+  return from_hip(
+      zsmpm::p2g_scatter(ps, t, sv, grid->data, grid->capacity_blocks, dt, grid->dx, false, pol->ws, pol->stream));
 }
 
 int32_t zsmpm_grid_update__rocm(zsmpm_policy pol, const zsmpm_grid* grid, const zsmpm_table* table, float dt,
@@ -632,6 +645,7 @@ int32_t zsmpm_g2p__rocm(zsmpm_policy pol, const zsmpm_particles* particles, cons
     ZS_TRY(run_buckets(pol, t, ps, grid->dx, sv));
   }
   StageScope sc(pol, ZSMPM_STAGE_G2P);
+  ZS_HIP(block_neighbours(t, pol->ws, pol->stream));
   return from_hip(
       zsmpm::g2p(ps, nullptr, t, sv, grid->data, grid->capacity_blocks, dt, grid->dx, pol->ws, pol->stream));
 }

@@ -1,7 +1,11 @@
 // Device-wide exclusive scan and stable LSB radix sort for gfx950 (wave64).
-// Element counts may live in device memory so a whole substep runs without a
-// host round trip; grids are sized from host-side capacities and surplus
-// workgroups find nothing to do.
+//
+// Element counts may live in device memory, so a whole substep runs without a
+// host round trip. Work follows the device-side count, not the host-side
+// capacity: a fixed grid of at most kGroups workgroups is launched and
+// workgroup g takes the contiguous range [g per, (g+1) per) of the n elements,
+// per = ceil(n / grid). A scan is reduce -> top -> down; a radix pass is
+// histogram -> one-workgroup scan of the digit counts -> stable scatter.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -12,16 +16,25 @@ namespace zsmpm {
 
 namespace {
 
-constexpr int kScanThreads = 256;
-constexpr int kScanItems = 16;
-constexpr int kScanTile = kScanThreads * kScanItems;  // 4096
-constexpr int kScanTopThreads = 1024;
-constexpr int kScanTopItems = 16;
+constexpr int kGroups = 128;       // grid of the range-per-workgroup kernels
+constexpr int kThreads = 256;
+constexpr int kItems = 16;         // scan: items per thread per step (blocked)
+constexpr int kStep = kThreads * kItems;
+constexpr int kSmallThreads = 1024;  // one-workgroup scan of up to kSmallThreads * kSmallItems values
+constexpr int kSmallItems = 32;
 
 __device__ __forceinline__ uint32_t load_n(const uint32_t* n_dev, uint32_t n_cap) {
   if (!n_dev) return n_cap;
   const uint32_t n = *n_dev;
   return n < n_cap ? n : n_cap;
+}
+
+// range [lo, hi) of workgroup g out of G over n elements
+__device__ __forceinline__ void group_range(uint32_t n, uint32_t G, uint32_t g, uint32_t& lo, uint32_t& hi) {
+  const uint32_t per = (n + G - 1) / G;
+  const uint64_t l = (uint64_t)g * per;
+  lo = l < n ? (uint32_t)l : n;
+  hi = (uint64_t)lo + per < n ? lo + per : n;
 }
 
 // inclusive scan across the wave
@@ -58,126 +71,139 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total,
   return res;
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in,
-                                                                   const uint32_t* n_dev, uint32_t n_cap,
-                                                                   uint32_t* __restrict__ tile_sums) {
-  __shared__ uint32_t lds[2 * (kScanThreads / kWave) + 1];
-  const uint32_t n = load_n(n_dev, n_cap);
-  const uint32_t base = blockIdx.x * (uint32_t)kScanTile;
-  uint32_t s = 0;
-  if (base < n) {
-    const uint32_t start = base + threadIdx.x * kScanItems;
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-      const uint32_t idx = start + i;
-      if (idx < n) s += in[idx];
-    }
-  }
+template <int NT>
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* lds) {
   uint32_t total;
-  block_excl_scan<kScanThreads>(s, total, lds);
-  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+  block_excl_scan<NT>(v, total, lds);
+  return total;
 }
 
-// single workgroup: exclusive scan of up to kScanTopThreads*kScanTopItems tile sums
-__global__ __launch_bounds__(kScanTopThreads) void scan_top_kernel(uint32_t* __restrict__ tile_sums,
-                                                                   uint32_t ntiles,
+__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in,
+                                                               const uint32_t* n_dev, uint32_t n_cap,
+                                                               uint32_t* __restrict__ group_sums) {
+  __shared__ uint32_t lds[2 * (kThreads / kWave) + 1];
+  const uint32_t n = load_n(n_dev, n_cap);
+  uint32_t lo, hi;
+  group_range(n, gridDim.x, blockIdx.x, lo, hi);
+  uint32_t s = 0;
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kThreads) s += in[i];
+  const uint32_t total = block_sum<kThreads>(s, lds);
+  if (threadIdx.x == 0) group_sums[blockIdx.x] = total;
+}
+
+// one workgroup: exclusive scan of up to kSmallThreads * kN values in place
+// (thread t owns values [t kN, (t+1) kN), read and written as uint4 when kN
+// is a multiple of 4; vals must then be 16-byte aligned); total to
+// *total_out when given
+template <int kN>
+__global__ __launch_bounds__(kSmallThreads) void scan_small_kernel(uint32_t* __restrict__ vals, uint32_t count,
                                                                    uint32_t* __restrict__ total_out) {
-  __shared__ uint32_t lds[2 * (kScanTopThreads / kWave) + 1];
-  uint32_t vals[kScanTopItems];
-  const uint32_t start = threadIdx.x * kScanTopItems;
+  __shared__ uint32_t lds[2 * (kSmallThreads / kWave) + 1];
+  const uint32_t start = threadIdx.x * kN;
+  uint32_t v[kN];
+  if (kN % 4 == 0 && start + kN <= count) {
+#pragma unroll
+    for (int i = 0; i < kN; i += 4) {
+      const uint4 q = *reinterpret_cast<const uint4*>(vals + start + i);
+      v[i] = q.x;
+      v[i + 1] = q.y;
+      v[i + 2] = q.z;
+      v[i + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kN; ++i) v[i] = start + i < count ? vals[start + i] : 0u;
+  }
   uint32_t s = 0;
 #pragma unroll
-  for (int i = 0; i < kScanTopItems; ++i) {
-    const uint32_t idx = start + i;
-    vals[i] = idx < ntiles ? tile_sums[idx] : 0u;
-    s += vals[i];
-  }
+  for (int i = 0; i < kN; ++i) s += v[i];
   uint32_t total;
-  uint32_t run = block_excl_scan<kScanTopThreads>(s, total, lds);
+  uint32_t run = block_excl_scan<kSmallThreads>(s, total, lds);
 #pragma unroll
-  for (int i = 0; i < kScanTopItems; ++i) {
-    const uint32_t idx = start + i;
-    if (idx < ntiles) tile_sums[idx] = run;
-    run += vals[i];
+  for (int i = 0; i < kN; ++i) {
+    const uint32_t x = v[i];
+    v[i] = run;
+    run += x;
+  }
+  if (kN % 4 == 0 && start + kN <= count) {
+#pragma unroll
+    for (int i = 0; i < kN; i += 4)
+      *reinterpret_cast<uint4*>(vals + start + i) = make_uint4(v[i], v[i + 1], v[i + 2], v[i + 3]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < kN; ++i)
+      if (start + i < count) vals[start + i] = v[i];
   }
   if (threadIdx.x == 0 && total_out) *total_out = total;
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_down_kernel(const uint32_t* __restrict__ in,
-                                                                 uint32_t* __restrict__ out,
-                                                                 const uint32_t* n_dev, uint32_t n_cap,
-                                                                 const uint32_t* __restrict__ tile_sums) {
-  __shared__ uint32_t lds[2 * (kScanThreads / kWave) + 1];
+__global__ __launch_bounds__(kThreads) void scan_down_kernel(const uint32_t* __restrict__ in,
+                                                             uint32_t* __restrict__ out, const uint32_t* n_dev,
+                                                             uint32_t n_cap, const uint32_t* __restrict__ group_base) {
+  __shared__ uint32_t lds[2 * (kThreads / kWave) + 1];
   const uint32_t n = load_n(n_dev, n_cap);
-  const uint32_t base = blockIdx.x * (uint32_t)kScanTile;
-  if (base >= n) return;
-  const uint32_t start = base + threadIdx.x * kScanItems;
-  uint32_t vals[kScanItems];
-  uint32_t s = 0;
+  uint32_t lo, hi;
+  group_range(n, gridDim.x, blockIdx.x, lo, hi);
+  uint32_t carry = group_base[blockIdx.x];
+  for (uint32_t base = lo; base < hi; base += kStep) {  // uniform across the workgroup
+    const uint32_t start = base + threadIdx.x * kItems;
+    uint32_t v[kItems];
+    uint32_t s = 0;
 #pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    const uint32_t idx = start + i;
-    vals[i] = idx < n ? in[idx] : 0u;
-    s += vals[i];
-  }
-  uint32_t total;
-  uint32_t run = block_excl_scan<kScanThreads>(s, total, lds) + tile_sums[blockIdx.x];
+    for (int i = 0; i < kItems; ++i) {
+      const uint32_t idx = start + i;
+      v[i] = idx < hi ? in[idx] : 0u;
+      s += v[i];
+    }
+    uint32_t total;
+    uint32_t run = block_excl_scan<kThreads>(s, total, lds) + carry;
 #pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    const uint32_t idx = start + i;
-    if (idx < n) out[idx] = run;
-    run += vals[i];
+    for (int i = 0; i < kItems; ++i) {
+      const uint32_t idx = start + i;
+      if (idx < hi) out[idx] = run;
+      run += v[i];
+    }
+    carry += total;
   }
 }
 
 // ---------------------------------------------------------------- radix sort
-constexpr int kRsThreads = 256;
-constexpr int kRsItems = 8;
-constexpr int kRsTile = kRsThreads * kRsItems;  // 2048
-constexpr int kRsWaves = kRsThreads / kWave;
+constexpr int kRsWaves = kThreads / kWave;
 
-__global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const uint32_t* __restrict__ keys,
-                                                             const uint32_t* n_dev, uint32_t n_cap,
-                                                             int shift, uint32_t* __restrict__ hist,
-                                                             uint32_t ntiles) {
+__global__ __launch_bounds__(kThreads) void rs_hist_kernel(const uint32_t* __restrict__ keys,
+                                                           const uint32_t* n_dev, uint32_t n_cap, int shift,
+                                                           uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
   const uint32_t n = load_n(n_dev, n_cap);
-  const uint32_t base = blockIdx.x * (uint32_t)kRsTile;
-  if (base < n) {
-#pragma unroll
-    for (int r = 0; r < kRsItems; ++r) {
-      const uint32_t idx = base + r * kRsThreads + threadIdx.x;
-      if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
-    }
-  }
+  uint32_t lo, hi;
+  group_range(n, gridDim.x, blockIdx.x, lo, hi);
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kThreads) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
   __syncthreads();
-  hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+  hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter: within a tile, elements keep their (round, thread) order.
-__global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(
-    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, const uint32_t* n_dev,
-    uint32_t n_cap, int shift, const uint32_t* __restrict__ hist_scanned, uint32_t ntiles,
-    uint32_t* __restrict__ okeys, uint32_t* __restrict__ ovals) {
+// Stable scatter of workgroup g's range: rounds of kThreads elements in
+// order; within a round, equal digits keep thread order (ballot matching).
+__global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, const uint32_t* n_dev, uint32_t n_cap,
+    int shift, const uint32_t* __restrict__ hist_scanned, uint32_t* __restrict__ okeys,
+    uint32_t* __restrict__ ovals) {
   __shared__ uint32_t run[256];
   __shared__ uint32_t wc[kRsWaves][256];
-  __shared__ uint32_t gbase[256];
   const uint32_t n = load_n(n_dev, n_cap);
-  const uint32_t base = blockIdx.x * (uint32_t)kRsTile;
-  if (base >= n) return;
-  run[threadIdx.x] = 0;
+  uint32_t lo, hi;
+  group_range(n, gridDim.x, blockIdx.x, lo, hi);
+  run[threadIdx.x] = hist_scanned[(size_t)threadIdx.x * gridDim.x + blockIdx.x];
 #pragma unroll
   for (int w = 0; w < kRsWaves; ++w) wc[w][threadIdx.x] = 0;
-  gbase[threadIdx.x] = hist_scanned[(size_t)threadIdx.x * ntiles + blockIdx.x];
   __syncthreads();
-  const int lane = lane_id();
   const int wave = threadIdx.x / kWave;
   const uint64_t lt = lanemask_lt();
-  for (int r = 0; r < kRsItems; ++r) {
-    const uint32_t idx = base + r * kRsThreads + threadIdx.x;
-    const bool valid = idx < n;
+  for (uint32_t base = lo; base < hi; base += kThreads) {  // uniform across the workgroup
+    const uint32_t idx = base + threadIdx.x;
+    const bool valid = idx < hi;
     const uint32_t key = valid ? keys[idx] : 0u;
     const uint32_t val = valid ? vals[idx] : 0u;
     const uint32_t d = (key >> shift) & 255u;
@@ -189,15 +215,13 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(
       peers &= on ? b : ~b;
     }
     const uint32_t below = (uint32_t)__popcll(peers & lt);
-    const bool leader = valid && below == 0;
-    if (leader) wc[wave][d] = (uint32_t)__popcll(peers);
+    if (valid && below == 0) wc[wave][d] = (uint32_t)__popcll(peers);
     __syncthreads();
     if (valid) {
       uint32_t pos = run[d] + below;
       for (int w = 0; w < wave; ++w) pos += wc[w][d];
-      const uint32_t dst = gbase[d] + pos;
-      okeys[dst] = key;
-      ovals[dst] = val;
+      okeys[pos] = key;
+      ovals[pos] = val;
     }
     __syncthreads();
     uint32_t add = 0;
@@ -209,29 +233,31 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(
     run[threadIdx.x] += add;
     __syncthreads();
   }
-  (void)lane;
+}
+
+// grid for n_cap elements: one workgroup per kStep elements, at most kGroups
+uint32_t groups_for_cap(uint32_t n_cap) {
+  const uint32_t g = (n_cap + kStep - 1) / kStep;
+  return g < 1 ? 1 : (g > (uint32_t)kGroups ? (uint32_t)kGroups : g);
 }
 
 }  // namespace
 
-uint32_t scan_tiles_for(uint32_t n_cap) { return (n_cap + kScanTile - 1) / kScanTile; }
-uint32_t radix_tiles_for(uint32_t n_cap) { return (n_cap + kRsTile - 1) / kRsTile; }
-
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_cap,
-                              uint32_t* tile_sums, uint32_t* total_out, hipStream_t s) {
+                              uint32_t* scratch, uint32_t* total_out, hipStream_t s) {
   if (n_cap == 0) return hipSuccess;
-  const uint32_t ntiles = scan_tiles_for(n_cap);
-  if (ntiles > (uint32_t)(kScanTopThreads * kScanTopItems)) return hipErrorInvalidValue;
-  scan_reduce_kernel<<<ntiles, kScanThreads, 0, s>>>(in, n_dev, n_cap, tile_sums);
-  scan_top_kernel<<<1, kScanTopThreads, 0, s>>>(tile_sums, ntiles, total_out);
-  scan_down_kernel<<<ntiles, kScanThreads, 0, s>>>(in, out, n_dev, n_cap, tile_sums);
+  const uint32_t G = groups_for_cap(n_cap);
+  scan_reduce_kernel<<<G, kThreads, 0, s>>>(in, n_dev, n_cap, scratch);
+  scan_small_kernel<1><<<1, kSmallThreads, 0, s>>>(scratch, G, total_out);
+  scan_down_kernel<<<G, kThreads, 0, s>>>(in, out, n_dev, n_cap, scratch);
   return hipGetLastError();
 }
 
+size_t scan_scratch_words() { return kGroups; }
+
 size_t radix_sort_scratch_words(uint32_t n_cap) {
-  const uint32_t nt = radix_tiles_for(n_cap);
-  const size_t hist = (size_t)256 * nt;
-  return hist * 2 + scan_tiles_for((uint32_t)hist) + 16;
+  (void)n_cap;
+  return (size_t)256 * kGroups;
 }
 
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
@@ -239,18 +265,14 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
                             hipStream_t s, bool* result_in_alt) {
   *result_in_alt = false;
   if (n_cap == 0 || bits <= 0) return hipSuccess;
-  const uint32_t nt = radix_tiles_for(n_cap);
-  const size_t hsz = (size_t)256 * nt;
-  if (hsz > 0xffffffffull) return hipErrorInvalidValue;
-  uint32_t* hist = scratch;
-  uint32_t* hist_scan = scratch + hsz;
-  uint32_t* tsums = scratch + 2 * hsz;
+  const uint32_t G = groups_for_cap(n_cap);
+  static_assert(256 * kGroups <= kSmallThreads * kSmallItems, "digit histogram exceeds the one-workgroup scan");
+  static_assert(kGroups <= kSmallThreads, "group sums exceed the one-workgroup scan");
   uint32_t *ka = keys, *va = vals, *kb = keys_alt, *vb = vals_alt;
   for (int shift = 0; shift < bits; shift += 8) {
-    rs_hist_kernel<<<nt, kRsThreads, 0, s>>>(ka, n_dev, n_cap, shift, hist, nt);
-    hipError_t e = exclusive_scan_u32(hist, hist_scan, nullptr, (uint32_t)hsz, tsums, nullptr, s);
-    if (e != hipSuccess) return e;
-    rs_scatter_kernel<<<nt, kRsThreads, 0, s>>>(ka, va, n_dev, n_cap, shift, hist_scan, nt, kb, vb);
+    rs_hist_kernel<<<G, kThreads, 0, s>>>(ka, n_dev, n_cap, shift, scratch);
+    scan_small_kernel<kSmallItems><<<1, kSmallThreads, 0, s>>>(scratch, 256 * G, nullptr);
+    rs_scatter_kernel<<<G, kThreads, 0, s>>>(ka, va, n_dev, n_cap, shift, scratch, kb, vb);
     uint32_t* t = ka;
     ka = kb;
     kb = t;

@@ -304,16 +304,25 @@ __global__ void zero_u32_kernel(uint32_t* p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
 }
 
+constexpr uint32_t kHeadBit = 0x80000000u;  // bkt[p] bit 31: p starts a run
+
 // Bucket of every particle (the block number its key maps to), per-bucket
-// counts and the number of run heads in each tile. Lanes query the table only
-// at run heads (and lane 0), the rest take the head's bucket.
+// counts, run heads and the number of run heads per tile. A run head is a
+// particle whose key differs from the previous particle's; bkt[p] carries the
+// head flag in bit 31 so every later pass sees the same runs. Lanes query the
+// table only at run heads (and lane 0), the rest take the head's bucket.
+// Particles of a tile are visited in order: round r, thread t -> tile*kTile +
+// r*kThreads + t.
 __global__ __launch_bounds__(kThreads) void bucket_kernel(const float* __restrict__ x, int64_t n, float dxinv,
                                                           int blockLen, int offset, float disp, TableView t,
                                                           uint32_t* __restrict__ bkt, uint32_t* counts,
-                                                          int32_t cap_blocks, uint32_t* err) {
+                                                          int32_t cap_blocks, uint32_t* __restrict__ tile_heads,
+                                                          uint32_t* err) {
+  __shared__ uint32_t wsum[kThreads / kWave];
   const int64_t ntiles = (n + kTile - 1) / kTile;
   const int lane = lane_id();
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint32_t heads = 0;
     for (int r = 0; r < kTileItems; ++r) {
       const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
       const bool valid = p < n;
@@ -338,7 +347,8 @@ __global__ __launch_bounds__(kThreads) void bucket_kernel(const float* __restric
       const uint64_t le = lanemask_lt() | (1ull << lane);
       const int src = 63 - __clzll(lm & le);
       b = shfl(b, src < 0 ? 0 : src);
-      if (valid) bkt[p] = (uint32_t)(b < 0 ? 0 : b);
+      if (valid) bkt[p] = (uint32_t)(b < 0 ? 0 : b) | (head ? kHeadBit : 0u);
+      heads += head ? 1u : 0u;
       // per-bucket counts: one atomic per (lookup segment) in the wave
       const uint64_t vm = ballot(valid);
       if (lookup && b >= 0) {
@@ -348,80 +358,40 @@ __global__ __launch_bounds__(kThreads) void bucket_kernel(const float* __restric
         atomicAdd(&counts[b], (uint32_t)__popcll(seg & vm));
       }
     }
-  }
-}
-
-// Run heads per tile, counted on the bucket ids themselves (the same test
-// run_emit and particle_scatter apply), so the three passes agree on the runs
-// even where a key found no bucket.
-__global__ __launch_bounds__(kThreads) void run_heads_kernel(const uint32_t* __restrict__ bkt, int64_t n,
-                                                             uint32_t* tile_heads) {
-  __shared__ uint32_t wsum[kThreads / kWave];
-  const int64_t ntiles = (n + kTile - 1) / kTile;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int64_t p0 = tile * kTile + (int64_t)threadIdx.x * kTileItems;
-    uint32_t prev = (p0 > 0 && p0 - 1 < n) ? bkt[p0 - 1] : 0xffffffffu;
-    uint32_t heads = 0;
-#pragma unroll
-    for (int i = 0; i < kTileItems; ++i) {
-      const int64_t p = p0 + i;
-      if (p < n) {
-        const uint32_t b = bkt[p];
-        heads += (p == 0 || b != prev) ? 1u : 0u;
-        prev = b;
-      }
-    }
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) heads += __shfl_xor(heads, o, kWave);
-    if (lane_id() == 0) wsum[threadIdx.x / kWave] = heads;
+    if (lane == 0) wsum[threadIdx.x / kWave] = heads;
     __syncthreads();
     if (threadIdx.x == 0) {
-      uint32_t t = 0;
-      for (int i = 0; i < kThreads / kWave; ++i) t += wsum[i];
-      tile_heads[tile] = t;
+      uint32_t tot = 0;
+      for (int i = 0; i < kThreads / kWave; ++i) tot += wsum[i];
+      tile_heads[tile] = tot;
     }
     __syncthreads();
   }
 }
 
-// Per tile, blocked layout (thread t owns kTileItems consecutive particles):
-// exclusive count of run heads before each particle. Returns the global run
-// id of the run containing particle p (valid only for p < n).
-__device__ __forceinline__ void tile_run_ids(const uint32_t* __restrict__ bkt, int64_t n, int64_t tile,
-                                             uint32_t tile_base, uint32_t (&rid)[kTileItems],
-                                             bool (&headf)[kTileItems], uint32_t* lds) {
-  const int64_t p0 = tile * kTile + (int64_t)threadIdx.x * kTileItems;
-  uint32_t prev = (p0 > 0 && p0 - 1 < n) ? bkt[p0 - 1] : 0xffffffffu;
-  uint32_t cnt = 0;
-#pragma unroll
-  for (int i = 0; i < kTileItems; ++i) {
-    const int64_t p = p0 + i;
-    const uint32_t b = p < n ? bkt[p] : prev;
-    headf[i] = p < n && (p == 0 || b != prev);
-    cnt += headf[i] ? 1u : 0u;
-    prev = b;
-  }
-  // workgroup exclusive scan of cnt
-  constexpr int kWaves = kThreads / kWave;
+// Run ids of one round of a tile (round r, thread t -> particle tile*kTile +
+// r*kThreads + t): `carry` holds the heads before the round; returns the id of
+// the run containing this thread's particle (the number of heads at or before
+// it, minus one) and adds the round's heads to carry. Workgroup-uniform call.
+__device__ __forceinline__ uint32_t round_run_id(bool head, uint32_t& carry, uint32_t* lds) {
   const int lane = lane_id(), wave = threadIdx.x / kWave;
-  uint32_t inc = cnt;
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    const uint32_t t = __shfl_up(inc, o, kWave);
-    if (lane >= o) inc += t;
-  }
-  if (lane == kWave - 1) lds[wave] = inc;
+  const uint64_t hm = ballot(head);
+  if (lane == 0) lds[wave] = (uint32_t)__popcll(hm);
   __syncthreads();
-  uint32_t wo = 0;
-  for (int w = 0; w < wave; ++w) wo += lds[w];
-  __syncthreads();
-  uint32_t run = tile_base + wo + inc - cnt;  // heads strictly before this thread's chunk
+  uint32_t before = 0, total = 0;
 #pragma unroll
-  for (int i = 0; i < kTileItems; ++i) {
-    run += headf[i] ? 1u : 0u;
-    rid[i] = run - 1u;  // id of the run containing p
+  for (int w = 0; w < kThreads / kWave; ++w) {
+    const uint32_t c = lds[w];
+    before += w < wave ? c : 0u;
+    total += c;
   }
-  (void)kWaves;
+  __syncthreads();
+  const uint64_t le = lanemask_lt() | (1ull << lane);
+  const uint32_t rid = carry + before + (uint32_t)__popcll(hm & le) - 1u;
+  carry += total;
+  return rid;
 }
 
 __global__ __launch_bounds__(kThreads) void run_emit_kernel(const uint32_t* __restrict__ bkt, int64_t n,
@@ -430,20 +400,19 @@ __global__ __launch_bounds__(kThreads) void run_emit_kernel(const uint32_t* __re
   __shared__ uint32_t lds[kThreads / kWave];
   const int64_t ntiles = (n + kTile - 1) / kTile;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    uint32_t rid[kTileItems];
-    bool headf[kTileItems];
-    tile_run_ids(bkt, n, tile, tile_base[tile], rid, headf, lds);
-    const int64_t p0 = tile * kTile + (int64_t)threadIdx.x * kTileItems;
-#pragma unroll
-    for (int i = 0; i < kTileItems; ++i) {
-      if (headf[i] && rid[i] < (uint32_t)n) {
-        const int64_t p = p0 + i;
-        run_start[rid[i]] = (uint32_t)p;
-        rkey[rid[i]] = bkt[p];
-        rval[rid[i]] = rid[i];
+    uint32_t carry = tile_base[tile];
+    for (int r = 0; r < kTileItems; ++r) {
+      const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
+      const bool valid = p < n;
+      const uint32_t v = valid ? bkt[p] : 0u;
+      const bool head = valid && (v & kHeadBit);
+      const uint32_t rid = round_run_id(head, carry, lds);
+      if (head && rid < (uint32_t)n) {
+        run_start[rid] = (uint32_t)p;
+        rkey[rid] = v & ~kHeadBit;
+        rval[rid] = rid;
       }
     }
-    __syncthreads();
   }
 }
 
@@ -478,23 +447,36 @@ __global__ __launch_bounds__(kThreads) void particle_scatter_kernel(const uint32
                                                                     const uint32_t* __restrict__ tile_base,
                                                                     const uint32_t* __restrict__ run_start,
                                                                     const uint32_t* __restrict__ run_dest,
+                                                                    const uint32_t* counters,
                                                                     int32_t* __restrict__ indices) {
   __shared__ uint32_t lds[kThreads / kWave];
+  const uint32_t R = min(counters[kCtrRuns], (uint32_t)n);
   const int64_t ntiles = (n + kTile - 1) / kTile;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    uint32_t rid[kTileItems];
-    bool headf[kTileItems];
-    tile_run_ids(bkt, n, tile, tile_base[tile], rid, headf, lds);
-    const int64_t p0 = tile * kTile + (int64_t)threadIdx.x * kTileItems;
-#pragma unroll
-    for (int i = 0; i < kTileItems; ++i) {
-      const int64_t p = p0 + i;
-      if (p < n && rid[i] < (uint32_t)n) {
-        const uint64_t dst = (uint64_t)run_dest[rid[i]] + ((uint32_t)p - run_start[rid[i]]);
+    uint32_t carry = tile_base[tile];
+    for (int r = 0; r < kTileItems; ++r) {
+      const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
+      const bool valid = p < n;
+      const bool head = valid && (bkt[p] & kHeadBit);
+      const uint32_t rid = round_run_id(head, carry, lds);
+      if (valid && rid < R) {
+        const uint64_t dst = (uint64_t)run_dest[rid] + ((uint32_t)p - run_start[rid]);
         if (dst < (uint64_t)n) indices[dst] = (int32_t)p;
       }
     }
-    __syncthreads();
+  }
+}
+
+// Neighbour blocks of every active block: nbr[8 b + d] = index of
+// activeKeys[b] + (d>>2&1, d>>1&1, d&1), -1 where absent.
+__global__ void neighbours_kernel(TableView t, int32_t* __restrict__ nbr, uint32_t cap) {
+  const int64_t total = 8ll * (int64_t)min((uint32_t)*t.cnt, cap);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += stride) {
+    const int64_t b = i >> 3;
+    const int d = (int)(i & 7);
+    nbr[i] = table_find(t, t.active[3 * b] + ((d >> 2) & 1), t.active[3 * b + 1] + ((d >> 1) & 1),
+                        t.active[3 * b + 2] + (d & 1));
   }
 }
 
@@ -515,7 +497,7 @@ hipError_t finish_insertion(const TableView& t, Workspace& ws, uint64_t order_bo
                                             ws.la_val, ws.list_cap, ws.err);
   bool alt = false;
   hipError_t e = radix_sort_pairs(ws.la_key, ws.la_val, ws.lb_key, ws.lb_val, ws.counters + kCtrList,
-                                  ws.list_cap, bits_for(order_bound), ws.rs_scratch, s, &alt);
+                                  ws.list_cap, bits_for(order_bound - 1), ws.rs_scratch, s, &alt);
   if (e != hipSuccess) return e;
   const uint32_t* sorted_slot = alt ? ws.lb_val : ws.la_val;
   const int gl = groups_for(ws.list_cap, kThreads);
@@ -558,6 +540,12 @@ hipError_t table_enlarge(const TableView& t, Workspace& ws, const int32_t lo[3],
   return finish_insertion(t, ws, (uint64_t)(total_cap > 0 ? total_cap : 1), s);
 }
 
+hipError_t block_neighbours(const TableView& t, Workspace& ws, hipStream_t s) {
+  const int g = groups_for(8ll * ws.ts_cap, kThreads);
+  neighbours_kernel<<<g < 1024 ? g : 1024, kThreads, 0, s>>>(t, ws.nbr, ws.ts_cap);
+  return hipGetLastError();
+}
+
 hipError_t table_query(const TableView& t, const int32_t* keys, int64_t n, int32_t* out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   query_kernel<<<groups_for(n, kThreads), kThreads, 0, s>>>(t, keys, n, out);
@@ -577,27 +565,26 @@ hipError_t bucket_sort(const TableView& t, Workspace& ws, const float* x, int64_
   const int gt = (int)(ntiles < 65536 ? ntiles : 65536);
   const float dxinv = (float)1.0 / dx;
   bucket_kernel<<<gt, kThreads, 0, s>>>(x, n, dxinv, blockLen, offset, disp, t, ws.bkt, counts, out.cap_blocks,
-                                        ws.err);
-  run_heads_kernel<<<gt, kThreads, 0, s>>>(ws.bkt, n, ws.tile_heads);
+                                        ws.tile_heads, ws.err);
   hipError_t e = exclusive_scan_u32(ws.tile_heads, ws.tile_base, nullptr, (uint32_t)ntiles, ws.scan_tmp,
                                     ws.counters + kCtrRuns, s);
   if (e != hipSuccess) return e;
   run_emit_kernel<<<gt, kThreads, 0, s>>>(ws.bkt, n, ws.tile_base, ws.run_start, ws.rk_a, ws.rv_a);
   bool alt = false;
   e = radix_sort_pairs(ws.rk_a, ws.rv_a, ws.rk_b, ws.rv_b, ws.counters + kCtrRuns, (uint32_t)n,
-                       bits_for((uint64_t)out.cap_blocks), ws.rs_scratch, s, &alt);
+                       bits_for((uint64_t)out.cap_blocks - 1), ws.rs_scratch, s, &alt);
   if (e != hipSuccess) return e;
   const uint32_t* sorted_ids = alt ? ws.rv_b : ws.rv_a;
   uint32_t* spare = alt ? ws.rk_a : ws.rk_b;  // free buffer for the sorted dest scan
-  run_len_kernel<<<groups_for(n, kThreads), kThreads, 0, s>>>(sorted_ids, ws.run_start, ws.counters, n,
-                                                               ws.run_len);
+  const int gr = groups_for(n, kThreads) < 1024 ? groups_for(n, kThreads) : 1024;  // grid-stride over R <= n runs
+  run_len_kernel<<<gr, kThreads, 0, s>>>(sorted_ids, ws.run_start, ws.counters, n, ws.run_len);
   e = exclusive_scan_u32(ws.run_len, spare, ws.counters + kCtrRuns, (uint32_t)n, ws.scan_tmp, nullptr, s);
   if (e != hipSuccess) return e;
-  run_dest_kernel<<<groups_for(n, kThreads), kThreads, 0, s>>>(sorted_ids, spare, ws.counters, n, ws.run_dest);
+  run_dest_kernel<<<gr, kThreads, 0, s>>>(sorted_ids, spare, ws.counters, n, ws.run_dest);
   e = exclusive_scan_u32(counts, (uint32_t*)out.offsets, nullptr, (uint32_t)nb1, ws.scan_tmp, nullptr, s);
   if (e != hipSuccess) return e;
   particle_scatter_kernel<<<gt, kThreads, 0, s>>>(ws.bkt, n, ws.tile_base, ws.run_start, ws.run_dest,
-                                                  out.indices);
+                                                  ws.counters, out.indices);
   return hipGetLastError();
 }
 

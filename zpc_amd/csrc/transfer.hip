@@ -43,6 +43,7 @@ constexpr int kWaves = kThreads / kWave;
 constexpr int kMainCells = 64;    // base cells [4b+1, 4b+5)^3
 constexpr int kArena6 = 216;      // nodes [4b+1, 4b+7)^3
 constexpr int kArena8 = 512;      // nodes [4b, 4b+8)^3
+constexpr int kG2PThreads = 256;  // one workgroup per bucket
 
 inline int groups_for(int64_t n, int threads) {
   int64_t g = (n + threads - 1) / threads;
@@ -82,15 +83,13 @@ __device__ __forceinline__ int32_t bounded_blocks(const TableView& t, const Sort
   return nb;
 }
 
-// neighbour blocks key + (d>>2&1, d>>1&1, d&1) of bucket b
-__device__ __forceinline__ void load_neighbours(const TableView& t, int32_t b, int32_t* nbr, uint32_t* err) {
+// neighbour blocks key + (d>>2&1, d>>1&1, d&1) of bucket b, from the
+// per-substep neighbour table
+__device__ __forceinline__ void load_neighbours(const int32_t* __restrict__ nbr_tab, int32_t b, int32_t* nbr,
+                                                uint32_t* err) {
   if (threadIdx.x < 8) {
-    const int d = threadIdx.x;
-    const int32_t k0 = t.active[3 * (int64_t)b] + ((d >> 2) & 1);
-    const int32_t k1 = t.active[3 * (int64_t)b + 1] + ((d >> 1) & 1);
-    const int32_t k2 = t.active[3 * (int64_t)b + 2] + (d & 1);
-    const int32_t r = table_find(t, k0, k1, k2);
-    nbr[d] = r;
+    const int32_t r = nbr_tab[8 * (int64_t)b + threadIdx.x];
+    nbr[threadIdx.x] = r;
     if (r < 0) atomicOr(err, kErrMissingBlock);
   }
 }
@@ -117,10 +116,45 @@ struct Rec {
   static constexpr int kChunk = kFold ? 512 : 256;
 };
 
-template <int kKind, bool kFold>
-__global__ __launch_bounds__(kThreads) void p2g_kernel(Model m, ParticlesView ps, TableView t, SortView sv,
+// Per-particle constitutive part of P2G (P2G.hpp:57-100): vol P F^T from the
+// stored F (fixed-corotated or the Drucker-Prager return mapping, which
+// writes logJp back). Particles in array order, 64 consecutive per wave, so
+// rows always move as 16-byte vectors.
+template <int kKind>
+__global__ __launch_bounds__(kThreads) void stress_kernel(Model m, ParticlesView ps, float* __restrict__ contrib,
+                                                          uint32_t* counters) {
+  __shared__ float4 stage4[kWaves][stage_floats(9) / 4];
+  const int wave = uniform((int32_t)(threadIdx.x / kWave)), lane = lane_id();
+  float* stage = reinterpret_cast<float*>(stage4[wave]);
+  const int64_t nwaves = (int64_t)gridDim.x * kWaves;
+  uint32_t fp64_used = 0;
+  for (int64_t w = (int64_t)blockIdx.x * kWaves + wave; w * kWave < ps.n; w += nwaves) {
+    const int64_t p0 = w * kWave;
+    const int32_t cw = (int32_t)min((int64_t)kWave, ps.n - p0);
+    const int64_t p = p0 + lane;
+    float F[9], PF[9];
+    float lj = (kKind != 0 && lane < cw) ? ps.logJp[p] : 0.f;
+    wave_load_rows<9>(ps.F, p0, cw, stage, F);
+    if (lane < cw) {
+      bool f64 = false;
+      if (kKind == 0) {
+        stress_fixedcorotated(m, F, PF, f64);
+      } else {
+        stress_sand(m, F, lj, PF, nullptr, f64);
+        ps.logJp[p] = lj;
+      }
+      fp64_used += f64 ? 1u : 0u;
+    }
+    wave_store_rows<9>(contrib, p0, cw, stage, PF);
+  }
+  if (fp64_used) atomicAdd(&counters[kCtrFp64], fp64_used);
+}
+
+template <bool kFold>
+__global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const float* __restrict__ contrib,
+                                                       TableView t, SortView sv, const int32_t* __restrict__ nbr_tab,
                                                        float* __restrict__ grid, int32_t grid_cap, float dt,
-                                                       float dx, uint32_t* err, uint32_t* counters) {
+                                                       float dx, uint32_t* err) {
   constexpr int kQ = Rec<kFold>::kQuads;
   constexpr int kChunk = Rec<kFold>::kChunk;
   constexpr int kNV = kFold ? 1 : 2;  // float4 per arena node
@@ -136,8 +170,7 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(Model m, ParticlesView ps
   const float dx_inv = (float)1.0 / dx;
   const float D_inv = 4.f * dx_inv * dx_inv;
   const float scale = -dt * D_inv;
-  const int wave = threadIdx.x / kWave, lane = lane_id();
-  uint32_t fp64_used = 0;
+  const int wave = uniform((int32_t)(threadIdx.x / kWave)), lane = lane_id();
 
   for (int32_t b = blockIdx.x; b < nb; b += gridDim.x) {
     const int32_t cnt = sv.counts[b];
@@ -145,7 +178,7 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(Model m, ParticlesView ps
     if (cnt <= 0 || off < 0 || (int64_t)off + cnt > ps.n) continue;
     for (int i = threadIdx.x; i < kWaves * kArena6 * kNV; i += kThreads)
       (&arena[0][0])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    load_neighbours(t, b, nbr, err);
+    load_neighbours(nbr_tab, b, nbr, err);
     const int32_t o0 = t.active[3 * (int64_t)b] * 4, o1 = t.active[3 * (int64_t)b + 1] * 4,
                   o2 = t.active[3 * (int64_t)b + 2] * 4;
 
@@ -156,33 +189,55 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(Model m, ParticlesView ps
         cfill[threadIdx.x] = 0;
       }
       __syncthreads();
-      // ---- A: stress + affine record per particle
-      for (int32_t i = threadIdx.x; i < nch; i += kThreads) {
-        const int64_t p = sv.indices[off + c0 + i];
-        if (p < 0 || p >= ps.n) {
+      // ---- A: stress + affine record per particle. Each wave takes 64
+      // consecutive chunk slots (i = threadIdx.x + k * kThreads, as phase B
+      // expects); a contiguous run of particles is read with 16-byte vectors,
+      // staged in the wave's own (not yet written) record slots.
+      for (int32_t i0 = wave * kWave; i0 < nch; i0 += kThreads) {
+        const int32_t i = i0 + lane;
+        const int32_t cw = min(kWave, nch - i0);
+        const bool valid = lane < cw;
+        const int64_t p = valid ? (int64_t)sv.indices[off + c0 + i] : -1;
+        const bool bad = valid && (p < 0 || p >= ps.n);
+        if (bad) {
           atomicOr(err, kErrCapacity);
           cellof[i] = 255;
           list[i] = 0;
-          continue;
         }
-        const float x0 = ps.x[3 * p], x1 = ps.x[3 * p + 1], x2 = ps.x[3 * p + 2];
-        const float mass = ps.m[p];
-        float F[9], C[9];
+        const bool live = valid && !bad;
+        int64_t p0;
+        const bool contig = wave_contiguous(p, live, p0) && ballot(bad) == 0;
+        float* stage = reinterpret_cast<float*>(rec + (int64_t)i0 * kQ);
+        float xv[3] = {0.f, 0.f, 0.f}, vv[3] = {0.f, 0.f, 0.f};
+        float PF[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        float C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        const float mass = live ? ps.m[p] : 0.f;
+        if (contig) {
+          RowLoad<3> lx, lv;
+          RowLoad<9> lp, lc;
+          lx.issue(ps.x, p0, cw);
+          lv.issue(ps.v, p0, cw);
+          lp.issue(contrib, p0, cw);
+          lc.issue(ps.C, p0, cw);
+          lx.finish(stage, cw, xv);
+          lv.finish(stage, cw, vv);
+          lp.finish(stage, cw, PF);
+          lc.finish(stage, cw, C);
+        } else if (live) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) F[k] = ps.F[9 * p + k];
-        float contrib[9];
-        bool f64 = false;
-        if (kKind == 0) {
-          stress_fixedcorotated(m, F, contrib, f64);
-        } else {
-          float lj = ps.logJp[p];
-          stress_sand(m, F, lj, contrib, nullptr, f64);
-          ps.logJp[p] = lj;
+          for (int d = 0; d < 3; ++d) {
+            xv[d] = ps.x[3 * p + d];
+            vv[d] = ps.v[3 * p + d];
+          }
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            PF[k] = contrib[9 * p + k];
+            C[k] = ps.C[9 * p + k];
+          }
         }
-        fp64_used += f64 ? 1u : 0u;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) C[k] = ps.C[9 * p + k];
-        const float v0 = ps.v[3 * p], v1 = ps.v[3 * p + 1], v2 = ps.v[3 * p + 2];
+        if (!live) continue;
+        const float x0 = xv[0], x1 = xv[1], x2 = xv[2];
+        const float v0 = vv[0], v1 = vv[1], v2 = vv[2];
         // base node and local position in grid units (LocalArena::init, Utils.hpp:51-70)
         float l[3];
         int32_t base[3];
@@ -212,7 +267,7 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(Model m, ParticlesView ps
         float Af[9], Bf[3];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-          const float cf = contrib[k] * scale;
+          const float cf = PF[k] * scale;
           if (kFold) {
             A[k] = (mass * C[k] + cf) * dx;
           } else {
@@ -446,7 +501,6 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(Model m, ParticlesView ps
     }
     __syncthreads();
   }
-  if (fp64_used) atomicAdd(&counters[kCtrFp64], fp64_used);
 }
 
 // v = (mv [+ rhs]) / m + extf dt where m != 0 (GridOp.hpp:71-110), max |v|^2,
@@ -528,24 +582,28 @@ __global__ __launch_bounds__(kThreads) void grid_update_kernel(float* __restrict
 }
 
 template <bool kReorder>
-__global__ __launch_bounds__(kThreads) void g2p_kernel(ParticlesView ps, ParticlesView out, TableView t,
-                                                       SortView sv, const float* __restrict__ grid,
-                                                       int32_t grid_cap, float dt, float dx, uint32_t* err) {
+__global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, ParticlesView out, TableView t,
+                                                          SortView sv, const int32_t* __restrict__ nbr_tab,
+                                                          const float* __restrict__ grid, int32_t grid_cap,
+                                                          float dt, float dx, uint32_t* err) {
   __shared__ float4 vel[kArena8];
   __shared__ int32_t nbr[8];
+  __shared__ float4 stage4[kG2PThreads / kWave][stage_floats(9) / 4];
   const int32_t nb = bounded_blocks(t, sv, grid_cap, err);
   const float dx_inv = (float)1 / dx;
   const float D_inv = 4.f * dx_inv * dx_inv;
+  const int wave = uniform((int32_t)(threadIdx.x / kWave)), lane = lane_id();
+  float* stage = reinterpret_cast<float*>(stage4[wave]);
   for (int32_t b = blockIdx.x; b < nb; b += gridDim.x) {
     const int32_t cnt = sv.counts[b];
     const int32_t off = sv.offsets[b];
     if (cnt <= 0 || off < 0 || (int64_t)off + cnt > ps.n) continue;
-    load_neighbours(t, b, nbr, err);
+    load_neighbours(nbr_tab, b, nbr, err);
     const int32_t o0 = t.active[3 * (int64_t)b] * 4, o1 = t.active[3 * (int64_t)b + 1] * 4,
                   o2 = t.active[3 * (int64_t)b + 2] * 4;
     __syncthreads();
     // stage node velocities: slot s -> (neighbour block s>>6, cell s&63), stored at (r0*8 + r1)*8 + r2
-    for (int s = threadIdx.x; s < kArena8; s += kThreads) {
+    for (int s = threadIdx.x; s < kArena8; s += kG2PThreads) {
       const int nbi = s >> 6, cell = s & 63;
       const int r0 = ((nbi >> 2) & 1) * 4 + ((cell >> 4) & 3);
       const int r1 = ((nbi >> 1) & 1) * 4 + ((cell >> 2) & 3);
@@ -559,90 +617,130 @@ __global__ __launch_bounds__(kThreads) void g2p_kernel(ParticlesView ps, Particl
       vel[(r0 * 8 + r1) * 8 + r2] = v;
     }
     __syncthreads();
-    for (int32_t i = threadIdx.x; i < cnt; i += kThreads) {
-      const int64_t p = sv.indices[off + i];
-      if (p < 0 || p >= ps.n) {
-        atomicOr(err, kErrCapacity);
-        continue;
+    // each wave takes 64 consecutive bucket slots at a time (wave-uniform loop)
+    for (int32_t i0 = wave * kWave; i0 < cnt; i0 += kG2PThreads) {
+      const int32_t cw = min(kWave, cnt - i0);
+      const bool valid = lane < cw;
+      const int64_t p = valid ? (int64_t)sv.indices[off + i0 + lane] : -1;
+      const bool bad = valid && (p < 0 || p >= ps.n);
+      if (bad) atomicOr(err, kErrCapacity);
+      const bool live = valid && !bad;
+      int64_t p0;
+      const bool contig = wave_contiguous(p, live, p0) && ballot(bad) == 0;
+      float x[3] = {0.f, 0.f, 0.f}, F[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      // the attributes the reorder only moves, loaded with the rest
+      float mcopy = 0.f, ljcopy = 0.f;
+      int32_t idcopy = 0;
+      if (kReorder && live) {
+        mcopy = ps.m[p];
+        if (ps.logJp) ljcopy = ps.logJp[p];
+        idcopy = ps.id ? ps.id[p] : (int32_t)p;
       }
-      // destination: the particle's own slot, or its sorted position
-      const int64_t q = kReorder ? (int64_t)off + i : p;
-      const ParticlesView& dst = kReorder ? out : ps;
-      if (kReorder) {
-        dst.m[q] = ps.m[p];
-        if (ps.logJp) dst.logJp[q] = ps.logJp[p];
-        if (dst.id) dst.id[q] = ps.id ? ps.id[p] : (int32_t)p;
+      if (contig) {
+        RowLoad<3> lx;
+        RowLoad<9> lf;
+        lx.issue(ps.x, p0, cw);
+        lf.issue(ps.F, p0, cw);
+        lx.finish(stage, cw, x);
+        lf.finish(stage, cw, F);
+      } else if (live) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x[d] = ps.x[3 * p + d];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = ps.F[9 * p + k];
       }
-      const float x[3] = {ps.x[3 * p], ps.x[3 * p + 1], ps.x[3 * p + 2]};
       Arena ar;
       make_arena(x[0], x[1], x[2], dx, ar);
       const int lb0 = ar.base[0] - o0, lb1 = ar.base[1] - o1, lb2 = ar.base[2] - o2;
-      if (lb0 < 0 || lb0 > 5 || lb1 < 0 || lb1 > 5 || lb2 < 0 || lb2 > 5) {
-        atomicOr(err, kErrArena);
-        if (kReorder) {  // keep the moved state whole; the error is reported at sync
-          for (int d = 0; d < 3; ++d) {
-            dst.x[3 * q + d] = x[d];
-            dst.v[3 * q + d] = ps.v[3 * p + d];
-          }
-          for (int k = 0; k < 9; ++k) {
-            dst.C[9 * q + k] = ps.C[9 * p + k];
-            dst.F[9 * q + k] = ps.F[9 * p + k];
-          }
-        }
-        continue;
-      }
+      const bool outside = live && (lb0 < 0 || lb0 > 5 || lb1 < 0 || lb1 > 5 || lb2 < 0 || lb2 > 5);
+      if (outside) atomicOr(err, kErrArena);
+      const bool ok = live && !outside;
       float v[3] = {0.f, 0.f, 0.f};
       float C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      float xn[3], Fn[9];
+      if (ok) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const float xa = (float)a * dx - ar.lp[0];
+        for (int a = 0; a < 3; ++a) {
+          const float xa = (float)a * dx - ar.lp[0];
 #pragma unroll
-        for (int bb = 0; bb < 3; ++bb) {
-          const float xb = (float)bb * dx - ar.lp[1];
-          const float wab = ar.w[0][a] * ar.w[1][bb];
+          for (int bb = 0; bb < 3; ++bb) {
+            const float xb = (float)bb * dx - ar.lp[1];
+            const float wab = ar.w[0][a] * ar.w[1][bb];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const float xc = (float)c * dx - ar.lp[2];
-            const float W = wab * ar.w[2][c];
-            const float4 vi = vel[((lb0 + a) * 8 + (lb1 + bb)) * 8 + (lb2 + c)];
-            v[0] += vi.x * W;
-            v[1] += vi.y * W;
-            v[2] += vi.z * W;
-            const float w0 = W * vi.x * D_inv, w1 = W * vi.y * D_inv, w2 = W * vi.z * D_inv;
-            C[0] += w0 * xa;
-            C[1] += w1 * xa;
-            C[2] += w2 * xa;
-            C[3] += w0 * xb;
-            C[4] += w1 * xb;
-            C[5] += w2 * xb;
-            C[6] += w0 * xc;
-            C[7] += w1 * xc;
-            C[8] += w2 * xc;
+            for (int c = 0; c < 3; ++c) {
+              const float xc = (float)c * dx - ar.lp[2];
+              const float W = wab * ar.w[2][c];
+              const float4 vi = vel[((lb0 + a) * 8 + (lb1 + bb)) * 8 + (lb2 + c)];
+              v[0] += vi.x * W;
+              v[1] += vi.y * W;
+              v[2] += vi.z * W;
+              const float w0 = W * vi.x * D_inv, w1 = W * vi.y * D_inv, w2 = W * vi.z * D_inv;
+              C[0] += w0 * xa;
+              C[1] += w1 * xa;
+              C[2] += w2 * xa;
+              C[3] += w0 * xb;
+              C[4] += w1 * xb;
+              C[5] += w2 * xb;
+              C[6] += w0 * xc;
+              C[7] += w1 * xc;
+              C[8] += w2 * xc;
+            }
           }
         }
-      }
-      float F[9], Fn[9];
+        // F <- (I + dt C) F, column-major (G2P.hpp:75-78)
 #pragma unroll
-      for (int k = 0; k < 9; ++k) F[k] = ps.F[9 * p + k];
-      // F <- (I + dt C) F, column-major (G2P.hpp:75-78)
+        for (int j = 0; j < 3; ++j)
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
+          for (int r = 0; r < 3; ++r) {
+            const float a0 = C[r] * dt + (r == 0 ? 1.f : 0.f);
+            const float a1 = C[r + 3] * dt + (r == 1 ? 1.f : 0.f);
+            const float a2 = C[r + 6] * dt + (r == 2 ? 1.f : 0.f);
+            Fn[r + 3 * j] = a0 * F[3 * j] + a1 * F[3 * j + 1] + a2 * F[3 * j + 2];
+          }
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          const float a0 = C[r] * dt + (r == 0 ? 1.f : 0.f);
-          const float a1 = C[r + 3] * dt + (r == 1 ? 1.f : 0.f);
-          const float a2 = C[r + 6] * dt + (r == 2 ? 1.f : 0.f);
-          Fn[r + 3 * j] = a0 * F[3 * j] + a1 * F[3 * j + 1] + a2 * F[3 * j + 2];
+        for (int d = 0; d < 3; ++d) xn[d] = x[d] + v[d] * dt;
+      } else {
+        // a particle outside its bucket's arena keeps its state (the error is
+        // reported at the next sync)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          xn[d] = x[d];
+          v[d] = outside ? ps.v[3 * p + d] : 0.f;
         }
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        dst.x[3 * q + d] = x[d] + v[d] * dt;
-        dst.v[3 * q + d] = v[d];
+        for (int k = 0; k < 9; ++k) {
+          Fn[k] = F[k];
+          C[k] = outside ? ps.C[9 * p + k] : 0.f;
+        }
       }
+      if (kReorder) {
+        // destination slots off + i0 + lane are contiguous by construction
+        const int64_t q = (int64_t)off + i0 + lane;
+        if (live) {
+          out.m[q] = mcopy;
+          if (ps.logJp) out.logJp[q] = ljcopy;
+          if (out.id) out.id[q] = idcopy;
+        }
+        wave_store_rows<3>(out.x, (int64_t)off + i0, cw, stage, xn);
+        wave_store_rows<3>(out.v, (int64_t)off + i0, cw, stage, v);
+        wave_store_rows<9>(out.C, (int64_t)off + i0, cw, stage, C);
+        wave_store_rows<9>(out.F, (int64_t)off + i0, cw, stage, Fn);
+      } else if (contig && ballot(outside) == 0) {
+        wave_store_rows<3>(ps.x, p0, cw, stage, xn);
+        wave_store_rows<3>(ps.v, p0, cw, stage, v);
+        wave_store_rows<9>(ps.C, p0, cw, stage, C);
+        wave_store_rows<9>(ps.F, p0, cw, stage, Fn);
+      } else if (ok) {
 #pragma unroll
-      for (int k = 0; k < 9; ++k) {
-        dst.C[9 * q + k] = C[k];
-        dst.F[9 * q + k] = Fn[k];
+        for (int d = 0; d < 3; ++d) {
+          ps.x[3 * p + d] = xn[d];
+          ps.v[3 * p + d] = v[d];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          ps.C[9 * p + k] = C[k];
+          ps.F[9 * p + k] = Fn[k];
+        }
       }
     }
     __syncthreads();
@@ -682,8 +780,8 @@ hipError_t check_launch_limits(int device) {
   hipError_t e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
   if (e != hipSuccess) return e;
   const void* kernels[] = {
-      reinterpret_cast<const void*>(&p2g_kernel<0, true>), reinterpret_cast<const void*>(&p2g_kernel<0, false>),
-      reinterpret_cast<const void*>(&p2g_kernel<1, true>), reinterpret_cast<const void*>(&p2g_kernel<1, false>),
+      reinterpret_cast<const void*>(&stress_kernel<0>), reinterpret_cast<const void*>(&stress_kernel<1>),
+      reinterpret_cast<const void*>(&p2g_kernel<true>), reinterpret_cast<const void*>(&p2g_kernel<false>),
       reinterpret_cast<const void*>(&g2p_kernel<false>), reinterpret_cast<const void*>(&g2p_kernel<true>)};
   for (const void* k : kernels) {
     hipFuncAttributes attr;
@@ -700,20 +798,24 @@ hipError_t grid_clear(float* grid, const int32_t* nblocks_dev, int32_t cap_block
   return hipGetLastError();
 }
 
-hipError_t p2g(const Model& m, const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
-               int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s) {
+hipError_t stress(const Model& m, const ParticlesView& ps, Workspace& ws, hipStream_t s) {
+  if (ps.n <= 0) return hipSuccess;
+  const int64_t waves = (ps.n + kWave - 1) / kWave;
+  const int g = groups_for(waves * kWave, kThreads);
+  if (m.kind == 0)
+    stress_kernel<0><<<g, kThreads, 0, s>>>(m, ps, ws.contrib, ws.counters);
+  else
+    stress_kernel<1><<<g, kThreads, 0, s>>>(m, ps, ws.contrib, ws.counters);
+  return hipGetLastError();
+}
+
+hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
+                       int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s) {
   const int g = persistent_groups();
-  if (m.kind == 0) {
-    if (fold)
-      p2g_kernel<0, true><<<g, kThreads, 0, s>>>(m, ps, t, sv, grid, grid_cap, dt, dx, ws.err, ws.counters);
-    else
-      p2g_kernel<0, false><<<g, kThreads, 0, s>>>(m, ps, t, sv, grid, grid_cap, dt, dx, ws.err, ws.counters);
-  } else {
-    if (fold)
-      p2g_kernel<1, true><<<g, kThreads, 0, s>>>(m, ps, t, sv, grid, grid_cap, dt, dx, ws.err, ws.counters);
-    else
-      p2g_kernel<1, false><<<g, kThreads, 0, s>>>(m, ps, t, sv, grid, grid_cap, dt, dx, ws.err, ws.counters);
-  }
+  if (fold)
+    p2g_kernel<true><<<g, kThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err);
+  else
+    p2g_kernel<false><<<g, kThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err);
   return hipGetLastError();
 }
 
@@ -727,9 +829,11 @@ hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float 
 hipError_t g2p(const ParticlesView& ps, const ParticlesView* out, const TableView& t, const SortView& sv,
                const float* grid, int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s) {
   if (out)
-    g2p_kernel<true><<<persistent_groups(), kThreads, 0, s>>>(ps, *out, t, sv, grid, grid_cap, dt, dx, ws.err);
+    g2p_kernel<true><<<persistent_groups(), kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx,
+                                                                   ws.err);
   else
-    g2p_kernel<false><<<persistent_groups(), kThreads, 0, s>>>(ps, ps, t, sv, grid, grid_cap, dt, dx, ws.err);
+    g2p_kernel<false><<<persistent_groups(), kG2PThreads, 0, s>>>(ps, ps, t, sv, ws.nbr, grid, grid_cap, dt, dx,
+                                                                    ws.err);
   return hipGetLastError();
 }
 

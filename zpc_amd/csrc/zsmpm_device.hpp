@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "rowspan.hpp"
+
 namespace zsmpm {
 
 constexpr int kWave = 64;
@@ -96,6 +98,105 @@ __device__ __forceinline__ T shfl(T v, int src) {
   return __shfl(v, src, kWave);
 }
 
+// Value of the first active lane as a wave-uniform (scalar) value. Called
+// with the whole wave active, that is lane 0.
+__device__ __forceinline__ int32_t uniform(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uniform(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Wave-uniform, called with the whole wave active: do the valid lanes hold
+// one contiguous run p0 + lane (lane 0 valid)? p0 receives lane 0's index.
+__device__ __forceinline__ bool wave_contiguous(int64_t p, bool valid, int64_t& p0) {
+  p0 = uniform(p);
+  const uint64_t vm = ballot(valid);
+  return (vm & 1ull) != 0 && ballot(valid && p == p0 + lane_id()) == vm;
+}
+
+// Orders a wave's own LDS writes before its later LDS reads by other lanes
+// (the LDS serves one wave's instructions in issue order; this keeps the
+// compiler from moving accesses across the point).
+__device__ __forceinline__ void wave_lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// Split-phase load of rows q0 .. q0+cnt-1 (cnt <= 64) of W floats: issue()
+// puts the wave's global vector loads in flight (no LDS, no wait), finish()
+// transposes them through the wave's LDS stage so lane l < cnt receives row
+// q0 + l. Several RowLoads issued back to back cost one memory round trip.
+// Both calls are made by the whole wave with wave-uniform q0, cnt.
+template <int W>
+struct RowLoad {
+  static constexpr int kVec = (W * kWave / 4 + kWave - 1) / kWave;  // float4 per lane
+  RowSpan r;
+  float4 v[kVec];
+  float head, tail;
+  __device__ __forceinline__ void issue(const float* __restrict__ base, int64_t q0, int32_t cnt) {
+    const int lane = lane_id();
+    r = row_span((uint64_t)(uintptr_t)base, q0, cnt, W);
+    const int64_t hn = r.a4 - r.a;
+    const int64_t t0 = r.a4 + 4 * r.nvec;
+    head = lane < hn ? base[r.a + lane] : 0.f;
+    tail = lane < r.b - t0 ? base[t0 + lane] : 0.f;
+#pragma unroll
+    for (int k = 0; k < kVec; ++k) {
+      const int64_t j = lane + (int64_t)k * kWave;
+      v[k] = j < r.nvec ? *reinterpret_cast<const float4*>(base + r.a4 + 4 * j) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+  __device__ __forceinline__ void finish(float* stage, int32_t cnt, float (&out)[W]) {
+    const int lane = lane_id();
+    const int64_t hn = r.a4 - r.a;
+    const int64_t t0 = r.a4 + 4 * r.nvec;
+    if (lane < hn) stage[r.shift + lane] = head;
+#pragma unroll
+    for (int k = 0; k < kVec; ++k) {
+      const int64_t j = lane + (int64_t)k * kWave;
+      if (j < r.nvec) *reinterpret_cast<float4*>(stage + r.shift + hn + 4 * j) = v[k];
+    }
+    if (lane < r.b - t0) stage[r.shift + (t0 - r.a) + lane] = tail;
+    wave_lds_barrier();
+    if (lane < cnt) {
+#pragma unroll
+      for (int k = 0; k < W; ++k) out[k] = stage[r.shift + lane * W + k];
+    }
+    wave_lds_barrier();
+  }
+};
+
+// One-shot form: issue and finish.
+template <int W>
+__device__ __forceinline__ void wave_load_rows(const float* __restrict__ base, int64_t q0, int32_t cnt,
+                                               float* stage, float (&out)[W]) {
+  RowLoad<W> ld;
+  ld.issue(base, q0, cnt);
+  ld.finish(stage, cnt, out);
+}
+
+// The store counterpart: lane l < cnt provides row q0 + l.
+template <int W>
+__device__ __forceinline__ void wave_store_rows(float* __restrict__ base, int64_t q0, int32_t cnt, float* stage,
+                                                const float (&in)[W]) {
+  const int lane = lane_id();
+  const RowSpan r = row_span((uint64_t)(uintptr_t)base, q0, cnt, W);
+  if (lane < cnt) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) stage[r.shift + lane * W + k] = in[k];
+  }
+  wave_lds_barrier();
+  const int64_t head = r.a4 - r.a;
+  if (lane < head) base[r.a + lane] = stage[r.shift + lane];
+  for (int64_t j = lane; j < r.nvec; j += kWave)
+    *reinterpret_cast<float4*>(base + r.a4 + 4 * j) = *reinterpret_cast<const float4*>(stage + r.shift + head + 4 * j);
+  const int64_t t0 = r.a4 + 4 * r.nvec;
+  if (lane < r.b - t0) base[t0 + lane] = stage[r.shift + (t0 - r.a) + lane];
+  wave_lds_barrier();
+}
+
 // ---------------------------------------------------------------------------
 // 3x3 SVD: McAdams et al. 2011 ("Computing the SVD of 3x3 matrices with
 // minimal branching and elementary floating point operations"), the method
@@ -112,7 +213,10 @@ struct Svd3 {
   static constexpr float kTiny = 1.e-20f;
   static constexpr float kSmall = 1.e-12f;
 
-  __device__ static __forceinline__ float rsq(float v) { return 1.f / sqrtf(v); }
+  // hardware reciprocal square root (v_rsq_f32, about 1 ulp): zpc's SVD calls
+  // zs::rsqrt (math/matrix/SVD.hpp:113), which is ::rsqrtf on its CUDA backend
+  // and 1/sqrtf on the host (ZpcMathUtils.hpp:805-846)
+  __device__ static __forceinline__ float rsq(float v) { return __builtin_amdgcn_rsqf(v); }
 
   // approximate Jacobi conjugation for pivot (P, Q), third index R (quaternion axis)
   template <int P, int Q, int R>

@@ -90,6 +90,8 @@ struct Workspace {
   size_t scan_words = 0;
   uint32_t* counters = nullptr;  // [kCtrCount]
   uint32_t* err = nullptr;       // [1] error bits
+  float* contrib = nullptr;      // [9 n_cap] vol P F^T of each particle (column-major)
+  int32_t* nbr = nullptr;        // [8 ts_cap] block index of key_b + (d>>2&1, d>>1&1, d&1), or -1
 };
 
 // Next slot of zpc's probe sequence, (s + 127) % T (HashTable.hpp:362), for
@@ -112,8 +114,7 @@ __device__ __forceinline__ int32_t table_find(const TableView& t, int32_t k0, in
 }
 
 // ---------------------------------------------------------------- primitives
-uint32_t scan_tiles_for(uint32_t n_cap);
-uint32_t radix_tiles_for(uint32_t n_cap);
+size_t scan_scratch_words();
 size_t radix_sort_scratch_words(uint32_t n_cap);
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_cap,
                               uint32_t* tile_sums, uint32_t* total_out, hipStream_t s);
@@ -130,13 +131,17 @@ hipError_t table_enlarge(const TableView& t, Workspace& ws, const int32_t lo[3],
 hipError_t table_query(const TableView& t, const int32_t* keys, int64_t n, int32_t* out, hipStream_t s);
 hipError_t bucket_sort(const TableView& t, Workspace& ws, const float* x, int64_t n, float dx,
                        int blockLen, int offset, float disp, const SortView& out, hipStream_t s);
+// ws.nbr for every active block (the 2x2x2 blocks a bucket's stencils cover)
+hipError_t block_neighbours(const TableView& t, Workspace& ws, hipStream_t s);
 
 // ---------------------------------------------------------------- transfer
 // every transfer kernel's static LDS fits the device's per-workgroup limit
 hipError_t check_launch_limits(int device);
 hipError_t grid_clear(float* grid, const int32_t* nblocks_dev, int32_t cap_blocks, hipStream_t s);
-hipError_t p2g(const Model& m, const ParticlesView& ps, const TableView& t, const SortView& sv,
-               float* grid, int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s);
+// P2G = stress (per particle, writes ws.contrib and logJp) + scatter (per bucket)
+hipError_t stress(const Model& m, const ParticlesView& ps, Workspace& ws, hipStream_t s);
+hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
+                       int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s);
 hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float dt, const float extf[3],
                        const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s);
 // out == nullptr updates ps in place; otherwise the updated particles (and

@@ -75,7 +75,7 @@ class PlaneCollider(C.Structure):
 _lib = None
 
 # zsmpm_stage order
-STAGES = ("table_clear", "activate", "enlarge", "bucket_sort", "grid_clear", "p2g", "grid_update", "g2p")
+STAGES = ("table_clear", "activate", "enlarge", "bucket_sort", "grid_clear", "p2g", "grid_update", "g2p", "stress")
 
 
 def lib():
