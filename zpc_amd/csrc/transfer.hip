@@ -43,6 +43,10 @@ constexpr int kWaves = kThreads / kWave;
 constexpr int kMainCells = 64;    // base cells [4b+1, 4b+5)^3
 constexpr int kArena6 = 216;      // nodes [4b+1, 4b+7)^3
 constexpr int kArena8 = 512;      // nodes [4b, 4b+8)^3
+// G2P velocity stage: node (r0, r1, r2) at r0*73 + r1*9 + r2, so a float4
+// node's LDS bank group is (r0 + r1 + r2) mod 8 rather than r2 alone
+constexpr int kVelSY = 9, kVelSX = 73, kVelNodes = 7 * kVelSX + 7 * kVelSY + 8;
+__device__ __forceinline__ int vel_at(int r0, int r1, int r2) { return r0 * kVelSX + r1 * kVelSY + r2; }
 constexpr int kG2PThreads = 256;  // one workgroup per bucket
 
 inline int groups_for(int64_t n, int threads) {
@@ -92,6 +96,19 @@ __device__ __forceinline__ void load_neighbours(const int32_t* __restrict__ nbr_
     nbr[threadIdx.x] = r;
     if (r < 0) atomicOr(err, kErrMissingBlock);
   }
+}
+
+// Main base cell (i, j, k) in [0,4)^3 <-> lane of the scatter rounds. Lane
+// bits are (k0, k1, i0, j0, i1, j1): the 8 consecutive lanes one LDS b128
+// access serves together cover 4 values of k and 2 of i, 36 nodes apart in
+// the 6^3 arena (36 = 4 mod 8), so they fall in 8 distinct bank groups.
+__device__ __forceinline__ int cell_lane(int i, int j, int k) {
+  return k | ((i & 1) << 2) | ((j & 1) << 3) | ((i >> 1) << 4) | ((j >> 1) << 5);
+}
+__device__ __forceinline__ void lane_cell(int l, int& i, int& j, int& k) {
+  k = l & 3;
+  i = ((l >> 2) & 1) | (((l >> 4) & 1) << 1);
+  j = ((l >> 3) & 1) | (((l >> 5) & 1) << 1);
 }
 
 // quadratic B-spline weights from the local position l in [0.5, 1.5)
@@ -254,7 +271,7 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const f
         const int r0 = base[0] - o0, r1 = base[1] - o1, r2 = base[2] - o2;  // in [0, 6)
         int cell;
         if (r0 >= 1 && r0 <= 4 && r1 >= 1 && r1 <= 4 && r2 >= 1 && r2 <= 4) {
-          cell = ((r0 - 1) << 4) | ((r1 - 1) << 2) | (r2 - 1);
+          cell = cell_lane(r0 - 1, r1 - 1, r2 - 1);
         } else if (r0 >= 0 && r0 <= 5 && r1 >= 0 && r1 <= 5 && r2 >= 0 && r2 <= 5) {
           cell = kMainCells;  // overflow: scattered with global atomics
         } else {
@@ -337,7 +354,8 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const f
       {
         const int32_t lcount = ccount[lane];
         const int32_t lstart = cstart[lane];
-        const int lr0 = lane >> 4, lr1 = (lane >> 2) & 3, lr2 = lane & 3;  // base - (4b+1)
+        int lr0, lr1, lr2;  // base - (4b+1) of the lane's main cell
+        lane_cell(lane, lr0, lr1, lr2);
         float4* ar = arena[wave];
         const int32_t rounds = maxc;
         for (int32_t r = wave; r < rounds; r += kWaves) {
@@ -586,7 +604,7 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
                                                           SortView sv, const int32_t* __restrict__ nbr_tab,
                                                           const float* __restrict__ grid, int32_t grid_cap,
                                                           float dt, float dx, uint32_t* err) {
-  __shared__ float4 vel[kArena8];
+  __shared__ float4 vel[kVelNodes];
   __shared__ int32_t nbr[8];
   __shared__ float4 stage4[kG2PThreads / kWave][stage_floats(9) / 4];
   const int32_t nb = bounded_blocks(t, sv, grid_cap, err);
@@ -602,7 +620,7 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
     const int32_t o0 = t.active[3 * (int64_t)b] * 4, o1 = t.active[3 * (int64_t)b + 1] * 4,
                   o2 = t.active[3 * (int64_t)b + 2] * 4;
     __syncthreads();
-    // stage node velocities: slot s -> (neighbour block s>>6, cell s&63), stored at (r0*8 + r1)*8 + r2
+    // stage node velocities: slot s -> (neighbour block s>>6, cell s&63), stored at vel_at(r0, r1, r2)
     for (int s = threadIdx.x; s < kArena8; s += kG2PThreads) {
       const int nbi = s >> 6, cell = s & 63;
       const int r0 = ((nbi >> 2) & 1) * 4 + ((cell >> 4) & 3);
@@ -614,7 +632,7 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
         const float* gp = grid + grid_at(gb, 1, cell);
         v = make_float4(gp[0], gp[kBlockCells], gp[2 * kBlockCells], 0.f);
       }
-      vel[(r0 * 8 + r1) * 8 + r2] = v;
+      vel[vel_at(r0, r1, r2)] = v;
     }
     __syncthreads();
     // each wave takes 64 consecutive bucket slots at a time (wave-uniform loop)
@@ -670,7 +688,7 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
             for (int c = 0; c < 3; ++c) {
               const float xc = (float)c * dx - ar.lp[2];
               const float W = wab * ar.w[2][c];
-              const float4 vi = vel[((lb0 + a) * 8 + (lb1 + bb)) * 8 + (lb2 + c)];
+              const float4 vi = vel[vel_at(lb0 + a, lb1 + bb, lb2 + c)];
               v[0] += vi.x * W;
               v[1] += vi.y * W;
               v[2] += vi.z * W;
