@@ -38,12 +38,14 @@ def stage_bytes(stage: str, n: int, blocks: int, plastic: bool, table_size: int)
     nodes = 64 * blocks
     lj = 4 if plastic else 0
     if stage == "p2g":
-        # read x, v, C, F, m (+ logJp), the bucket permutation; write logJp;
-        # flush m and the force-folded momentum (4 channels) once per node
-        return n * (12 + 12 + 36 + 36 + 4 + lj + 4 + lj) + nodes * 16
+        # read x, v, C, vol P F^T, m and the bucket permutation; flush m and the
+        # force-folded momentum (4 channels) once per node
+        return n * (12 + 12 + 36 + 36 + 4 + 4) + nodes * 16
     if stage == "g2p":
         # read x, F and the permutation; write x, v, C, F; gather v once per node
         return n * (12 + 36 + 4 + 12 + 12 + 36 + 36) + nodes * 12
+    if stage == "stress":
+        return n * (36 + lj + 36 + lj)  # read F (+ logJp), write vol P F^T (+ logJp)
     if stage == "grid_update":
         return nodes * (16 + 12)  # read m + momentum, write velocity
     if stage == "grid_clear":
@@ -120,6 +122,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--in-place", action="store_true",
                     help="keep the particles in their original order (no reorder after G2P)")
+    ap.add_argument("--single", action="store_true",
+                    help="one zsmpm_substep_sorted__rocm call per substep instead of one multi-substep call")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (the reference kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -159,8 +163,12 @@ def main() -> int:
         del src, dst
 
     sim = rocm.Simulator(sc, reorder=not args.in_place)
-    for _ in range(args.warmup):
-        sim.substep()
+    multi = not (args.in_place or args.single)
+    if multi:
+        sim.substeps(args.warmup)
+    else:
+        for _ in range(args.warmup):
+            sim.substep()
     sim.sync()
     b0 = sim.nblocks()
     print(f"# warm-up done ({args.warmup} substeps), {b0} active blocks", file=sys.stderr, flush=True)
@@ -169,8 +177,11 @@ def main() -> int:
     sim.pol.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sim.substep()
+    if multi:
+        sim.substeps(args.steps)
+    else:
+        for _ in range(args.steps):
+            sim.substep()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     sim.sync()
@@ -214,6 +225,9 @@ def main() -> int:
             "path": "plain",
             "particle_order": "original (in place)" if args.in_place else
                               "bucket order, reordered inside G2P every substep",
+            "driver": "zsmpm_substeps_sorted__rocm (one call for all timed substeps)" if multi else
+                      ("zsmpm_substep__rocm per substep" if args.in_place else
+                       "zsmpm_substep_sorted__rocm per substep"),
         },
         "roofline": {
             "bound": "hbm",

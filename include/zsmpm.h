@@ -221,6 +221,21 @@ int32_t zsmpm_substep_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, c
                                    const float extf[3], const zsmpm_plane_collider* colliders,
                                    int32_t ncolliders);
 
+/* zsmpm_substeps_sorted__rocm flags */
+#define ZSMPM_SUBSTEPS_FUSE_STRESS 1 /* G2P computes the next substep's stress (see below) */
+
+/* nsteps substeps of zsmpm_substep_sorted__rocm, ping-ponging a -> b -> a ...:
+ * the result is in b when nsteps is odd (*final_in_b = nsteps & 1); the
+ * particle state after every substep is what the single-substep calls
+ * produce. With ZSMPM_SUBSTEPS_FUSE_STRESS, G2P also computes the next
+ * substep's stress from the F it stores (P2G.hpp:57-100), so the separate
+ * per-particle constitutive pass runs once per call. a and b must not overlap. */
+int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* a,
+                                    const zsmpm_particles* b, const zsmpm_table* table,
+                                    const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt,
+                                    const float extf[3], const zsmpm_plane_collider* colliders, int32_t ncolliders,
+                                    int32_t nsteps, int32_t flags, int32_t* final_in_b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -196,6 +196,8 @@ int32_t reserve(zsmpm_policy_s* p, int64_t n, int32_t T) {
   ZA(w.err, 4);
   ZA(w.contrib, (size_t)9 * w.n_cap);
   ZA(w.nbr, (size_t)8 * w.ts_cap);
+  ZA(w.lj_next[0], w.n_cap);
+  ZA(w.lj_next[1], w.n_cap);
   p->own_cap_blocks = (int32_t)w.ts_cap;
   ZA(p->own_counts, (size_t)p->own_cap_blocks + 1);
   ZA(p->own_offsets, (size_t)p->own_cap_blocks + 1);
@@ -344,12 +346,21 @@ int32_t run_buckets(zsmpm_policy_s* p, const TableView& t, const ParticlesView& 
   return from_hip(bucket_sort(t, p->ws, ps.x, ps.n, dx, 4, -2, 0.5f, sv, p->stream));
 }
 
+// Multi-substep chaining of the sorted driver: the stress of substep k+1 is
+// computed by G2P of substep k from the updated F.
+struct Chain {
+  bool cached_stress = false;     // ws.contrib already holds this substep's stress
+  const float* lj_src = nullptr;  // this substep's logJp (the previous G2P's cache)
+  bool compute_next = false;      // G2P computes the next substep's stress
+  float* lj_next = nullptr;       // where G2P puts the next substep's logJp
+};
+
 // The explicit driver's substep (stage order of the reconstructed zpc
 // driver); with `particles_out`, G2P also performs the Vector::reorder gather.
 int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_particles* particles,
                     const zsmpm_particles* particles_out, const zsmpm_table* table, const zsmpm_buckets* buckets,
                     const zsmpm_grid* grid, float dt, const float extf[3], const zsmpm_plane_collider* colliders,
-                    int32_t ncolliders) {
+                    int32_t ncolliders, const Chain* chain = nullptr) {
   if (!pol || !extf) return ZSMPM_INVALID_ARGUMENT;
   Model m;
   ParticlesView ps, po;
@@ -398,7 +409,7 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
     StageScope sc(pol, ZSMPM_STAGE_GRID_CLEAR);
     ZS_HIP(zsmpm::grid_clear(grid->data, t.cnt, grid->capacity_blocks, s));
   }
-  {
+  if (!chain || !chain->cached_stress) {
     StageScope sc(pol, ZSMPM_STAGE_STRESS);
     ZS_HIP(zsmpm::stress(m, ps, ws, s));
   }
@@ -415,7 +426,17 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_G2P);
-    ZS_HIP(zsmpm::g2p(ps, particles_out ? &po : nullptr, t, sv, grid->data, grid->capacity_blocks, dt, dx, ws, s));
+    NextStress ns{};
+    ns.m = m;
+    if (chain) {
+      ns.lj_src = chain->lj_src;
+      if (chain->compute_next) {
+        ns.contrib = ws.contrib;
+        ns.lj_next = chain->lj_next;
+      }
+    }
+    ZS_HIP(zsmpm::g2p(ps, particles_out ? &po : nullptr, t, sv, grid->data, grid->capacity_blocks, dt, dx, ws, s,
+                      &ns));
   }
   return ZSMPM_OK;
 }
@@ -664,6 +685,39 @@ int32_t zsmpm_substep_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, c
                                    int32_t ncolliders) {
   if (!out) return ZSMPM_INVALID_ARGUMENT;
   return run_substep(pol, model, in, out, table, buckets, grid, dt, extf, colliders, ncolliders);
+}
+
+int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* a,
+                                    const zsmpm_particles* b, const zsmpm_table* table,
+                                    const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt,
+                                    const float extf[3], const zsmpm_plane_collider* colliders, int32_t ncolliders,
+                                    int32_t nsteps, int32_t flags, int32_t* final_in_b) {
+  if (!pol || !a || !b || nsteps < 0 || (flags & ~ZSMPM_SUBSTEPS_FUSE_STRESS)) return ZSMPM_INVALID_ARGUMENT;
+  const bool fuse = (flags & ZSMPM_SUBSTEPS_FUSE_STRESS) != 0;
+  if (final_in_b) *final_in_b = nsteps & 1;
+  if (nsteps == 0) return ZSMPM_OK;
+  {
+    // validate once; run_substep repeats the cheap checks per substep
+    ParticlesView pa, pb;
+    ZS_TRY(view_particles(a, pa, true));
+    ZS_TRY(view_particles(b, pb, true));
+    ZS_TRY(check_out_particles(pa, pb));
+    DeviceGuard g(pol->devid);
+    ZS_TRY(ensure(pol, pa.n, table ? table->table_size : 0));
+  }
+  for (int32_t k = 0; k < nsteps; ++k) {
+    Chain c;
+    if (fuse) {
+      c.cached_stress = k > 0;
+      c.lj_src = k > 0 ? pol->ws.lj_next[(k - 1) & 1] : nullptr;
+      c.compute_next = k + 1 < nsteps;
+      c.lj_next = pol->ws.lj_next[k & 1];
+    }
+    const zsmpm_particles* in = (k & 1) ? b : a;
+    const zsmpm_particles* out = (k & 1) ? a : b;
+    ZS_TRY(run_substep(pol, model, in, out, table, buckets, grid, dt, extf, colliders, ncolliders, &c));
+  }
+  return ZSMPM_OK;
 }
 
 int32_t zsmpm_reorder__rocm(zsmpm_policy pol, const zsmpm_buckets* buckets, const zsmpm_particles* in,

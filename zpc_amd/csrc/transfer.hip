@@ -358,7 +358,67 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const f
         lane_cell(lane, lr0, lr1, lr2);
         float4* ar = arena[wave];
         const int32_t rounds = maxc;
-        for (int32_t r = wave; r < rounds; r += kWaves) {
+        if constexpr (kFold) {
+          // All particles of one base cell touch the same 27 nodes: the lane
+          // sums its share of its cell's particles (every kWaves-th, starting
+          // at the wave index) in registers and adds the node sums to the
+          // wave's arena once per cell, instead of 27 LDS read-modify-writes
+          // per particle. One pass per x-slab of the stencil keeps the
+          // accumulators at 9 nodes x 4 channels.
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            float acc[9][4];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.f;
+            for (int32_t r = wave; r < lcount; r += kWaves) {
+              const int32_t i = list[lstart + r];
+              const float4* q = rec + i * kQ;
+              const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+              float w0[3], w1[3], w2[3];
+              bspline_w(q1.x, w0);
+              bspline_w(q1.y, w1);
+              bspline_w(q1.z, w2);
+              const float fa = (float)a;
+#pragma unroll
+              for (int bb = 0; bb < 3; ++bb) {
+                const float w01 = w0[a] * w1[bb];
+                const float fb = (float)bb;
+                const float s0 = q0.y + q2.x * fa + q2.w * fb;
+                const float s1 = q0.z + q2.y * fa + q3.x * fb;
+                const float s2 = q0.w + q2.z * fa + q3.y * fb;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                  const float W = w01 * w2[c];
+                  const float fc = (float)c;
+                  float* e = acc[bb * 3 + c];
+                  e[0] += q0.x * W;
+                  e[1] += W * (s0 + q3.z * fc);
+                  e[2] += W * (s1 + q3.w * fc);
+                  e[3] += W * (s2 + q1.w * fc);
+                }
+              }
+            }
+            if (wave < lcount) {
+#pragma unroll
+              for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                  const int n = (lr0 + a) * 36 + (lr1 + bb) * 6 + (lr2 + c);
+                  const float* e = acc[bb * 3 + c];
+                  // lanes of one step add to distinct nodes; steps stay in
+                  // issue order (see the unfolded path below)
+                  asm volatile("" ::: "memory");
+                  float4 v = ar[n];
+                  v.x += e[0];
+                  v.y += e[1];
+                  v.z += e[2];
+                  v.w += e[3];
+                  ar[n] = v;
+                }
+            }
+          }
+        }
+        for (int32_t r = wave; !kFold && r < rounds; r += kWaves) {
           if (r < lcount) {
             const int32_t i = list[lstart + r];
             const float4* q = rec + i * kQ;
@@ -599,11 +659,16 @@ __global__ __launch_bounds__(kThreads) void grid_update_kernel(float* __restrict
   }
 }
 
-template <bool kReorder>
+// kNext: -1 plain G2P; 0 / 1 (reorder only): also the next substep's
+// fixed-corotated / Drucker-Prager stress from the updated F (NextStress)
+template <bool kReorder, int kNext>
 __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, ParticlesView out, TableView t,
                                                           SortView sv, const int32_t* __restrict__ nbr_tab,
                                                           const float* __restrict__ grid, int32_t grid_cap,
-                                                          float dt, float dx, uint32_t* err) {
+                                                          float dt, float dx, NextStress ns, uint32_t* err,
+                                                          uint32_t* counters) {
+  static_assert(kNext < 0 || kReorder, "the fused stress writes in output slot order");
+  uint32_t fp64_used = 0;
   __shared__ float4 vel[kVelNodes];
   __shared__ int32_t nbr[8];
   __shared__ float4 stage4[kG2PThreads / kWave][stage_floats(9) / 4];
@@ -651,7 +716,7 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
       int32_t idcopy = 0;
       if (kReorder && live) {
         mcopy = ps.m[p];
-        if (ps.logJp) ljcopy = ps.logJp[p];
+        if (ps.logJp) ljcopy = ns.lj_src ? ns.lj_src[p] : ps.logJp[p];
         idcopy = ps.id ? ps.id[p] : (int32_t)p;
       }
       if (contig) {
@@ -743,6 +808,23 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
         wave_store_rows<3>(out.v, (int64_t)off + i0, cw, stage, v);
         wave_store_rows<9>(out.C, (int64_t)off + i0, cw, stage, C);
         wave_store_rows<9>(out.F, (int64_t)off + i0, cw, stage, Fn);
+        if constexpr (kNext >= 0) {
+          // the next substep's P2G stress from F_new and the logJp this
+          // substep leaves (P2G.hpp:57-100 on the stored F)
+          float PF[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+          if (live) {
+            bool f64 = false;
+            if (kNext == 0) {
+              stress_fixedcorotated(ns.m, Fn, PF, f64);
+            } else {
+              float lj = ljcopy;
+              stress_sand(ns.m, Fn, lj, PF, nullptr, f64);
+              ns.lj_next[q] = lj;
+            }
+            fp64_used += f64 ? 1u : 0u;
+          }
+          wave_store_rows<9>(ns.contrib, (int64_t)off + i0, cw, stage, PF);
+        }
       } else if (contig && ballot(outside) == 0) {
         wave_store_rows<3>(ps.x, p0, cw, stage, xn);
         wave_store_rows<3>(ps.v, p0, cw, stage, v);
@@ -763,6 +845,7 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
     }
     __syncthreads();
   }
+  if (kNext >= 0 && fp64_used) atomicAdd(&counters[kCtrFp64], fp64_used);
 }
 
 // Vector::reorder gather (container/Vector.hpp:617-646) of every attribute
@@ -800,7 +883,8 @@ hipError_t check_launch_limits(int device) {
   const void* kernels[] = {
       reinterpret_cast<const void*>(&stress_kernel<0>), reinterpret_cast<const void*>(&stress_kernel<1>),
       reinterpret_cast<const void*>(&p2g_kernel<true>), reinterpret_cast<const void*>(&p2g_kernel<false>),
-      reinterpret_cast<const void*>(&g2p_kernel<false>), reinterpret_cast<const void*>(&g2p_kernel<true>)};
+      reinterpret_cast<const void*>(&g2p_kernel<false, -1>), reinterpret_cast<const void*>(&g2p_kernel<true, -1>),
+      reinterpret_cast<const void*>(&g2p_kernel<true, 0>), reinterpret_cast<const void*>(&g2p_kernel<true, 1>)};
   for (const void* k : kernels) {
     hipFuncAttributes attr;
     e = hipFuncGetAttributes(&attr, k);
@@ -845,13 +929,26 @@ hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float 
 }
 
 hipError_t g2p(const ParticlesView& ps, const ParticlesView* out, const TableView& t, const SortView& sv,
-               const float* grid, int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s) {
-  if (out)
-    g2p_kernel<true><<<persistent_groups(), kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx,
-                                                                   ws.err);
-  else
-    g2p_kernel<false><<<persistent_groups(), kG2PThreads, 0, s>>>(ps, ps, t, sv, ws.nbr, grid, grid_cap, dt, dx,
-                                                                    ws.err);
+               const float* grid, int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s,
+               const NextStress* next) {
+  const int g = persistent_groups();
+  NextStress ns{};
+  if (next) ns = *next;
+  if (next && next->contrib) {
+    if (!out) return hipErrorInvalidValue;
+    if (ns.m.kind == 0)
+      g2p_kernel<true, 0><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
+                                                    ws.counters);
+    else
+      g2p_kernel<true, 1><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
+                                                    ws.counters);
+  } else if (out) {
+    g2p_kernel<true, -1><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
+                                                   ws.counters);
+  } else {
+    g2p_kernel<false, -1><<<g, kG2PThreads, 0, s>>>(ps, ps, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
+                                                    ws.counters);
+  }
   return hipGetLastError();
 }
 

@@ -92,6 +92,17 @@ struct Workspace {
   uint32_t* err = nullptr;       // [1] error bits
   float* contrib = nullptr;      // [9 n_cap] vol P F^T of each particle (column-major)
   int32_t* nbr = nullptr;        // [8 ts_cap] block index of key_b + (d>>2&1, d>>1&1, d&1), or -1
+  float* lj_next[2] = {nullptr, nullptr};  // [n_cap] each: logJp caches of the fused multi-substep driver
+};
+
+// The next substep's constitutive part computed inside G2P from the updated
+// F (the sorted multi-substep driver): vol P F^T and the return-mapped logJp
+// of every particle, written in output slot order.
+struct NextStress {
+  Model m;
+  const float* lj_src = nullptr;  // logJp carried to the output (the previous G2P's cache), or null: ps.logJp
+  float* contrib = nullptr;       // [9 n] next substep's vol P F^T; null: not computed (last substep)
+  float* lj_next = nullptr;       // [n] next substep's logJp after the return mapping
 };
 
 // Next slot of zpc's probe sequence, (s + 127) % T (HashTable.hpp:362), for
@@ -147,7 +158,8 @@ hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float 
 // out == nullptr updates ps in place; otherwise the updated particles (and
 // m, logJp, id) are written to out in bucket order (slot = sorted position)
 hipError_t g2p(const ParticlesView& ps, const ParticlesView* out, const TableView& t, const SortView& sv,
-               const float* grid, int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s);
+               const float* grid, int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s,
+               const NextStress* next = nullptr);
 // out.attr[i] = in.attr[map[i]] for every attribute; out.id[i] = in.id ? in.id[map[i]] : map[i]
 hipError_t reorder_particles(const ParticlesView& in, const ParticlesView& out, const int32_t* map,
                              Workspace& ws, hipStream_t s);

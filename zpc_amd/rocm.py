@@ -113,6 +113,9 @@ def lib():
         "zsmpm_substep_sorted__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(ParticlesDesc), P(TableDesc),
                                        P(BucketsDesc), P(GridDesc), f32, P(C.c_float * 3), P(PlaneCollider), i32],
         "zsmpm_reorder__rocm": [vp, P(BucketsDesc), P(ParticlesDesc), P(ParticlesDesc)],
+        "zsmpm_substeps_sorted__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(ParticlesDesc), P(TableDesc),
+                                        P(BucketsDesc), P(GridDesc), f32, P(C.c_float * 3), P(PlaneCollider), i32,
+                                        i32, i32, P(C.c_int32)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -353,6 +356,24 @@ class Simulator:
                                          C.byref(self.table.desc), C.byref(self.buckets.desc),
                                          C.byref(self.grid.desc), dt, C.byref(self.extf), self.cols, self.ncols),
               "substep")
+
+    def substeps(self, n: int, dt: Optional[float] = None, fuse_stress: bool = False):
+        """n substeps in one call (sorted driver); fuse_stress: G2P computes the
+        next substep's stress."""
+        if not self.reorder:
+            for _ in range(n):
+                self.substep(dt)
+            return
+        dt = float(self.sc.dt if dt is None else dt)
+        in_b = C.c_int32(0)
+        check(self.L.zsmpm_substeps_sorted__rocm(self.pol.h, C.byref(self.model), C.byref(self.parts.desc),
+                                                 C.byref(self.spare.desc), C.byref(self.table.desc),
+                                                 C.byref(self.buckets.desc), C.byref(self.grid.desc), dt,
+                                                 C.byref(self.extf), self.cols, self.ncols, int(n),
+                                                 1 if fuse_stress else 0, C.byref(in_b)),
+              "substeps_sorted")
+        if in_b.value:
+            self.parts, self.spare = self.spare, self.parts
 
     # stage-level operators (zpc functor equivalents)
     def clear_table(self):
