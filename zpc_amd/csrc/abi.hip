@@ -56,12 +56,15 @@ int32_t from_hip(hipError_t e) {
   return ZSMPM_DEVICE_ERROR;
 }
 
+// Several bits can be raised by one cause (a bucket beyond the grid capacity
+// also leaves its particles outside the arena they land in), so causes are
+// reported before their consequences.
 int32_t from_err_bits(uint32_t b) {
   if (b & kErrTableFull) return ZSMPM_TABLE_FULL;
   if (b & kErrKeyRange) return ZSMPM_KEY_RANGE;
-  if (b & kErrArena) return ZSMPM_STENCIL_OUTSIDE;
-  if (b & kErrMissingBlock) return ZSMPM_MISSING_BLOCK;
   if (b & kErrCapacity) return ZSMPM_CAPACITY;
+  if (b & kErrMissingBlock) return ZSMPM_MISSING_BLOCK;
+  if (b & kErrArena) return ZSMPM_STENCIL_OUTSIDE;
   return ZSMPM_OK;
 }
 
