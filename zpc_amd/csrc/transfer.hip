@@ -48,6 +48,8 @@ constexpr int kArena8 = 512;      // nodes [4b, 4b+8)^3
 constexpr int kVelSY = 9, kVelSX = 73, kVelNodes = 7 * kVelSX + 7 * kVelSY + 8;
 __device__ __forceinline__ int vel_at(int r0, int r1, int r2) { return r0 * kVelSX + r1 * kVelSY + r2; }
 constexpr int kG2PThreads = 256;  // one workgroup per bucket
+constexpr int kP2GThreads = 256;  // one workgroup per bucket
+constexpr int kP2GWaves = kP2GThreads / kWave;
 
 inline int groups_for(int64_t n, int threads) {
   int64_t g = (n + threads - 1) / threads;
@@ -86,6 +88,19 @@ __device__ __forceinline__ int32_t bounded_blocks(const TableView& t, const Sort
   }
   return nb;
 }
+
+// Buckets are handed out in order from a queue (one atomic per bucket), so
+// a workgroup that drew small buckets takes more: bucket sizes range from 0
+// (enlarged-only blocks) to several hundred particles. Workgroup-uniform;
+// every workgroup leaves once the queue passes the bucket count.
+__device__ __forceinline__ int32_t next_bucket(uint32_t* queue, int32_t* slot) {
+  __syncthreads();  // every thread has read the previous bucket number
+  if (threadIdx.x == 0) *slot = (int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu);
+  __syncthreads();
+  return *slot;
+}
+
+__global__ void reset_queue_kernel(uint32_t* counters) { counters[kCtrQueue] = 0u; }
 
 // neighbour blocks key + (d>>2&1, d>>1&1, d&1) of bucket b, from the
 // per-substep neighbour table
@@ -168,14 +183,19 @@ __global__ __launch_bounds__(kThreads) void stress_kernel(Model m, ParticlesView
 }
 
 template <bool kFold>
-__global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const float* __restrict__ contrib,
+__global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, const float* __restrict__ contrib,
                                                        TableView t, SortView sv, const int32_t* __restrict__ nbr_tab,
                                                        float* __restrict__ grid, int32_t grid_cap, float dt,
-                                                       float dx, uint32_t* err) {
+                                                       float dx, uint32_t* err, uint32_t* queue) {
   constexpr int kQ = Rec<kFold>::kQuads;
   constexpr int kChunk = Rec<kFold>::kChunk;
+  static_assert(kP2GThreads % kWave == 0 && kChunk % kP2GThreads == 0,
+                "whole waves; a chunk is a whole number of workgroup passes");
   constexpr int kNV = kFold ? 1 : 2;  // float4 per arena node
-  __shared__ float4 arena[kWaves][kArena6 * kNV];
+  __shared__ float4 arena[kP2GWaves][kArena6 * kNV];
+  // records in component-major order: component k of record i at
+  // rec[k * kChunk + i], so consecutive records are consecutive float4 (no
+  // LDS bank conflicts when lanes write consecutive records)
   __shared__ float4 rec[kChunk * kQ];
   __shared__ uint16_t list[kChunk];
   __shared__ uint8_t cellof[kChunk];
@@ -188,29 +208,28 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const f
   const float D_inv = 4.f * dx_inv * dx_inv;
   const float scale = -dt * D_inv;
   const int wave = uniform((int32_t)(threadIdx.x / kWave)), lane = lane_id();
+  __shared__ int32_t bslot;
 
-  for (int32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+  for (int32_t b = next_bucket(queue, &bslot); b < nb; b = next_bucket(queue, &bslot)) {
     const int32_t cnt = sv.counts[b];
     const int32_t off = sv.offsets[b];
     if (cnt <= 0 || off < 0 || (int64_t)off + cnt > ps.n) continue;
-    for (int i = threadIdx.x; i < kWaves * kArena6 * kNV; i += kThreads)
-      (&arena[0][0])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     load_neighbours(nbr_tab, b, nbr, err);
     const int32_t o0 = t.active[3 * (int64_t)b] * 4, o1 = t.active[3 * (int64_t)b + 1] * 4,
                   o2 = t.active[3 * (int64_t)b + 2] * 4;
 
     for (int32_t c0 = 0; c0 < cnt; c0 += kChunk) {
       const int32_t nch = min(kChunk, cnt - c0);
-      if (threadIdx.x <= kMainCells) {
-        ccount[threadIdx.x] = 0;
-        cfill[threadIdx.x] = 0;
+      for (int c = threadIdx.x; c <= kMainCells; c += kP2GThreads) {  // 64 main cells + overflow
+        ccount[c] = 0;
+        cfill[c] = 0;
       }
       __syncthreads();
       // ---- A: stress + affine record per particle. Each wave takes 64
-      // consecutive chunk slots (i = threadIdx.x + k * kThreads, as phase B
+      // consecutive chunk slots (i = threadIdx.x + k * kP2GThreads, as phase B
       // expects); a contiguous run of particles is read with 16-byte vectors,
       // staged in the wave's own (not yet written) record slots.
-      for (int32_t i0 = wave * kWave; i0 < nch; i0 += kThreads) {
+      for (int32_t i0 = wave * kWave; i0 < nch; i0 += kP2GThreads) {
         const int32_t i = i0 + lane;
         const int32_t cw = min(kWave, nch - i0);
         const bool valid = lane < cw;
@@ -223,8 +242,10 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const f
         }
         const bool live = valid && !bad;
         int64_t p0;
-        const bool contig = wave_contiguous(p, live, p0) && ballot(bad) == 0;
-        float* stage = reinterpret_cast<float*>(rec + (int64_t)i0 * kQ);
+        // the wave's arena is free during the first chunk's phase A (it is
+        // cleared right after): it stages the vector row loads
+        const bool contig = c0 == 0 && wave_contiguous(p, live, p0) && ballot(bad) == 0;
+        float* stage = reinterpret_cast<float*>(arena[wave]);
         float xv[3] = {0.f, 0.f, 0.f}, vv[3] = {0.f, 0.f, 0.f};
         float PF[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         float C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -297,19 +318,24 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const f
           Bv[d] = mv[d] - (A[d] * l[0] + A[d + 3] * l[1] + A[d + 6] * l[2]);
           if (!kFold) Bf[d] = -(Af[d] * l[0] + Af[d + 3] * l[1] + Af[d + 6] * l[2]);
         }
-        float4* q = rec + i * kQ;
-        q[0] = make_float4(mass, Bv[0], Bv[1], Bv[2]);
-        q[1] = make_float4(l[0], l[1], l[2], A[8]);
-        q[2] = make_float4(A[0], A[1], A[2], A[3]);
-        q[3] = make_float4(A[4], A[5], A[6], A[7]);
+        rec[0 * kChunk + i] = make_float4(mass, Bv[0], Bv[1], Bv[2]);
+        rec[1 * kChunk + i] = make_float4(l[0], l[1], l[2], A[8]);
+        rec[2 * kChunk + i] = make_float4(A[0], A[1], A[2], A[3]);
+        rec[3 * kChunk + i] = make_float4(A[4], A[5], A[6], A[7]);
         if (!kFold) {
-          q[4 % kQ] = make_float4(Bf[0], Bf[1], Bf[2], Af[8]);
-          q[5 % kQ] = make_float4(Af[0], Af[1], Af[2], Af[3]);
-          q[6 % kQ] = make_float4(Af[4], Af[5], Af[6], Af[7]);
+          rec[(4 % kQ) * kChunk + i] = make_float4(Bf[0], Bf[1], Bf[2], Af[8]);
+          rec[(5 % kQ) * kChunk + i] = make_float4(Af[0], Af[1], Af[2], Af[3]);
+          rec[(6 % kQ) * kChunk + i] = make_float4(Af[4], Af[5], Af[6], Af[7]);
         }
         cellof[i] = (uint8_t)cell;
         if (cell <= kMainCells) atomicAdd(&ccount[cell], 1);
         list[i] = (uint16_t)((r0 << 6) | (r1 << 3) | r2);  // base offsets; the list is rebuilt in B
+      }
+      if (c0 == 0) {
+        // the first chunk's loads are done with the wave's arena as stage:
+        // clear it for the bucket's accumulation
+        wave_lds_barrier();
+        for (int k = lane; k < kArena6 * kNV; k += kWave) arena[wave][k] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
       __syncthreads();
       // ---- B: counting sort of the chunk by base cell
@@ -332,18 +358,18 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const f
       }
       // each thread keeps the base offsets and cells of its own particles
       // across the rebuild of list[]
-      uint16_t keep[kChunk / kThreads];
-      uint8_t keepc[kChunk / kThreads];
+      uint16_t keep[kChunk / kP2GThreads];
+      uint8_t keepc[kChunk / kP2GThreads];
 #pragma unroll
-      for (int k = 0; k < kChunk / kThreads; ++k) {
-        const int32_t i = threadIdx.x + k * kThreads;
+      for (int k = 0; k < kChunk / kP2GThreads; ++k) {
+        const int32_t i = threadIdx.x + k * kP2GThreads;
         keep[k] = i < nch ? list[i] : (uint16_t)0;
         keepc[k] = i < nch ? cellof[i] : (uint8_t)255;
       }
       __syncthreads();
 #pragma unroll
-      for (int k = 0; k < kChunk / kThreads; ++k) {
-        const int32_t i = threadIdx.x + k * kThreads;
+      for (int k = 0; k < kChunk / kP2GThreads; ++k) {
+        const int32_t i = threadIdx.x + k * kP2GThreads;
         if (i < nch && keepc[k] <= kMainCells) {
           const int32_t slot = atomicAdd(&cfill[keepc[k]], 1);
           list[cstart[keepc[k]] + slot] = (uint16_t)i;
@@ -360,7 +386,7 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const f
         const int32_t rounds = maxc;
         if constexpr (kFold) {
           // All particles of one base cell touch the same 27 nodes: the lane
-          // sums its share of its cell's particles (every kWaves-th, starting
+          // sums its share of its cell's particles (every kP2GWaves-th, starting
           // at the wave index) in registers and adds the node sums to the
           // wave's arena once per cell, instead of 27 LDS read-modify-writes
           // per particle. One pass per x-slab of the stencil keeps the
@@ -370,10 +396,9 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const f
             float acc[9][4];
 #pragma unroll
             for (int k = 0; k < 9; ++k) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.f;
-            for (int32_t r = wave; r < lcount; r += kWaves) {
+            for (int32_t r = wave; r < lcount; r += kP2GWaves) {
               const int32_t i = list[lstart + r];
-              const float4* q = rec + i * kQ;
-              const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+              const float4 q0 = rec[i], q1 = rec[kChunk + i], q2 = rec[2 * kChunk + i], q3 = rec[3 * kChunk + i];
               float w0[3], w1[3], w2[3];
               bspline_w(q1.x, w0);
               bspline_w(q1.y, w1);
@@ -418,15 +443,15 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const f
             }
           }
         }
-        for (int32_t r = wave; !kFold && r < rounds; r += kWaves) {
+        for (int32_t r = wave; !kFold && r < rounds; r += kP2GWaves) {
           if (r < lcount) {
             const int32_t i = list[lstart + r];
-            const float4* q = rec + i * kQ;
-            const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+            const float4 q0 = rec[i], q1 = rec[kChunk + i], q2 = rec[2 * kChunk + i], q3 = rec[3 * kChunk + i];
             const float Am[9] = {q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w, q1.w};
             float Af[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, Bf[3] = {0.f, 0.f, 0.f};
             if (!kFold) {
-              const float4 qf0 = q[4 % kQ], qf1 = q[5 % kQ], qf2 = q[6 % kQ];
+              const float4 qf0 = rec[(4 % kQ) * kChunk + i], qf1 = rec[(5 % kQ) * kChunk + i],
+                         qf2 = rec[(6 % kQ) * kChunk + i];
               Bf[0] = qf0.x;
               Bf[1] = qf0.y;
               Bf[2] = qf0.z;
@@ -490,16 +515,16 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const f
       }
       // ---- overflow particles: global atomics straight into the grid
 #pragma unroll
-      for (int k = 0; k < kChunk / kThreads; ++k) {
-        const int32_t i = threadIdx.x + k * kThreads;
+      for (int k = 0; k < kChunk / kP2GThreads; ++k) {
+        const int32_t i = threadIdx.x + k * kP2GThreads;
         if (i >= nch || keepc[k] != kMainCells) continue;
         const int r0 = (keep[k] >> 6) & 7, r1 = (keep[k] >> 3) & 7, r2 = keep[k] & 7;
-        const float4* q = rec + i * kQ;
-        const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+        const float4 q0 = rec[i], q1 = rec[kChunk + i], q2 = rec[2 * kChunk + i], q3 = rec[3 * kChunk + i];
         const float Am[9] = {q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w, q1.w};
         float Af[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, Bf[3] = {0.f, 0.f, 0.f};
         if (!kFold) {
-          const float4 qf0 = q[4 % kQ], qf1 = q[5 % kQ], qf2 = q[6 % kQ];
+          const float4 qf0 = rec[(4 % kQ) * kChunk + i], qf1 = rec[(5 % kQ) * kChunk + i],
+                         qf2 = rec[(6 % kQ) * kChunk + i];
           Bf[0] = qf0.x;
           Bf[1] = qf0.y;
           Bf[2] = qf0.z;
@@ -541,10 +566,10 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const f
       __syncthreads();
     }
     // ---- flush: sum the wave arenas, one global atomic per touched node/channel
-    for (int n = threadIdx.x; n < kArena6; n += kThreads) {
+    for (int n = threadIdx.x; n < kArena6; n += kP2GThreads) {
       float4 s = arena[0][n * kNV];
 #pragma unroll
-      for (int w = 1; w < kWaves; ++w) {
+      for (int w = 1; w < kP2GWaves; ++w) {
         const float4 u = arena[w][n * kNV];
         s.x += u.x;
         s.y += u.y;
@@ -564,7 +589,7 @@ __global__ __launch_bounds__(kThreads) void p2g_kernel(ParticlesView ps, const f
           if (!kFold) {
             float4 f = arena[0][n * kNV + 1];
 #pragma unroll
-            for (int w = 1; w < kWaves; ++w) {
+            for (int w = 1; w < kP2GWaves; ++w) {
               const float4 u = arena[w][n * kNV + 1];
               f.x += u.x;
               f.y += u.y;
@@ -914,10 +939,12 @@ hipError_t stress(const Model& m, const ParticlesView& ps, Workspace& ws, hipStr
 hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
                        int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s) {
   const int g = persistent_groups();
+  reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
+  uint32_t* q = ws.counters + kCtrQueue;
   if (fold)
-    p2g_kernel<true><<<g, kThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err);
+    p2g_kernel<true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q);
   else
-    p2g_kernel<false><<<g, kThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err);
+    p2g_kernel<false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q);
   return hipGetLastError();
 }
 

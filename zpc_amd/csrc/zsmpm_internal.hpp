@@ -59,6 +59,7 @@ enum : int {
   kCtrFp64 = 3,     // particles that took the FP64 SVD path
   kCtrFallback = 4, // particles that took the SVD fallback of the learned closure
   kCtrMaxVel = 5,   // max |v|^2 as float bits
+  kCtrQueue = 6,    // next bucket of the running per-bucket launch
   kCtrCount = 16
 };
 
