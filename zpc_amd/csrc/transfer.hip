@@ -709,6 +709,9 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
     load_neighbours(nbr_tab, b, nbr, err);
     const int32_t o0 = t.active[3 * (int64_t)b] * 4, o1 = t.active[3 * (int64_t)b + 1] * 4,
                   o2 = t.active[3 * (int64_t)b + 2] * 4;
+    // the wave's first 64 particle indices load while the velocities are staged
+    int32_t first = -1;
+    if (wave * kWave + lane < cnt) first = sv.indices[off + wave * kWave + lane];
     __syncthreads();
     // stage node velocities: slot s -> (neighbour block s>>6, cell s&63), stored at vel_at(r0, r1, r2)
     for (int s = threadIdx.x; s < kArena8; s += kG2PThreads) {
@@ -729,7 +732,7 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
     for (int32_t i0 = wave * kWave; i0 < cnt; i0 += kG2PThreads) {
       const int32_t cw = min(kWave, cnt - i0);
       const bool valid = lane < cw;
-      const int64_t p = valid ? (int64_t)sv.indices[off + i0 + lane] : -1;
+      const int64_t p = valid ? (int64_t)(i0 == wave * kWave ? first : sv.indices[off + i0 + lane]) : -1;
       const bool bad = valid && (p < 0 || p >= ps.n);
       if (bad) atomicOr(err, kErrCapacity);
       const bool live = valid && !bad;
