@@ -201,6 +201,8 @@ int32_t reserve(zsmpm_policy_s* p, int64_t n, int32_t T) {
   ZA(w.nbr, (size_t)8 * w.ts_cap);
   ZA(w.lj_next[0], w.n_cap);
   ZA(w.lj_next[1], w.n_cap);
+  ZA(w.pslot, w.n_cap);
+  ZA(w.slot_block, w.ts_cap);
   p->own_cap_blocks = (int32_t)w.ts_cap;
   ZA(p->own_counts, (size_t)p->own_cap_blocks + 1);
   ZA(p->own_offsets, (size_t)p->own_cap_blocks + 1);
@@ -397,7 +399,7 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_ACTIVATE);
-    ZS_HIP(table_activate(t, ws, ps.x, ps.n, dx, 4, -2, 0.5f, s));
+    ZS_HIP(table_activate(t, ws, ps.x, ps.n, dx, 4, -2, 0.5f, s, /*record_slots=*/true));
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_ENLARGE);
@@ -405,7 +407,9 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_BUCKET_SORT);
-    ZS_HIP(bucket_sort(t, ws, ps.x, ps.n, dx, 4, -2, 0.5f, sv, s));
+    // the activation above recorded every particle's key slot in the table it
+    // just built from empty: the bucket pass reads those instead of x
+    ZS_HIP(bucket_sort_slots(ws, ps.n, sv, s));
     ZS_HIP(block_neighbours(t, ws, s));
   }
   {

@@ -61,11 +61,12 @@ __global__ void clear_table_kernel(TableView t, uint64_t* skey, uint32_t* sorder
 }
 
 __global__ void clear_scratch_kernel(uint64_t* skey, uint32_t* sorder, uint32_t ts,
-                                     const int32_t* cnt, uint32_t* counters) {
+                                     const int32_t* cnt, uint32_t* counters, int32_t* slot_block) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ts; e += stride) {
     skey[e] = kEmpty64;
     sorder[e] = 0xffffffffu;
+    if (slot_block) slot_block[e] = -1;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     counters[kCtrList] = 0;
@@ -74,28 +75,32 @@ __global__ void clear_scratch_kernel(uint64_t* skey, uint32_t* sorder, uint32_t 
 }
 
 // ----------------------------------------------------------- dedup inserts
-__device__ __forceinline__ void dedup_insert(uint64_t pk, uint32_t order, uint64_t* skey,
-                                             uint32_t* sorder, uint32_t ts, uint32_t* err) {
+// returns the slot holding pk, or ~0u when the scratch table is full
+__device__ __forceinline__ uint32_t dedup_insert(uint64_t pk, uint32_t order, uint64_t* skey,
+                                                 uint32_t* sorder, uint32_t ts, uint32_t* err) {
   uint32_t s = (uint32_t)mix64(pk) & (ts - 1u);
   for (uint32_t probe = 0; probe < ts; ++probe) {
     const uint64_t prev = atomicCAS((unsigned long long*)&skey[s], (unsigned long long)kEmpty64,
                                     (unsigned long long)pk);
     if (prev == kEmpty64 || prev == pk) {
       atomicMin(&sorder[s], order);
-      return;
+      return s;
     }
     s = (s + 1u) & (ts - 1u);
   }
   atomicOr(err, kErrTableFull);
+  return ~0u;
 }
 
 // ComputeSparsity candidates: one per particle, order = particle index. Each
 // distinct key of a wave is inserted once, by its lowest lane (the smallest
-// particle index holding it), so hot keys see one atomic per wave.
+// particle index holding it), so hot keys see one atomic per wave. With
+// pslot, every particle's scratch slot (the identity of its key) is recorded
+// for the bucket sort that follows.
 __global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restrict__ x, int64_t n,
                                                             float dxinv, int blockLen, int offset,
                                                             float disp, uint64_t* skey, uint32_t* sorder,
-                                                            uint32_t ts, uint32_t* err) {
+                                                            uint32_t ts, uint32_t* err, uint32_t* pslot) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int lane = lane_id();
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -114,13 +119,21 @@ __global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restr
     // peel off one distinct key per iteration: the lowest pending lane leads
     bool pending = pk != kEmpty64;
     bool head = false;
+    int my_leader = lane;
     for (uint64_t m = ballot(pending); m != 0; m = ballot(pending)) {
       const int leader = __ffsll((long long)m) - 1;
       const uint64_t lk = shfl(pk, leader);
+      const bool same = pending && pk == lk;
+      my_leader = same ? leader : my_leader;
       head = head || lane == leader;
-      pending = pending && pk != lk;
+      pending = pending && !same;
     }
-    if (head) dedup_insert(pk, (uint32_t)p, skey, sorder, ts, err);
+    uint32_t slot = ~0u;
+    if (head) slot = dedup_insert(pk, (uint32_t)p, skey, sorder, ts, err);
+    if (pslot) {
+      const uint32_t ls = (uint32_t)__shfl((int)slot, my_leader, kWave);
+      if (valid) pslot[p] = pk != kEmpty64 ? ls : ~0u;
+    }
   }
 }
 
@@ -144,11 +157,11 @@ __global__ __launch_bounds__(kThreads) void enlarge_kernel(const int32_t* __rest
       atomicOr(err, kErrKeyRange);
       continue;
     }
-    dedup_insert(pack_key(k0, k1, k2), (uint32_t)c, skey, sorder, ts, err);
+    (void)dedup_insert(pack_key(k0, k1, k2), (uint32_t)c, skey, sorder, ts, err);
   }
 }
 
-constexpr int kCollectGroups = 256;
+constexpr int kCollectGroups = 1024;
 constexpr int kCollectBuf = 2048;
 
 __device__ __forceinline__ void list_put(uint32_t slot, uint32_t key, uint32_t val, uint32_t* lkey,
@@ -223,12 +236,13 @@ __global__ __launch_bounds__(kThreads) void collect_new_kernel(const uint64_t* _
 __global__ __launch_bounds__(kThreads) void assign_kernel(const uint32_t* __restrict__ sorted_slot,
                                                           const uint64_t* __restrict__ skey,
                                                           TableView t, const uint32_t* counters,
-                                                          uint32_t list_cap, uint32_t* err) {
+                                                          uint32_t list_cap, uint32_t* err, int32_t* slot_block) {
   const uint32_t nl = min(counters[kCtrList], list_cap);
   const uint32_t cnt0 = counters[kCtrCnt0];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nl; i += stride) {
-    const uint64_t pk = skey[sorted_slot[i]];
+    const uint32_t sl = sorted_slot[i];
+    const uint64_t pk = skey[sl];
     int32_t k0, k1, k2;
     unpack_key(pk, k0, k1, k2);
     const uint32_t rank = cnt0 + (uint32_t)i;
@@ -236,6 +250,7 @@ __global__ __launch_bounds__(kThreads) void assign_kernel(const uint32_t* __rest
       atomicOr(err, kErrTableFull);
       continue;
     }
+    if (slot_block) slot_block[sl] = (int32_t)rank;  // the block number of the key in scratch slot sl
     t.active[3 * (int64_t)rank] = k0;
     t.active[3 * (int64_t)rank + 1] = k1;
     t.active[3 * (int64_t)rank + 2] = k2;
@@ -350,6 +365,64 @@ __global__ __launch_bounds__(kThreads) void bucket_kernel(const float* __restric
       if (valid) bkt[p] = (uint32_t)(b < 0 ? 0 : b) | (head ? kHeadBit : 0u);
       heads += head ? 1u : 0u;
       // per-bucket counts: one atomic per (lookup segment) in the wave
+      const uint64_t vm = ballot(valid);
+      if (lookup && b >= 0) {
+        const uint64_t above = lm & ~le;
+        const int next = above ? (__ffsll((long long)above) - 1) : 64;
+        const uint64_t seg = (next >= 64 ? ~0ull : ((1ull << next) - 1ull)) & ~((1ull << lane) - 1ull);
+        atomicAdd(&counts[b], (uint32_t)__popcll(seg & vm));
+      }
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) heads += __shfl_xor(heads, o, kWave);
+    if (lane == 0) wsum[threadIdx.x / kWave] = heads;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t tot = 0;
+      for (int i = 0; i < kThreads / kWave; ++i) tot += wsum[i];
+      tile_heads[tile] = tot;
+    }
+    __syncthreads();
+  }
+}
+
+// The same outputs as bucket_kernel from the scratch slots recorded by the
+// activation that built this table (activate_kernel with pslot): a particle's
+// slot identifies its key, and slot_block maps it to the block number, so
+// neither x nor the table is read again.
+__global__ __launch_bounds__(kThreads) void bucket_slots_kernel(const uint32_t* __restrict__ pslot, int64_t n,
+                                                                const int32_t* __restrict__ slot_block, uint32_t ts,
+                                                                uint32_t* __restrict__ bkt, uint32_t* counts,
+                                                                int32_t cap_blocks, uint32_t* __restrict__ tile_heads,
+                                                                uint32_t* err) {
+  __shared__ uint32_t wsum[kThreads / kWave];
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  const int lane = lane_id();
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint32_t heads = 0;
+    for (int r = 0; r < kTileItems; ++r) {
+      const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
+      const bool valid = p < n;
+      const uint32_t sl = valid ? pslot[p] : ~0u;
+      uint32_t prev = __shfl_up(sl, 1, kWave);
+      if (lane == 0 && valid && p > 0) prev = pslot[p - 1];
+      const bool head = valid && (p == 0 || sl != prev);
+      const bool lookup = valid && (head || lane == 0);
+      int32_t b = -1;
+      if (lookup) {
+        b = sl < ts ? slot_block[sl] : -1;
+        if (b < 0) atomicOr(err, kErrMissingBlock);
+        if (b >= cap_blocks) {
+          atomicOr(err, kErrCapacity);
+          b = -1;
+        }
+      }
+      const uint64_t lm = ballot(lookup);
+      const uint64_t le = lanemask_lt() | (1ull << lane);
+      const int src = 63 - __clzll(lm & le);
+      b = shfl(b, src < 0 ? 0 : src);
+      if (valid) bkt[p] = (uint32_t)(b < 0 ? 0 : b) | (head ? kHeadBit : 0u);
+      heads += head ? 1u : 0u;
       const uint64_t vm = ballot(valid);
       if (lookup && b >= 0) {
         const uint64_t above = lm & ~le;
@@ -492,7 +565,8 @@ int bits_for(uint64_t maxval) {
 
 // dedup entries -> ranked, placed and keyed table entries (shared tail of
 // activation and enlargement)
-hipError_t finish_insertion(const TableView& t, Workspace& ws, uint64_t order_bound, hipStream_t s) {
+hipError_t finish_insertion(const TableView& t, Workspace& ws, uint64_t order_bound, hipStream_t s,
+                            int32_t* slot_block = nullptr) {
   collect_new_kernel<<<kCollectGroups, kThreads, 0, s>>>(ws.skey, ws.sorder, ws.ts_cap, t, ws.counters, ws.la_key,
                                             ws.la_val, ws.list_cap, ws.err);
   bool alt = false;
@@ -501,7 +575,7 @@ hipError_t finish_insertion(const TableView& t, Workspace& ws, uint64_t order_bo
   if (e != hipSuccess) return e;
   const uint32_t* sorted_slot = alt ? ws.lb_val : ws.la_val;
   const int gl = groups_for(ws.list_cap, kThreads);
-  assign_kernel<<<gl, kThreads, 0, s>>>(sorted_slot, ws.skey, t, ws.counters, ws.list_cap, ws.err);
+  assign_kernel<<<gl, kThreads, 0, s>>>(sorted_slot, ws.skey, t, ws.counters, ws.list_cap, ws.err, slot_block);
   fill_keys_kernel<<<gl, kThreads, 0, s>>>(t, ws.counters, ws.list_cap, ws.err);
   bump_cnt_kernel<<<1, 1, 0, s>>>(t, ws.counters, ws.list_cap);
   return hipGetLastError();
@@ -517,15 +591,17 @@ hipError_t table_clear(const TableView& t, Workspace& ws, hipStream_t s) {
 }
 
 hipError_t table_activate(const TableView& t, Workspace& ws, const float* x, int64_t n, float dx,
-                          int blockLen, int offset, float disp, hipStream_t s) {
+                          int blockLen, int offset, float disp, hipStream_t s, bool record_slots) {
   if (blockLen <= 0) return hipErrorInvalidValue;
+  int32_t* slot_block = record_slots ? ws.slot_block : nullptr;
   clear_scratch_kernel<<<groups_for(ws.ts_cap, kThreads), kThreads, 0, s>>>(ws.skey, ws.sorder, ws.ts_cap,
-                                                                             t.cnt, ws.counters);
+                                                                             t.cnt, ws.counters, slot_block);
   const float dxinv = (float)1.0 / dx;
   if (n > 0)
     activate_kernel<<<groups_for(n, kThreads), kThreads, 0, s>>>(x, n, dxinv, blockLen, offset, disp,
-                                                                  ws.skey, ws.sorder, ws.ts_cap, ws.err);
-  return finish_insertion(t, ws, (uint64_t)(n > 0 ? n : 1), s);
+                                                                  ws.skey, ws.sorder, ws.ts_cap, ws.err,
+                                                                  record_slots ? ws.pslot : nullptr);
+  return finish_insertion(t, ws, (uint64_t)(n > 0 ? n : 1), s, slot_block);
 }
 
 hipError_t table_enlarge(const TableView& t, Workspace& ws, const int32_t lo[3], const int32_t hi[3],
@@ -533,7 +609,7 @@ hipError_t table_enlarge(const TableView& t, Workspace& ws, const int32_t lo[3],
   const int32_t e0 = hi[0] - lo[0], e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
   if (e0 <= 0 || e1 <= 0 || e2 <= 0) return hipSuccess;
   clear_scratch_kernel<<<groups_for(ws.ts_cap, kThreads), kThreads, 0, s>>>(ws.skey, ws.sorder, ws.ts_cap,
-                                                                             t.cnt, ws.counters);
+                                                                             t.cnt, ws.counters, nullptr);
   const int64_t total_cap = (int64_t)n0_cap * e0 * e1 * e2;
   enlarge_kernel<<<groups_for(total_cap, kThreads), kThreads, 0, s>>>(
       t.active, ws.counters, lo[0], lo[1], lo[2], e0, e1, e2, ws.skey, ws.sorder, ws.ts_cap, ws.err);
@@ -552,20 +628,13 @@ hipError_t table_query(const TableView& t, const int32_t* keys, int64_t n, int32
   return hipGetLastError();
 }
 
-hipError_t bucket_sort(const TableView& t, Workspace& ws, const float* x, int64_t n, float dx, int blockLen,
-                       int offset, float disp, const SortView& out, hipStream_t s) {
-  if (n > (int64_t)ws.n_cap) return hipErrorInvalidValue;
+namespace {
+
+// after the bucket pass (bkt with run heads, counts, tile_heads): stable
+// placement of the particles through the runs
+hipError_t place_runs(Workspace& ws, int64_t n, const SortView& out, int64_t ntiles, int gt, hipStream_t s) {
   const int64_t nb1 = (int64_t)out.cap_blocks + 1;
   uint32_t* counts = (uint32_t*)out.counts;
-  zero_u32_kernel<<<groups_for(nb1, kThreads), kThreads, 0, s>>>(counts, nb1);
-  if (n <= 0) {
-    return exclusive_scan_u32(counts, (uint32_t*)out.offsets, nullptr, (uint32_t)nb1, ws.scan_tmp, nullptr, s);
-  }
-  const int64_t ntiles = (n + kTile - 1) / kTile;
-  const int gt = (int)(ntiles < 65536 ? ntiles : 65536);
-  const float dxinv = (float)1.0 / dx;
-  bucket_kernel<<<gt, kThreads, 0, s>>>(x, n, dxinv, blockLen, offset, disp, t, ws.bkt, counts, out.cap_blocks,
-                                        ws.tile_heads, ws.err);
   hipError_t e = exclusive_scan_u32(ws.tile_heads, ws.tile_base, nullptr, (uint32_t)ntiles, ws.scan_tmp,
                                     ws.counters + kCtrRuns, s);
   if (e != hipSuccess) return e;
@@ -586,6 +655,40 @@ hipError_t bucket_sort(const TableView& t, Workspace& ws, const float* x, int64_
   particle_scatter_kernel<<<gt, kThreads, 0, s>>>(ws.bkt, n, ws.tile_base, ws.run_start, ws.run_dest,
                                                   ws.counters, out.indices);
   return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t bucket_sort(const TableView& t, Workspace& ws, const float* x, int64_t n, float dx, int blockLen,
+                       int offset, float disp, const SortView& out, hipStream_t s) {
+  if (n > (int64_t)ws.n_cap) return hipErrorInvalidValue;
+  const int64_t nb1 = (int64_t)out.cap_blocks + 1;
+  uint32_t* counts = (uint32_t*)out.counts;
+  zero_u32_kernel<<<groups_for(nb1, kThreads), kThreads, 0, s>>>(counts, nb1);
+  if (n <= 0) {
+    return exclusive_scan_u32(counts, (uint32_t*)out.offsets, nullptr, (uint32_t)nb1, ws.scan_tmp, nullptr, s);
+  }
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  const int gt = (int)(ntiles < 65536 ? ntiles : 65536);
+  const float dxinv = (float)1.0 / dx;
+  bucket_kernel<<<gt, kThreads, 0, s>>>(x, n, dxinv, blockLen, offset, disp, t, ws.bkt, counts, out.cap_blocks,
+                                        ws.tile_heads, ws.err);
+  return place_runs(ws, n, out, ntiles, gt, s);
+}
+
+hipError_t bucket_sort_slots(Workspace& ws, int64_t n, const SortView& out, hipStream_t s) {
+  if (n > (int64_t)ws.n_cap) return hipErrorInvalidValue;
+  const int64_t nb1 = (int64_t)out.cap_blocks + 1;
+  uint32_t* counts = (uint32_t*)out.counts;
+  zero_u32_kernel<<<groups_for(nb1, kThreads), kThreads, 0, s>>>(counts, nb1);
+  if (n <= 0) {
+    return exclusive_scan_u32(counts, (uint32_t*)out.offsets, nullptr, (uint32_t)nb1, ws.scan_tmp, nullptr, s);
+  }
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  const int gt = (int)(ntiles < 65536 ? ntiles : 65536);
+  bucket_slots_kernel<<<gt, kThreads, 0, s>>>(ws.pslot, n, ws.slot_block, ws.ts_cap, ws.bkt, counts,
+                                              out.cap_blocks, ws.tile_heads, ws.err);
+  return place_runs(ws, n, out, ntiles, gt, s);
 }
 
 }  // namespace zsmpm

@@ -55,8 +55,12 @@ __device__ __forceinline__ void block_key(float x0, float x1, float x2, float dx
   for (int d = 0; d < 3; ++d) {
     const float t = xs[d] * dxinv;
     const int32_t c = (int32_t)floorf(t + disp) + offset;
-    const int32_t b = c + (c < 0 ? -blockLen + 1 : 0);
-    out[d] = b / blockLen;
+    if ((blockLen & (blockLen - 1)) == 0) {
+      out[d] = c >> (__ffs(blockLen) - 1);  // floor division by a power of two
+    } else {
+      const int32_t b = c + (c < 0 ? -blockLen + 1 : 0);
+      out[d] = b / blockLen;
+    }
   }
   k0 = out[0];
   k1 = out[1];

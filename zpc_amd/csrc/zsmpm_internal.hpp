@@ -94,6 +94,8 @@ struct Workspace {
   float* contrib = nullptr;      // [9 n_cap] vol P F^T of each particle (column-major)
   int32_t* nbr = nullptr;        // [8 ts_cap] block index of key_b + (d>>2&1, d>>1&1, d&1), or -1
   float* lj_next[2] = {nullptr, nullptr};  // [n_cap] each: logJp caches of the fused multi-substep driver
+  uint32_t* pslot = nullptr;     // [n_cap] scratch slot of each particle's key (recording activation)
+  int32_t* slot_block = nullptr; // [ts_cap] block number of the key in each scratch slot
 };
 
 // The next substep's constitutive part computed inside G2P from the updated
@@ -136,13 +138,18 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
 
 // ---------------------------------------------------------------- sparsity
 hipError_t table_clear(const TableView& t, Workspace& ws, hipStream_t s);
+// record_slots: also fill ws.pslot / ws.slot_block for bucket_sort_slots
+// (the table must be empty: every key is then new)
 hipError_t table_activate(const TableView& t, Workspace& ws, const float* x, int64_t n, float dx,
-                          int blockLen, int offset, float disp, hipStream_t s);
+                          int blockLen, int offset, float disp, hipStream_t s, bool record_slots = false);
 hipError_t table_enlarge(const TableView& t, Workspace& ws, const int32_t lo[3], const int32_t hi[3],
                          int32_t n0_cap, hipStream_t s);
 hipError_t table_query(const TableView& t, const int32_t* keys, int64_t n, int32_t* out, hipStream_t s);
 hipError_t bucket_sort(const TableView& t, Workspace& ws, const float* x, int64_t n, float dx,
                        int blockLen, int offset, float disp, const SortView& out, hipStream_t s);
+// bucket_sort of the particles a recording activation just inserted into an
+// emptied table (blockLen 4, offset -2, displacement 0.5 as that activation)
+hipError_t bucket_sort_slots(Workspace& ws, int64_t n, const SortView& out, hipStream_t s);
 // ws.nbr for every active block (the 2x2x2 blocks a bucket's stencils cover)
 hipError_t block_neighbours(const TableView& t, Workspace& ws, hipStream_t s);
 
