@@ -201,6 +201,13 @@ int32_t reserve(zsmpm_policy_s* p, int64_t n, int32_t T) {
   ZA(w.nbr, (size_t)8 * w.ts_cap);
   ZA(w.lj_next[0], w.n_cap);
   ZA(w.lj_next[1], w.n_cap);
+  {
+    // insertion orders: particle indices (< n) or enlargement candidates (< 8 T)
+    const uint64_t orders = (uint64_t)w.n_cap > 8ull * w.ts_cap ? (uint64_t)w.n_cap : 8ull * w.ts_cap;
+    w.bm_words = (uint32_t)(orders / 32 + 2);
+  }
+  ZA(w.bitmap, w.bm_words);
+  ZA(w.bm_prefix, w.bm_words);
   ZA(w.pslot, w.n_cap);
   ZA(w.slot_block, w.ts_cap);
   p->own_cap_blocks = (int32_t)w.ts_cap;

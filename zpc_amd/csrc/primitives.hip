@@ -78,6 +78,13 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* lds) {
   return total;
 }
 
+// kPopc: scan the population counts of the input words instead of the words
+template <bool kPopc>
+__device__ __forceinline__ uint32_t scan_in(uint32_t v) {
+  return kPopc ? (uint32_t)__popc(v) : v;
+}
+
+template <bool kPopc>
 __global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in,
                                                                const uint32_t* n_dev, uint32_t n_cap,
                                                                uint32_t* __restrict__ group_sums) {
@@ -86,7 +93,7 @@ __global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const uint32_t* _
   uint32_t lo, hi;
   group_range(n, gridDim.x, blockIdx.x, lo, hi);
   uint32_t s = 0;
-  for (uint32_t i = lo + threadIdx.x; i < hi; i += kThreads) s += in[i];
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kThreads) s += scan_in<kPopc>(in[i]);
   const uint32_t total = block_sum<kThreads>(s, lds);
   if (threadIdx.x == 0) group_sums[blockIdx.x] = total;
 }
@@ -137,6 +144,7 @@ __global__ __launch_bounds__(kSmallThreads) void scan_small_kernel(uint32_t* __r
   if (threadIdx.x == 0 && total_out) *total_out = total;
 }
 
+template <bool kPopc>
 __global__ __launch_bounds__(kThreads) void scan_down_kernel(const uint32_t* __restrict__ in,
                                                              uint32_t* __restrict__ out, const uint32_t* n_dev,
                                                              uint32_t n_cap, const uint32_t* __restrict__ group_base) {
@@ -152,7 +160,7 @@ __global__ __launch_bounds__(kThreads) void scan_down_kernel(const uint32_t* __r
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
       const uint32_t idx = start + i;
-      v[i] = idx < hi ? in[idx] : 0u;
+      v[i] = idx < hi ? scan_in<kPopc>(in[idx]) : 0u;
       s += v[i];
     }
     uint32_t total;
@@ -247,9 +255,19 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, const uint32_t*
                               uint32_t* scratch, uint32_t* total_out, hipStream_t s) {
   if (n_cap == 0) return hipSuccess;
   const uint32_t G = groups_for_cap(n_cap);
-  scan_reduce_kernel<<<G, kThreads, 0, s>>>(in, n_dev, n_cap, scratch);
+  scan_reduce_kernel<false><<<G, kThreads, 0, s>>>(in, n_dev, n_cap, scratch);
   scan_small_kernel<1><<<1, kSmallThreads, 0, s>>>(scratch, G, total_out);
-  scan_down_kernel<<<G, kThreads, 0, s>>>(in, out, n_dev, n_cap, scratch);
+  scan_down_kernel<false><<<G, kThreads, 0, s>>>(in, out, n_dev, n_cap, scratch);
+  return hipGetLastError();
+}
+
+hipError_t exclusive_scan_popc_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch,
+                                   hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t G = groups_for_cap(n);
+  scan_reduce_kernel<true><<<G, kThreads, 0, s>>>(in, nullptr, n, scratch);
+  scan_small_kernel<1><<<1, kSmallThreads, 0, s>>>(scratch, G, nullptr);
+  scan_down_kernel<true><<<G, kThreads, 0, s>>>(in, out, nullptr, n, scratch);
   return hipGetLastError();
 }
 

@@ -183,7 +183,8 @@ __global__ __launch_bounds__(kThreads) void collect_new_kernel(const uint64_t* _
                                                                const uint32_t* __restrict__ sorder,
                                                                uint32_t ts, TableView t, uint32_t* counters,
                                                                uint32_t* lkey, uint32_t* lval,
-                                                               uint32_t list_cap, uint32_t* err) {
+                                                               uint32_t list_cap, uint32_t* err,
+                                                               uint32_t* bitmap, uint32_t bm_words) {
   __shared__ uint32_t bkey[kCollectBuf], bval[kCollectBuf];
   __shared__ uint32_t bcount, gbase;
   const bool table_empty = counters[kCtrCnt0] == 0;
@@ -214,6 +215,10 @@ __global__ __launch_bounds__(kThreads) void collect_new_kernel(const uint64_t* _
     if (lane == leader) wbase = atomicAdd(&bcount, (uint32_t)__popcll(m));
     wbase = shfl(wbase, leader);
     if (take) {
+      // mark the key's order: its rank among the new keys is the number of
+      // marked orders below it (orders of distinct keys are distinct)
+      const uint32_t o = sorder[s];
+      if ((o >> 5) < bm_words) atomicOr(&bitmap[o >> 5], 1u << (o & 31u));
       const uint32_t slot = wbase + (uint32_t)__popcll(m & lanemask_lt());
       if (slot < (uint32_t)kCollectBuf) {
         bkey[slot] = sorder[s];
@@ -233,7 +238,10 @@ __global__ __launch_bounds__(kThreads) void collect_new_kernel(const uint64_t* _
 
 // Number the new keys cnt0 + rank, record _activeKeys and place them by
 // priority linear probing on indices[] (lower block number wins a slot).
-__global__ __launch_bounds__(kThreads) void assign_kernel(const uint32_t* __restrict__ sorted_slot,
+__global__ __launch_bounds__(kThreads) void assign_kernel(const uint32_t* __restrict__ lorder,
+                                                          const uint32_t* __restrict__ lslot,
+                                                          const uint32_t* __restrict__ bitmap,
+                                                          const uint32_t* __restrict__ bm_prefix, uint32_t bm_words,
                                                           const uint64_t* __restrict__ skey,
                                                           TableView t, const uint32_t* counters,
                                                           uint32_t list_cap, uint32_t* err, int32_t* slot_block) {
@@ -241,11 +249,18 @@ __global__ __launch_bounds__(kThreads) void assign_kernel(const uint32_t* __rest
   const uint32_t cnt0 = counters[kCtrCnt0];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nl; i += stride) {
-    const uint32_t sl = sorted_slot[i];
+    const uint32_t sl = lslot[i];
+    const uint32_t o = lorder[i];
+    if ((o >> 5) >= bm_words) {
+      atomicOr(err, kErrCapacity);
+      continue;
+    }
+    // first-occurrence rank: new keys with a smaller order
+    const uint32_t below = bm_prefix[o >> 5] + (uint32_t)__popc(bitmap[o >> 5] & ((1u << (o & 31u)) - 1u));
     const uint64_t pk = skey[sl];
     int32_t k0, k1, k2;
     unpack_key(pk, k0, k1, k2);
-    const uint32_t rank = cnt0 + (uint32_t)i;
+    const uint32_t rank = cnt0 + below;
     if (rank >= (uint32_t)t.T) {
       atomicOr(err, kErrTableFull);
       continue;
@@ -567,15 +582,17 @@ int bits_for(uint64_t maxval) {
 // activation and enlargement)
 hipError_t finish_insertion(const TableView& t, Workspace& ws, uint64_t order_bound, hipStream_t s,
                             int32_t* slot_block = nullptr) {
+  uint64_t words = order_bound / 32 + 1;
+  if (words > ws.bm_words) words = ws.bm_words;  // orders beyond the bitmap are reported by assign
+  zero_u32_kernel<<<groups_for((int64_t)words, kThreads) < 2048 ? groups_for((int64_t)words, kThreads) : 2048,
+                    kThreads, 0, s>>>(ws.bitmap, (int64_t)words);
   collect_new_kernel<<<kCollectGroups, kThreads, 0, s>>>(ws.skey, ws.sorder, ws.ts_cap, t, ws.counters, ws.la_key,
-                                            ws.la_val, ws.list_cap, ws.err);
-  bool alt = false;
-  hipError_t e = radix_sort_pairs(ws.la_key, ws.la_val, ws.lb_key, ws.lb_val, ws.counters + kCtrList,
-                                  ws.list_cap, bits_for(order_bound - 1), ws.rs_scratch, s, &alt);
+                                                         ws.la_val, ws.list_cap, ws.err, ws.bitmap, (uint32_t)words);
+  hipError_t e = exclusive_scan_popc_u32(ws.bitmap, ws.bm_prefix, (uint32_t)words, ws.scan_tmp, s);
   if (e != hipSuccess) return e;
-  const uint32_t* sorted_slot = alt ? ws.lb_val : ws.la_val;
   const int gl = groups_for(ws.list_cap, kThreads);
-  assign_kernel<<<gl, kThreads, 0, s>>>(sorted_slot, ws.skey, t, ws.counters, ws.list_cap, ws.err, slot_block);
+  assign_kernel<<<gl, kThreads, 0, s>>>(ws.la_key, ws.la_val, ws.bitmap, ws.bm_prefix, (uint32_t)words, ws.skey, t,
+                                        ws.counters, ws.list_cap, ws.err, slot_block);
   fill_keys_kernel<<<gl, kThreads, 0, s>>>(t, ws.counters, ws.list_cap, ws.err);
   bump_cnt_kernel<<<1, 1, 0, s>>>(t, ws.counters, ws.list_cap);
   return hipGetLastError();

@@ -94,6 +94,9 @@ struct Workspace {
   float* contrib = nullptr;      // [9 n_cap] vol P F^T of each particle (column-major)
   int32_t* nbr = nullptr;        // [8 ts_cap] block index of key_b + (d>>2&1, d>>1&1, d&1), or -1
   float* lj_next[2] = {nullptr, nullptr};  // [n_cap] each: logJp caches of the fused multi-substep driver
+  uint32_t* bitmap = nullptr;    // [bm_words] one bit per insertion order: the orders of new keys
+  uint32_t* bm_prefix = nullptr; // [bm_words] exclusive prefix of the bit counts
+  uint32_t bm_words = 0;
   uint32_t* pslot = nullptr;     // [n_cap] scratch slot of each particle's key (recording activation)
   int32_t* slot_block = nullptr; // [ts_cap] block number of the key in each scratch slot
 };
@@ -132,6 +135,9 @@ size_t scan_scratch_words();
 size_t radix_sort_scratch_words(uint32_t n_cap);
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_cap,
                               uint32_t* tile_sums, uint32_t* total_out, hipStream_t s);
+// exclusive scan of the population counts of n words
+hipError_t exclusive_scan_popc_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch,
+                                   hipStream_t s);
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
                             const uint32_t* n_dev, uint32_t n_cap, int bits, uint32_t* scratch,
                             hipStream_t s, bool* result_in_alt);
