@@ -15,6 +15,10 @@ using namespace zsmpm;
 struct zsmpm_policy_s {
   int devid = 0;
   hipStream_t stream = nullptr;
+  // second stream of the substep: the stress pass runs on it while the
+  // sparsity stages run on `stream` (created on first use, default priority)
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   Workspace ws{};
   std::vector<void*> allocs;
   int64_t reserved_n = 0;
@@ -106,11 +110,13 @@ hipEvent_t take_event(zsmpm_policy_s* p) {
 struct StageScope {
   zsmpm_policy_s* p;
   int stage;
+  hipStream_t s;
   hipEvent_t a = nullptr;
-  StageScope(zsmpm_policy_s* pol, int st) : p(pol), stage(st) {
+  StageScope(zsmpm_policy_s* pol, int st) : StageScope(pol, st, pol->stream) {}
+  StageScope(zsmpm_policy_s* pol, int st, hipStream_t stream) : p(pol), stage(st), s(stream) {
     if (!p->profile) return;
     a = take_event(p);
-    if (a && hipEventRecord(a, p->stream) != hipSuccess) {
+    if (a && hipEventRecord(a, s) != hipSuccess) {
       p->ev_free.push_back(a);
       a = nullptr;
     }
@@ -118,7 +124,7 @@ struct StageScope {
   ~StageScope() {
     if (!a) return;
     hipEvent_t b = take_event(p);
-    if (!b || hipEventRecord(b, p->stream) != hipSuccess) {
+    if (!b || hipEventRecord(b, s) != hipSuccess) {
       p->ev_free.push_back(a);
       if (b) p->ev_free.push_back(b);
       return;
@@ -145,7 +151,51 @@ void destroy_events(zsmpm_policy_s* p) {
   harvest_events(p);
   for (hipEvent_t e : p->ev_free) (void)hipEventDestroy(e);
   p->ev_free.clear();
+  if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
+  if (p->join_ev) (void)hipEventDestroy(p->join_ev);
+  if (p->side) (void)hipStreamDestroy(p->side);
+  p->fork_ev = p->join_ev = nullptr;
+  p->side = nullptr;
 }
+
+hipError_t ensure_side(zsmpm_policy_s* p) {
+  if (p->side) return hipSuccess;
+  hipError_t e = hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
+    if (p->join_ev) (void)hipEventDestroy(p->join_ev);
+    p->fork_ev = p->join_ev = nullptr;
+    p->side = nullptr;
+  }
+  return e;
+}
+
+// Work enqueued on the side stream after fork() is ordered after everything
+// already on the policy's stream; the policy's stream waits for it when the
+// object goes out of scope (every return path), so a sync of the policy's
+// stream still covers all of a call's work.
+struct SideFork {
+  zsmpm_policy_s* p;
+  bool open = false;
+  explicit SideFork(zsmpm_policy_s* pol) : p(pol) {}
+  hipError_t fork() {
+    hipError_t e = hipEventRecord(p->fork_ev, p->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(p->side, p->fork_ev, 0);
+    open = e == hipSuccess;
+    return e;
+  }
+  hipError_t join() {
+    if (!open) return hipSuccess;
+    open = false;
+    hipError_t e = hipEventRecord(p->join_ev, p->side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(p->stream, p->join_ev, 0);
+    if (e != hipSuccess) (void)hipStreamSynchronize(p->side);
+    return e;
+  }
+  ~SideFork() { (void)join(); }
+};
 
 void free_all(zsmpm_policy_s* p) {
   for (void* a : p->allocs) (void)hipFree(a);
@@ -400,6 +450,16 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
   hipStream_t s = pol->stream;
   Workspace& ws = pol->ws;
   const int32_t lo[3] = {0, 0, 0}, hi[3] = {2, 2, 2};
+  // The stress pass reads F and logJp and writes ws.contrib; the sparsity
+  // stages read x only. It runs on the side stream beside them and P2G waits
+  // for both.
+  SideFork side(pol);
+  if (!chain || !chain->cached_stress) {
+    ZS_HIP(ensure_side(pol));
+    ZS_HIP(side.fork());
+    StageScope sc(pol, ZSMPM_STAGE_STRESS, pol->side);
+    ZS_HIP(zsmpm::stress(m, ps, ws, pol->side));
+  }
   {
     StageScope sc(pol, ZSMPM_STAGE_TABLE_CLEAR);
     ZS_HIP(table_clear(t, ws, s));
@@ -423,10 +483,7 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
     StageScope sc(pol, ZSMPM_STAGE_GRID_CLEAR);
     ZS_HIP(zsmpm::grid_clear(grid->data, t.cnt, grid->capacity_blocks, s));
   }
-  if (!chain || !chain->cached_stress) {
-    StageScope sc(pol, ZSMPM_STAGE_STRESS);
-    ZS_HIP(zsmpm::stress(m, ps, ws, s));
-  }
+  ZS_HIP(side.join());
   {
     // P2G folds the force into the momentum channels (the driver's
     // v = (mv + rhs)/m convention), so the update must not add channels 4..6
