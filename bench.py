@@ -10,11 +10,17 @@ zsmpm_substep_sorted__rocm: G2P also writes the particles back in bucket order
 (zpc's Vector::reorder), so every substep reads them block by block
 (--in-place: zsmpm_substep__rocm, particles never move in memory).
 
-Prints one JSON line: whole-job particle-substeps/s, the roofline of the
-dominant kernel (algorithmic bytes per launch over its mean duration, from HIP
-events recorded on the policy's stream inside the timed region), the same for
-the whole substep, and the host-core baseline (the C++/OpenMP restatement of
-the reference CPU path, timed on a bounded sample of the same workload).
+The timed substeps are replays of a captured two-substep hipGraph
+(ZSMPM_SUBSTEPS_GRAPH; --eager times the launches themselves). Before them, a
+profiled eager pass of the same number of substeps records HIP events around
+every stage on the stream it runs on; the per-stage times and the roofline of
+the dominant kernel (algorithmic bytes per launch over its mean duration) come
+from that pass.
+
+Prints one JSON line: whole-job particle-substeps/s, the dominant kernel's
+roofline, the same for the whole substep, and the host-core baseline (the
+C++/OpenMP restatement of the reference CPU path, timed on a bounded sample of
+the same workload).
 
     python bench.py [--steps K] [--warmup W] [--dims 128,256,64] [--no-cpu]
 """
@@ -124,6 +130,8 @@ def main() -> int:
                     help="keep the particles in their original order (no reorder after G2P)")
     ap.add_argument("--single", action="store_true",
                     help="one zsmpm_substep_sorted__rocm call per substep instead of one multi-substep call")
+    ap.add_argument("--eager", action="store_true",
+                    help="time the profiled eager launches instead of hipGraph replays")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (the reference kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -164,6 +172,7 @@ def main() -> int:
 
     sim = rocm.Simulator(sc, reorder=not args.in_place)
     multi = not (args.in_place or args.single)
+    graph = multi and not args.eager
     if multi:
         sim.substeps(args.warmup)
     else:
@@ -173,6 +182,8 @@ def main() -> int:
     b0 = sim.nblocks()
     print(f"# warm-up done ({args.warmup} substeps), {b0} active blocks", file=sys.stderr, flush=True)
 
+    # profiled eager pass: every stage bracketed by HIP events on the stream it
+    # runs on (per-stage and per-kernel times, the roofline below)
     sim.pol.stage_times(reset=True)
     sim.pol.profile(True)
     torch.cuda.synchronize()
@@ -187,10 +198,26 @@ def main() -> int:
     sim.sync()
     sim.pol.profile(False)
     stages = sim.pol.stage_times(reset=True)
+    eager_ms = (t1 - t0) * 1e3 / args.steps
+    print(f"# profiled eager pass: {eager_ms:.3f} ms/substep", file=sys.stderr, flush=True)
+
+    wall = t1 - t0
+    if graph:
+        # hipGraph replays of the captured two-substep pair; the capture and one
+        # replay happen before the timed region
+        sim.substeps(2, graph=True)
+        sim.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.substeps(args.steps, graph=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sim.sync()
+        wall = t1 - t0
+        print(f"# graph replays: {wall * 1e3 / args.steps:.3f} ms/substep", file=sys.stderr, flush=True)
     b1 = sim.nblocks()
     blocks = 0.5 * (b0 + b1)
 
-    wall = t1 - t0
     ms_per_step = wall * 1e3 / args.steps
     value = sc.n * args.steps / wall
 
@@ -225,10 +252,14 @@ def main() -> int:
             "path": "plain",
             "particle_order": "original (in place)" if args.in_place else
                               "bucket order, reordered inside G2P every substep",
-            "driver": "zsmpm_substeps_sorted__rocm (one call for all timed substeps)" if multi else
+            "driver": ("zsmpm_substeps_sorted__rocm with ZSMPM_SUBSTEPS_GRAPH (a captured two-substep "
+                       "hipGraph replayed)" if graph else
+                       "zsmpm_substeps_sorted__rocm (one call for all timed substeps)") if multi else
                       ("zsmpm_substep__rocm per substep" if args.in_place else
                        "zsmpm_substep_sorted__rocm per substep"),
         },
+        "eager_ms_per_step": eager_ms,
+        "stage_timing": f"HIP events around every stage of a profiled eager pass of the same {args.steps} substeps",
         "roofline": {
             "bound": "hbm",
             "kernel": dom,

@@ -223,13 +223,21 @@ int32_t zsmpm_substep_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, c
 
 /* zsmpm_substeps_sorted__rocm flags */
 #define ZSMPM_SUBSTEPS_FUSE_STRESS 1 /* G2P computes the next substep's stress (see below) */
+#define ZSMPM_SUBSTEPS_GRAPH 2        /* replay a captured hipGraph of two substeps (see below) */
 
 /* nsteps substeps of zsmpm_substep_sorted__rocm, ping-ponging a -> b -> a ...:
  * the result is in b when nsteps is odd (*final_in_b = nsteps & 1); the
  * particle state after every substep is what the single-substep calls
  * produce. With ZSMPM_SUBSTEPS_FUSE_STRESS, G2P also computes the next
  * substep's stress from the F it stores (P2G.hpp:57-100), so the separate
- * per-particle constitutive pass runs once per call. a and b must not overlap. */
+ * per-particle constitutive pass runs once per call. a and b must not overlap.
+ * With ZSMPM_SUBSTEPS_GRAPH (not combinable with FUSE_STRESS), the pair of
+ * substeps a -> b -> a is captured once into a hipGraph (zpc launches every
+ * kernel separately; graph replay has no counterpart there) and replayed
+ * nsteps / 2 times on the policy's stream; an odd
+ * remainder runs as one ordinary substep. The graph is kept on the policy and
+ * captured again when any argument or the policy's scratch changes. Stage
+ * profiling does not see graph replays. */
 int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* a,
                                     const zsmpm_particles* b, const zsmpm_table* table,
                                     const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt,

@@ -382,6 +382,35 @@ def test_fused_substeps_match_single_substeps(scene, steps, tol, request):
         check_field(f"{name} (single vs oracle)", b[name], ps.by_id(name), ps64.by_id(name), tol)
 
 
+@pytest.mark.parametrize("scene", ["cube", "sand"])
+def test_graph_substeps_replays(scene, request):
+    """ZSMPM_SUBSTEPS_GRAPH: the captured pair of substeps is replayed three
+    times with eager work (another simulator's substeps, host read-backs)
+    between the replays; the state after every replay, and after an odd call
+    (one replay plus one ordinary substep), stays on the oracle."""
+    sc = request.getfixturevalue(scene)
+    rocm = _rocm()
+    sim = rocm.Simulator(sc, reorder=True)
+    other = rocm.Simulator(sc, reorder=True)
+    ps = _oracle_particles(sc)
+    ps64 = _oracle_particles(sc).astype(np.float64)
+    params = O.make_params(sc)
+    done = 0
+    for calls, tol in ((2, TOL_1STEP), (2, 1e-4), (2, 1e-4), (3, 1e-4)):
+        sim.substeps(calls, graph=True)
+        other.substep()
+        other.sync()
+        sim.sync()
+        got = sim.parts.host_by_id()
+        O.Context().run(params, ps, calls, reorder=True)
+        O.Context(np.float64).run(params, ps64, calls, reorder=True)
+        done += calls
+        for name in ("x", "v", "C", "F", "m", "logJp"):
+            if got[name] is None:
+                continue
+            check_field(f"{name} after {done} substeps", got[name], ps.by_id(name), ps64.by_id(name), tol)
+
+
 def test_fused_substeps_drift_100(sand):
     """100 fused substeps of the sand column against the plain oracle, per id:
     x, v and F within 1e-3. logJp is not bounded here: it follows the

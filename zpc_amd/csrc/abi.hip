@@ -19,6 +19,12 @@ struct zsmpm_policy_s {
   // sparsity stages run on `stream` (created on first use, default priority)
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // captured pair of substeps (ZSMPM_SUBSTEPS_GRAPH): the capture stream, the
+  // executable graph and the bytes of the arguments it was captured with
+  hipStream_t cap = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<unsigned char> gkey;
+  uint64_t ws_epoch = 0;  // bumped whenever the scratch is reallocated
   Workspace ws{};
   std::vector<void*> allocs;
   int64_t reserved_n = 0;
@@ -197,7 +203,15 @@ struct SideFork {
   ~SideFork() { (void)join(); }
 };
 
+void destroy_graph(zsmpm_policy_s* p) {
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  p->gexec = nullptr;
+  p->gkey.clear();
+}
+
 void free_all(zsmpm_policy_s* p) {
+  destroy_graph(p);  // the graph holds scratch addresses
+  ++p->ws_epoch;
   for (void* a : p->allocs) (void)hipFree(a);
   p->allocs.clear();
   p->ws = Workspace{};
@@ -512,6 +526,67 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
   return ZSMPM_OK;
 }
 
+template <typename D>
+void key_put(std::vector<unsigned char>& k, const D* d, size_t bytes = sizeof(D)) {
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(d);
+  if (!d) {
+    k.push_back(0);
+    return;
+  }
+  k.push_back(1);
+  k.insert(k.end(), b, b + bytes);
+}
+
+// Captures the substeps a -> b and b -> a into one executable graph on the
+// policy's capture stream, unless the graph kept on the policy was captured
+// with the same arguments and scratch. Profiling is off while capturing.
+int32_t capture_pair(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_particles* a,
+                     const zsmpm_particles* b, const zsmpm_table* table, const zsmpm_buckets* buckets,
+                     const zsmpm_grid* grid, float dt, const float extf[3], const zsmpm_plane_collider* colliders,
+                     int32_t ncolliders) {
+  if (!model || !table || !grid || !extf || ncolliders < 0 || ncolliders > kMaxColliders)
+    return ZSMPM_INVALID_ARGUMENT;
+  std::vector<unsigned char> key;
+  key_put(key, model);
+  key_put(key, a);
+  key_put(key, b);
+  key_put(key, table);
+  key_put(key, buckets);
+  key_put(key, grid);
+  key_put(key, &dt);
+  key_put(key, extf, 3 * sizeof(float));
+  key_put(key, &ncolliders);
+  if (ncolliders > 0) key_put(key, colliders, (size_t)ncolliders * sizeof(zsmpm_plane_collider));
+  key_put(key, &pol->ws_epoch);
+  if (pol->gexec && key == pol->gkey) return ZSMPM_OK;
+  destroy_graph(pol);
+  ZS_HIP(ensure_side(pol));
+  if (!pol->cap) ZS_HIP(hipStreamCreateWithFlags(&pol->cap, hipStreamNonBlocking));
+  const hipStream_t saved = pol->stream;
+  const bool prof = pol->profile;
+  pol->stream = pol->cap;
+  pol->profile = false;
+  int32_t r = from_hip(hipStreamBeginCapture(pol->cap, hipStreamCaptureModeThreadLocal));
+  const bool begun = r == ZSMPM_OK;
+  if (r == ZSMPM_OK) r = run_substep(pol, model, a, b, table, buckets, grid, dt, extf, colliders, ncolliders);
+  if (r == ZSMPM_OK) r = run_substep(pol, model, b, a, table, buckets, grid, dt, extf, colliders, ncolliders);
+  hipGraph_t g = nullptr;
+  if (begun) {
+    const hipError_t e = hipStreamEndCapture(pol->cap, &g);
+    if (r == ZSMPM_OK) r = from_hip(e);
+  }
+  pol->stream = saved;
+  pol->profile = prof;
+  if (r == ZSMPM_OK && g) r = from_hip(hipGraphInstantiate(&pol->gexec, g, nullptr, nullptr, 0));
+  if (g) (void)hipGraphDestroy(g);
+  if (r != ZSMPM_OK) {
+    pol->gexec = nullptr;
+    return r;
+  }
+  pol->gkey.swap(key);
+  return ZSMPM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -547,6 +622,7 @@ int32_t zsmpm_del_policy__rocm(zsmpm_policy pol) {
     (void)hipStreamSynchronize(pol->stream);
     destroy_events(pol);
     free_all(pol);
+    if (pol->cap) (void)hipStreamDestroy(pol->cap);
   }
   delete pol;
   return ZSMPM_OK;
@@ -763,8 +839,11 @@ int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, 
                                     const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt,
                                     const float extf[3], const zsmpm_plane_collider* colliders, int32_t ncolliders,
                                     int32_t nsteps, int32_t flags, int32_t* final_in_b) {
-  if (!pol || !a || !b || nsteps < 0 || (flags & ~ZSMPM_SUBSTEPS_FUSE_STRESS)) return ZSMPM_INVALID_ARGUMENT;
+  if (!pol || !a || !b || nsteps < 0 || (flags & ~(ZSMPM_SUBSTEPS_FUSE_STRESS | ZSMPM_SUBSTEPS_GRAPH)))
+    return ZSMPM_INVALID_ARGUMENT;
   const bool fuse = (flags & ZSMPM_SUBSTEPS_FUSE_STRESS) != 0;
+  const bool graph = (flags & ZSMPM_SUBSTEPS_GRAPH) != 0;
+  if (fuse && graph) return ZSMPM_INVALID_ARGUMENT;
   if (final_in_b) *final_in_b = nsteps & 1;
   if (nsteps == 0) return ZSMPM_OK;
   {
@@ -775,6 +854,15 @@ int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, 
     ZS_TRY(check_out_particles(pa, pb));
     DeviceGuard g(pol->devid);
     ZS_TRY(ensure(pol, pa.n, table ? table->table_size : 0));
+  }
+  if (graph) {
+    DeviceGuard g(pol->devid);
+    if (nsteps >= 2) {
+      ZS_TRY(capture_pair(pol, model, a, b, table, buckets, grid, dt, extf, colliders, ncolliders));
+      for (int32_t k = 0; k + 1 < nsteps; k += 2) ZS_HIP(hipGraphLaunch(pol->gexec, pol->stream));
+    }
+    if (nsteps & 1) ZS_TRY(run_substep(pol, model, a, b, table, buckets, grid, dt, extf, colliders, ncolliders));
+    return ZSMPM_OK;
   }
   for (int32_t k = 0; k < nsteps; ++k) {
     Chain c;

@@ -357,9 +357,10 @@ class Simulator:
                                          C.byref(self.grid.desc), dt, C.byref(self.extf), self.cols, self.ncols),
               "substep")
 
-    def substeps(self, n: int, dt: Optional[float] = None, fuse_stress: bool = False):
+    def substeps(self, n: int, dt: Optional[float] = None, fuse_stress: bool = False, graph: bool = False):
         """n substeps in one call (sorted driver); fuse_stress: G2P computes the
-        next substep's stress."""
+        next substep's stress; graph: replay a captured hipGraph of two
+        substeps (kept by the library while the arguments stay the same)."""
         if not self.reorder:
             for _ in range(n):
                 self.substep(dt)
@@ -370,7 +371,8 @@ class Simulator:
                                                  C.byref(self.spare.desc), C.byref(self.table.desc),
                                                  C.byref(self.buckets.desc), C.byref(self.grid.desc), dt,
                                                  C.byref(self.extf), self.cols, self.ncols, int(n),
-                                                 1 if fuse_stress else 0, C.byref(in_b)),
+                                                 (1 if fuse_stress else 0) | (2 if graph else 0),
+                                                 C.byref(in_b)),
               "substeps_sorted")
         if in_b.value:
             self.parts, self.spare = self.spare, self.parts
