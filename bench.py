@@ -117,6 +117,119 @@ def cpu_baseline(sc, warmup: int, steps: int) -> dict:
     }
 
 
+def x_by_id(torch, sim):
+    """Positions in original particle order (ids carried by the sorted driver)."""
+    x = sim.parts.x
+    if sim.parts.id is None:
+        return x.clone()
+    out = torch.empty_like(x)
+    out[sim.parts.id.long()] = x
+    return out
+
+
+def run_path(torch, rocm, sc, args, multi, graph, closure=None):
+    """Warm-up, a profiled eager pass and (graph) the timed replays of one
+    simulator; closure: (params or None) selects the learned closure."""
+    sim = rocm.Simulator(sc, reorder=not args.in_place)
+    if closure is not None:
+        sim.pol.set_closure(closure[0], tol=closure[1], always_network=closure[2])
+    if multi:
+        sim.substeps(args.warmup)
+    else:
+        for _ in range(args.warmup):
+            sim.substep()
+    sim.sync()
+    b0 = sim.nblocks()
+    print(f"# warm-up done ({args.warmup} substeps), {b0} active blocks", file=sys.stderr, flush=True)
+    fb0 = int(sim.pol.counters()[4])
+    # profiled eager pass: every stage bracketed by HIP events on the stream it
+    # runs on (per-stage and per-kernel times, the roofline)
+    sim.pol.stage_times(reset=True)
+    sim.pol.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if multi:
+        sim.substeps(args.steps)
+    else:
+        for _ in range(args.steps):
+            sim.substep()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sim.sync()
+    sim.pol.profile(False)
+    stages = sim.pol.stage_times(reset=True)
+    eager_ms = (t1 - t0) * 1e3 / args.steps
+    print(f"# profiled eager pass: {eager_ms:.3f} ms/substep", file=sys.stderr, flush=True)
+    wall = t1 - t0
+    if graph:
+        # hipGraph replays of the captured two-substep pair; the capture and one
+        # replay happen before the timed region
+        sim.substeps(2, graph=True)
+        sim.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.substeps(args.steps, graph=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sim.sync()
+        wall = t1 - t0
+        print(f"# graph replays: {wall * 1e3 / args.steps:.3f} ms/substep", file=sys.stderr, flush=True)
+    b1 = sim.nblocks()
+    substeps_run = args.warmup + args.steps + (2 + args.steps if graph else 0)
+    out = {"stages": stages, "blocks": 0.5 * (b0 + b1), "ms": wall * 1e3 / args.steps, "eager_ms": eager_ms,
+           "fallbacks": int(sim.pol.counters()[4]) - fb0, "substeps": substeps_run, "x": x_by_id(torch, sim)}
+    del sim
+    return out
+
+
+def surrogate_leg(torch, rocm, sc, args, graph, plain):
+    """The same run with the learned closure: the network is trained on the
+    GPU (outside the timed regions) on states of the scene after a few plain
+    substeps plus random large-strain states; positions are compared with the
+    plain run particle by particle after the same substeps."""
+    from zpc_amd import closure as CL
+
+    t0 = time.perf_counter()
+    F, lj = CL.training_states(sc, sim_steps=args.warmup, synthetic=1 << 18, seed=0)
+    keep = torch.randperm(F.shape[0], device=F.device)[: 1 << 20]
+    model = rocm.model_desc(sc.material)
+    pol = rocm.Policy()
+    tr = CL.train(pol, model, F[keep].contiguous(), lj[keep].contiguous(), steps=args.train_steps, batch=8192,
+                  lr=3e-3, seed=0)
+    del F, lj, keep
+    prep = time.perf_counter() - t0
+    print(f"# closure trained: {args.train_steps} Adam steps in {tr.seconds:.2f} s, loss {tr.losses[0]:.3e} -> "
+          f"{tr.losses[-1]:.3e}", file=sys.stderr, flush=True)
+    res = {}
+    for name, params, always in (("trained", tr.params, False), ("trained_network_every_wave", tr.params, True),
+                                 ("zero_start", None, False)):
+        r = run_path(torch, rocm, sc, args, True, graph, closure=(params, 1e-4, always))
+        dx_err = (r["x"] - plain["x"]).abs().max().item() / sc.dx
+        n_pass = r["substeps"] - args.warmup
+        per_launch = {k: (ms / c if c else 0.0) for k, (ms, c) in r["stages"].items()}
+        res[name] = {
+            "value": sc.n / (r["ms"] * 1e-3),
+            "ms_per_step": r["ms"],
+            "eager_ms_per_step": r["eager_ms"],
+            "speedup_vs_plain": plain["ms"] / r["ms"],
+            "stress_ms_per_launch": per_launch.get("stress", 0.0),
+            "plain_stress_ms_per_launch": plain["stages"]["stress"][0] / max(plain["stages"]["stress"][1], 1),
+            "fallback_fraction": r["fallbacks"] / (sc.n * n_pass),
+            "max_position_deviation_from_plain_dx": dx_err,
+            "substeps_compared": r["substeps"],
+        }
+        print(f"# surrogate ({name}): {r['ms']:.3f} ms/substep, fallback {res[name]['fallback_fraction']:.2e}, "
+              f"max |x - x_plain| = {dx_err:.2e} dx", file=sys.stderr, flush=True)
+        del r
+    res["network"] = ("MLP 9-64-64-7 GELU, BF16 MFMA / FP32 accumulate, trained on this GPU; 'trained' evaluates "
+                      "it only in waves that one Newton step from X0 = 0 cannot certify a priori, "
+                      "'trained_network_every_wave' in every wave, 'zero_start' never")
+    res["training"] = {"adam_steps": args.train_steps, "batch": 8192, "samples": tr.samples,
+                       "seconds": tr.seconds, "loss_first": tr.losses[0], "loss_last": tr.losses[-1],
+                       "data_seconds": prep - tr.seconds}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +245,8 @@ def main() -> int:
                     help="one zsmpm_substep_sorted__rocm call per substep instead of one multi-substep call")
     ap.add_argument("--eager", action="store_true",
                     help="time the profiled eager launches instead of hipGraph replays")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-closure leg")
+    ap.add_argument("--train-steps", type=int, default=2000, help="Adam steps of the closure network")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (the reference kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -170,57 +285,15 @@ def main() -> int:
         print(f"# copy kernel: {4 * words} bytes read, {4 * words} bytes written", file=sys.stderr, flush=True)
         del src, dst
 
-    sim = rocm.Simulator(sc, reorder=not args.in_place)
     multi = not (args.in_place or args.single)
     graph = multi and not args.eager
-    if multi:
-        sim.substeps(args.warmup)
-    else:
-        for _ in range(args.warmup):
-            sim.substep()
-    sim.sync()
-    b0 = sim.nblocks()
-    print(f"# warm-up done ({args.warmup} substeps), {b0} active blocks", file=sys.stderr, flush=True)
-
-    # profiled eager pass: every stage bracketed by HIP events on the stream it
-    # runs on (per-stage and per-kernel times, the roofline below)
-    sim.pol.stage_times(reset=True)
-    sim.pol.profile(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    if multi:
-        sim.substeps(args.steps)
-    else:
-        for _ in range(args.steps):
-            sim.substep()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    sim.sync()
-    sim.pol.profile(False)
-    stages = sim.pol.stage_times(reset=True)
-    eager_ms = (t1 - t0) * 1e3 / args.steps
-    print(f"# profiled eager pass: {eager_ms:.3f} ms/substep", file=sys.stderr, flush=True)
-
-    wall = t1 - t0
-    if graph:
-        # hipGraph replays of the captured two-substep pair; the capture and one
-        # replay happen before the timed region
-        sim.substeps(2, graph=True)
-        sim.sync()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sim.substeps(args.steps, graph=True)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        sim.sync()
-        wall = t1 - t0
-        print(f"# graph replays: {wall * 1e3 / args.steps:.3f} ms/substep", file=sys.stderr, flush=True)
-    b1 = sim.nblocks()
-    blocks = 0.5 * (b0 + b1)
-
-    ms_per_step = wall * 1e3 / args.steps
-    value = sc.n * args.steps / wall
-
+    plain = run_path(torch, rocm, sc, args, multi, graph)
+    stages, blocks, ms_per_step, eager_ms = plain["stages"], plain["blocks"], plain["ms"], plain["eager_ms"]
+    value = sc.n / (ms_per_step * 1e-3)
+    surrogate = None
+    if not args.no_surrogate and multi:
+        surrogate = surrogate_leg(torch, rocm, sc, args, graph, plain)
+    x_plain = plain.pop("x")
     per_launch = {k: (ms / calls if calls else 0.0) for k, (ms, calls) in stages.items()}
     dom = max(per_launch, key=lambda k: stages[k][0])
     dom_bytes = stage_bytes(dom, sc.n, blocks, plastic, sc.table_size)
@@ -279,9 +352,10 @@ def main() -> int:
             "frac": sub_gbs / HBM_PEAK_GBS,
         },
         "stage_ms_per_launch": per_launch,
+        "surrogate": surrogate,
         "cpu_baseline": None,
     }
-    del sim
+    del x_plain
     torch.cuda.synchronize()
     if not args.no_cpu:
         print(f"# cpu baseline: {args.cpu_warmup} + {args.cpu_steps} substeps on {cpu_threads()} threads",

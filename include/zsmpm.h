@@ -244,6 +244,81 @@ int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, 
                                     const float extf[3], const zsmpm_plane_collider* colliders, int32_t ncolliders,
                                     int32_t nsteps, int32_t flags, int32_t* final_in_b);
 
+/* ------------------------------------------------------------------------
+ * Learned stress closure (an addition: zpc has no learned constitutive path).
+ * For Drucker-Prager particles it replaces the SVD of the return mapping
+ * (physics/ConstitutiveModel_Vol_dP.hpp:241-325, called from
+ * simulation/transfer/P2G.hpp:84-100): with A = C - I, C = F^T F, n = |A|_F
+ * and U = A / n, an MLP 9 -> 64 -> 64 -> 7 (GELU, bfloat16 operands on MFMA,
+ * float accumulation) predicts E = log(C)/2 as n (psi0 I + psi1 U + psi2 U^2)
+ * (and the projected strain and plastic multiplier, outputs 3..6); one Newton
+ * step with series exp/log refines X = log C; the return mapping is applied to
+ * E in tensor form and vol P F_proj^T = vol F S F^T, S = tau C^-1. Particles
+ * whose a-posteriori residual bound exceeds tol (or with |D| >= 1/2,
+ * |log C| > 1, det F <= 0) take the SVD path; their count accumulates in
+ * counter 4 (zsmpm_policy_counters).
+ *
+ * Parameters (float32, ZSMPM_MLP_PARAMS): W1[9][64], b1[64], W2[64][64],
+ * b2[64], W3[64][7], b3[7] (W[in][out]). Inputs per sample (9): tr U, tr U^3,
+ * det U, log(n)/8 + 1, min(n, 1), clamp(logJp/n, -8, 8)/8, the friction
+ * coefficient k = yield_surface (3 lambda + 2 mu) / (2 mu),
+ * clamp(cohesion/n, -8, 8)/8, lambda / (lambda + 2 mu). Training records (10):
+ * the eigenvalues m_i of U, e_i / n, lp_i / n (principal Hencky and projected
+ * strains) and clamp(dg/n, -8, 8)/8. The loss per sample is
+ * sum_i (psi0 + psi1 m_i + psi2 m_i^2 - e_i/n)^2 + the same for the projected
+ * strain + (y6 - dg feature)^2; a pass minimises its mean over the samples.
+ * ---------------------------------------------------------------------- */
+#define ZSMPM_MLP_INPUTS 9
+#define ZSMPM_MLP_HIDDEN 64
+#define ZSMPM_MLP_OUTPUTS 7
+#define ZSMPM_MLP_RECORD 10
+#define ZSMPM_MLP_PARAMS 5255
+
+#define ZSMPM_CLOSURE_ALWAYS_NETWORK 1 /* evaluate the network in every wave (see below) */
+
+typedef struct zsmpm_closure {
+  zsmpm_abi_header hdr;
+  const float* params; /* [ZSMPM_MLP_PARAMS] network, or NULL: Newton from X0 = 0 */
+  float tol;           /* relative residual bound above which the SVD path runs (1e-4) */
+  int32_t flags;       /* ZSMPM_CLOSURE_*; by default a wave of 64 particles skips the
+                          network when one step from X0 = 0 certifies all of them a priori
+                          (|C - I|^3 / (4 (1 - |C - I|) (1 - |C - I|/2)) <= tol / 2) */
+} zsmpm_closure;
+
+/* Selects the closure of every later stress pass of Drucker-Prager models on
+ * this policy (the plain SVD return mapping when closure is NULL). The weights
+ * are read by each pass, not copied. Not combinable with
+ * ZSMPM_SUBSTEPS_FUSE_STRESS (ZSMPM_UNSUPPORTED_OPERATION). */
+int32_t zsmpm_policy_closure__rocm(zsmpm_policy pol, const zsmpm_closure* closure);
+
+/* The stress pass on plain arrays: PF[9 i] = vol P F^T of F[9 i] (column-major)
+ * with the policy's closure; logJp is updated (Drucker-Prager). The per-
+ * particle part of P2G (P2G.hpp:57-100; compute_stress_* of
+ * physics/ConstitutiveModel_Vol_dP.hpp:10-46, 241-325). */
+int32_t zsmpm_stress__rocm(zsmpm_policy pol, const zsmpm_model* model, const float* F, float* logJp, int64_t n,
+                           float* PF);
+
+/* Training data from the plain path: features[9 i] and records[10 i] of
+ * (F[9 i], logJp[i]), the targets from the device SVD return mapping. */
+int32_t zsmpm_mlp_harvest__rocm(zsmpm_policy pol, const zsmpm_model* model, const float* F, const float* logJp,
+                                int64_t n, float* features, float* records);
+
+/* outputs[7 i] = MLP(features[9 i]). */
+int32_t zsmpm_mlp_infer__rocm(zsmpm_policy pol, const float* params, const float* features, int64_t n,
+                              float* outputs);
+
+/* Gradient of the mean loss over the n samples (grad: ZSMPM_MLP_PARAMS floats)
+ * and the loss (1 float, may be NULL), both device pointers. */
+int32_t zsmpm_mlp_grad__rocm(zsmpm_policy pol, const float* params, const float* features, const float* records,
+                             int64_t n, float* grad, float* loss);
+
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8; step counts from 1) on the batch:
+ * params, adam_m and adam_v (ZSMPM_MLP_PARAMS floats each) are updated in
+ * place; loss (device, may be NULL) receives the batch loss before the step. */
+int32_t zsmpm_mlp_train_step__rocm(zsmpm_policy pol, float* params, float* adam_m, float* adam_v,
+                                   const float* features, const float* records, int64_t n, float lr, int32_t step,
+                                   float* loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -224,6 +224,18 @@ def stress_sand(vol, mu, lam, coh, beta, ys, volcorr, logJp, F):
     return PF, lj, Fp, which
 
 
+def stress_sand64(vol, mu, lam, coh, beta, ys, volcorr, logJp, F):
+    """The float64 twin of stress_sand (Jacobi SVD)."""
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(-1, 9)
+    lj = np.ascontiguousarray(logJp, dtype=np.float64).copy()
+    PF = np.zeros_like(F)
+    Fp = np.zeros_like(F)
+    which = np.zeros(F.shape[0], np.int32)
+    lib().zso_stress_sand_f64(F.shape[0], vol, mu, lam, coh, beta, ys, int(volcorr), lj, F, PF, _ptr(Fp),
+                              _ptr(which))
+    return PF, lj, Fp, which
+
+
 def arena(dx, x):
     x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
     n = x.shape[0]

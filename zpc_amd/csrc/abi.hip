@@ -2,6 +2,7 @@
 // their scratch workspace, descriptor validation and the stage entry points.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <math.h>
 #include <string.h>
 
 #include <new>
@@ -25,6 +26,11 @@ struct zsmpm_policy_s {
   hipGraphExec_t gexec = nullptr;
   std::vector<unsigned char> gkey;
   uint64_t ws_epoch = 0;  // bumped whenever the scratch is reallocated
+  // learned closure of the stress pass (zsmpm_policy_closure__rocm)
+  bool closure_on = false;
+  ClosureView closure{};
+  float* mlp_partial = nullptr;  // [kMlpGroups * kMlpParams] gradient partials
+  float* mlp_loss = nullptr;     // [kMlpGroups]
   Workspace ws{};
   std::vector<void*> allocs;
   int64_t reserved_n = 0;
@@ -215,6 +221,7 @@ void free_all(zsmpm_policy_s* p) {
   for (void* a : p->allocs) (void)hipFree(a);
   p->allocs.clear();
   p->ws = Workspace{};
+  p->mlp_partial = p->mlp_loss = nullptr;
   p->own_counts = p->own_offsets = p->own_indices = nullptr;
   p->own_cap_blocks = 0;
   p->reserved_n = 0;
@@ -263,6 +270,7 @@ int32_t reserve(zsmpm_policy_s* p, int64_t n, int32_t T) {
   ZA(w.err, 4);
   ZA(w.contrib, (size_t)9 * w.n_cap);
   ZA(w.nbr, (size_t)8 * w.ts_cap);
+  ZA(w.fb, w.n_cap);
   ZA(w.lj_next[0], w.n_cap);
   ZA(w.lj_next[1], w.n_cap);
   {
@@ -472,7 +480,7 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
     ZS_HIP(ensure_side(pol));
     ZS_HIP(side.fork());
     StageScope sc(pol, ZSMPM_STAGE_STRESS, pol->side);
-    ZS_HIP(zsmpm::stress(m, ps, ws, pol->side));
+    ZS_HIP(zsmpm::stress(m, ps, ws, pol->side, pol->closure_on ? &pol->closure : nullptr));
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_TABLE_CLEAR);
@@ -558,6 +566,8 @@ int32_t capture_pair(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_
   key_put(key, &ncolliders);
   if (ncolliders > 0) key_put(key, colliders, (size_t)ncolliders * sizeof(zsmpm_plane_collider));
   key_put(key, &pol->ws_epoch);
+  key_put(key, &pol->closure_on);
+  key_put(key, &pol->closure);
   if (pol->gexec && key == pol->gkey) return ZSMPM_OK;
   destroy_graph(pol);
   ZS_HIP(ensure_side(pol));
@@ -768,7 +778,7 @@ int32_t zsmpm_p2g__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_STRESS);
-    ZS_HIP(zsmpm::stress(m, ps, pol->ws, pol->stream));
+    ZS_HIP(zsmpm::stress(m, ps, pol->ws, pol->stream, pol->closure_on ? &pol->closure : nullptr));
   }
   StageScope sc(pol, ZSMPM_STAGE_P2G);
   ZS_HIP(block_neighbours(t, pol->ws, pol->stream));
@@ -844,6 +854,7 @@ int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, 
   const bool fuse = (flags & ZSMPM_SUBSTEPS_FUSE_STRESS) != 0;
   const bool graph = (flags & ZSMPM_SUBSTEPS_GRAPH) != 0;
   if (fuse && graph) return ZSMPM_INVALID_ARGUMENT;
+  if (fuse && pol->closure_on) return ZSMPM_UNSUPPORTED_OPERATION;  // G2P's fused stress is the SVD path
   if (final_in_b) *final_in_b = nsteps & 1;
   if (nsteps == 0) return ZSMPM_OK;
   {
@@ -891,6 +902,92 @@ int32_t zsmpm_reorder__rocm(zsmpm_policy pol, const zsmpm_buckets* buckets, cons
   DeviceGuard g(pol->devid);
   ZS_TRY(ensure(pol, pi.n, pol->reserved_T));
   return from_hip(reorder_particles(pi, po, sv.indices, pol->ws, pol->stream));
+}
+
+int32_t zsmpm_policy_closure__rocm(zsmpm_policy pol, const zsmpm_closure* closure) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
+  if (!closure) {
+    pol->closure_on = false;
+    pol->closure = ClosureView{};
+    return ZSMPM_OK;
+  }
+  ZS_TRY(check_header(closure));
+  if (!(closure->tol > 0.f)) return ZSMPM_INVALID_ARGUMENT;
+  if (closure->flags & ~ZSMPM_CLOSURE_ALWAYS_NETWORK) return ZSMPM_INVALID_ARGUMENT;
+  pol->closure.params = closure->params;
+  pol->closure.tol = closure->tol;
+  pol->closure.always_net = (closure->flags & ZSMPM_CLOSURE_ALWAYS_NETWORK) ? 1 : 0;
+  pol->closure_on = true;
+  return ZSMPM_OK;
+}
+
+int32_t zsmpm_stress__rocm(zsmpm_policy pol, const zsmpm_model* model, const float* F, float* logJp, int64_t n,
+                           float* PF) {
+  if (!pol || n < 0 || (n > 0 && (!F || !PF))) return ZSMPM_INVALID_ARGUMENT;
+  Model m;
+  ZS_TRY(make_model(model, m));
+  if (m.kind == ZSMPM_DRUCKER_PRAGER && n > 0 && !logJp) return ZSMPM_INVALID_ARGUMENT;
+  if (n == 0) return ZSMPM_OK;
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, n, pol->reserved_T));
+  ParticlesView ps;
+  ps.n = n;
+  ps.F = const_cast<float*>(F);
+  ps.logJp = logJp;
+  StageScope sc(pol, ZSMPM_STAGE_STRESS);
+  return from_hip(zsmpm::stress(m, ps, pol->ws, pol->stream, pol->closure_on ? &pol->closure : nullptr, PF));
+}
+
+int32_t zsmpm_mlp_harvest__rocm(zsmpm_policy pol, const zsmpm_model* model, const float* F, const float* logJp,
+                                int64_t n, float* features, float* records) {
+  if (!pol || n < 0 || (n > 0 && (!F || !features || !records))) return ZSMPM_INVALID_ARGUMENT;
+  Model m;
+  ZS_TRY(make_model(model, m));
+  if (m.kind != ZSMPM_DRUCKER_PRAGER || m.mu == 0.f) return ZSMPM_UNSUPPORTED_OPERATION;
+  DeviceGuard g(pol->devid);
+  return from_hip(mlp_harvest(m, F, logJp, n, features, records, pol->stream));
+}
+
+int32_t zsmpm_mlp_infer__rocm(zsmpm_policy pol, const float* params, const float* features, int64_t n,
+                              float* outputs) {
+  if (!pol || n < 0 || (n > 0 && (!params || !features || !outputs))) return ZSMPM_INVALID_ARGUMENT;
+  DeviceGuard g(pol->devid);
+  return from_hip(mlp_infer(params, features, n, outputs, pol->stream));
+}
+
+namespace {
+int32_t mlp_pass(zsmpm_policy_s* pol, float* params, float* adam_m, float* adam_v, const float* features,
+                 const float* records, int64_t n, float lr, int32_t step, float* grad, float* loss) {
+  if (!pol || n <= 0 || !params || !features || !records) return ZSMPM_INVALID_ARGUMENT;
+  DeviceGuard g(pol->devid);
+  if (!pol->mlp_partial) {
+    ZS_HIP(alloc(pol, &pol->mlp_partial, (size_t)kMlpGroups * kMlpParams));
+    ZS_HIP(alloc(pol, &pol->mlp_loss, (size_t)kMlpGroups));
+  }
+  const int64_t tiles = (n + 15) / 16;
+  const int G = (int)(tiles < kMlpGroups ? tiles : kMlpGroups);
+  ZS_HIP(mlp_grad(params, features, records, n, pol->mlp_partial, pol->mlp_loss, G, pol->stream));
+  float bc1 = 1.f, bc2 = 1.f;
+  if (adam_m) {
+    bc1 = (float)(1.0 - pow(0.9, (double)step));
+    bc2 = (float)(1.0 - pow(0.999, (double)step));
+  }
+  return from_hip(mlp_reduce(adam_m ? params : nullptr, adam_m, adam_v, pol->mlp_partial, pol->mlp_loss, G, lr, bc1,
+                             bc2, grad, loss, pol->stream));
+}
+}  // namespace
+
+int32_t zsmpm_mlp_grad__rocm(zsmpm_policy pol, const float* params, const float* features, const float* records,
+                             int64_t n, float* grad, float* loss) {
+  if (!grad) return ZSMPM_INVALID_ARGUMENT;
+  return mlp_pass(pol, const_cast<float*>(params), nullptr, nullptr, features, records, n, 0.f, 0, grad, loss);
+}
+
+int32_t zsmpm_mlp_train_step__rocm(zsmpm_policy pol, float* params, float* adam_m, float* adam_v,
+                                   const float* features, const float* records, int64_t n, float lr, int32_t step,
+                                   float* loss) {
+  if (!adam_m || !adam_v || step < 1 || !(lr > 0.f)) return ZSMPM_INVALID_ARGUMENT;
+  return mlp_pass(pol, params, adam_m, adam_v, features, records, n, lr, step, nullptr, loss);
 }
 
 }  // extern "C"

@@ -919,7 +919,7 @@ hipError_t check_launch_limits(int device) {
     if (e != hipSuccess) return e;
     if ((int)attr.sharedSizeBytes > max_lds) return hipErrorInvalidConfiguration;
   }
-  return hipSuccess;
+  return closure_launch_limits();
 }
 
 hipError_t grid_clear(float* grid, const int32_t* nblocks_dev, int32_t cap_blocks, hipStream_t s) {
@@ -928,14 +928,17 @@ hipError_t grid_clear(float* grid, const int32_t* nblocks_dev, int32_t cap_block
   return hipGetLastError();
 }
 
-hipError_t stress(const Model& m, const ParticlesView& ps, Workspace& ws, hipStream_t s) {
+hipError_t stress(const Model& m, const ParticlesView& ps, Workspace& ws, hipStream_t s, const ClosureView* closure,
+                  float* contrib) {
   if (ps.n <= 0) return hipSuccess;
+  float* out = contrib ? contrib : ws.contrib;
+  if (closure && m.kind == 1 && m.mu != 0.f) return stress_closure(m, *closure, ps, out, ws, s);
   const int64_t waves = (ps.n + kWave - 1) / kWave;
   const int g = groups_for(waves * kWave, kThreads);
   if (m.kind == 0)
-    stress_kernel<0><<<g, kThreads, 0, s>>>(m, ps, ws.contrib, ws.counters);
+    stress_kernel<0><<<g, kThreads, 0, s>>>(m, ps, out, ws.counters);
   else
-    stress_kernel<1><<<g, kThreads, 0, s>>>(m, ps, ws.contrib, ws.counters);
+    stress_kernel<1><<<g, kThreads, 0, s>>>(m, ps, out, ws.counters);
   return hipGetLastError();
 }
 

@@ -60,7 +60,22 @@ enum : int {
   kCtrFallback = 4, // particles that took the SVD fallback of the learned closure
   kCtrMaxVel = 5,   // max |v|^2 as float bits
   kCtrQueue = 6,    // next bucket of the running per-bucket launch
+  kCtrFbQueue = 7,  // length of the closure's SVD fallback queue (per pass)
   kCtrCount = 16
+};
+
+// learned closure network: 9 inputs, 64 x 64 hidden (GELU), 7 outputs;
+// training records of 10 floats (eigen-space targets)
+constexpr int kMlpIn = 9, kMlpOut = 7, kMlpRec = 10;
+constexpr int kMlpParams = 9 * 64 + 64 + 64 * 64 + 64 + 64 * 7 + 7;  // 5255
+constexpr int kMlpGroups = 512;  // workgroups (and gradient partials) of one training pass
+
+// The closure a stress pass uses for Drucker-Prager particles: the network
+// weights, or null for Newton from X0 = 0, and the residual tolerance.
+struct ClosureView {
+  const float* params = nullptr;
+  float tol = 1e-4f;
+  int32_t always_net = 0;  // evaluate the network even where X0 = 0 certifies a priori
 };
 
 /// Scratch device memory owned by a policy (the get_temporary_memory_source
@@ -93,6 +108,7 @@ struct Workspace {
   uint32_t* err = nullptr;       // [1] error bits
   float* contrib = nullptr;      // [9 n_cap] vol P F^T of each particle (column-major)
   int32_t* nbr = nullptr;        // [8 ts_cap] block index of key_b + (d>>2&1, d>>1&1, d&1), or -1
+  int32_t* fb = nullptr;         // [n_cap] particles queued for the closure's SVD fallback
   float* lj_next[2] = {nullptr, nullptr};  // [n_cap] each: logJp caches of the fused multi-substep driver
   uint32_t* bitmap = nullptr;    // [bm_words] one bit per insertion order: the orders of new keys
   uint32_t* bm_prefix = nullptr; // [bm_words] exclusive prefix of the bit counts
@@ -164,7 +180,21 @@ hipError_t block_neighbours(const TableView& t, Workspace& ws, hipStream_t s);
 hipError_t check_launch_limits(int device);
 hipError_t grid_clear(float* grid, const int32_t* nblocks_dev, int32_t cap_blocks, hipStream_t s);
 // P2G = stress (per particle, writes ws.contrib and logJp) + scatter (per bucket)
-hipError_t stress(const Model& m, const ParticlesView& ps, Workspace& ws, hipStream_t s);
+hipError_t stress(const Model& m, const ParticlesView& ps, Workspace& ws, hipStream_t s,
+                  const ClosureView* closure = nullptr, float* contrib = nullptr);
+
+// ---------------------------------------------------------------- closure
+// stress pass of the learned closure into contrib (SVD fallback queued)
+hipError_t stress_closure(const Model& m, const ClosureView& cl, const ParticlesView& ps, float* contrib,
+                          Workspace& ws, hipStream_t s);
+hipError_t mlp_infer(const float* P, const float* X, int64_t n, float* Y, hipStream_t s);
+hipError_t mlp_grad(const float* P, const float* X, const float* R, int64_t n, float* partial, float* loss_part,
+                    int G, hipStream_t s);
+hipError_t mlp_reduce(float* P, float* m, float* v, const float* partial, const float* loss_part, int G, float lr,
+                      float bc1, float bc2, float* grad_out, float* loss_out, hipStream_t s);
+hipError_t mlp_harvest(const Model& m, const float* F, const float* logJp, int64_t n, float* X, float* R,
+                       hipStream_t s);
+hipError_t closure_launch_limits();
 hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
                        int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s);
 hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float dt, const float extf[3],

@@ -68,6 +68,14 @@ class ModelDesc(C.Structure):
                 ("volume_correction", C.c_int32), ("svd_fp64_gap", C.c_float)]
 
 
+class ClosureDesc(C.Structure):
+    _fields_ = [("hdr", Header), ("params", C.c_void_p), ("tol", C.c_float), ("flags", C.c_int32)]
+
+
+# learned closure network sizes (include/zsmpm.h)
+MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_RECORD, MLP_PARAMS = 9, 64, 7, 10, 5255
+
+
 class PlaneCollider(C.Structure):
     _fields_ = [("type", C.c_int32), ("origin", C.c_float * 3), ("normal", C.c_float * 3)]
 
@@ -116,6 +124,12 @@ def lib():
         "zsmpm_substeps_sorted__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(ParticlesDesc), P(TableDesc),
                                         P(BucketsDesc), P(GridDesc), f32, P(C.c_float * 3), P(PlaneCollider), i32,
                                         i32, i32, P(C.c_int32)],
+        "zsmpm_policy_closure__rocm": [vp, P(ClosureDesc)],
+        "zsmpm_stress__rocm": [vp, P(ModelDesc), vp, vp, i64, vp],
+        "zsmpm_mlp_harvest__rocm": [vp, P(ModelDesc), vp, vp, i64, vp, vp],
+        "zsmpm_mlp_infer__rocm": [vp, vp, vp, i64, vp],
+        "zsmpm_mlp_grad__rocm": [vp, vp, vp, vp, i64, vp, vp],
+        "zsmpm_mlp_train_step__rocm": [vp, vp, vp, vp, vp, vp, i64, f32, i32, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -179,6 +193,25 @@ class Policy:
     def profile(self, enable: bool = True):
         """Time every stage with HIP events on the policy's stream."""
         check(self.L.zsmpm_policy_profile(self.h, 1 if enable else 0), "profile")
+
+    def set_closure(self, params=None, tol: float = 1e-4, enable: bool = True, always_network: bool = False):
+        """Stress passes of Drucker-Prager models use the learned closure:
+        params (a device float32 tensor of MLP_PARAMS) or None for Newton from
+        X0 = 0; enable=False restores the plain SVD return mapping;
+        always_network evaluates the network even in waves that one step from
+        X0 = 0 certifies a priori. The policy reads params at every pass: keep
+        the tensor alive."""
+        if not enable:
+            check(self.L.zsmpm_policy_closure__rocm(self.h, None), "closure")
+            self._closure_params = None
+            return
+        d = ClosureDesc()
+        d.hdr = _hdr(ClosureDesc)
+        d.params = None if params is None else params.data_ptr()
+        d.tol = float(tol)
+        d.flags = 1 if always_network else 0
+        check(self.L.zsmpm_policy_closure__rocm(self.h, C.byref(d)), "closure")
+        self._closure_params = params
 
     def stage_times(self, reset: bool = False) -> dict:
         """{stage: (total ms, calls)} accumulated since the last reset."""
