@@ -230,6 +230,64 @@ def surrogate_leg(torch, rocm, sc, args, graph, plain):
     return res
 
 
+def batched_leg(torch, rocm, args) -> dict:
+    """The survey's config 4: 64 independent scenes of 10K..500K particles
+    (boxes and spheres, fixed-corotated and sand, dx 1/128 and 1/256) advanced
+    together by zsmpm_substeps_batched__rocm; the lane count is autotuned on the
+    warm-up substeps. Baseline: the same scenes one after another, one
+    multi-substep call per scene."""
+    from zpc_amd import scenes as S
+
+    import ctypes as C
+
+    scenes = S.batch_scenes(64)
+    b = rocm.Batch(scenes)
+    n = b.particles
+    b.substeps(30, lanes=0)  # warm-up, and the lane autotuning (on these substeps)
+    b.sync()
+    steps = args.batch_steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.substeps(steps, lanes=0)  # the tuned lane count (same scene set)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    b.sync()
+    ms = (t1 - t0) * 1e3 / steps
+    print(f"# batched 64 scenes, {n} particles: {ms:.3f} ms/substep with {b.lanes_used} lanes "
+          f"(autotune ms per 2 substeps: {b.tune_ms})", file=sys.stderr, flush=True)
+    # one scene after another: a multi-substep call per scene on one policy
+    L = b.L
+    in_b = C.c_int32(0)
+    seq_steps = 10
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for st in b.states:
+        rocm.check(L.zsmpm_substeps_sorted__rocm(b.pol.h, C.byref(st.model), C.byref(st.parts.desc),
+                                                 C.byref(st.spare.desc), C.byref(st.table.desc),
+                                                 C.byref(st.buckets.desc), C.byref(st.grid.desc),
+                                                 float(st.sc.dt), (C.c_float * 3)(*st.sc.gravity), st.cols,
+                                                 st.ncols, seq_steps, 0, C.byref(in_b)), "substeps_sorted")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    b.sync()
+    seq_ms = (t1 - t0) * 1e3 / seq_steps
+    print(f"# one scene at a time: {seq_ms:.3f} ms/substep", file=sys.stderr, flush=True)
+    return {
+        "workload": "64 scenes, particle counts log-uniform in [10K, 500K] (seeded), jittered boxes and spheres, "
+                    "fixed-corotated / Drucker-Prager, dx 1/128 or 1/256, dt 1e-4",
+        "particles": n,
+        "value": n / (ms * 1e-3),
+        "unit": "particle-substeps/s",
+        "ms_per_step": ms,
+        "steps": steps,
+        "lanes": b.lanes_used,
+        "autotune_ms_per_two_substeps": {"1": b.tune_ms[0], "2": b.tune_ms[1], "4": b.tune_ms[2],
+                                          "8": b.tune_ms[3]},
+        "one_scene_at_a_time_ms_per_step": seq_ms,
+        "speedup_vs_one_scene_at_a_time": seq_ms / ms,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,6 +305,9 @@ def main() -> int:
                     help="time the profiled eager launches instead of hipGraph replays")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-closure leg")
     ap.add_argument("--train-steps", type=int, default=2000, help="Adam steps of the closure network")
+    ap.add_argument("--batch", action="store_true",
+                    help="run the batched multi-scene leg (its entry point is switched off at present)")
+    ap.add_argument("--batch-steps", type=int, default=100, help="timed substeps of the batched leg")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (the reference kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -294,6 +355,12 @@ def main() -> int:
     if not args.no_surrogate and multi:
         surrogate = surrogate_leg(torch, rocm, sc, args, graph, plain)
     x_plain = plain.pop("x")
+    batched = None
+    if args.batch:
+        del x_plain
+        x_plain = None
+        torch.cuda.empty_cache()
+        batched = batched_leg(torch, rocm, args)
     per_launch = {k: (ms / calls if calls else 0.0) for k, (ms, calls) in stages.items()}
     dom = max(per_launch, key=lambda k: stages[k][0])
     dom_bytes = stage_bytes(dom, sc.n, blocks, plastic, sc.table_size)
@@ -353,6 +420,7 @@ def main() -> int:
         },
         "stage_ms_per_launch": per_launch,
         "surrogate": surrogate,
+        "batched": batched,
         "cpu_baseline": None,
     }
     del x_plain

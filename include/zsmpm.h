@@ -245,6 +245,46 @@ int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, 
                                     int32_t nsteps, int32_t flags, int32_t* final_in_b);
 
 /* ------------------------------------------------------------------------
+ * Batches of independent scenes (an addition: zpc advances one scene per
+ * driver). Every scene has its own model, particles, table, buckets, grid,
+ * time step, gravity and colliders; a call advances all of them nsteps
+ * substeps.
+ * ---------------------------------------------------------------------- */
+#define ZSMPM_MAX_LANES 8
+
+typedef struct zsmpm_scene {
+  zsmpm_abi_header hdr;
+  const zsmpm_model* model;
+  const zsmpm_particles* a;        /* particle state */
+  const zsmpm_particles* b;        /* NULL: substeps in place (zsmpm_substep__rocm); else the sorted
+                                      driver's second set, ping-ponged a -> b -> a (same for every scene) */
+  const zsmpm_table* table;
+  const zsmpm_buckets* buckets;    /* may be NULL (the lane's own) */
+  const zsmpm_grid* grid;
+  float dt;
+  float extf[3];
+  const zsmpm_plane_collider* colliders;
+  int32_t ncolliders;
+} zsmpm_scene;
+
+/* nsteps substeps of every scene. The scenes are spread over `lanes` lanes
+ * (largest first, each to the lane with the fewest particles); a lane is a
+ * stream with its own scratch, and one hipGraph holds two substeps of every
+ * scene with the lanes forked from and joined to the capture stream. The graph
+ * is replayed nsteps / 2 times on the policy's stream (an odd remainder runs
+ * as ordinary substeps) and kept on the policy while the scenes stay the same.
+ * lanes = 0 autotunes: when nsteps >= 24 the first replays (which are the
+ * caller's substeps, not extra ones) time 1, 2, 4 and 8 lanes and the fastest
+ * is kept for this scene set; tune_ms (4 floats, may be NULL) receives the
+ * milliseconds per replay of each count (-1 where not measured) and lanes_used
+ * the count used. final_in_b as zsmpm_substeps_sorted__rocm.
+ * Currently returns ZSMPM_UNSUPPORTED_OPERATION after validating its
+ * arguments: the path is switched off until a host fault seen in its first
+ * run is explained. */
+int32_t zsmpm_substeps_batched__rocm(zsmpm_policy pol, const zsmpm_scene* scenes, int32_t nscenes, int32_t nsteps,
+                                     int32_t lanes, int32_t* lanes_used, float* tune_ms, int32_t* final_in_b);
+
+/* ------------------------------------------------------------------------
  * Learned stress closure (an addition: zpc has no learned constitutive path).
  * For Drucker-Prager particles it replaces the SVD of the return mapping
  * (physics/ConstitutiveModel_Vol_dP.hpp:241-325, called from

@@ -72,6 +72,13 @@ class ClosureDesc(C.Structure):
     _fields_ = [("hdr", Header), ("params", C.c_void_p), ("tol", C.c_float), ("flags", C.c_int32)]
 
 
+class SceneDesc(C.Structure):
+    _fields_ = [("hdr", Header), ("model", C.POINTER(ModelDesc)), ("a", C.POINTER(ParticlesDesc)),
+                ("b", C.POINTER(ParticlesDesc)), ("table", C.POINTER(TableDesc)),
+                ("buckets", C.POINTER(BucketsDesc)), ("grid", C.POINTER(GridDesc)), ("dt", C.c_float),
+                ("extf", C.c_float * 3), ("colliders", C.c_void_p), ("ncolliders", C.c_int32)]
+
+
 # learned closure network sizes (include/zsmpm.h)
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_RECORD, MLP_PARAMS = 9, 64, 7, 10, 5255
 
@@ -125,6 +132,8 @@ def lib():
                                         P(BucketsDesc), P(GridDesc), f32, P(C.c_float * 3), P(PlaneCollider), i32,
                                         i32, i32, P(C.c_int32)],
         "zsmpm_policy_closure__rocm": [vp, P(ClosureDesc)],
+        "zsmpm_substeps_batched__rocm": [vp, P(SceneDesc), i32, i32, i32, P(C.c_int32), P(C.c_float),
+                                         P(C.c_int32)],
         "zsmpm_stress__rocm": [vp, P(ModelDesc), vp, vp, i64, vp],
         "zsmpm_mlp_harvest__rocm": [vp, P(ModelDesc), vp, vp, i64, vp, vp],
         "zsmpm_mlp_infer__rocm": [vp, vp, vp, i64, vp],
@@ -467,3 +476,74 @@ class Simulator:
 
     def nblocks(self) -> int:
         return int(self.table.cnt.item())
+
+
+class SceneState:
+    """Device buffers of one scene of a batch (no policy of its own)."""
+
+    def __init__(self, sc: S.Scene, sorted_driver: bool = True):
+        self.sc = sc
+        T = int(sc.table_size)
+        self.table = DeviceTable(T)
+        cap = max(T // 16, 64)
+        self.grid = DeviceGrid(sc.dx, cap)
+        self.parts = DeviceParticles.from_scene(sc, ids=sorted_driver)
+        self.spare = self.parts.empty_like() if sorted_driver else None
+        self.buckets = DeviceBuckets(cap, sc.n)
+        self.model = model_desc(sc.material)
+        self.cols = colliders_array(sc.colliders)
+        self.ncols = len(sc.colliders)
+
+    def desc(self) -> SceneDesc:
+        d = SceneDesc()
+        d.hdr = _hdr(SceneDesc)
+        d.model = C.pointer(self.model)
+        d.a = C.pointer(self.parts.desc)
+        d.b = C.pointer(self.spare.desc) if self.spare is not None else C.POINTER(ParticlesDesc)()
+        d.table = C.pointer(self.table.desc)
+        d.buckets = C.pointer(self.buckets.desc)
+        d.grid = C.pointer(self.grid.desc)
+        d.dt = float(self.sc.dt)
+        for k in range(3):
+            d.extf[k] = float(self.sc.gravity[k])
+        d.colliders = C.cast(self.cols, C.c_void_p) if self.ncols else None
+        d.ncolliders = self.ncols
+        return d
+
+
+class Batch:
+    """Independent scenes advanced together by zsmpm_substeps_batched__rocm
+    (lanes of streams, one hipGraph holding two substeps of every scene)."""
+
+    def __init__(self, scenes: Sequence[S.Scene], sorted_driver: bool = True, device: int = 0):
+        self.pol = Policy(device)
+        self.L = lib()
+        self.states = [SceneState(sc, sorted_driver) for sc in scenes]
+        self._build()
+        self.lanes_used = 0
+        self.tune_ms = [-1.0] * 4
+
+    def _build(self):
+        self._descs = (SceneDesc * len(self.states))(*[st.desc() for st in self.states])
+
+    def substeps(self, n: int, lanes: int = 0):
+        used = C.c_int32(0)
+        tune = (C.c_float * 4)()
+        in_b = C.c_int32(0)
+        check(self.L.zsmpm_substeps_batched__rocm(self.pol.h, self._descs, len(self.states), int(n), int(lanes),
+                                                  C.byref(used), tune, C.byref(in_b)), "substeps_batched")
+        self.lanes_used = used.value
+        if any(t >= 0 for t in tune):
+            self.tune_ms = [float(t) for t in tune]
+        if in_b.value:
+            for st in self.states:
+                st.parts, st.spare = st.spare, st.parts
+            self._build()
+
+    def sync(self):
+        self.pol.sync()
+
+    @property
+    def particles(self) -> int:
+        return sum(st.sc.n for st in self.states)
+

@@ -149,10 +149,15 @@ def _state(x: np.ndarray, material: Material, vel=(0.0, 0.0, 0.0)):
 def _table_size_for(x: np.ndarray, dx: float, slack: float = 2.0) -> int:
     """T = evaluate_table_size(nExpected) with nExpected = slack x the enlarged block count."""
     c = np.floor(x.astype(np.float32) * np.float32(1.0 / dx) + np.float32(0.5)).astype(np.int64) - 2
-    b = np.floor_divide(c, 4)
-    keys = np.unique(b, axis=0)
-    enl = np.unique((keys[:, None, :] + np.array([[i, j, k] for i in (0, 1) for j in (0, 1)
-                                                   for k in (0, 1)])[None]).reshape(-1, 3), axis=0)
+    b = np.floor_divide(c, 4) + (1 << 20)  # non-negative, 21 bits per axis
+
+    def pack(k):
+        return (k[..., 0] << 42) | (k[..., 1] << 21) | k[..., 2]
+
+    keys = np.unique(pack(b))
+    kb = np.stack([keys >> 42, (keys >> 21) & ((1 << 21) - 1), keys & ((1 << 21) - 1)], axis=1)
+    offs = np.array([[i, j, k] for i in (0, 1) for j in (0, 1) for k in (0, 1)], np.int64)
+    enl = np.unique(pack(kb[:, None, :] + offs[None]))
     return evaluate_table_size(int(math.ceil(slack * enl.shape[0])))
 
 
@@ -203,6 +208,42 @@ def sand_scene(dims=(128, 256, 64), seed: int = 20261201 + 3, dx: float = 1.0 / 
     v, C, F, m, logJp = _state(x, mat)
     return Scene(f"sand{nx}x{ny}x{nz}", dx, dt, mat, x, v, C, F, m, logJp, colliders=box_walls(dx),
                  table_size=_table_size_for(x, dx), steps=steps)
+
+
+def batch_scenes(count: int = 64, seed: int = 20261201 + 4, lo: int = 10_000, hi: int = 500_000,
+                 ppc: int = 8) -> List[Scene]:
+    """Config 4: `count` independent scenes with particle counts log-uniform in
+    [lo, hi] (seeded), each a jittered box or sphere at 8 ppc resting in the
+    middle of the unit cube, half fixed-corotated (E = 1e4, nu = 0.3) and half
+    Drucker-Prager sand (E = 3.5e5, nu = 0.3, 30 deg), dx 1/128 or 1/256,
+    dt = 1e-4, sticky walls, gravity along -y."""
+    out = []
+    for i in range(count):
+        u = uniform24(seed + 7919 * i, 4)
+        n_target = math.exp(math.log(lo) + float(u[0]) * (math.log(hi) - math.log(lo)))
+        sphere = u[1] >= 0.5
+        sand = u[2] >= 0.5
+        dx = 1.0 / 128 if u[3] < 0.5 else 1.0 / 256
+        cells = n_target / ppc
+        if sphere:
+            r = (3.0 * cells / (4.0 * math.pi)) ** (1.0 / 3.0) * dx
+            c = np.array([0.5, 0.5, 0.5])
+            lo_c = np.floor((c - r) / dx).astype(int)
+            hi_c = np.ceil((c + r) / dx).astype(int)
+            x = jittered_cells(seed + 31 * i + 1, lo_c, hi_c - lo_c, dx, ppc,
+                               keep=lambda p, c=c, r=r: np.sum((p - c) ** 2, axis=1) <= r * r)
+        else:
+            side = max(2, int(round(cells ** (1.0 / 3.0))))
+            lo_c = [int(round(0.5 / dx)) - side // 2] * 3
+            x = jittered_cells(seed + 31 * i + 1, lo_c, (side,) * 3, dx, ppc)
+        if sand:
+            mat = Material.drucker_prager(3.5e5, 0.3, 1e3, volume=dx ** 3 / ppc)
+        else:
+            mat = Material(kind=FIXED_COROTATED, E=1e4, nu=0.3, rho=1e3, volume=dx ** 3 / ppc)
+        v, C, F, m, logJp = _state(x, mat)
+        out.append(Scene(f"batch{i}_{'sphere' if sphere else 'box'}_{'sand' if sand else 'fc'}", dx, 1e-4, mat,
+                         x, v, C, F, m, logJp, colliders=box_walls(dx), table_size=_table_size_for(x, dx)))
+    return out
 
 
 def tiny_scene(n_cells=(3, 3, 3), seed: int = 7, dx: float = 1.0 / 32, ppc: int = 2,
