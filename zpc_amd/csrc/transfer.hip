@@ -89,17 +89,10 @@ __device__ __forceinline__ int32_t bounded_blocks(const TableView& t, const Sort
   return nb;
 }
 
-// Buckets are handed out in order from a queue (one atomic per bucket), so
-// a workgroup that drew small buckets takes more: bucket sizes range from 0
-// (enlarged-only blocks) to several hundred particles. Workgroup-uniform;
-// every workgroup leaves once the queue passes the bucket count.
-__device__ __forceinline__ int32_t next_bucket(uint32_t* queue, int32_t* slot) {
-  __syncthreads();  // every thread has read the previous bucket number
-  if (threadIdx.x == 0) *slot = (int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu);
-  __syncthreads();
-  return *slot;
-}
-
+// P2G hands buckets out in order from a queue (one atomic per bucket), so a
+// workgroup that drew small buckets takes more: bucket sizes range from 0
+// (enlarged-only blocks) to several hundred particles. Every workgroup leaves
+// once the queue passes the bucket count.
 __global__ void reset_queue_kernel(uint32_t* counters) { counters[kCtrQueue] = 0u; }
 
 // neighbour blocks key + (d>>2&1, d>>1&1, d&1) of bucket b, from the
@@ -111,6 +104,27 @@ __device__ __forceinline__ void load_neighbours(const int32_t* __restrict__ nbr_
     nbr[threadIdx.x] = r;
     if (r < 0) atomicOr(err, kErrMissingBlock);
   }
+}
+
+// Count, offset and block key of bucket b (0 for b >= nb or a bucket whose
+// range is not within the particles), written to m[0..5] by one thread.
+__device__ __forceinline__ void bucket_meta(int32_t b, int32_t nb, const SortView& sv, const TableView& t,
+                                            int64_t n, int32_t* m) {
+  int32_t cnt = 0, off = 0, k0 = 0, k1 = 0, k2 = 0;
+  if (b < nb) {
+    cnt = sv.counts[b];
+    off = sv.offsets[b];
+    if (cnt <= 0 || off < 0 || (int64_t)off + cnt > n) cnt = 0;
+    k0 = t.active[3 * (int64_t)b];
+    k1 = t.active[3 * (int64_t)b + 1];
+    k2 = t.active[3 * (int64_t)b + 2];
+  }
+  m[0] = b;
+  m[1] = cnt;
+  m[2] = off;
+  m[3] = k0;
+  m[4] = k1;
+  m[5] = k2;
 }
 
 // Main base cell (i, j, k) in [0,4)^3 <-> lane of the scatter rounds. Lane
@@ -208,15 +222,40 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
   const float D_inv = 4.f * dx_inv * dx_inv;
   const float scale = -dt * D_inv;
   const int wave = uniform((int32_t)(threadIdx.x / kWave)), lane = lane_id();
-  __shared__ int32_t bslot;
+  // Buckets come from the queue one ahead: thread 0 pops the next bucket
+  // after the last chunk's counting sort and loads its count, offset and key
+  // while phase C runs (bmeta, published by the barrier that ends the chunk);
+  // every thread then issues the loads of its particle indices in the next
+  // bucket's first chunk before the flush, so they are in flight across the
+  // flush and the next bucket's set-up instead of starting phase A.
+  constexpr int kPre = kChunk / kP2GThreads;
+  __shared__ int32_t bmeta[6];  // bucket, count, offset, key
+  if (threadIdx.x == 0) bucket_meta((int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu), nb, sv, t, ps.n, bmeta);
+  __syncthreads();
+  int32_t pre[kPre];
+  auto prefetch = [&]() {
+    const int32_t lim = min(bmeta[1], kChunk), po = bmeta[2];
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+      const int32_t i = threadIdx.x + k * kP2GThreads;
+      pre[k] = i < lim ? sv.indices[po + i] : -1;
+    }
+  };
+  prefetch();
 
-  for (int32_t b = next_bucket(queue, &bslot); b < nb; b = next_bucket(queue, &bslot)) {
-    const int32_t cnt = sv.counts[b];
-    const int32_t off = sv.offsets[b];
-    if (cnt <= 0 || off < 0 || (int64_t)off + cnt > ps.n) continue;
+  for (;;) {
+    const int32_t b = bmeta[0], cnt = bmeta[1], off = bmeta[2];
+    if (b >= nb) break;
+    const int32_t o0 = bmeta[3] * 4, o1 = bmeta[4] * 4, o2 = bmeta[5] * 4;
+    if (cnt == 0) {  // empty or out-of-range bucket: take the next one now
+      __syncthreads();
+      if (threadIdx.x == 0) bucket_meta((int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu), nb, sv, t, ps.n, bmeta);
+      __syncthreads();
+      prefetch();
+      continue;
+    }
     load_neighbours(nbr_tab, b, nbr, err);
-    const int32_t o0 = t.active[3 * (int64_t)b] * 4, o1 = t.active[3 * (int64_t)b + 1] * 4,
-                  o2 = t.active[3 * (int64_t)b + 2] * 4;
+    uint32_t nxt = 0;  // thread 0: the next bucket, popped during the last chunk
 
     for (int32_t c0 = 0; c0 < cnt; c0 += kChunk) {
       const int32_t nch = min(kChunk, cnt - c0);
@@ -229,11 +268,14 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
       // consecutive chunk slots (i = threadIdx.x + k * kP2GThreads, as phase B
       // expects); a contiguous run of particles is read with 16-byte vectors,
       // staged in the wave's own (not yet written) record slots.
-      for (int32_t i0 = wave * kWave; i0 < nch; i0 += kP2GThreads) {
+#pragma unroll
+      for (int kk = 0; kk < kPre; ++kk) {
+        const int32_t i0 = wave * kWave + kk * kP2GThreads;
+        if (i0 >= nch) break;
         const int32_t i = i0 + lane;
         const int32_t cw = min(kWave, nch - i0);
         const bool valid = lane < cw;
-        const int64_t p = valid ? (int64_t)sv.indices[off + c0 + i] : -1;
+        const int64_t p = valid ? (int64_t)(c0 == 0 ? pre[kk] : sv.indices[off + c0 + i]) : -1;
         const bool bad = valid && (p < 0 || p >= ps.n);
         if (bad) {
           atomicOr(err, kErrCapacity);
@@ -376,6 +418,8 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
         }
       }
       __syncthreads();
+      const bool last = c0 + kChunk >= cnt;
+      if (last && threadIdx.x == 0) nxt = atomicAdd(queue, 1u);
       // ---- C: rounds, lane = main cell
       {
         const int32_t lcount = ccount[lane];
@@ -563,8 +607,10 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
               }
             }
       }
+      if (last && threadIdx.x == 0) bucket_meta((int32_t)min(nxt, 0x7fffffffu), nb, sv, t, ps.n, bmeta);
       __syncthreads();
     }
+    prefetch();  // the next bucket's first-chunk indices (bmeta is the next bucket's now)
     // ---- flush: sum the wave arenas, one global atomic per touched node/channel
     for (int n = threadIdx.x; n < kArena6; n += kP2GThreads) {
       float4 s = arena[0][n * kNV];
