@@ -415,17 +415,38 @@ __global__ __launch_bounds__(kThreads) void bucket_slots_kernel(const uint32_t* 
   const int lane = lane_id();
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t heads = 0;
+    // all rounds' slots (and, for lane 0, the slot before the wave) load
+    // together, then all run heads' block lookups: two memory round trips per
+    // tile instead of two per round
+    uint32_t sl[kTileItems], pv[kTileItems];
+    int32_t bl[kTileItems];
+#pragma unroll
+    for (int r = 0; r < kTileItems; ++r) {
+      const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
+      sl[r] = p < n ? pslot[p] : ~0u;
+      pv[r] = (lane == 0 && p < n && p > 0) ? pslot[p - 1] : ~0u;
+    }
+    uint32_t headm = 0;
+#pragma unroll
     for (int r = 0; r < kTileItems; ++r) {
       const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
       const bool valid = p < n;
-      const uint32_t sl = valid ? pslot[p] : ~0u;
-      uint32_t prev = __shfl_up(sl, 1, kWave);
-      if (lane == 0 && valid && p > 0) prev = pslot[p - 1];
-      const bool head = valid && (p == 0 || sl != prev);
+      uint32_t prev = __shfl_up(sl[r], 1, kWave);
+      if (lane == 0) prev = pv[r];
+      const bool head = valid && (p == 0 || sl[r] != prev);
+      headm |= head ? (1u << r) : 0u;
+      const bool lookup = valid && (head || lane == 0);
+      bl[r] = lookup ? (sl[r] < ts ? slot_block[sl[r]] : -1) : -1;
+    }
+#pragma unroll
+    for (int r = 0; r < kTileItems; ++r) {
+      const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
+      const bool valid = p < n;
+      const bool head = (headm >> r) & 1u;
       const bool lookup = valid && (head || lane == 0);
       int32_t b = -1;
       if (lookup) {
-        b = sl < ts ? slot_block[sl] : -1;
+        b = bl[r];
         if (b < 0) atomicOr(err, kErrMissingBlock);
         if (b >= cap_blocks) {
           atomicOr(err, kErrCapacity);
@@ -489,15 +510,20 @@ __global__ __launch_bounds__(kThreads) void run_emit_kernel(const uint32_t* __re
   const int64_t ntiles = (n + kTile - 1) / kTile;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t carry = tile_base[tile];
+    uint32_t v[kTileItems];  // all rounds load before the first barrier
+#pragma unroll
     for (int r = 0; r < kTileItems; ++r) {
       const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
-      const bool valid = p < n;
-      const uint32_t v = valid ? bkt[p] : 0u;
-      const bool head = valid && (v & kHeadBit);
+      v[r] = p < n ? bkt[p] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kTileItems; ++r) {
+      const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
+      const bool head = p < n && (v[r] & kHeadBit);
       const uint32_t rid = round_run_id(head, carry, lds);
       if (head && rid < (uint32_t)n) {
         run_start[rid] = (uint32_t)p;
-        rkey[rid] = v & ~kHeadBit;
+        rkey[rid] = v[r] & ~kHeadBit;
         rval[rid] = rid;
       }
     }
@@ -542,13 +568,32 @@ __global__ __launch_bounds__(kThreads) void particle_scatter_kernel(const uint32
   const int64_t ntiles = (n + kTile - 1) / kTile;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t carry = tile_base[tile];
+    // three phases, each with its loads in flight together: the tile's
+    // bucket words, the run ids (LDS scans only), the runs' destinations
+    uint32_t w[kTileItems], rid[kTileItems];
+#pragma unroll
     for (int r = 0; r < kTileItems; ++r) {
       const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
-      const bool valid = p < n;
-      const bool head = valid && (bkt[p] & kHeadBit);
-      const uint32_t rid = round_run_id(head, carry, lds);
-      if (valid && rid < R) {
-        const uint64_t dst = (uint64_t)run_dest[rid] + ((uint32_t)p - run_start[rid]);
+      w[r] = p < n ? bkt[p] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kTileItems; ++r) {
+      const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
+      rid[r] = round_run_id(p < n && (w[r] & kHeadBit), carry, lds);
+    }
+    uint32_t dest[kTileItems], start[kTileItems];
+#pragma unroll
+    for (int r = 0; r < kTileItems; ++r) {
+      const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
+      const bool ok = p < n && rid[r] < R;
+      dest[r] = ok ? run_dest[rid[r]] : 0u;
+      start[r] = ok ? run_start[rid[r]] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kTileItems; ++r) {
+      const int64_t p = tile * kTile + (int64_t)r * kThreads + threadIdx.x;
+      if (p < n && rid[r] < R) {
+        const uint64_t dst = (uint64_t)dest[r] + ((uint32_t)p - start[r]);
         if (dst < (uint64_t)n) indices[dst] = (int32_t)p;
       }
     }
