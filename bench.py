@@ -117,6 +117,26 @@ def cpu_baseline(sc, warmup: int, steps: int) -> dict:
     }
 
 
+def copy_peak(torch, nbytes: int = 1 << 30, reps: int = 5) -> float:
+    """Measured device-to-device copy bandwidth (bytes read + written per
+    second), the practical HBM ceiling beside the 8 TB/s nominal peak."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    sec = a.elapsed_time(b) * 1e-3 / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / sec
+
+
 def x_by_id(torch, sim):
     """Positions in original particle order (ids carried by the sorted driver)."""
     x = sim.parts.x
@@ -346,6 +366,8 @@ def main() -> int:
         print(f"# copy kernel: {4 * words} bytes read, {4 * words} bytes written", file=sys.stderr, flush=True)
         del src, dst
 
+    copy_gbs = copy_peak(torch) / 1e9
+    print(f"# device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     multi = not (args.in_place or args.single)
     graph = multi and not args.eager
     plain = run_path(torch, rocm, sc, args, multi, graph)
@@ -410,6 +432,8 @@ def main() -> int:
             "traffic": measured_traffic(dom) if dims == (128, 256, 64) else None,
             "bytes_per_launch": dom_bytes,
             "ms_per_launch": per_launch[dom],
+            "copy_peak": copy_gbs,
+            "frac_of_copy_peak": dom_gbs / copy_gbs,
         },
         "substep_roofline": {
             "bytes_per_substep": sub_bytes,
@@ -417,6 +441,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": sub_gbs / HBM_PEAK_GBS,
+            "copy_peak": copy_gbs,
+            "frac_of_copy_peak": sub_gbs / copy_gbs,
         },
         "stage_ms_per_launch": per_launch,
         "surrogate": surrogate,
