@@ -464,6 +464,79 @@ def test_batched_autotune_and_odd_steps():
     _batch_vs_oracle(b, scenes, 25, ("x", "v", "F"), TOL_100STEP, with64=False)
 
 
+def _first_occurrence(packed):
+    """Distinct values of `packed` in order of first occurrence."""
+    u, first = np.unique(packed, return_index=True)
+    return u[np.argsort(first, kind="stable")]
+
+
+def _pack(k):
+    k = k.astype(np.int64) + (1 << 20)
+    return (k[:, 0] << 42) | (k[:, 1] << 21) | k[:, 2]
+
+
+def test_full_size_properties():
+    """The 16,777,216-particle sand column (the bench workload) after two
+    sorted substeps, stage by stage: block numbering after activation and
+    enlargement equals a numpy restatement of first-occurrence order; the
+    bucket sort equals a stable argsort of the block numbers; P2G conserves
+    mass and momentum (the stress terms sum to zero over a particle's
+    stencil); a sorted substep keeps the particle multiset (ids a permutation,
+    total mass)."""
+    sc = S.sand_scene()
+    rocm = _rocm()
+    sim = rocm.Simulator(sc, reorder=True)
+    sim.substeps(2)
+    sim.sync()
+    x = sim.parts.x.cpu().numpy()
+    sim.clear_table()
+    sim.activate()
+    sim.sync()
+    t = sim.table.host()
+    keys = O.block_keys(x, sc.dx)
+    pk = _pack(keys)
+    order = _first_occurrence(pk)
+    assert t["cnt"] == order.size
+    assert np.array_equal(_pack(t["active"]), order)
+    sim.enlarge()
+    sim.sync()
+    t = sim.table.host()
+    offs = np.array([[i, j, k] for i in (0, 1) for j in (0, 1) for k in (0, 1)], np.int64)
+    akeys = t["active"][: order.size].astype(np.int64)
+    cand = _pack((akeys[:, None, :] + offs[None]).reshape(-1, 3))
+    new = _first_occurrence(cand[~np.isin(cand, order)])
+    want = np.concatenate([order, new])
+    assert t["cnt"] == want.size
+    assert np.array_equal(_pack(t["active"]), want)
+    sim.bucket_sort()
+    sim.sync()
+    nb = t["cnt"]
+    srt = np.argsort(want, kind="stable")
+    block = srt[np.searchsorted(want[srt], pk)]
+    counts, offsets, indices = sim.buckets.host(nb, sc.n)
+    bc = np.bincount(block, minlength=nb)
+    assert np.array_equal(counts[:nb], bc)
+    assert np.array_equal(offsets[:nb], np.concatenate([[0], np.cumsum(bc)[:-1]]))
+    assert np.array_equal(indices, np.argsort(block, kind="stable"))
+    sim.clear_grid()
+    sim.p2g()
+    sim.sync()
+    g = sim.grid.data[:nb].cpu().numpy().astype(np.float64)
+    m = sim.parts.m.cpu().numpy().astype(np.float64)
+    v = sim.parts.v.cpu().numpy().astype(np.float64)
+    assert abs(g[:, 0].sum() - m.sum()) <= 1e-5 * m.sum()
+    mom_p = (m[:, None] * v).sum(0)
+    mom_g = g[:, 1:4].sum((0, 2)) + g[:, 4:7].sum((0, 2))
+    scale = (m[:, None] * np.abs(v)).sum()
+    assert np.all(np.abs(mom_g - mom_p) <= 1e-4 * scale), (mom_g, mom_p, scale)
+    # one sorted substep: ids stay a permutation, the mass is moved, not changed
+    sim.substeps(1)
+    sim.sync()
+    ids = sim.parts.id.cpu().numpy()
+    assert np.array_equal(np.sort(ids), np.arange(sc.n, dtype=ids.dtype))
+    assert abs(float(sim.parts.m.double().sum().cpu()) - m.sum()) <= 1e-9 * m.sum()
+
+
 def test_fused_substeps_drift_100(sand):
     """100 fused substeps of the sand column against the plain oracle, per id:
     x, v and F within 1e-3. logJp is not bounded here: it follows the
