@@ -231,6 +231,10 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
   constexpr int kPre = kChunk / kP2GThreads;
   __shared__ int32_t bmeta[6];  // bucket, count, offset, key
   if (threadIdx.x == 0) bucket_meta((int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu), nb, sv, t, ps.n, bmeta);
+  for (int c = threadIdx.x; c <= kMainCells; c += kP2GThreads) {
+    ccount[c] = 0;
+    cfill[c] = 0;
+  }
   __syncthreads();
   int32_t pre[kPre];
   auto prefetch = [&]() {
@@ -259,11 +263,15 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
 
     for (int32_t c0 = 0; c0 < cnt; c0 += kChunk) {
       const int32_t nch = min(kChunk, cnt - c0);
-      for (int c = threadIdx.x; c <= kMainCells; c += kP2GThreads) {  // 64 main cells + overflow
-        ccount[c] = 0;
-        cfill[c] = 0;
+      if (c0 > 0) {
+        // later chunks clear the cell counters here; a bucket's first chunk
+        // finds them cleared by the previous bucket's flush (or the prologue)
+        for (int c = threadIdx.x; c <= kMainCells; c += kP2GThreads) {  // 64 main cells + overflow
+          ccount[c] = 0;
+          cfill[c] = 0;
+        }
+        __syncthreads();
       }
-      __syncthreads();
       // ---- A: stress + affine record per particle. Each wave takes 64
       // consecutive chunk slots (i = threadIdx.x + k * kP2GThreads, as phase B
       // expects); a contiguous run of particles is read with 16-byte vectors,
@@ -611,6 +619,12 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
       __syncthreads();
     }
     prefetch();  // the next bucket's first-chunk indices (bmeta is the next bucket's now)
+    // the next bucket's first chunk finds its cell counters cleared (they are
+    // not read from here on; the barrier after the flush orders the clear)
+    for (int c = threadIdx.x; c <= kMainCells; c += kP2GThreads) {
+      ccount[c] = 0;
+      cfill[c] = 0;
+    }
     // ---- flush: sum the wave arenas, one global atomic per touched node/channel
     for (int n = threadIdx.x; n < kArena6; n += kP2GThreads) {
       float4 s = arena[0][n * kNV];
