@@ -154,7 +154,8 @@ def run_path(torch, rocm, sc, args, multi, graph, closure=None):
     if closure is not None:
         sim.pol.set_closure(closure[0], tol=closure[1], always_network=closure[2])
     if multi:
-        sim.substeps(args.warmup)
+        # the warm-up substeps also pick the P2G / G2P workgroup counts
+        sim.substeps(args.warmup, autotune=True)
     else:
         for _ in range(args.warmup):
             sim.substep()
@@ -197,6 +198,7 @@ def run_path(torch, rocm, sc, args, multi, graph, closure=None):
     b1 = sim.nblocks()
     substeps_run = args.warmup + args.steps + (2 + args.steps if graph else 0)
     out = {"stages": stages, "blocks": 0.5 * (b0 + b1), "ms": wall * 1e3 / args.steps, "eager_ms": eager_ms,
+           "launch_widths": sim.pol.launch_widths(),
            "fallbacks": int(sim.pol.counters()[4]) - fb0, "substeps": substeps_run, "x": x_by_id(torch, sim)}
     del sim
     return out
@@ -414,6 +416,7 @@ def main() -> int:
             "path": "plain",
             "particle_order": "original (in place)" if args.in_place else
                               "bucket order, reordered inside G2P every substep",
+            "p2g_g2p_workgroups": list(plain["launch_widths"]),
             "driver": ("zsmpm_substeps_sorted__rocm with ZSMPM_SUBSTEPS_GRAPH (a captured two-substep "
                        "hipGraph replayed)" if graph else
                        "zsmpm_substeps_sorted__rocm (one call for all timed substeps)") if multi else

@@ -224,6 +224,7 @@ int32_t zsmpm_substep_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, c
 /* zsmpm_substeps_sorted__rocm flags */
 #define ZSMPM_SUBSTEPS_FUSE_STRESS 1 /* G2P computes the next substep's stress (see below) */
 #define ZSMPM_SUBSTEPS_GRAPH 2        /* replay a captured hipGraph of two substeps (see below) */
+#define ZSMPM_SUBSTEPS_AUTOTUNE 4     /* pick the P2G / G2P workgroup counts on the first substeps */
 
 /* nsteps substeps of zsmpm_substep_sorted__rocm, ping-ponging a -> b -> a ...:
  * the result is in b when nsteps is odd (*final_in_b = nsteps & 1); the
@@ -237,12 +238,20 @@ int32_t zsmpm_substep_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, c
  * nsteps / 2 times on the policy's stream; an odd
  * remainder runs as one ordinary substep. The graph is kept on the policy and
  * captured again when any argument or the policy's scratch changes. Stage
- * profiling does not see graph replays. */
+ * profiling does not see graph replays.
+ * With ZSMPM_SUBSTEPS_AUTOTUNE (and nsteps >= 8, not with FUSE_STRESS) the
+ * first 8 substeps, which are the caller's own, run in pairs with four P2G /
+ * G2P workgroup counts timed per kernel; the policy keeps the fastest count
+ * of each kernel for all later calls (zsmpm_policy_launch_widths__rocm). */
 int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* a,
                                     const zsmpm_particles* b, const zsmpm_table* table,
                                     const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt,
                                     const float extf[3], const zsmpm_plane_collider* colliders, int32_t ncolliders,
                                     int32_t nsteps, int32_t flags, int32_t* final_in_b);
+
+/* The P2G and G2P workgroup counts the policy launches with (autotuned or the
+ * defaults). No counterpart in the reference (its launch shapes are fixed). */
+int32_t zsmpm_policy_launch_widths__rocm(zsmpm_policy pol, int32_t* p2g_groups, int32_t* g2p_groups);
 
 /* ------------------------------------------------------------------------
  * Batches of independent scenes (an addition: zpc advances one scene per

@@ -48,6 +48,10 @@ int main(int argc, char** argv) {
   expect("substeps graph", zsmpm_substeps_sorted__rocm(pol, &s0.model, &s0.a, &s0.b, &s0.t, &s0.bk, &s0.g, 1e-4f, g3, s0.cols, 6, 6, ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
   expect("substeps graph again", zsmpm_substeps_sorted__rocm(pol, &s0.model, &s0.a, &s0.b, &s0.t, &s0.bk, &s0.g, 1e-4f, g3, s0.cols, 6, 5, ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
   expect("odd count ends in b", inb, 1);
+  expect("autotune", zsmpm_substeps_sorted__rocm(pol, &s0.model, &s0.a, &s0.b, &s0.t, &s0.bk, &s0.g, 1e-4f, g3, s0.cols, 6, 10, ZSMPM_SUBSTEPS_AUTOTUNE | ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
+  int32_t wp = 0, wg = 0;
+  expect("launch widths", zsmpm_policy_launch_widths__rocm(pol, &wp, &wg), 0);
+  expect("autotuned widths are candidates", (wp == 512 || wp == 768 || wp == 1024 || wp == 2048) && (wg == 8192 || wg == 16384 || wg == 32768 || wg == 65536), 1);
   expect("graph + fuse rejected", zsmpm_substeps_sorted__rocm(pol, &s0.model, &s0.a, &s0.b, &s0.t, &s0.bk, &s0.g, 1e-4f, g3, s0.cols, 6, 2, ZSMPM_SUBSTEPS_GRAPH | ZSMPM_SUBSTEPS_FUSE_STRESS, &inb), ZSMPM_INVALID_ARGUMENT);
   // closure selection
   zsmpm_closure cl = mk<zsmpm_closure>();

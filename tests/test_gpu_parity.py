@@ -464,6 +464,28 @@ def test_batched_autotune_and_odd_steps():
     _batch_vs_oracle(b, scenes, 25, ("x", "v", "F"), TOL_100STEP, with64=False)
 
 
+@pytest.mark.parametrize("scene", ["cube", "sand"])
+def test_autotuned_substeps(scene, request):
+    """ZSMPM_SUBSTEPS_AUTOTUNE: the 8 tuning substeps (four P2G / G2P
+    workgroup counts) and the graph-replayed rest are all real substeps;
+    the state after 10 stays on the oracle and the counts kept are
+    candidates."""
+    sc = request.getfixturevalue(scene)
+    rocm = _rocm()
+    sim = rocm.Simulator(sc, reorder=True)
+    sim.substeps(10, graph=True, autotune=True)
+    sim.sync()
+    p2g, g2p = sim.pol.launch_widths()
+    assert p2g in (512, 768, 1024, 2048) and g2p in (8192, 16384, 32768, 65536)
+    got = sim.parts.host_by_id()
+    ps = _oracle_particles(sc)
+    O.Context().run(O.make_params(sc), ps, 10, reorder=True)
+    ps64 = _oracle_particles(sc).astype(np.float64)
+    O.Context(np.float64).run(O.make_params(sc), ps64, 10, reorder=True)
+    for name in ("x", "v", "F"):
+        check_field(f"{name} after 10 autotuned substeps", got[name], ps.by_id(name), ps64.by_id(name), 1e-4)
+
+
 def _first_occurrence(packed):
     """Distinct values of `packed` in order of first occurrence."""
     u, first = np.unique(packed, return_index=True)

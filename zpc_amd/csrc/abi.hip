@@ -41,6 +41,8 @@ struct zsmpm_policy_s {
   std::vector<unsigned char> bkey;
   std::vector<unsigned char> tune_key;  // scene set the autotuned lane count belongs to
   int32_t tuned_lanes = 0;
+  // P2G / G2P workgroup counts picked by ZSMPM_SUBSTEPS_AUTOTUNE (0: default)
+  int32_t tuned_p2g = 0, tuned_g2p = 0;
   Workspace ws{};
   std::vector<void*> allocs;
   int64_t reserved_n = 0;
@@ -481,6 +483,8 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
   const float dx = grid->dx;
   hipStream_t s = pol->stream;
   Workspace& ws = pol->ws;
+  ws.grid_p2g = pol->tuned_p2g;
+  ws.grid_g2p = pol->tuned_g2p;
   const int32_t lo[3] = {0, 0, 0}, hi[3] = {2, 2, 2};
   // The stress pass reads F and logJp and writes ws.contrib; the sparsity
   // stages read x only. It runs on the side stream beside them and P2G waits
@@ -578,6 +582,8 @@ int32_t capture_pair(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_
   key_put(key, &pol->ws_epoch);
   key_put(key, &pol->closure_on);
   key_put(key, &pol->closure);
+  key_put(key, &pol->tuned_p2g);
+  key_put(key, &pol->tuned_g2p);
   if (pol->gexec && key == pol->gkey) return ZSMPM_OK;
   destroy_graph(pol);
   ZS_HIP(ensure_side(pol));
@@ -1043,7 +1049,8 @@ int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, 
                                     const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt,
                                     const float extf[3], const zsmpm_plane_collider* colliders, int32_t ncolliders,
                                     int32_t nsteps, int32_t flags, int32_t* final_in_b) {
-  if (!pol || !a || !b || nsteps < 0 || (flags & ~(ZSMPM_SUBSTEPS_FUSE_STRESS | ZSMPM_SUBSTEPS_GRAPH)))
+  if (!pol || !a || !b || nsteps < 0 ||
+      (flags & ~(ZSMPM_SUBSTEPS_FUSE_STRESS | ZSMPM_SUBSTEPS_GRAPH | ZSMPM_SUBSTEPS_AUTOTUNE)))
     return ZSMPM_INVALID_ARGUMENT;
   const bool fuse = (flags & ZSMPM_SUBSTEPS_FUSE_STRESS) != 0;
   const bool graph = (flags & ZSMPM_SUBSTEPS_GRAPH) != 0;
@@ -1060,6 +1067,45 @@ int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, 
     DeviceGuard g(pol->devid);
     ZS_TRY(ensure(pol, pa.n, table ? table->table_size : 0));
   }
+  // Autotuning: the first 8 substeps (real ones, in pairs a -> b -> a) run
+  // with four P2G / G2P workgroup counts, timed per kernel with the stage
+  // events; each kernel keeps its fastest count.
+  int32_t done = 0;
+  if ((flags & ZSMPM_SUBSTEPS_AUTOTUNE) && nsteps >= 8 && !fuse) {
+    DeviceGuard g(pol->devid);
+    const int32_t cand_p2g[4] = {512, 768, 1024, 2048}, cand_g2p[4] = {8192, 16384, 32768, 65536};
+    const bool prof = pol->profile;
+    ZS_HIP(hipStreamSynchronize(pol->stream));
+    harvest_events(pol);
+    pol->profile = true;
+    double best_p = 0.0, best_g = 0.0;
+    int32_t pick_p = 0, pick_g = 0;
+    int32_t r = ZSMPM_OK;
+    for (int c = 0; c < 4 && r == ZSMPM_OK; ++c) {
+      pol->tuned_p2g = cand_p2g[c];
+      pol->tuned_g2p = cand_g2p[c];
+      const double p0 = pol->stage_ms[ZSMPM_STAGE_P2G], g0 = pol->stage_ms[ZSMPM_STAGE_G2P];
+      for (int k = 0; k < 2 && r == ZSMPM_OK; ++k, ++done)
+        r = run_substep(pol, model, (done & 1) ? b : a, (done & 1) ? a : b, table, buckets, grid, dt, extf,
+                        colliders, ncolliders);
+      if (r == ZSMPM_OK) r = from_hip(hipStreamSynchronize(pol->stream));
+      harvest_events(pol);
+      const double dp = pol->stage_ms[ZSMPM_STAGE_P2G] - p0, dg = pol->stage_ms[ZSMPM_STAGE_G2P] - g0;
+      if (c == 0 || dp < best_p) {
+        best_p = dp;
+        pick_p = cand_p2g[c];
+      }
+      if (c == 0 || dg < best_g) {
+        best_g = dg;
+        pick_g = cand_g2p[c];
+      }
+    }
+    pol->profile = prof;
+    if (r != ZSMPM_OK) return r;
+    pol->tuned_p2g = pick_p;
+    pol->tuned_g2p = pick_g;
+  }
+  nsteps -= done;  // an even count: the particles are back in a
   if (graph) {
     DeviceGuard g(pol->devid);
     if (nsteps >= 2) {
@@ -1163,6 +1209,13 @@ int32_t zsmpm_substeps_batched__rocm(zsmpm_policy pol, const zsmpm_scene* scenes
   }
   if (nsteps & 1)
     for (int32_t i = 0; i < nscenes; ++i) ZS_TRY(scene_substep(pol, scenes[i], false));
+  return ZSMPM_OK;
+}
+
+int32_t zsmpm_policy_launch_widths__rocm(zsmpm_policy pol, int32_t* p2g_groups, int32_t* g2p_groups) {
+  if (!pol || !p2g_groups || !g2p_groups) return ZSMPM_INVALID_ARGUMENT;
+  *p2g_groups = pol->tuned_p2g > 0 ? pol->tuned_p2g : persistent_groups();
+  *g2p_groups = pol->tuned_g2p > 0 ? pol->tuned_g2p : persistent_groups();
   return ZSMPM_OK;
 }
 

@@ -1004,7 +1004,7 @@ hipError_t stress(const Model& m, const ParticlesView& ps, Workspace& ws, hipStr
 
 hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
                        int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s) {
-  const int g = persistent_groups();
+  const int g = ws.grid_p2g > 0 ? ws.grid_p2g : persistent_groups();
   reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
   uint32_t* q = ws.counters + kCtrQueue;
   if (fold)
@@ -1024,7 +1024,7 @@ hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float 
 hipError_t g2p(const ParticlesView& ps, const ParticlesView* out, const TableView& t, const SortView& sv,
                const float* grid, int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s,
                const NextStress* next) {
-  const int g = persistent_groups();
+  const int g = ws.grid_g2p > 0 ? ws.grid_g2p : persistent_groups();
   NextStress ns{};
   if (next) ns = *next;
   if (next && next->contrib) {

@@ -109,6 +109,8 @@ struct Workspace {
   float* contrib = nullptr;      // [9 n_cap] vol P F^T of each particle (column-major)
   int32_t* nbr = nullptr;        // [8 ts_cap] block index of key_b + (d>>2&1, d>>1&1, d&1), or -1
   int32_t* fb = nullptr;         // [n_cap] particles queued for the closure's SVD fallback
+  int32_t grid_p2g = 0;          // P2G / G2P workgroups (0: persistent_groups()), set by the policy
+  int32_t grid_g2p = 0;
   float* lj_next[2] = {nullptr, nullptr};  // [n_cap] each: logJp caches of the fused multi-substep driver
   uint32_t* bitmap = nullptr;    // [bm_words] one bit per insertion order: the orders of new keys
   uint32_t* bm_prefix = nullptr; // [bm_words] exclusive prefix of the bit counts

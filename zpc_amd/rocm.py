@@ -132,6 +132,7 @@ def lib():
                                         P(BucketsDesc), P(GridDesc), f32, P(C.c_float * 3), P(PlaneCollider), i32,
                                         i32, i32, P(C.c_int32)],
         "zsmpm_policy_closure__rocm": [vp, P(ClosureDesc)],
+        "zsmpm_policy_launch_widths__rocm": [vp, P(C.c_int32), P(C.c_int32)],
         "zsmpm_substeps_batched__rocm": [vp, P(SceneDesc), i32, i32, i32, P(C.c_int32), P(C.c_float),
                                          P(C.c_int32)],
         "zsmpm_stress__rocm": [vp, P(ModelDesc), vp, vp, i64, vp],
@@ -221,6 +222,12 @@ class Policy:
         d.flags = 1 if always_network else 0
         check(self.L.zsmpm_policy_closure__rocm(self.h, C.byref(d)), "closure")
         self._closure_params = params
+
+    def launch_widths(self):
+        """(P2G, G2P) workgroup counts the policy launches with."""
+        p, g = C.c_int32(0), C.c_int32(0)
+        check(self.L.zsmpm_policy_launch_widths__rocm(self.h, C.byref(p), C.byref(g)), "launch_widths")
+        return p.value, g.value
 
     def stage_times(self, reset: bool = False) -> dict:
         """{stage: (total ms, calls)} accumulated since the last reset."""
@@ -399,10 +406,12 @@ class Simulator:
                                          C.byref(self.grid.desc), dt, C.byref(self.extf), self.cols, self.ncols),
               "substep")
 
-    def substeps(self, n: int, dt: Optional[float] = None, fuse_stress: bool = False, graph: bool = False):
+    def substeps(self, n: int, dt: Optional[float] = None, fuse_stress: bool = False, graph: bool = False,
+                 autotune: bool = False):
         """n substeps in one call (sorted driver); fuse_stress: G2P computes the
         next substep's stress; graph: replay a captured hipGraph of two
-        substeps (kept by the library while the arguments stay the same)."""
+        substeps (kept by the library while the arguments stay the same);
+        autotune: the first 8 substeps pick the P2G / G2P workgroup counts."""
         if not self.reorder:
             for _ in range(n):
                 self.substep(dt)
@@ -413,7 +422,8 @@ class Simulator:
                                                  C.byref(self.spare.desc), C.byref(self.table.desc),
                                                  C.byref(self.buckets.desc), C.byref(self.grid.desc), dt,
                                                  C.byref(self.extf), self.cols, self.ncols, int(n),
-                                                 (1 if fuse_stress else 0) | (2 if graph else 0),
+                                                 (1 if fuse_stress else 0) | (2 if graph else 0) |
+                                                 (4 if autotune else 0),
                                                  C.byref(in_b)),
               "substeps_sorted")
         if in_b.value:
