@@ -181,6 +181,37 @@ def test_bucket_sort_shuffled_exact(cube):
     assert np.array_equal(gi, indices)
 
 
+@pytest.mark.parametrize("cap_log2", [17, 18])
+def test_bucket_sort_wide_buckets_exact(cube, cap_log2):
+    """Bucket capacities of 2^17 and 2^18 blocks: the runs are radix-sorted
+    in two 9-bit passes (three 8-bit passes below 2^16 keys and above 2^18).
+    Shuffled particles make nearly every particle its own run."""
+    import torch
+
+    rng = np.random.default_rng(23 + cap_log2)
+    perm = rng.permutation(cube.n)
+    x = np.ascontiguousarray(cube.x[perm])
+    rocm = _rocm()
+    sim = rocm.Simulator(cube)
+    sim.buckets = rocm.DeviceBuckets(1 << cap_log2, cube.n)
+    sim.parts.x.copy_(torch.as_tensor(x).cuda())
+    sim.clear_table()
+    sim.activate()
+    sim.enlarge()
+    sim.bucket_sort()
+    sim.sync()
+    ctx = O.Context()
+    ctx.activate(cube.table_size, x, cube.dx)
+    ctx.enlarge()
+    ctx.bucket_sort(x, cube.dx)
+    counts, offsets, indices = ctx.sort()
+    gc, go, gi = sim.buckets.host(ctx.sizes()["cnt"], cube.n)
+    assert sim.pol.counters()[2] > cube.n // 2  # runs
+    assert np.array_equal(gc, counts)
+    assert np.array_equal(go, offsets)
+    assert np.array_equal(gi, indices)
+
+
 @pytest.mark.parametrize("scene", ["cube", "sand"])
 def test_stages_parity(scene, request):
     """P2G grid (m, mv, rhs), grid update (velocities) and G2P state, stage by stage."""

@@ -494,11 +494,13 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
     ZS_HIP(ensure_side(pol));
     ZS_HIP(side.fork());
     StageScope sc(pol, ZSMPM_STAGE_STRESS, pol->side);
-    ZS_HIP(zsmpm::stress(m, ps, ws, pol->side, pol->closure_on ? &pol->closure : nullptr));
+    ZS_HIP(zsmpm::stress(m, ps, ws, pol->side, pol->closure_on ? &pol->closure : nullptr, nullptr,
+                         /*beside_sparsity=*/true));
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_TABLE_CLEAR);
-    ZS_HIP(table_clear(t, ws, s));
+    // the same launch clears the bucket counts the bucket pass adds to
+    ZS_HIP(table_clear(t, ws, s, reinterpret_cast<uint32_t*>(sv.counts), (int64_t)sv.cap_blocks + 1));
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_ACTIVATE);
@@ -512,23 +514,24 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
     StageScope sc(pol, ZSMPM_STAGE_BUCKET_SORT);
     // the activation above recorded every particle's key slot in the table it
     // just built from empty: the bucket pass reads those instead of x
-    ZS_HIP(bucket_sort_slots(ws, ps.n, sv, s));
+    ZS_HIP(bucket_sort_slots(ws, ps.n, sv, s, /*counts_zeroed=*/true));
     ZS_HIP(block_neighbours(t, ws, s));
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_GRID_CLEAR);
-    ZS_HIP(zsmpm::grid_clear(grid->data, t.cnt, grid->capacity_blocks, s));
+    // also resets the P2G bucket queue and the max |v|^2 counter
+    ZS_HIP(zsmpm::grid_clear(grid->data, t.cnt, grid->capacity_blocks, s, ws.counters));
   }
   ZS_HIP(side.join());
   {
     // P2G folds the force into the momentum channels (the driver's
     // v = (mv + rhs)/m convention), so the update must not add channels 4..6
     StageScope sc(pol, ZSMPM_STAGE_P2G);
-    ZS_HIP(zsmpm::p2g_scatter(ps, t, sv, grid->data, grid->capacity_blocks, dt, dx, true, ws, s));
+    ZS_HIP(zsmpm::p2g_scatter(ps, t, sv, grid->data, grid->capacity_blocks, dt, dx, true, ws, s,
+                              /*reset_queue=*/false));
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_GRID_UPDATE);
-    set_maxvel_kernel<<<1, 1, 0, s>>>(ws.counters);
     ZS_HIP(zsmpm::grid_update(grid->data, grid->capacity_blocks, t, dt, extf, cols, dx, 0, ws, s));
   }
   {

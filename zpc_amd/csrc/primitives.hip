@@ -178,34 +178,41 @@ __global__ __launch_bounds__(kThreads) void scan_down_kernel(const uint32_t* __r
 // ---------------------------------------------------------------- radix sort
 constexpr int kRsWaves = kThreads / kWave;
 
+// digit histogram of workgroup g's range, written digit-major: hist[d G + g]
+template <int kBits>
 __global__ __launch_bounds__(kThreads) void rs_hist_kernel(const uint32_t* __restrict__ keys,
                                                            const uint32_t* n_dev, uint32_t n_cap, int shift,
                                                            uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  constexpr uint32_t kBins = 1u << kBits;
+  __shared__ uint32_t h[kBins];
+  for (uint32_t d = threadIdx.x; d < kBins; d += kThreads) h[d] = 0;
   __syncthreads();
   const uint32_t n = load_n(n_dev, n_cap);
   uint32_t lo, hi;
   group_range(n, gridDim.x, blockIdx.x, lo, hi);
-  for (uint32_t i = lo + threadIdx.x; i < hi; i += kThreads) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kThreads) atomicAdd(&h[(keys[i] >> shift) & (kBins - 1u)], 1u);
   __syncthreads();
-  hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+  for (uint32_t d = threadIdx.x; d < kBins; d += kThreads) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
 }
 
 // Stable scatter of workgroup g's range: rounds of kThreads elements in
 // order; within a round, equal digits keep thread order (ballot matching).
+template <int kBits>
 __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, const uint32_t* n_dev, uint32_t n_cap,
     int shift, const uint32_t* __restrict__ hist_scanned, uint32_t* __restrict__ okeys,
     uint32_t* __restrict__ ovals) {
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wc[kRsWaves][256];
+  constexpr uint32_t kBins = 1u << kBits;
+  __shared__ uint32_t run[kBins];
+  __shared__ uint32_t wc[kRsWaves][kBins];
   const uint32_t n = load_n(n_dev, n_cap);
   uint32_t lo, hi;
   group_range(n, gridDim.x, blockIdx.x, lo, hi);
-  run[threadIdx.x] = hist_scanned[(size_t)threadIdx.x * gridDim.x + blockIdx.x];
+  for (uint32_t d = threadIdx.x; d < kBins; d += kThreads) {
+    run[d] = hist_scanned[(size_t)d * gridDim.x + blockIdx.x];
 #pragma unroll
-  for (int w = 0; w < kRsWaves; ++w) wc[w][threadIdx.x] = 0;
+    for (int w = 0; w < kRsWaves; ++w) wc[w][d] = 0;
+  }
   __syncthreads();
   const int wave = threadIdx.x / kWave;
   const uint64_t lt = lanemask_lt();
@@ -214,10 +221,10 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
     const bool valid = idx < hi;
     const uint32_t key = valid ? keys[idx] : 0u;
     const uint32_t val = valid ? vals[idx] : 0u;
-    const uint32_t d = (key >> shift) & 255u;
+    const uint32_t d = (key >> shift) & (kBins - 1u);
     uint64_t peers = ballot(valid);
 #pragma unroll
-    for (int bit = 0; bit < 8; ++bit) {
+    for (int bit = 0; bit < kBits; ++bit) {
       const bool on = (d >> bit) & 1u;
       const uint64_t b = ballot(on);
       peers &= on ? b : ~b;
@@ -232,13 +239,15 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
       ovals[pos] = val;
     }
     __syncthreads();
-    uint32_t add = 0;
+    for (uint32_t e = threadIdx.x; e < kBins; e += kThreads) {
+      uint32_t add = 0;
 #pragma unroll
-    for (int w = 0; w < kRsWaves; ++w) {
-      add += wc[w][threadIdx.x];
-      wc[w][threadIdx.x] = 0;
+      for (int w = 0; w < kRsWaves; ++w) {
+        add += wc[w][e];
+        wc[w][e] = 0;
+      }
+      run[e] += add;
     }
-    run[threadIdx.x] += add;
     __syncthreads();
   }
 }
@@ -283,14 +292,26 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
                             hipStream_t s, bool* result_in_alt) {
   *result_in_alt = false;
   if (n_cap == 0 || bits <= 0) return hipSuccess;
-  const uint32_t G = groups_for_cap(n_cap);
   static_assert(256 * kGroups <= kSmallThreads * kSmallItems, "digit histogram exceeds the one-workgroup scan");
+  static_assert(512 * (kGroups / 2) <= kSmallThreads * kSmallItems, "digit histogram exceeds the one-workgroup scan");
   static_assert(kGroups <= kSmallThreads, "group sums exceed the one-workgroup scan");
+  // 17- and 18-bit keys take two 9-bit passes instead of three 8-bit ones, on
+  // half the workgroups so that the 512-digit histogram fits the scan
+  const bool nine = bits > 16 && bits <= 18;
+  const uint32_t G8 = groups_for_cap(n_cap);
+  const uint32_t G = nine ? (G8 > (uint32_t)(kGroups / 2) ? (uint32_t)(kGroups / 2) : G8) : G8;
+  const int step = nine ? 9 : 8;
   uint32_t *ka = keys, *va = vals, *kb = keys_alt, *vb = vals_alt;
-  for (int shift = 0; shift < bits; shift += 8) {
-    rs_hist_kernel<<<G, kThreads, 0, s>>>(ka, n_dev, n_cap, shift, scratch);
-    scan_small_kernel<kSmallItems><<<1, kSmallThreads, 0, s>>>(scratch, 256 * G, nullptr);
-    rs_scatter_kernel<<<G, kThreads, 0, s>>>(ka, va, n_dev, n_cap, shift, scratch, kb, vb);
+  for (int shift = 0; shift < bits; shift += step) {
+    if (nine) {
+      rs_hist_kernel<9><<<G, kThreads, 0, s>>>(ka, n_dev, n_cap, shift, scratch);
+      scan_small_kernel<kSmallItems><<<1, kSmallThreads, 0, s>>>(scratch, 512 * G, nullptr);
+      rs_scatter_kernel<9><<<G, kThreads, 0, s>>>(ka, va, n_dev, n_cap, shift, scratch, kb, vb);
+    } else {
+      rs_hist_kernel<8><<<G, kThreads, 0, s>>>(ka, n_dev, n_cap, shift, scratch);
+      scan_small_kernel<kSmallItems><<<1, kSmallThreads, 0, s>>>(scratch, 256 * G, nullptr);
+      rs_scatter_kernel<8><<<G, kThreads, 0, s>>>(ka, va, n_dev, n_cap, shift, scratch, kb, vb);
+    }
     uint32_t* t = ka;
     ka = kb;
     kb = t;

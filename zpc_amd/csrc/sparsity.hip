@@ -41,8 +41,9 @@ inline int groups_for(int64_t n, int threads) {
 
 // --------------------------------------------------------------- clearing
 __global__ void clear_table_kernel(TableView t, uint64_t* skey, uint32_t* sorder, uint32_t ts,
-                                   uint32_t* counters) {
+                                   uint32_t* counters, uint32_t* zero, int64_t nzero) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nzero; e += stride) zero[e] = 0u;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < t.T; e += stride) {
     t.keys[3 * e] = kKeySentinel;
     t.keys[3 * e + 1] = kKeySentinel;
@@ -61,8 +62,10 @@ __global__ void clear_table_kernel(TableView t, uint64_t* skey, uint32_t* sorder
 }
 
 __global__ void clear_scratch_kernel(uint64_t* skey, uint32_t* sorder, uint32_t ts,
-                                     const int32_t* cnt, uint32_t* counters, int32_t* slot_block) {
+                                     const int32_t* cnt, uint32_t* counters, int32_t* slot_block,
+                                     uint32_t* bitmap, int64_t bm_n) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < bm_n; e += stride) bitmap[e] = 0u;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ts; e += stride) {
     skey[e] = kEmpty64;
     sorder[e] = 0xffffffffu;
@@ -75,64 +78,117 @@ __global__ void clear_scratch_kernel(uint64_t* skey, uint32_t* sorder, uint32_t 
 }
 
 // ----------------------------------------------------------- dedup inserts
-// returns the slot holding pk, or ~0u when the scratch table is full
-__device__ __forceinline__ uint32_t dedup_insert(uint64_t pk, uint32_t order, uint64_t* skey,
-                                                 uint32_t* sorder, uint32_t ts, uint32_t* err) {
-  uint32_t s = (uint32_t)mix64(pk) & (ts - 1u);
-  for (uint32_t probe = 0; probe < ts; ++probe) {
-    const uint64_t prev = atomicCAS((unsigned long long*)&skey[s], (unsigned long long)kEmpty64,
-                                    (unsigned long long)pk);
-    if (prev == kEmpty64 || prev == pk) {
-      atomicMin(&sorder[s], order);
-      return s;
+__device__ __forceinline__ uint32_t dedup_home(uint64_t pk, uint32_t ts) { return (uint32_t)mix64(pk) & (ts - 1u); }
+
+// Finishes an insert whose probe at slot s returned prev: probes on until the
+// slot is free or holds pk (ts probes in all), then lowers the slot's order.
+// Returns the slot holding pk, or ~0u when the scratch table is full.
+__device__ __forceinline__ uint32_t dedup_finish(uint64_t pk, uint32_t order, uint32_t s, uint64_t prev,
+                                                 uint64_t* skey, uint32_t* sorder, uint32_t ts, uint32_t* err) {
+  for (uint32_t probe = 1; prev != kEmpty64 && prev != pk; ++probe) {
+    if (probe >= ts) {
+      atomicOr(err, kErrTableFull);
+      return ~0u;
     }
     s = (s + 1u) & (ts - 1u);
+    prev = atomicCAS((unsigned long long*)&skey[s], (unsigned long long)kEmpty64, (unsigned long long)pk);
   }
-  atomicOr(err, kErrTableFull);
-  return ~0u;
+  atomicMin(&sorder[s], order);
+  return s;
 }
 
+__device__ __forceinline__ uint32_t dedup_insert(uint64_t pk, uint32_t order, uint64_t* skey,
+                                                 uint32_t* sorder, uint32_t ts, uint32_t* err) {
+  const uint32_t s = dedup_home(pk, ts);
+  const uint64_t prev =
+      atomicCAS((unsigned long long*)&skey[s], (unsigned long long)kEmpty64, (unsigned long long)pk);
+  return dedup_finish(pk, order, s, prev, skey, sorder, ts, err);
+}
+
+constexpr int kActItems = 4;  // 64-particle groups per wave, their loads and first probes issued together
+
 // ComputeSparsity candidates: one per particle, order = particle index. Each
-// distinct key of a wave is inserted once, by its lowest lane (the smallest
-// particle index holding it), so hot keys see one atomic per wave. With
-// pslot, every particle's scratch slot (the identity of its key) is recorded
-// for the bucket sort that follows.
+// distinct key of a 64-particle group is inserted once, by its lowest lane
+// (the smallest particle index holding it), so hot keys see one atomic per
+// group. A wave takes kActItems groups and issues their x loads, then their
+// first probes, together: the kernel is bound by those round trips, not by
+// bytes. The dedup hash keeps the minimum order per key, so the result does
+// not depend on the order of the inserts. With pslot, every particle's
+// scratch slot (the identity of its key) is recorded for the bucket sort.
 __global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restrict__ x, int64_t n,
                                                             float dxinv, int blockLen, int offset,
                                                             float disp, uint64_t* skey, uint32_t* sorder,
                                                             uint32_t ts, uint32_t* err, uint32_t* pslot) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t span = (int64_t)kThreads * kActItems;
+  const int64_t stride = (int64_t)gridDim.x * span;
   const int lane = lane_id();
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
-    const int64_t p = base + threadIdx.x;
-    const bool valid = p < n;
-    uint64_t pk = kEmpty64;
-    if (valid) {
-      int32_t k0, k1, k2;
-      block_key(x[3 * p], x[3 * p + 1], x[3 * p + 2], dxinv, blockLen, offset, disp, k0, k1, k2);
-      if (key_packable(k0, k1, k2)) {
-        pk = pack_key(k0, k1, k2);
-      } else {
-        atomicOr(err, kErrKeyRange);
+  for (int64_t base = (int64_t)blockIdx.x * span; base < n; base += stride) {
+    float cx[kActItems], cy[kActItems], cz[kActItems];
+#pragma unroll
+    for (int r = 0; r < kActItems; ++r) {
+      const int64_t p = base + (int64_t)r * kThreads + threadIdx.x;
+      cx[r] = cy[r] = cz[r] = 0.f;
+      if (p < n) {
+        cx[r] = x[3 * p];
+        cy[r] = x[3 * p + 1];
+        cz[r] = x[3 * p + 2];
       }
     }
-    // peel off one distinct key per iteration: the lowest pending lane leads
-    bool pending = pk != kEmpty64;
-    bool head = false;
-    int my_leader = lane;
-    for (uint64_t m = ballot(pending); m != 0; m = ballot(pending)) {
-      const int leader = __ffsll((long long)m) - 1;
-      const uint64_t lk = shfl(pk, leader);
-      const bool same = pending && pk == lk;
-      my_leader = same ? leader : my_leader;
-      head = head || lane == leader;
-      pending = pending && !same;
+    uint64_t pk[kActItems];
+    int leader[kActItems];
+    bool head[kActItems];
+#pragma unroll
+    for (int r = 0; r < kActItems; ++r) {
+      const int64_t p = base + (int64_t)r * kThreads + threadIdx.x;
+      pk[r] = kEmpty64;
+      if (p < n) {
+        int32_t k0, k1, k2;
+        block_key(cx[r], cy[r], cz[r], dxinv, blockLen, offset, disp, k0, k1, k2);
+        if (key_packable(k0, k1, k2)) {
+          pk[r] = pack_key(k0, k1, k2);
+        } else {
+          atomicOr(err, kErrKeyRange);
+        }
+      }
+      // peel off one distinct key per iteration: the lowest pending lane leads
+      bool pending = pk[r] != kEmpty64;
+      bool h = false;
+      int my_leader = lane;
+      for (uint64_t m = ballot(pending); m != 0; m = ballot(pending)) {
+        const int ld = __ffsll((long long)m) - 1;
+        const uint64_t lk = shfl(pk[r], ld);
+        const bool same = pending && pk[r] == lk;
+        my_leader = same ? ld : my_leader;
+        h = h || lane == ld;
+        pending = pending && !same;
+      }
+      head[r] = h;
+      leader[r] = my_leader;
     }
-    uint32_t slot = ~0u;
-    if (head) slot = dedup_insert(pk, (uint32_t)p, skey, sorder, ts, err);
+    uint32_t slot[kActItems];
+    uint64_t prev[kActItems];
+#pragma unroll
+    for (int r = 0; r < kActItems; ++r) {
+      slot[r] = ~0u;
+      prev[r] = kEmpty64;
+      if (head[r]) {
+        slot[r] = dedup_home(pk[r], ts);
+        prev[r] = atomicCAS((unsigned long long*)&skey[slot[r]], (unsigned long long)kEmpty64,
+                            (unsigned long long)pk[r]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kActItems; ++r) {
+      const int64_t p = base + (int64_t)r * kThreads + threadIdx.x;
+      if (head[r]) slot[r] = dedup_finish(pk[r], (uint32_t)p, slot[r], prev[r], skey, sorder, ts, err);
+    }
     if (pslot) {
-      const uint32_t ls = (uint32_t)__shfl((int)slot, my_leader, kWave);
-      if (valid) pslot[p] = pk != kEmpty64 ? ls : ~0u;
+#pragma unroll
+      for (int r = 0; r < kActItems; ++r) {
+        const int64_t p = base + (int64_t)r * kThreads + threadIdx.x;
+        const uint32_t ls = (uint32_t)__shfl((int)slot[r], leader[r], kWave);
+        if (p < n) pslot[p] = pk[r] != kEmpty64 ? ls : ~0u;
+      }
     }
   }
 }
@@ -294,6 +350,10 @@ __global__ __launch_bounds__(kThreads) void fill_keys_kernel(TableView t, const 
                                                              uint32_t list_cap, uint32_t* err) {
   const uint32_t nl = min(counters[kCtrList], list_cap);
   const uint32_t cnt0 = counters[kCtrCnt0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {  // nothing in this kernel reads the count
+    const int64_t c = (int64_t)cnt0 + nl;
+    *t.cnt = (int32_t)(c < t.T ? c : t.T);
+  }
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nl; i += stride) {
     const int32_t r = (int32_t)(cnt0 + (uint32_t)i);
@@ -314,12 +374,6 @@ __global__ __launch_bounds__(kThreads) void fill_keys_kernel(TableView t, const 
     t.keys[3 * (int64_t)s + 1] = k1;
     t.keys[3 * (int64_t)s + 2] = k2;
   }
-}
-
-__global__ void bump_cnt_kernel(TableView t, const uint32_t* counters, uint32_t list_cap) {
-  const uint32_t nl = min(counters[kCtrList], list_cap);
-  const int64_t c = (int64_t)counters[kCtrCnt0] + nl;
-  *t.cnt = (int32_t)(c < t.T ? c : t.T);
 }
 
 __global__ void query_kernel(TableView t, const int32_t* __restrict__ keys, int64_t n, int32_t* out) {
@@ -623,14 +677,18 @@ int bits_for(uint64_t maxval) {
   return b;
 }
 
+// words of the first-occurrence bitmap for orders below order_bound (orders
+// beyond the bitmap are reported by assign)
+uint64_t bitmap_words(const Workspace& ws, uint64_t order_bound) {
+  const uint64_t words = order_bound / 32 + 1;
+  return words > ws.bm_words ? ws.bm_words : words;
+}
+
 // dedup entries -> ranked, placed and keyed table entries (shared tail of
-// activation and enlargement)
+// activation and enlargement); the bitmap was cleared with the dedup scratch
 hipError_t finish_insertion(const TableView& t, Workspace& ws, uint64_t order_bound, hipStream_t s,
                             int32_t* slot_block = nullptr) {
-  uint64_t words = order_bound / 32 + 1;
-  if (words > ws.bm_words) words = ws.bm_words;  // orders beyond the bitmap are reported by assign
-  zero_u32_kernel<<<groups_for((int64_t)words, kThreads) < 2048 ? groups_for((int64_t)words, kThreads) : 2048,
-                    kThreads, 0, s>>>(ws.bitmap, (int64_t)words);
+  const uint64_t words = bitmap_words(ws, order_bound);
   collect_new_kernel<<<kCollectGroups, kThreads, 0, s>>>(ws.skey, ws.sorder, ws.ts_cap, t, ws.counters, ws.la_key,
                                                          ws.la_val, ws.list_cap, ws.err, ws.bitmap, (uint32_t)words);
   hipError_t e = exclusive_scan_popc_u32(ws.bitmap, ws.bm_prefix, (uint32_t)words, ws.scan_tmp, s);
@@ -639,16 +697,15 @@ hipError_t finish_insertion(const TableView& t, Workspace& ws, uint64_t order_bo
   assign_kernel<<<gl, kThreads, 0, s>>>(ws.la_key, ws.la_val, ws.bitmap, ws.bm_prefix, (uint32_t)words, ws.skey, t,
                                         ws.counters, ws.list_cap, ws.err, slot_block);
   fill_keys_kernel<<<gl, kThreads, 0, s>>>(t, ws.counters, ws.list_cap, ws.err);
-  bump_cnt_kernel<<<1, 1, 0, s>>>(t, ws.counters, ws.list_cap);
   return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t table_clear(const TableView& t, Workspace& ws, hipStream_t s) {
+hipError_t table_clear(const TableView& t, Workspace& ws, hipStream_t s, uint32_t* zero, int64_t nzero) {
   const int64_t m = (int64_t)t.T > (int64_t)ws.ts_cap ? t.T : ws.ts_cap;
   clear_table_kernel<<<groups_for(m, kThreads), kThreads, 0, s>>>(t, ws.skey, ws.sorder, ws.ts_cap,
-                                                                   ws.counters);
+                                                                   ws.counters, zero, zero ? nzero : 0);
   return hipGetLastError();
 }
 
@@ -656,26 +713,28 @@ hipError_t table_activate(const TableView& t, Workspace& ws, const float* x, int
                           int blockLen, int offset, float disp, hipStream_t s, bool record_slots) {
   if (blockLen <= 0) return hipErrorInvalidValue;
   int32_t* slot_block = record_slots ? ws.slot_block : nullptr;
-  clear_scratch_kernel<<<groups_for(ws.ts_cap, kThreads), kThreads, 0, s>>>(ws.skey, ws.sorder, ws.ts_cap,
-                                                                             t.cnt, ws.counters, slot_block);
+  const uint64_t order_bound = (uint64_t)(n > 0 ? n : 1);
+  clear_scratch_kernel<<<groups_for(ws.ts_cap, kThreads), kThreads, 0, s>>>(
+      ws.skey, ws.sorder, ws.ts_cap, t.cnt, ws.counters, slot_block, ws.bitmap, (int64_t)bitmap_words(ws, order_bound));
   const float dxinv = (float)1.0 / dx;
   if (n > 0)
-    activate_kernel<<<groups_for(n, kThreads), kThreads, 0, s>>>(x, n, dxinv, blockLen, offset, disp,
+    activate_kernel<<<groups_for(n, kThreads * kActItems), kThreads, 0, s>>>(x, n, dxinv, blockLen, offset, disp,
                                                                   ws.skey, ws.sorder, ws.ts_cap, ws.err,
                                                                   record_slots ? ws.pslot : nullptr);
-  return finish_insertion(t, ws, (uint64_t)(n > 0 ? n : 1), s, slot_block);
+  return finish_insertion(t, ws, order_bound, s, slot_block);
 }
 
 hipError_t table_enlarge(const TableView& t, Workspace& ws, const int32_t lo[3], const int32_t hi[3],
                          int32_t n0_cap, hipStream_t s) {
   const int32_t e0 = hi[0] - lo[0], e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
   if (e0 <= 0 || e1 <= 0 || e2 <= 0) return hipSuccess;
-  clear_scratch_kernel<<<groups_for(ws.ts_cap, kThreads), kThreads, 0, s>>>(ws.skey, ws.sorder, ws.ts_cap,
-                                                                             t.cnt, ws.counters, nullptr);
   const int64_t total_cap = (int64_t)n0_cap * e0 * e1 * e2;
+  const uint64_t order_bound = (uint64_t)(total_cap > 0 ? total_cap : 1);
+  clear_scratch_kernel<<<groups_for(ws.ts_cap, kThreads), kThreads, 0, s>>>(
+      ws.skey, ws.sorder, ws.ts_cap, t.cnt, ws.counters, nullptr, ws.bitmap, (int64_t)bitmap_words(ws, order_bound));
   enlarge_kernel<<<groups_for(total_cap, kThreads), kThreads, 0, s>>>(
       t.active, ws.counters, lo[0], lo[1], lo[2], e0, e1, e2, ws.skey, ws.sorder, ws.ts_cap, ws.err);
-  return finish_insertion(t, ws, (uint64_t)(total_cap > 0 ? total_cap : 1), s);
+  return finish_insertion(t, ws, order_bound, s);
 }
 
 hipError_t block_neighbours(const TableView& t, Workspace& ws, hipStream_t s) {
@@ -738,11 +797,11 @@ hipError_t bucket_sort(const TableView& t, Workspace& ws, const float* x, int64_
   return place_runs(ws, n, out, ntiles, gt, s);
 }
 
-hipError_t bucket_sort_slots(Workspace& ws, int64_t n, const SortView& out, hipStream_t s) {
+hipError_t bucket_sort_slots(Workspace& ws, int64_t n, const SortView& out, hipStream_t s, bool counts_zeroed) {
   if (n > (int64_t)ws.n_cap) return hipErrorInvalidValue;
   const int64_t nb1 = (int64_t)out.cap_blocks + 1;
   uint32_t* counts = (uint32_t*)out.counts;
-  zero_u32_kernel<<<groups_for(nb1, kThreads), kThreads, 0, s>>>(counts, nb1);
+  if (!counts_zeroed) zero_u32_kernel<<<groups_for(nb1, kThreads), kThreads, 0, s>>>(counts, nb1);
   if (n <= 0) {
     return exclusive_scan_u32(counts, (uint32_t*)out.offsets, nullptr, (uint32_t)nb1, ws.scan_tmp, nullptr, s);
   }

@@ -68,7 +68,11 @@ __device__ __forceinline__ void node_home(int r0, int r1, int r2, int& nb, int& 
   cell = ((r0 & 3) << 4) | ((r1 & 3) << 2) | (r2 & 3);
 }
 
-__global__ void grid_clear_kernel(float* grid, const int32_t* nblocks, int32_t cap) {
+__global__ void grid_clear_kernel(float* grid, const int32_t* nblocks, int32_t cap, uint32_t* counters) {
+  if (counters && blockIdx.x == 0 && threadIdx.x == 0) {
+    counters[kCtrQueue] = 0u;
+    counters[kCtrMaxVel] = 0u;
+  }
   const int32_t nb = min(*nblocks, cap);
   const int64_t total = (int64_t)nb * kChannels * kBlockCells / 4;
   float4* g4 = reinterpret_cast<float4*>(grid);
@@ -964,6 +968,14 @@ __global__ __launch_bounds__(kThreads) void reorder_kernel(ParticlesView in, Par
 
 }  // namespace
 
+// Dynamic LDS the stress pass reserves (and does not use) when it runs beside
+// the sparsity stages: with its 9,280 bytes of static LDS, three workgroups
+// fit in a CU's 160 KiB and a fourth does not. Uncapped, its workgroups (120
+// VGPRs, four waves per SIMD) fill the register files and the sparsity
+// kernels wait for CUs; capped, both finish earlier (measured on the 16M
+// sand column: 2.52 -> 2.46 ms per substep; a cap of two is slower again).
+constexpr size_t kStressLdsReserve = 40960;
+
 hipError_t check_launch_limits(int device) {
   int max_lds = 0;
   hipError_t e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
@@ -973,39 +985,43 @@ hipError_t check_launch_limits(int device) {
       reinterpret_cast<const void*>(&p2g_kernel<true>), reinterpret_cast<const void*>(&p2g_kernel<false>),
       reinterpret_cast<const void*>(&g2p_kernel<false, -1>), reinterpret_cast<const void*>(&g2p_kernel<true, -1>),
       reinterpret_cast<const void*>(&g2p_kernel<true, 0>), reinterpret_cast<const void*>(&g2p_kernel<true, 1>)};
-  for (const void* k : kernels) {
+  for (int i = 0; i < (int)(sizeof(kernels) / sizeof(kernels[0])); ++i) {
     hipFuncAttributes attr;
-    e = hipFuncGetAttributes(&attr, k);
+    e = hipFuncGetAttributes(&attr, kernels[i]);
     if (e != hipSuccess) return e;
-    if ((int)attr.sharedSizeBytes > max_lds) return hipErrorInvalidConfiguration;
+    const size_t extra = i < 2 ? kStressLdsReserve : 0;  // the stress kernels' reserve
+    if (attr.sharedSizeBytes + extra > (size_t)max_lds) return hipErrorInvalidConfiguration;
   }
   return closure_launch_limits();
 }
 
-hipError_t grid_clear(float* grid, const int32_t* nblocks_dev, int32_t cap_blocks, hipStream_t s) {
+hipError_t grid_clear(float* grid, const int32_t* nblocks_dev, int32_t cap_blocks, hipStream_t s,
+                      uint32_t* counters) {
   const int64_t total = (int64_t)cap_blocks * kChannels * kBlockCells / 4;
-  grid_clear_kernel<<<groups_for(total, kThreads), kThreads, 0, s>>>(grid, nblocks_dev, cap_blocks);
+  grid_clear_kernel<<<groups_for(total, kThreads), kThreads, 0, s>>>(grid, nblocks_dev, cap_blocks, counters);
   return hipGetLastError();
 }
 
 hipError_t stress(const Model& m, const ParticlesView& ps, Workspace& ws, hipStream_t s, const ClosureView* closure,
-                  float* contrib) {
+                  float* contrib, bool beside_sparsity) {
   if (ps.n <= 0) return hipSuccess;
   float* out = contrib ? contrib : ws.contrib;
   if (closure && m.kind == 1 && m.mu != 0.f) return stress_closure(m, *closure, ps, out, ws, s);
   const int64_t waves = (ps.n + kWave - 1) / kWave;
   const int g = groups_for(waves * kWave, kThreads);
+  const size_t lds = beside_sparsity ? kStressLdsReserve : 0;
   if (m.kind == 0)
-    stress_kernel<0><<<g, kThreads, 0, s>>>(m, ps, out, ws.counters);
+    stress_kernel<0><<<g, kThreads, lds, s>>>(m, ps, out, ws.counters);
   else
-    stress_kernel<1><<<g, kThreads, 0, s>>>(m, ps, out, ws.counters);
+    stress_kernel<1><<<g, kThreads, lds, s>>>(m, ps, out, ws.counters);
   return hipGetLastError();
 }
 
 hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
-                       int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s) {
+                       int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s,
+                       bool reset_queue) {
   const int g = ws.grid_p2g > 0 ? ws.grid_p2g : persistent_groups();
-  reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
+  if (reset_queue) reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
   uint32_t* q = ws.counters + kCtrQueue;
   if (fold)
     p2g_kernel<true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q);

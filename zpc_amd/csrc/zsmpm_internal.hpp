@@ -161,7 +161,9 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
                             hipStream_t s, bool* result_in_alt);
 
 // ---------------------------------------------------------------- sparsity
-hipError_t table_clear(const TableView& t, Workspace& ws, hipStream_t s);
+// zero[0, nzero) is cleared by the same launch (the substep's bucket counts)
+hipError_t table_clear(const TableView& t, Workspace& ws, hipStream_t s, uint32_t* zero = nullptr,
+                       int64_t nzero = 0);
 // record_slots: also fill ws.pslot / ws.slot_block for bucket_sort_slots
 // (the table must be empty: every key is then new)
 hipError_t table_activate(const TableView& t, Workspace& ws, const float* x, int64_t n, float dx,
@@ -173,17 +175,25 @@ hipError_t bucket_sort(const TableView& t, Workspace& ws, const float* x, int64_
                        int blockLen, int offset, float disp, const SortView& out, hipStream_t s);
 // bucket_sort of the particles a recording activation just inserted into an
 // emptied table (blockLen 4, offset -2, displacement 0.5 as that activation)
-hipError_t bucket_sort_slots(Workspace& ws, int64_t n, const SortView& out, hipStream_t s);
+// counts_zeroed: out.counts[0, cap_blocks] is already zero (table_clear did it)
+hipError_t bucket_sort_slots(Workspace& ws, int64_t n, const SortView& out, hipStream_t s,
+                             bool counts_zeroed = false);
 // ws.nbr for every active block (the 2x2x2 blocks a bucket's stencils cover)
 hipError_t block_neighbours(const TableView& t, Workspace& ws, hipStream_t s);
 
 // ---------------------------------------------------------------- transfer
 // every transfer kernel's static LDS fits the device's per-workgroup limit
 hipError_t check_launch_limits(int device);
-hipError_t grid_clear(float* grid, const int32_t* nblocks_dev, int32_t cap_blocks, hipStream_t s);
-// P2G = stress (per particle, writes ws.contrib and logJp) + scatter (per bucket)
+// with counters: also resets the P2G bucket queue and the max |v|^2 counter
+// (then p2g_scatter is called with reset_queue = false)
+hipError_t grid_clear(float* grid, const int32_t* nblocks_dev, int32_t cap_blocks, hipStream_t s,
+                      uint32_t* counters = nullptr);
+// P2G = stress (per particle, writes ws.contrib and logJp) + scatter (per bucket).
+// beside_sparsity: the pass runs on a second stream next to the sparsity
+// stages, and leaves room on every CU for them (at most three of its
+// workgroups per CU)
 hipError_t stress(const Model& m, const ParticlesView& ps, Workspace& ws, hipStream_t s,
-                  const ClosureView* closure = nullptr, float* contrib = nullptr);
+                  const ClosureView* closure = nullptr, float* contrib = nullptr, bool beside_sparsity = false);
 
 // ---------------------------------------------------------------- closure
 // stress pass of the learned closure into contrib (SVD fallback queued)
@@ -198,7 +208,8 @@ hipError_t mlp_harvest(const Model& m, const float* F, const float* logJp, int64
                        hipStream_t s);
 hipError_t closure_launch_limits();
 hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
-                       int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s);
+                       int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s,
+                       bool reset_queue = true);
 hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float dt, const float extf[3],
                        const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s);
 // out == nullptr updates ps in place; otherwise the updated particles (and
