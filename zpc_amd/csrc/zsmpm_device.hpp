@@ -128,6 +128,13 @@ __device__ __forceinline__ void wave_lds_barrier() {
   asm volatile("" ::: "memory");
 }
 
+// Particle rows are streamed: every kernel reads a row once and writes it
+// once, and the next reader is a later kernel. Their 16-byte vector loads and
+// stores are non-temporal, so the streams do not evict what is re-used from
+// L2 (staged grid velocities, table and bucket arrays). On the 16M sand
+// column this took the substep from 2.46 to 2.36 ms (G2P 0.86 -> 0.82 ms).
+typedef float zs_f4 __attribute__((ext_vector_type(4)));
+
 // Split-phase load of rows q0 .. q0+cnt-1 (cnt <= 64) of W floats: issue()
 // puts the wave's global vector loads in flight (no LDS, no wait), finish()
 // transposes them through the wave's LDS stage so lane l < cnt receives row
@@ -149,7 +156,12 @@ struct RowLoad {
 #pragma unroll
     for (int k = 0; k < kVec; ++k) {
       const int64_t j = lane + (int64_t)k * kWave;
-      v[k] = j < r.nvec ? *reinterpret_cast<const float4*>(base + r.a4 + 4 * j) : make_float4(0.f, 0.f, 0.f, 0.f);
+      if (j < r.nvec) {
+        const zs_f4 t = __builtin_nontemporal_load(reinterpret_cast<const zs_f4*>(base + r.a4 + 4 * j));
+        v[k] = make_float4(t.x, t.y, t.z, t.w);
+      } else {
+        v[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
     }
   }
   __device__ __forceinline__ void finish(float* stage, int32_t cnt, float (&out)[W]) {
@@ -195,7 +207,8 @@ __device__ __forceinline__ void wave_store_rows(float* __restrict__ base, int64_
   const int64_t head = r.a4 - r.a;
   if (lane < head) base[r.a + lane] = stage[r.shift + lane];
   for (int64_t j = lane; j < r.nvec; j += kWave)
-    *reinterpret_cast<float4*>(base + r.a4 + 4 * j) = *reinterpret_cast<const float4*>(stage + r.shift + head + 4 * j);
+    __builtin_nontemporal_store(*reinterpret_cast<const zs_f4*>(stage + r.shift + head + 4 * j),
+                                reinterpret_cast<zs_f4*>(base + r.a4 + 4 * j));
   const int64_t t0 = r.a4 + 4 * r.nvec;
   if (lane < r.b - t0) base[t0 + lane] = stage[r.shift + (t0 - r.a) + lane];
   wave_lds_barrier();
