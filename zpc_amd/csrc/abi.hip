@@ -294,6 +294,7 @@ int32_t reserve(zsmpm_policy_s* p, int64_t n, int32_t T) {
   ZA(w.bm_prefix, w.bm_words);
   ZA(w.pslot, w.n_cap);
   ZA(w.slot_block, w.ts_cap);
+  ZA(w.dmask, w.ts_cap / 32 + 1);
   p->own_cap_blocks = (int32_t)w.ts_cap;
   ZA(p->own_counts, (size_t)p->own_cap_blocks + 1);
   ZA(p->own_offsets, (size_t)p->own_cap_blocks + 1);

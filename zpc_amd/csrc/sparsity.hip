@@ -40,9 +40,10 @@ inline int groups_for(int64_t n, int threads) {
 }
 
 // --------------------------------------------------------------- clearing
-__global__ void clear_table_kernel(TableView t, uint64_t* skey, uint32_t* sorder, uint32_t ts,
+__global__ void clear_table_kernel(TableView t, uint64_t* skey, uint32_t* sorder, uint32_t* dmask, uint32_t ts,
                                    uint32_t* counters, uint32_t* zero, int64_t nzero) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)((ts + 31u) >> 5); e += stride) dmask[e] = 0u;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nzero; e += stride) zero[e] = 0u;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < t.T; e += stride) {
     t.keys[3 * e] = kKeySentinel;
@@ -61,10 +62,11 @@ __global__ void clear_table_kernel(TableView t, uint64_t* skey, uint32_t* sorder
   }
 }
 
-__global__ void clear_scratch_kernel(uint64_t* skey, uint32_t* sorder, uint32_t ts,
+__global__ void clear_scratch_kernel(uint64_t* skey, uint32_t* sorder, uint32_t* dmask, uint32_t ts,
                                      const int32_t* cnt, uint32_t* counters, int32_t* slot_block,
                                      uint32_t* bitmap, int64_t bm_n) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)((ts + 31u) >> 5); e += stride) dmask[e] = 0u;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < bm_n; e += stride) bitmap[e] = 0u;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ts; e += stride) {
     skey[e] = kEmpty64;
@@ -78,31 +80,45 @@ __global__ void clear_scratch_kernel(uint64_t* skey, uint32_t* sorder, uint32_t 
 }
 
 // ----------------------------------------------------------- dedup inserts
+// The dedup scratch of one insertion pass: packed keys and minimum orders per
+// slot, and one bit per slot the pass claimed, so the pass's keys are found
+// by reading ts/32 words instead of all ts slots. (A list of claimed slots
+// behind one atomic counter was slower: tens of thousands of same-address
+// atomics serialize.)
+struct Dedup {
+  uint64_t* skey;
+  uint32_t* sorder;
+  uint32_t ts;
+  uint32_t* err;
+  uint32_t* dmask;  // [ts / 32]
+};
+
 __device__ __forceinline__ uint32_t dedup_home(uint64_t pk, uint32_t ts) { return (uint32_t)mix64(pk) & (ts - 1u); }
 
 // Finishes an insert whose probe at slot s returned prev: probes on until the
-// slot is free or holds pk (ts probes in all), then lowers the slot's order.
-// Returns the slot holding pk, or ~0u when the scratch table is full.
+// slot is free or holds pk (ts probes in all), then lowers the slot's order;
+// the insert that claims a free slot marks it. Returns the slot holding pk, or
+// ~0u when the scratch table is full.
 __device__ __forceinline__ uint32_t dedup_finish(uint64_t pk, uint32_t order, uint32_t s, uint64_t prev,
-                                                 uint64_t* skey, uint32_t* sorder, uint32_t ts, uint32_t* err) {
+                                                 const Dedup& d) {
   for (uint32_t probe = 1; prev != kEmpty64 && prev != pk; ++probe) {
-    if (probe >= ts) {
-      atomicOr(err, kErrTableFull);
+    if (probe >= d.ts) {
+      atomicOr(d.err, kErrTableFull);
       return ~0u;
     }
-    s = (s + 1u) & (ts - 1u);
-    prev = atomicCAS((unsigned long long*)&skey[s], (unsigned long long)kEmpty64, (unsigned long long)pk);
+    s = (s + 1u) & (d.ts - 1u);
+    prev = atomicCAS((unsigned long long*)&d.skey[s], (unsigned long long)kEmpty64, (unsigned long long)pk);
   }
-  atomicMin(&sorder[s], order);
+  atomicMin(&d.sorder[s], order);
+  if (prev == kEmpty64) atomicOr(&d.dmask[s >> 5], 1u << (s & 31u));
   return s;
 }
 
-__device__ __forceinline__ uint32_t dedup_insert(uint64_t pk, uint32_t order, uint64_t* skey,
-                                                 uint32_t* sorder, uint32_t ts, uint32_t* err) {
-  const uint32_t s = dedup_home(pk, ts);
+__device__ __forceinline__ uint32_t dedup_insert(uint64_t pk, uint32_t order, const Dedup& d) {
+  const uint32_t s = dedup_home(pk, d.ts);
   const uint64_t prev =
-      atomicCAS((unsigned long long*)&skey[s], (unsigned long long)kEmpty64, (unsigned long long)pk);
-  return dedup_finish(pk, order, s, prev, skey, sorder, ts, err);
+      atomicCAS((unsigned long long*)&d.skey[s], (unsigned long long)kEmpty64, (unsigned long long)pk);
+  return dedup_finish(pk, order, s, prev, d);
 }
 
 constexpr int kActItems = 4;  // 64-particle groups per wave, their loads and first probes issued together
@@ -117,8 +133,8 @@ constexpr int kActItems = 4;  // 64-particle groups per wave, their loads and fi
 // scratch slot (the identity of its key) is recorded for the bucket sort.
 __global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restrict__ x, int64_t n,
                                                             float dxinv, int blockLen, int offset,
-                                                            float disp, uint64_t* skey, uint32_t* sorder,
-                                                            uint32_t ts, uint32_t* err, uint32_t* pslot) {
+                                                            float disp, Dedup dd, uint32_t* pslot) {
+  uint32_t* err = dd.err;
   const int64_t span = (int64_t)kThreads * kActItems;
   const int64_t stride = (int64_t)gridDim.x * span;
   const int lane = lane_id();
@@ -172,15 +188,15 @@ __global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restr
       slot[r] = ~0u;
       prev[r] = kEmpty64;
       if (head[r]) {
-        slot[r] = dedup_home(pk[r], ts);
-        prev[r] = atomicCAS((unsigned long long*)&skey[slot[r]], (unsigned long long)kEmpty64,
+        slot[r] = dedup_home(pk[r], dd.ts);
+        prev[r] = atomicCAS((unsigned long long*)&dd.skey[slot[r]], (unsigned long long)kEmpty64,
                             (unsigned long long)pk[r]);
       }
     }
 #pragma unroll
     for (int r = 0; r < kActItems; ++r) {
       const int64_t p = base + (int64_t)r * kThreads + threadIdx.x;
-      if (head[r]) slot[r] = dedup_finish(pk[r], (uint32_t)p, slot[r], prev[r], skey, sorder, ts, err);
+      if (head[r]) slot[r] = dedup_finish(pk[r], (uint32_t)p, slot[r], prev[r], dd);
     }
     if (pslot) {
 #pragma unroll
@@ -197,8 +213,8 @@ __global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restr
 __global__ __launch_bounds__(kThreads) void enlarge_kernel(const int32_t* __restrict__ active,
                                                            const uint32_t* counters, int32_t lo0, int32_t lo1,
                                                            int32_t lo2, int32_t e0, int32_t e1, int32_t e2,
-                                                           uint64_t* skey, uint32_t* sorder, uint32_t ts,
-                                                           uint32_t* err) {
+                                                           Dedup dd) {
+  uint32_t* err = dd.err;
   const int64_t n0 = counters[kCtrCnt0];
   const int64_t nd = (int64_t)e0 * e1 * e2;
   const int64_t total = n0 * nd;
@@ -213,7 +229,7 @@ __global__ __launch_bounds__(kThreads) void enlarge_kernel(const int32_t* __rest
       atomicOr(err, kErrKeyRange);
       continue;
     }
-    (void)dedup_insert(pack_key(k0, k1, k2), (uint32_t)c, skey, sorder, ts, err);
+    (void)dedup_insert(pack_key(k0, k1, k2), (uint32_t)c, dd);
   }
 }
 
@@ -230,13 +246,16 @@ __device__ __forceinline__ void list_put(uint32_t slot, uint32_t key, uint32_t v
   }
 }
 
-// Gather the dedup entries whose key is not in the table yet: (order, slot).
-// Each workgroup scans one contiguous range, collects its entries in LDS and
-// reserves list space with a single global atomic (entries beyond the LDS
-// buffer go straight to the list). The list order is irrelevant: it is
-// sorted by order next.
+// Gather the dedup entries whose key is not in the table yet: (order, slot),
+// and mark each one's order in the first-occurrence bitmap. Only the slots
+// the pass claimed (set bits of dmask) are visited, not all ts. Each
+// workgroup takes one contiguous range of dmask words, collects its entries in
+// LDS and reserves list space with a single global atomic (entries beyond the
+// LDS buffer go straight to the list). The list order is irrelevant: ranks
+// come from the bitmap.
 __global__ __launch_bounds__(kThreads) void collect_new_kernel(const uint64_t* __restrict__ skey,
                                                                const uint32_t* __restrict__ sorder,
+                                                               const uint32_t* __restrict__ dmask,
                                                                uint32_t ts, TableView t, uint32_t* counters,
                                                                uint32_t* lkey, uint32_t* lval,
                                                                uint32_t list_cap, uint32_t* err,
@@ -244,43 +263,50 @@ __global__ __launch_bounds__(kThreads) void collect_new_kernel(const uint64_t* _
   __shared__ uint32_t bkey[kCollectBuf], bval[kCollectBuf];
   __shared__ uint32_t bcount, gbase;
   const bool table_empty = counters[kCtrCnt0] == 0;
-  const int64_t per = ((int64_t)ts + gridDim.x - 1) / gridDim.x;
+  const int64_t nw = (int64_t)((ts + 31u) >> 5);
+  const int64_t per = (nw + gridDim.x - 1) / gridDim.x;
   const int64_t lo = (int64_t)blockIdx.x * per;
-  const int64_t hi = min((int64_t)ts, lo + per);
+  const int64_t hi = min(nw, lo + per);
   if (threadIdx.x == 0) bcount = 0;
   __syncthreads();
   const int lane = lane_id();
-  for (int64_t base = lo; base < hi; base += blockDim.x) {
-    const int64_t s = base + threadIdx.x;
-    bool take = false;
-    if (s < hi) {
-      const uint64_t pk = skey[s];
-      if (pk != kEmpty64) {
-        take = true;
-        if (!table_empty) {
-          int32_t k0, k1, k2;
-          unpack_key(pk, k0, k1, k2);
-          take = table_find(t, k0, k1, k2) < 0;
+  for (int64_t base = lo; base < hi; base += blockDim.x) {  // uniform across the workgroup
+    const int64_t j = base + threadIdx.x;
+    uint32_t bits = j < hi ? dmask[j] : 0u;  // slots beyond ts are never marked
+    while (ballot(bits != 0u) != 0) {  // one claimed slot per lane per step (wave-uniform loop)
+      bool take = false;
+      uint32_t s = 0;
+      if (bits != 0u) {
+        s = (uint32_t)(j * 32) + (uint32_t)(__ffs(bits) - 1);
+        bits &= bits - 1u;
+        const uint64_t pk = skey[s];
+        if (pk != kEmpty64) {
+          take = true;
+          if (!table_empty) {
+            int32_t k0, k1, k2;
+            unpack_key(pk, k0, k1, k2);
+            take = table_find(t, k0, k1, k2) < 0;
+          }
         }
       }
-    }
-    const uint64_t m = ballot(take);
-    if (m == 0) continue;
-    const int leader = __ffsll((long long)m) - 1;
-    uint32_t wbase = 0;
-    if (lane == leader) wbase = atomicAdd(&bcount, (uint32_t)__popcll(m));
-    wbase = shfl(wbase, leader);
-    if (take) {
-      // mark the key's order: its rank among the new keys is the number of
-      // marked orders below it (orders of distinct keys are distinct)
-      const uint32_t o = sorder[s];
-      if ((o >> 5) < bm_words) atomicOr(&bitmap[o >> 5], 1u << (o & 31u));
-      const uint32_t slot = wbase + (uint32_t)__popcll(m & lanemask_lt());
-      if (slot < (uint32_t)kCollectBuf) {
-        bkey[slot] = sorder[s];
-        bval[slot] = (uint32_t)s;
-      } else {
-        list_put(atomicAdd(&counters[kCtrList], 1u), sorder[s], (uint32_t)s, lkey, lval, list_cap, err);
+      const uint64_t m = ballot(take);
+      if (m == 0) continue;
+      const int leader = __ffsll((long long)m) - 1;
+      uint32_t wbase = 0;
+      if (lane == leader) wbase = atomicAdd(&bcount, (uint32_t)__popcll(m));
+      wbase = shfl(wbase, leader);
+      if (take) {
+        // mark the key's order: its rank among the new keys is the number of
+        // marked orders below it (orders of distinct keys are distinct)
+        const uint32_t o = sorder[s];
+        if ((o >> 5) < bm_words) atomicOr(&bitmap[o >> 5], 1u << (o & 31u));
+        const uint32_t slot = wbase + (uint32_t)__popcll(m & lanemask_lt());
+        if (slot < (uint32_t)kCollectBuf) {
+          bkey[slot] = sorder[s];
+          bval[slot] = (uint32_t)s;
+        } else {
+          list_put(atomicAdd(&counters[kCtrList], 1u), sorder[s], (uint32_t)s, lkey, lval, list_cap, err);
+        }
       }
     }
   }
@@ -677,6 +703,8 @@ int bits_for(uint64_t maxval) {
   return b;
 }
 
+Dedup dedup_of(const Workspace& ws) { return Dedup{ws.skey, ws.sorder, ws.ts_cap, ws.err, ws.dmask}; }
+
 // words of the first-occurrence bitmap for orders below order_bound (orders
 // beyond the bitmap are reported by assign)
 uint64_t bitmap_words(const Workspace& ws, uint64_t order_bound) {
@@ -689,7 +717,8 @@ uint64_t bitmap_words(const Workspace& ws, uint64_t order_bound) {
 hipError_t finish_insertion(const TableView& t, Workspace& ws, uint64_t order_bound, hipStream_t s,
                             int32_t* slot_block = nullptr) {
   const uint64_t words = bitmap_words(ws, order_bound);
-  collect_new_kernel<<<kCollectGroups, kThreads, 0, s>>>(ws.skey, ws.sorder, ws.ts_cap, t, ws.counters, ws.la_key,
+  collect_new_kernel<<<kCollectGroups, kThreads, 0, s>>>(ws.skey, ws.sorder, ws.dmask, ws.ts_cap, t, ws.counters,
+                                                         ws.la_key,
                                                          ws.la_val, ws.list_cap, ws.err, ws.bitmap, (uint32_t)words);
   hipError_t e = exclusive_scan_popc_u32(ws.bitmap, ws.bm_prefix, (uint32_t)words, ws.scan_tmp, s);
   if (e != hipSuccess) return e;
@@ -704,7 +733,7 @@ hipError_t finish_insertion(const TableView& t, Workspace& ws, uint64_t order_bo
 
 hipError_t table_clear(const TableView& t, Workspace& ws, hipStream_t s, uint32_t* zero, int64_t nzero) {
   const int64_t m = (int64_t)t.T > (int64_t)ws.ts_cap ? t.T : ws.ts_cap;
-  clear_table_kernel<<<groups_for(m, kThreads), kThreads, 0, s>>>(t, ws.skey, ws.sorder, ws.ts_cap,
+  clear_table_kernel<<<groups_for(m, kThreads), kThreads, 0, s>>>(t, ws.skey, ws.sorder, ws.dmask, ws.ts_cap,
                                                                    ws.counters, zero, zero ? nzero : 0);
   return hipGetLastError();
 }
@@ -715,12 +744,13 @@ hipError_t table_activate(const TableView& t, Workspace& ws, const float* x, int
   int32_t* slot_block = record_slots ? ws.slot_block : nullptr;
   const uint64_t order_bound = (uint64_t)(n > 0 ? n : 1);
   clear_scratch_kernel<<<groups_for(ws.ts_cap, kThreads), kThreads, 0, s>>>(
-      ws.skey, ws.sorder, ws.ts_cap, t.cnt, ws.counters, slot_block, ws.bitmap, (int64_t)bitmap_words(ws, order_bound));
+      ws.skey, ws.sorder, ws.dmask, ws.ts_cap, t.cnt, ws.counters, slot_block, ws.bitmap,
+      (int64_t)bitmap_words(ws, order_bound));
   const float dxinv = (float)1.0 / dx;
   if (n > 0)
     activate_kernel<<<groups_for(n, kThreads * kActItems), kThreads, 0, s>>>(x, n, dxinv, blockLen, offset, disp,
-                                                                  ws.skey, ws.sorder, ws.ts_cap, ws.err,
-                                                                  record_slots ? ws.pslot : nullptr);
+                                                                              dedup_of(ws),
+                                                                              record_slots ? ws.pslot : nullptr);
   return finish_insertion(t, ws, order_bound, s, slot_block);
 }
 
@@ -731,9 +761,10 @@ hipError_t table_enlarge(const TableView& t, Workspace& ws, const int32_t lo[3],
   const int64_t total_cap = (int64_t)n0_cap * e0 * e1 * e2;
   const uint64_t order_bound = (uint64_t)(total_cap > 0 ? total_cap : 1);
   clear_scratch_kernel<<<groups_for(ws.ts_cap, kThreads), kThreads, 0, s>>>(
-      ws.skey, ws.sorder, ws.ts_cap, t.cnt, ws.counters, nullptr, ws.bitmap, (int64_t)bitmap_words(ws, order_bound));
-  enlarge_kernel<<<groups_for(total_cap, kThreads), kThreads, 0, s>>>(
-      t.active, ws.counters, lo[0], lo[1], lo[2], e0, e1, e2, ws.skey, ws.sorder, ws.ts_cap, ws.err);
+      ws.skey, ws.sorder, ws.dmask, ws.ts_cap, t.cnt, ws.counters, nullptr, ws.bitmap,
+      (int64_t)bitmap_words(ws, order_bound));
+  enlarge_kernel<<<groups_for(total_cap, kThreads), kThreads, 0, s>>>(t.active, ws.counters, lo[0], lo[1], lo[2],
+                                                                       e0, e1, e2, dedup_of(ws));
   return finish_insertion(t, ws, order_bound, s);
 }
 

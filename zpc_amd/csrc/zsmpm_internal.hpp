@@ -117,6 +117,7 @@ struct Workspace {
   uint32_t bm_words = 0;
   uint32_t* pslot = nullptr;     // [n_cap] scratch slot of each particle's key (recording activation)
   int32_t* slot_block = nullptr; // [ts_cap] block number of the key in each scratch slot
+  uint32_t* dmask = nullptr;     // [ts_cap / 32 + 1] one bit per dedup slot claimed by the current pass
 };
 
 // The next substep's constitutive part computed inside G2P from the updated
