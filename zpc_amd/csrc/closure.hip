@@ -442,20 +442,52 @@ __global__ __launch_bounds__(64) void mlp_grad_kernel(const float* __restrict__ 
 
 // Sum of the partials; optional gradient output; optional Adam update
 // (Kingma & Ba 2015) with bias corrections bc1 = 1 - b1^t, bc2 = 1 - b2^t.
+// A workgroup takes kRedParams parameters; its kRedParts slices of threads
+// sum every kRedParts-th partial (four loads in flight per thread), and the
+// slices are combined through LDS. Workgroup 0 also sums the loss partials.
+constexpr int kRedParams = 32;
+constexpr int kRedParts = 256 / kRedParams;
+
 __global__ __launch_bounds__(256) void mlp_reduce_kernel(float* __restrict__ P, float* __restrict__ m,
                                                          float* __restrict__ v, const float* __restrict__ partial,
                                                          const float* __restrict__ loss_part, int G, float lr,
                                                          float bc1, float bc2, float* __restrict__ grad_out,
                                                          float* __restrict__ loss_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0 && loss_out) {
+  __shared__ float red[kRedParts][kRedParams];
+  __shared__ float lred[256 / kWave];
+  const int pl = (int)threadIdx.x % kRedParams, part = (int)threadIdx.x / kRedParams;
+  const int i = (int)blockIdx.x * kRedParams + pl;
+  const bool loss_here = blockIdx.x == 0 && loss_out != nullptr;
+  if (loss_here) {
     float ls = 0.f;
-    for (int b = 0; b < G; ++b) ls += loss_part[b];
+    for (int b = (int)threadIdx.x; b < G; b += (int)blockDim.x) ls += loss_part[b];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) ls += __shfl_xor(ls, o, kWave);
+    if (lane_id() == 0) lred[threadIdx.x / kWave] = ls;
+  }
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+  if (i < kNP) {
+    int b = part;
+    for (; b + 3 * kRedParts < G; b += 4 * kRedParts) {
+      a0 += partial[(int64_t)b * kNP + i];
+      a1 += partial[(int64_t)(b + kRedParts) * kNP + i];
+      a2 += partial[(int64_t)(b + 2 * kRedParts) * kNP + i];
+      a3 += partial[(int64_t)(b + 3 * kRedParts) * kNP + i];
+    }
+    for (; b < G; b += kRedParts) a0 += partial[(int64_t)b * kNP + i];
+  }
+  red[part][pl] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  if (loss_here && threadIdx.x == 0) {
+    float ls = 0.f;
+#pragma unroll
+    for (int w = 0; w < 256 / kWave; ++w) ls += lred[w];
     *loss_out = ls;
   }
-  if (i >= kNP) return;
+  if (part != 0 || i >= kNP) return;
   float gsum = 0.f;
-  for (int b = 0; b < G; ++b) gsum += partial[(int64_t)b * kNP + i];
+#pragma unroll
+  for (int q = 0; q < kRedParts; ++q) gsum += red[q][pl];
   if (grad_out) grad_out[i] = gsum;
   if (P && m && v) {
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -841,8 +873,8 @@ hipError_t mlp_grad(const float* P, const float* X, const float* R, int64_t n, f
 
 hipError_t mlp_reduce(float* P, float* m, float* v, const float* partial, const float* loss_part, int G, float lr,
                       float bc1, float bc2, float* grad_out, float* loss_out, hipStream_t s) {
-  mlp_reduce_kernel<<<groups(kNP, 256), 256, 0, s>>>(P, m, v, partial, loss_part, G, lr, bc1, bc2, grad_out,
-                                                     loss_out);
+  mlp_reduce_kernel<<<groups(kNP, kRedParams), 256, 0, s>>>(P, m, v, partial, loss_part, G, lr, bc1, bc2,
+                                                            grad_out, loss_out);
   return hipGetLastError();
 }
 

@@ -8,7 +8,8 @@
 //   * numbering: a key gets the next block number in order of its first
 //     occurrence (particle index for activation, (block, dx, dy, dz) loop
 //     order for enlargement) -- found with a dedup hash holding the minimum
-//     order per key, then a radix sort of the new keys by that order;
+//     order per key; each new key's order sets one bit of a bitmap, and an
+//     exclusive popcount scan of the bitmap ranks the new keys;
 //   * slots: keys are placed by priority linear probing on the reference
 //     probe sequence (home = (h % T + T) % T, step 127) with priority = block
 //     number; the result equals inserting serially in block-number order
