@@ -283,6 +283,7 @@ int32_t reserve(zsmpm_policy_s* p, int64_t n, int32_t T) {
   ZA(w.contrib, (size_t)9 * w.n_cap);
   ZA(w.nbr, (size_t)8 * w.ts_cap);
   ZA(w.fb, w.n_cap);
+  ZA(w.wq, w.n_cap / 64 + 2);
   ZA(w.lj_next[0], w.n_cap);
   ZA(w.lj_next[1], w.n_cap);
   {

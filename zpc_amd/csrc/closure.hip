@@ -762,15 +762,23 @@ __device__ __forceinline__ bool zero_start_certifies(float nrm, float tol) {
 template <bool kNet>
 __global__ __launch_bounds__(kCThreads) void closure_kernel(Model m, ParticlesView ps, float* __restrict__ contrib,
                                                             const float* __restrict__ P, float tol, int always,
-                                                            int32_t* __restrict__ fb, uint32_t* queue) {
+                                                            int32_t* __restrict__ fb, uint32_t* queue,
+                                                            uint32_t* __restrict__ wq, uint32_t* wq_n) {
   __shared__ float4 stage4[kCWaves][stage_floats(9) / 4];
   __shared__ float sy[kCWaves][kWave][4];
   const int wave = uniform((int32_t)(threadIdx.x / kWave)), lane = lane_id();
   float* stage = reinterpret_cast<float*>(stage4[wave]);
+  const int64_t total_waves = (ps.n + kWave - 1) / kWave;
+  // kNet with a wave queue: only the queued waves, which the zero-start pass
+  // could not certify; without kNet a queue receives those waves instead
+  const bool from_queue = kNet && wq != nullptr;
+  const int64_t nw = from_queue ? (int64_t)min(*wq_n, (uint32_t)total_waves) : total_waves;
+  if (nw == 0) return;
   NetRegs r;
   if constexpr (kNet) load_net(P, r);
   const int64_t nwaves = (int64_t)gridDim.x * kCWaves;
-  for (int64_t w = (int64_t)blockIdx.x * kCWaves + wave; w * kWave < ps.n; w += nwaves) {
+  for (int64_t i = (int64_t)blockIdx.x * kCWaves + wave; i < nw; i += nwaves) {
+    const int64_t w = from_queue ? (int64_t)wq[i] : i;
     const int64_t p0 = w * kWave;
     const int32_t cw = (int32_t)min((int64_t)kWave, ps.n - p0);
     const int64_t p = p0 + lane;
@@ -782,7 +790,17 @@ __global__ __launch_bounds__(kCThreads) void closure_kernel(Model m, ParticlesVi
     const float nrm = closure_features(A, lj, m, f);
     float psi[3] = {0.f, 0.f, 0.f};
     bool use_net = kNet;
-    if constexpr (kNet) use_net = always || ballot(lane < cw && !zero_start_certifies(nrm, tol)) != 0;
+    if constexpr (kNet) {
+      use_net = from_queue || always || ballot(lane < cw && !zero_start_certifies(nrm, tol)) != 0;
+    } else if (wq != nullptr) {
+      if (ballot(lane < cw && !zero_start_certifies(nrm, tol)) != 0) {  // the network's wave
+        if (lane == 0) {
+          const uint32_t q = atomicAdd(wq_n, 1u);
+          if (q < (uint32_t)total_waves) wq[q] = (uint32_t)w;
+        }
+        continue;
+      }
+    }
     if (use_net) {
       // features of particle 16 t + (l & 15) in the B layout: through LDS
       float* sf = reinterpret_cast<float*>(stage4[wave]);  // 64 x 9 floats fit the row stage
@@ -826,7 +844,10 @@ __global__ __launch_bounds__(kCThreads) void closure_kernel(Model m, ParticlesVi
   }
 }
 
-__global__ void reset_queue_kernel(uint32_t* queue) { *queue = 0u; }
+__global__ void reset_queue_kernel(uint32_t* queue, uint32_t* queue2) {
+  *queue = 0u;
+  if (queue2) *queue2 = 0u;
+}
 
 // the plain SVD return mapping for the queued particles
 __global__ __launch_bounds__(256) void fallback_kernel(Model m, ParticlesView ps, float* __restrict__ contrib,
@@ -889,14 +910,27 @@ hipError_t stress_closure(const Model& m, const ClosureView& cl, const Particles
                           Workspace& ws, hipStream_t s) {
   if (ps.n <= 0) return hipSuccess;
   uint32_t* queue = ws.counters + kCtrFbQueue;
-  reset_queue_kernel<<<1, 1, 0, s>>>(queue);
+  uint32_t* wq_n = ws.counters + kCtrNetQueue;
+  // With a network and without always_net: the zero-start pass takes every
+  // wave (80 VGPRs, six waves per SIMD) and queues the waves it cannot
+  // certify; the network pass (176 VGPRs) then takes only those. Each wave
+  // gets exactly what the single network pass would give it.
+  const bool two_pass = cl.params && !cl.always_net;
+  reset_queue_kernel<<<1, 1, 0, s>>>(queue, two_pass ? wq_n : nullptr);
   const int64_t waves = (ps.n + kWave - 1) / kWave;
-  const int want = groups(waves * kWave, kCThreads), cap = cl.params ? 2048 : 65536;
-  const int g = want < cap ? want : cap;
-  if (cl.params)
-    closure_kernel<true><<<g, kCThreads, 0, s>>>(m, ps, contrib, cl.params, cl.tol, cl.always_net, ws.fb, queue);
-  else
-    closure_kernel<false><<<g, kCThreads, 0, s>>>(m, ps, contrib, nullptr, cl.tol, 0, ws.fb, queue);
+  const int want = groups(waves * kWave, kCThreads);
+  const int g_zero = want < 65536 ? want : 65536, g_net = want < 2048 ? want : 2048;
+  if (two_pass) {
+    closure_kernel<false><<<g_zero, kCThreads, 0, s>>>(m, ps, contrib, nullptr, cl.tol, 0, ws.fb, queue, ws.wq, wq_n);
+    closure_kernel<true><<<g_net, kCThreads, 0, s>>>(m, ps, contrib, cl.params, cl.tol, 0, ws.fb, queue, ws.wq,
+                                                     wq_n);
+  } else if (cl.params) {
+    closure_kernel<true><<<g_net, kCThreads, 0, s>>>(m, ps, contrib, cl.params, cl.tol, cl.always_net, ws.fb, queue,
+                                                     nullptr, nullptr);
+  } else {
+    closure_kernel<false><<<g_zero, kCThreads, 0, s>>>(m, ps, contrib, nullptr, cl.tol, 0, ws.fb, queue, nullptr,
+                                                       nullptr);
+  }
   fallback_kernel<<<persistent_groups(), 256, 0, s>>>(m, ps, contrib, ws.fb, queue, ws.counters);
   return hipGetLastError();
 }
