@@ -213,15 +213,18 @@ def surrogate_leg(torch, rocm, sc, args, graph, plain):
 
     t0 = time.perf_counter()
     F, lj = CL.training_states(sc, sim_steps=args.warmup, synthetic=1 << 18, seed=0)
-    keep = torch.randperm(F.shape[0], device=F.device)[: 1 << 20]
+    gen = torch.Generator(device=F.device)
+    gen.manual_seed(0)
+    keep = torch.randperm(F.shape[0], device=F.device, generator=gen)[: 1 << 20]
     model = rocm.model_desc(sc.material)
     pol = rocm.Policy()
     tr = CL.train(pol, model, F[keep].contiguous(), lj[keep].contiguous(), steps=args.train_steps, batch=8192,
                   lr=3e-3, seed=0)
     del F, lj, keep
     prep = time.perf_counter() - t0
-    print(f"# closure trained: {args.train_steps} Adam steps in {tr.seconds:.2f} s, loss {tr.losses[0]:.3e} -> "
-          f"{tr.losses[-1]:.3e}", file=sys.stderr, flush=True)
+    print(f"# closure trained: {args.train_steps} Adam steps in {tr.seconds:.2f} s, batch loss {tr.losses[0]:.3e} -> "
+          f"{tr.losses[-1]:.3e}, kept step {tr.best_step} (mean loss {tr.eval_loss:.3e} on {tr.eval_samples} "
+          f"states)", file=sys.stderr, flush=True)
     res = {}
     for name, params, always in (("trained", tr.params, False), ("trained_network_every_wave", tr.params, True),
                                  ("zero_start", None, False)):
@@ -248,6 +251,7 @@ def surrogate_leg(torch, rocm, sc, args, graph, plain):
                       "'trained_network_every_wave' in every wave, 'zero_start' never")
     res["training"] = {"adam_steps": args.train_steps, "batch": 8192, "samples": tr.samples,
                        "seconds": tr.seconds, "loss_first": tr.losses[0], "loss_last": tr.losses[-1],
+                       "loss_eval": tr.eval_loss, "kept_step": tr.best_step, "eval_samples": tr.eval_samples,
                        "data_seconds": prep - tr.seconds}
     return res
 

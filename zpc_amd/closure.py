@@ -94,17 +94,31 @@ def stress(pol: rocm.Policy, model: rocm.ModelDesc, F, logJp):
 
 @dataclass
 class Trained:
-    params: object            # device float32 [NPARAMS]
+    params: object            # device float32 [NPARAMS]: the checkpoint with the lowest eval_loss
     losses: List[float] = field(default_factory=list)  # batch loss every `log_every` steps
-    seconds: float = 0.0      # GPU training time (harvest excluded)
+    seconds: float = 0.0      # GPU training time (harvest and checkpoint evaluations excluded)
     samples: int = 0
+    eval_loss: float = float("nan")  # mean loss of `params` on the evaluation sample
+    best_step: int = 0        # Adam steps behind `params`
+    eval_samples: int = 0
+
+
+def eval_loss(pol: rocm.Policy, params, X, R) -> float:
+    """Mean loss of params over the samples (X, R)."""
+    _, loss = grad(pol, params, X, R)
+    return float(loss.item())
 
 
 def train(pol: rocm.Policy, model: rocm.ModelDesc, F, logJp, steps: int = 2000, batch: int = 8192,
-          lr: float = 2e-3, seed: int = 0, log_every: int = 100, params=None) -> Trained:
+          lr: float = 2e-3, seed: int = 0, log_every: int = 100, params=None, check_every: int = 500,
+          eval_max: int = 1 << 16) -> Trained:
     """Adam on minibatches drawn from the states (F, logJp) (device tensors):
     the records come from zsmpm_mlp_harvest__rocm, every step is one
-    zsmpm_mlp_train_step__rocm. The learning rate decays by cosine to lr/20."""
+    zsmpm_mlp_train_step__rocm. The learning rate decays by cosine to lr/20.
+    Every check_every steps (and at the end) the mean loss over a fixed
+    strided sample of at most eval_max states is evaluated, outside the timed
+    segments, and the parameters with the lowest one are returned: one bad
+    stretch of minibatches cannot leave a worse network behind."""
     torch = rocm._torch()
     dev = F.device
     X, R = harvest(pol, model, F, logJp)
@@ -115,6 +129,10 @@ def train(pol: rocm.Policy, model: rocm.ModelDesc, F, logJp, steps: int = 2000, 
     loss = torch.zeros(steps, dtype=torch.float32, device=dev)
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
+    ev = torch.arange(0, n, max(n // eval_max, 1), device=dev)[:eval_max]
+    Xe, Re = X.index_select(0, ev).contiguous(), R.index_select(0, ev).contiguous()
+    best, best_P, best_step = eval_loss(pol, P, Xe, Re), P.clone(), 0
+    secs = 0.0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
@@ -126,10 +144,17 @@ def train(pol: rocm.Policy, model: rocm.ModelDesc, F, logJp, steps: int = 2000, 
         rocm.check(pol.L.zsmpm_mlp_train_step__rocm(pol.h, P.data_ptr(), m.data_ptr(), v.data_ptr(),
                                                     xb.data_ptr(), rb.data_ptr(), batch, float(rate), k + 1,
                                                     loss[k:k + 1].data_ptr()), "mlp_train_step")
-    torch.cuda.synchronize()
-    secs = time.perf_counter() - t0
+        if (k + 1) % check_every == 0 or k + 1 == steps:
+            torch.cuda.synchronize()
+            secs += time.perf_counter() - t0
+            e = eval_loss(pol, P, Xe, Re)
+            if e < best:
+                best, best_P, best_step = e, P.clone(), k + 1
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
     lh = loss.cpu().numpy()
-    return Trained(P, [float(lh[i]) for i in range(0, steps, log_every)] + [float(lh[-1])], secs, n)
+    return Trained(best_P, [float(lh[i]) for i in range(0, steps, log_every)] + [float(lh[-1])], secs, n,
+                   eval_loss=best, best_step=best_step, eval_samples=int(ev.numel()))
 
 
 def training_states(sc, sim_steps: int = 0, synthetic: int = 1 << 18, seed: int = 0, device="cuda"):

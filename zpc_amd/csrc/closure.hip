@@ -922,8 +922,11 @@ hipError_t stress_closure(const Model& m, const ClosureView& cl, const Particles
   const int g_zero = want < 65536 ? want : 65536, g_net = want < 2048 ? want : 2048;
   if (two_pass) {
     closure_kernel<false><<<g_zero, kCThreads, 0, s>>>(m, ps, contrib, nullptr, cl.tol, 0, ws.fb, queue, ws.wq, wq_n);
-    closure_kernel<true><<<g_net, kCThreads, 0, s>>>(m, ps, contrib, cl.params, cl.tol, 0, ws.fb, queue, ws.wq,
-                                                     wq_n);
+    // 512 workgroups of four waves fill every SIMD at the network's two waves
+    // per SIMD; more would only cost dispatch when the queue is short
+    const int g_queue = g_net < 512 ? g_net : 512;
+    closure_kernel<true><<<g_queue, kCThreads, 0, s>>>(m, ps, contrib, cl.params, cl.tol, 0, ws.fb, queue, ws.wq,
+                                                       wq_n);
   } else if (cl.params) {
     closure_kernel<true><<<g_net, kCThreads, 0, s>>>(m, ps, contrib, cl.params, cl.tol, cl.always_net, ws.fb, queue,
                                                      nullptr, nullptr);
