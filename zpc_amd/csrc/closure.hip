@@ -84,14 +84,19 @@ __device__ __forceinline__ int hid(int ks, int g, int e) { return e < 4 ? 32 * k
 
 // tanh form of GELU; tanh(u) = 1 - 2 / (e^{2u} + 1) with the hardware
 // exponential and reciprocal (1 ulp each; e^{2u} = inf gives tanh = 1)
-__device__ __forceinline__ float gelu_t(float x) {
-  const float u = kGeluC * (x + kGeluA * x * x * x);
-  return 1.f - 2.f * __builtin_amdgcn_rcpf(__expf(2.f * u) + 1.f);
+// tanh-form GELU, 0.5 x (1 + tanh u) with u = C (x + A x^3), evaluated as
+// x s with s = sigmoid(2 u) = 1 / (1 + 2^(x (k1 + k2 x^2))): one exp2 and one
+// rcp on the hardware units and five other operations. Its derivative is
+// s + 2 C x s (1 - s) (1 + 3 A x^2). Large |x| saturate to 0 and x.
+constexpr float kGeluK1 = (float)(-2.0 * 0.7978845608028654 * 1.4426950408889634);
+constexpr float kGeluK2 = (float)(-2.0 * 0.7978845608028654 * 0.044715 * 1.4426950408889634);
+__device__ __forceinline__ float gelu_s(float x) {
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(x * fmaf(kGeluK2, x * x, kGeluK1)));
 }
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + gelu_t(x)); }
+__device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float t = gelu_t(x);
-  return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * kGeluC * (1.f + 3.f * kGeluA * x * x);
+  const float sg = gelu_s(x);
+  return sg + 2.f * kGeluC * x * sg * (1.f - sg) * fmaf(3.f * kGeluA, x * x, 1.f);
 }
 
 // The forward weights of one lane, loaded once per wave from the float
