@@ -101,15 +101,27 @@ __device__ __forceinline__ float gelu_grad(float x) {
 
 // The forward weights of one lane, loaded once per wave from the float
 // master copy and rounded to bfloat16.
+// The biases stay in a per-wave LDS block (kNetBias floats: b1 at 0, b2 at
+// 64, b3 at 128 padded with zeros to 16), read as one 16-byte vector per tile
+// row: in registers they took 36 VGPRs, which with the weights held the
+// network kernels to two waves per SIMD. The sums are the same.
+constexpr int kNetBias = 144;
+
 struct NetRegs {
   v4s a1[4];      // W1^T, m-tile mt: out 16 mt + (l & 15), in 4 g + e
   v8bf a2[4][2];  // W2^T, m-tile mt, K step ks: out 16 mt + (l & 15), in hid(ks, g, e)
   v8bf a3[2];     // W3^T: out (l & 15) < 7, in hid(ks, g, e)
-  float b1[4][4], b2[4][4], b3[4];  // bias of accumulator row 4 g + e of each tile
+  const float* bias;  // the wave's LDS bias block, 16-byte aligned
 };
 
-__device__ __forceinline__ void load_net(const float* __restrict__ P, NetRegs& r) {
+// wb: the calling wave's LDS block of kNetBias floats (16-byte aligned)
+__device__ __forceinline__ void load_net(const float* __restrict__ P, NetRegs& r, float* wb) {
   const int l = lane_id(), i = l & 15, g = l >> 4;
+  wb[l] = P[kOffB1 + l];
+  wb[kH + l] = P[kOffB2 + l];
+  if (l < 16) wb[2 * kH + l] = l < kOut ? P[kOffB3 + l] : 0.f;
+  wave_lds_barrier();
+  r.bias = wb;
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     float v[4];
@@ -126,11 +138,6 @@ __device__ __forceinline__ void load_net(const float* __restrict__ P, NetRegs& r
       for (int e = 0; e < 8; ++e) w[e] = P[kOffW2 + hid(ks, g, e) * kH + 16 * mt + i];
       r.a2[mt][ks] = pack8(w);
     }
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      r.b1[mt][e] = P[kOffB1 + 16 * mt + 4 * g + e];
-      r.b2[mt][e] = P[kOffB2 + 16 * mt + 4 * g + e];
-    }
   }
 #pragma unroll
   for (int ks = 0; ks < 2; ++ks) {
@@ -139,8 +146,15 @@ __device__ __forceinline__ void load_net(const float* __restrict__ P, NetRegs& r
     for (int e = 0; e < 8; ++e) w[e] = i < kOut ? P[kOffW3 + hid(ks, g, e) * kOut + i] : 0.f;
     r.a3[ks] = pack8(w);
   }
-#pragma unroll
-  for (int e = 0; e < 4; ++e) r.b3[e] = (4 * g + e) < kOut ? P[kOffB3 + 4 * g + e] : 0.f;
+}
+
+// bias of accumulator rows 4 g + e (e = 0..3) of row block `base`; the empty
+// asm makes the index opaque so that the compiler reloads the vector per
+// tile instead of hoisting all biases back into registers
+__device__ __forceinline__ float4 net_bias(const NetRegs& r, int base) {
+  int q = (base >> 2) + (lane_id() >> 4);
+  asm volatile("" : "+v"(q));
+  return reinterpret_cast<const float4*>(r.bias)[q];
 }
 
 // Intermediates the backward pass needs.
@@ -158,9 +172,11 @@ __device__ __forceinline__ v4f net_tile(const NetRegs& r, v4s x, Acts* acts) {
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     v4f d = mfma16(r.a1[mt], x, zero);
+    const float4 bb = net_bias(r, 16 * mt);
+    const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      d[e] += r.b1[mt][e];
+      d[e] += bv[e];
       a1[mt][e] = gelu(d[e]);
       if constexpr (kKeep) acts->a1[mt][e] = bf16_bits(a1[mt][e]);
     }
@@ -178,9 +194,11 @@ __device__ __forceinline__ v4f net_tile(const NetRegs& r, v4s x, Acts* acts) {
   for (int mt = 0; mt < 4; ++mt) {
     v4f d = mfma32(r.a2[mt][0], b2[0], zero);
     d = mfma32(r.a2[mt][1], b2[1], d);
+    const float4 bb = net_bias(r, kH + 16 * mt);
+    const float bv2[4] = {bb.x, bb.y, bb.z, bb.w};
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      d[e] += r.b2[mt][e];
+      d[e] += bv2[e];
       a2[mt][e] = gelu(d[e]);
       if constexpr (kKeep) acts->a2[mt][e] = bf16_bits(a2[mt][e]);
     }
@@ -195,8 +213,12 @@ __device__ __forceinline__ v4f net_tile(const NetRegs& r, v4s x, Acts* acts) {
   }
   v4f y = mfma32(r.a3[0], b3[0], zero);
   y = mfma32(r.a3[1], b3[1], y);
+  {
+    const float4 bb = net_bias(r, 2 * kH);
+    const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
 #pragma unroll
-  for (int e = 0; e < 4; ++e) y[e] += r.b3[e];
+    for (int e = 0; e < 4; ++e) y[e] += bv[e];
+  }
   return y;
 }
 
@@ -205,8 +227,9 @@ __device__ __forceinline__ v4f net_tile(const NetRegs& r, v4s x, Acts* acts) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void mlp_infer_kernel(const float* __restrict__ P, const float* __restrict__ X,
                                                         int64_t n, float* __restrict__ Y) {
+  __shared__ float4 nbias[256 / kWave][kNetBias / 4];
   NetRegs r;
-  load_net(P, r);
+  load_net(P, r, reinterpret_cast<float*>(nbias[threadIdx.x / kWave]));
   const int l = lane_id(), j = l & 15, g = l >> 4;
   const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
   const int64_t w0 = uniform((int64_t)(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
@@ -236,8 +259,9 @@ __global__ __launch_bounds__(64) void mlp_grad_kernel(const float* __restrict__ 
                                                       float* __restrict__ partial, float* __restrict__ loss_part) {
   __shared__ uint16_t sx[16][16], sa1[16][kH], sa2[16][kH], sdz1[16][kH], sdz2[16][kH], sdy[16][16];
   __shared__ float sy[16][8], sdyf[16][8];
+  __shared__ float4 nbias[kNetBias / 4];
   NetRegs r;
-  load_net(P, r);
+  load_net(P, r, reinterpret_cast<float*>(nbias));
   const int l = lane_id(), j = l & 15, g = l >> 4;
   // backward operands: W3 as A (rows hidden 16 mt + (l & 15), K = outputs
   // 4 g + e) and W2 as A (rows in-unit 16 mt + (l & 15), K = out-units 16 ks + 4 g + e)
@@ -779,8 +803,9 @@ __global__ __launch_bounds__(kCThreads) void closure_kernel(Model m, ParticlesVi
   const bool from_queue = kNet && wq != nullptr;
   const int64_t nw = from_queue ? (int64_t)min(*wq_n, (uint32_t)total_waves) : total_waves;
   if (nw == 0) return;
+  __shared__ float4 nbias[kNet ? kCWaves : 1][kNet ? kNetBias / 4 : 1];
   NetRegs r;
-  if constexpr (kNet) load_net(P, r);
+  if constexpr (kNet) load_net(P, r, reinterpret_cast<float*>(nbias[wave]));
   const int64_t nwaves = (int64_t)gridDim.x * kCWaves;
   for (int64_t i = (int64_t)blockIdx.x * kCWaves + wave; i < nw; i += nwaves) {
     const int64_t w = from_queue ? (int64_t)wq[i] : i;
@@ -927,9 +952,9 @@ hipError_t stress_closure(const Model& m, const ClosureView& cl, const Particles
   const int g_zero = want < 65536 ? want : 65536, g_net = want < 2048 ? want : 2048;
   if (two_pass) {
     closure_kernel<false><<<g_zero, kCThreads, 0, s>>>(m, ps, contrib, nullptr, cl.tol, 0, ws.fb, queue, ws.wq, wq_n);
-    // 512 workgroups of four waves fill every SIMD at the network's two waves
-    // per SIMD; more would only cost dispatch when the queue is short
-    const int g_queue = g_net < 512 ? g_net : 512;
+    // 768 workgroups of four waves fill every SIMD at the network's three
+    // waves per SIMD; more would only cost dispatch when the queue is short
+    const int g_queue = g_net < 768 ? g_net : 768;
     closure_kernel<true><<<g_queue, kCThreads, 0, s>>>(m, ps, contrib, cl.params, cl.tol, 0, ws.fb, queue, ws.wq,
                                                        wq_n);
   } else if (cl.params) {
