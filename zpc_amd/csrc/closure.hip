@@ -47,11 +47,15 @@ constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
 constexpr float kGeluA = 0.044715f;
 
 // bfloat16 bits of a float, round to nearest, ties to even
-__device__ __forceinline__ uint32_t bf16_bits(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+// float -> bfloat16, round to nearest even: gfx950's v_cvt_pk_bf16_f32 rounds
+// two values in one instruction (the integer form, (u + 0x7fff + lsb) >> 16,
+// gives the same bits for finite values in about six operations per value)
+typedef __bf16 zs_bf2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
+  const zs_bf2 v = {(__bf16)lo, (__bf16)hi};
+  return __builtin_bit_cast(uint32_t, v);
 }
-__device__ __forceinline__ uint32_t pack2(float lo, float hi) { return bf16_bits(lo) | (bf16_bits(hi) << 16); }
+__device__ __forceinline__ uint32_t bf16_bits(float f) { return pack2(f, 0.f) & 0xffffu; }
 __device__ __forceinline__ v4s pack4(const float (&v)[4]) {
   v2u u;
   u.x = pack2(v[0], v[1]);
