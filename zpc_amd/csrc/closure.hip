@@ -111,21 +111,41 @@ __device__ __forceinline__ float gelu_grad(float x) {
 // network kernels to two waves per SIMD. The sums are the same.
 constexpr int kNetBias = 144;
 
+// W2 stays in one LDS block per workgroup, in the MFMA operand layout
+// (kNetW2 vectors: [m-tile mt][K step ks][lane]), read per tile as 16-byte
+// vectors; in registers it took 32 VGPRs. Same operands, same sums.
+constexpr int kNetW2 = 4 * 2 * 64;
+
 struct NetRegs {
   v4s a1[4];      // W1^T, m-tile mt: out 16 mt + (l & 15), in 4 g + e
-  v8bf a2[4][2];  // W2^T, m-tile mt, K step ks: out 16 mt + (l & 15), in hid(ks, g, e)
   v8bf a3[2];     // W3^T: out (l & 15) < 7, in hid(ks, g, e)
   const float* bias;  // the wave's LDS bias block, 16-byte aligned
+  const v8bf* w2;     // the workgroup's LDS block of W2^T: out 16 mt + (l & 15), in hid(ks, g, e)
 };
 
-// wb: the calling wave's LDS block of kNetBias floats (16-byte aligned)
-__device__ __forceinline__ void load_net(const float* __restrict__ P, NetRegs& r, float* wb) {
+// wb: the calling wave's LDS block of kNetBias floats (16-byte aligned);
+// w2: the workgroup's LDS block of kNetW2 vectors, written by its first wave.
+// Every thread of the workgroup must call it (it ends with a barrier).
+__device__ __forceinline__ void load_net(const float* __restrict__ P, NetRegs& r, float* wb, v8bf* w2) {
   const int l = lane_id(), i = l & 15, g = l >> 4;
   wb[l] = P[kOffB1 + l];
   wb[kH + l] = P[kOffB2 + l];
   if (l < 16) wb[2 * kH + l] = l < kOut ? P[kOffB3 + l] : 0.f;
-  wave_lds_barrier();
   r.bias = wb;
+  r.w2 = w2;
+  if (threadIdx.x < kWave) {
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) {
+        float w[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) w[e] = P[kOffW2 + hid(ks, g, e) * kH + 16 * mt + i];
+        w2[(2 * mt + ks) * kWave + l] = pack8(w);
+      }
+    }
+  }
+  __syncthreads();
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     float v[4];
@@ -135,13 +155,6 @@ __device__ __forceinline__ void load_net(const float* __restrict__ P, NetRegs& r
       v[e] = k < kIn ? P[kOffW1 + k * kH + 16 * mt + i] : 0.f;
     }
     r.a1[mt] = pack4(v);
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      float w[8];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) w[e] = P[kOffW2 + hid(ks, g, e) * kH + 16 * mt + i];
-      r.a2[mt][ks] = pack8(w);
-    }
   }
 #pragma unroll
   for (int ks = 0; ks < 2; ++ks) {
@@ -196,8 +209,10 @@ __device__ __forceinline__ v4f net_tile(const NetRegs& r, v4s x, Acts* acts) {
   float a2[4][4];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
-    v4f d = mfma32(r.a2[mt][0], b2[0], zero);
-    d = mfma32(r.a2[mt][1], b2[1], d);
+    int q = 2 * mt * kWave + lane_id();
+    asm volatile("" : "+v"(q));  // reload per tile: keep W2 out of registers
+    v4f d = mfma32(r.w2[q], b2[0], zero);
+    d = mfma32(r.w2[q + kWave], b2[1], d);
     const float4 bb = net_bias(r, kH + 16 * mt);
     const float bv2[4] = {bb.x, bb.y, bb.z, bb.w};
 #pragma unroll
@@ -232,8 +247,9 @@ __device__ __forceinline__ v4f net_tile(const NetRegs& r, v4s x, Acts* acts) {
 __global__ __launch_bounds__(256) void mlp_infer_kernel(const float* __restrict__ P, const float* __restrict__ X,
                                                         int64_t n, float* __restrict__ Y) {
   __shared__ float4 nbias[256 / kWave][kNetBias / 4];
+  __shared__ v8bf nw2[kNetW2];
   NetRegs r;
-  load_net(P, r, reinterpret_cast<float*>(nbias[threadIdx.x / kWave]));
+  load_net(P, r, reinterpret_cast<float*>(nbias[threadIdx.x / kWave]), nw2);
   const int l = lane_id(), j = l & 15, g = l >> 4;
   const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
   const int64_t w0 = uniform((int64_t)(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
@@ -264,8 +280,9 @@ __global__ __launch_bounds__(64) void mlp_grad_kernel(const float* __restrict__ 
   __shared__ uint16_t sx[16][16], sa1[16][kH], sa2[16][kH], sdz1[16][kH], sdz2[16][kH], sdy[16][16];
   __shared__ float sy[16][8], sdyf[16][8];
   __shared__ float4 nbias[kNetBias / 4];
+  __shared__ v8bf nw2[kNetW2];
   NetRegs r;
-  load_net(P, r, reinterpret_cast<float*>(nbias));
+  load_net(P, r, reinterpret_cast<float*>(nbias), nw2);
   const int l = lane_id(), j = l & 15, g = l >> 4;
   // backward operands: W3 as A (rows hidden 16 mt + (l & 15), K = outputs
   // 4 g + e) and W2 as A (rows in-unit 16 mt + (l & 15), K = out-units 16 ks + 4 g + e)
@@ -793,7 +810,7 @@ __device__ __forceinline__ bool zero_start_certifies(float nrm, float tol) {
 // are appended to the fallback queue (queue[0] counts them) and keep their
 // logJp.
 template <bool kNet>
-__global__ __launch_bounds__(kCThreads) void closure_kernel(Model m, ParticlesView ps, float* __restrict__ contrib,
+__global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void closure_kernel(Model m, ParticlesView ps, float* __restrict__ contrib,
                                                             const float* __restrict__ P, float tol, int always,
                                                             int32_t* __restrict__ fb, uint32_t* queue,
                                                             uint32_t* __restrict__ wq, uint32_t* wq_n) {
@@ -808,8 +825,9 @@ __global__ __launch_bounds__(kCThreads) void closure_kernel(Model m, ParticlesVi
   const int64_t nw = from_queue ? (int64_t)min(*wq_n, (uint32_t)total_waves) : total_waves;
   if (nw == 0) return;
   __shared__ float4 nbias[kNet ? kCWaves : 1][kNet ? kNetBias / 4 : 1];
+  __shared__ v8bf nw2[kNet ? kNetW2 : 1];
   NetRegs r;
-  if constexpr (kNet) load_net(P, r, reinterpret_cast<float*>(nbias[wave]));
+  if constexpr (kNet) load_net(P, r, reinterpret_cast<float*>(nbias[wave]), nw2);
   const int64_t nwaves = (int64_t)gridDim.x * kCWaves;
   for (int64_t i = (int64_t)blockIdx.x * kCWaves + wave; i < nw; i += nwaves) {
     const int64_t w = from_queue ? (int64_t)wq[i] : i;
@@ -956,9 +974,9 @@ hipError_t stress_closure(const Model& m, const ClosureView& cl, const Particles
   const int g_zero = want < 65536 ? want : 65536, g_net = want < 2048 ? want : 2048;
   if (two_pass) {
     closure_kernel<false><<<g_zero, kCThreads, 0, s>>>(m, ps, contrib, nullptr, cl.tol, 0, ws.fb, queue, ws.wq, wq_n);
-    // 768 workgroups of four waves fill every SIMD at the network's three
+    // 1024 workgroups of four waves fill every SIMD at the network's four
     // waves per SIMD; more would only cost dispatch when the queue is short
-    const int g_queue = g_net < 768 ? g_net : 768;
+    const int g_queue = g_net < 1024 ? g_net : 1024;
     closure_kernel<true><<<g_queue, kCThreads, 0, s>>>(m, ps, contrib, cl.params, cl.tol, 0, ws.fb, queue, ws.wq,
                                                        wq_n);
   } else if (cl.params) {
