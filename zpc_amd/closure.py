@@ -110,7 +110,7 @@ def eval_loss(pol: rocm.Policy, params, X, R) -> float:
 
 
 def train(pol: rocm.Policy, model: rocm.ModelDesc, F, logJp, steps: int = 2000, batch: int = 8192,
-          lr: float = 2e-3, seed: int = 0, log_every: int = 100, params=None, check_every: int = 500,
+          lr: float = 2e-3, seed: int = 0, log_every: int = 100, params=None, check_every: int = 250,
           eval_max: int = 1 << 16) -> Trained:
     """Adam on minibatches drawn from the states (F, logJp) (device tensors):
     the records come from zsmpm_mlp_harvest__rocm, every step is one
@@ -118,7 +118,9 @@ def train(pol: rocm.Policy, model: rocm.ModelDesc, F, logJp, steps: int = 2000, 
     Every check_every steps (and at the end) the mean loss over a fixed
     strided sample of at most eval_max states is evaluated, outside the timed
     segments, and the parameters with the lowest one are returned: one bad
-    stretch of minibatches cannot leave a worse network behind."""
+    stretch of minibatches cannot leave a worse network behind. When a
+    checkpoint's loss exceeds 1.5 times the best, the parameters and Adam
+    moments go back to the best checkpoint and the learning rate is halved."""
     torch = rocm._torch()
     dev = F.device
     X, R = harvest(pol, model, F, logJp)
@@ -132,6 +134,8 @@ def train(pol: rocm.Policy, model: rocm.ModelDesc, F, logJp, steps: int = 2000, 
     ev = torch.arange(0, n, max(n // eval_max, 1), device=dev)[:eval_max]
     Xe, Re = X.index_select(0, ev).contiguous(), R.index_select(0, ev).contiguous()
     best, best_P, best_step = eval_loss(pol, P, Xe, Re), P.clone(), 0
+    best_m, best_v = m.clone(), v.clone()
+    scale = 1.0
     secs = 0.0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -140,7 +144,7 @@ def train(pol: rocm.Policy, model: rocm.ModelDesc, F, logJp, steps: int = 2000, 
         xb = X.index_select(0, idx)
         rb = R.index_select(0, idx)
         a = 0.5 * (1.0 + np.cos(np.pi * k / max(steps - 1, 1)))
-        rate = lr * (0.05 + 0.95 * a)
+        rate = lr * scale * (0.05 + 0.95 * a)
         rocm.check(pol.L.zsmpm_mlp_train_step__rocm(pol.h, P.data_ptr(), m.data_ptr(), v.data_ptr(),
                                                     xb.data_ptr(), rb.data_ptr(), batch, float(rate), k + 1,
                                                     loss[k:k + 1].data_ptr()), "mlp_train_step")
@@ -150,6 +154,12 @@ def train(pol: rocm.Policy, model: rocm.ModelDesc, F, logJp, steps: int = 2000, 
             e = eval_loss(pol, P, Xe, Re)
             if e < best:
                 best, best_P, best_step = e, P.clone(), k + 1
+                best_m, best_v = m.clone(), v.clone()
+            elif not (e <= 1.5 * best):  # diverging (or NaN): back to the best checkpoint, half the rate
+                P.copy_(best_P)
+                m.copy_(best_m)
+                v.copy_(best_v)
+                scale *= 0.5
             torch.cuda.synchronize()
             t0 = time.perf_counter()
     lh = loss.cpu().numpy()
