@@ -291,3 +291,25 @@ def test_surrogate_substeps_match_oracle_and_plain(trained):
     print(f"100 substeps: |x_closure - x_plain| = {d_sur / sc2.dx:.2e} dx, plain run-to-run {d_plain / sc2.dx:.2e} dx",
           flush=True)
     assert d_sur <= max(1e-3 * sc2.dx, 2.0 * d_plain), (d_sur, d_plain, sc2.dx)
+
+
+def test_training_never_ends_worse_than_its_best_checkpoint(pol):
+    """At an absurd learning rate (1.0) Adam diverges; the checkpoints restore
+    the best parameters and halve the rate, so the network returned is never
+    worse (on the fixed evaluation sample) than the one training started from,
+    and the reported loss is that network's."""
+    C = _closure()
+    _, model = _sand_model()
+    F, lj = C.synthetic_states(1 << 15, seed=19, lo=1e-4, hi=0.25)
+    Fd, ljd = _dev(F), _dev(lj)
+    tr = C.train(pol, model, Fd, ljd, steps=600, batch=2048, lr=1.0, seed=2, check_every=100)
+    X, R = C.harvest(pol, model, Fd, ljd)
+    n = X.shape[0]
+    import torch
+
+    ev = torch.arange(0, n, max(n // (1 << 16), 1), device=X.device)[: 1 << 16]
+    Xe, Re = X.index_select(0, ev).contiguous(), R.index_select(0, ev).contiguous()
+    start = C.eval_loss(pol, torch.as_tensor(C.init_params(2)).to(X.device), Xe, Re)
+    kept = C.eval_loss(pol, tr.params, Xe, Re)
+    assert np.isfinite(kept) and kept <= start * (1 + 1e-6), (start, kept, tr.best_step)
+    assert abs(kept - tr.eval_loss) <= 1e-6 * max(kept, 1e-30), (kept, tr.eval_loss)
