@@ -187,8 +187,9 @@ int32_t zsmpm_p2g__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_
                         float dt);
 /* ComputeGridBlockVelocity + ApplyBoundaryConditionOnGridBlocks per collider,
  * GridOp.hpp:71-164. fold_rhs != 0 first adds the rhs channels to momentum
- * (the explicit driver's convention). max_vel_sqr: device float (atomic max,
- * caller-initialised like zpc's maxVel) or NULL. */
+ * (the explicit driver's convention). max_vel_sqr: device float or NULL;
+ * receives max(*max_vel_sqr, max |v|^2 over nodes with mass) like zpc's
+ * atomic_max on maxVel (any initial value, negative ones included). */
 int32_t zsmpm_grid_update__rocm(zsmpm_policy pol, const zsmpm_grid* grid, const zsmpm_table* table, float dt,
                                 const float extf[3], const zsmpm_plane_collider* colliders, int32_t ncolliders,
                                 int32_t fold_rhs, float* max_vel_sqr);

@@ -83,6 +83,9 @@ def lib():
         L.zso_ctx_grids_f32.restype = C.c_float
         L.zso_ctx_grids_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.zso_ctx_grids_f64.restype = C.c_double
+        L.zso_ctx_set_grid_f32.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.zso_ctx_angular_momentum_f32.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.c_int32, C.c_int32,
+                                                   _f64p]
         L.zso_activate_f32.argtypes = [C.c_void_p, C.c_int32, C.c_int64, _f32p, C.c_float, C.c_int32,
                                        C.c_int32, C.c_float]
         L.zso_enlarge_f32.argtypes = [C.c_void_p, _i32p, _i32p]
@@ -312,6 +315,22 @@ class Context:
         gv = np.zeros(s["grid"], self.dtype)
         mv = getattr(self.L, f"zso_ctx_grids_{self.sfx}")(self.h, _ptr(g), _ptr(gv))
         return g.reshape(-1, 7, 64), gv.reshape(-1, 7, 64), float(mv)
+
+    def set_grids(self, grid_p2g=None, grid_v=None):
+        """Replace the P2G grid and/or the updated grid (table.cnt x 7 x 64)."""
+        n = self.sizes()["cnt"] * 7 * 64
+        a = None if grid_p2g is None else np.ascontiguousarray(grid_p2g, np.float32).reshape(-1)
+        b = None if grid_v is None else np.ascontiguousarray(grid_v, np.float32).reshape(-1)
+        assert a is None or a.size == n
+        assert b is None or b.size == n
+        self.L.zso_ctx_set_grid_f32(self.h, _ptr(a), _ptr(b))
+
+    def angular_momentum(self, dx, which=0, m_channel=0, mv_channel=1):
+        """GridAngularMomentum (GridOp.hpp:216-259) of the P2G grid (which=0)
+        or the updated grid (which=1): [x cross mv (3), mv (3)] in float64."""
+        out = np.zeros(6)
+        self.L.zso_ctx_angular_momentum_f32(self.h, which, dx, m_channel, mv_channel, out)
+        return out
 
     def p2g(self, params, ps):
         self.L.zso_p2g_f32(self.h, C.byref(params), ps.n, ps.x, ps.v, ps.C, ps.F, ps.m, _ptr(ps.logJp))

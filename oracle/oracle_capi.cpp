@@ -196,6 +196,24 @@ double zso_ctx_grids_f64(void* c, double* grid_p2g, double* grid_v) {
   return o.maxVel;
 }
 
+/// Replace the grids (after P2G, and after update + boundary) with caller
+/// data of table.cnt * 7 * 64 floats: synthetic node values for the known-
+/// answer checks of the grid update and G2P.
+void zso_ctx_set_grid_f32(void* c, const float* grid_p2g, const float* grid_v) {
+  auto& o = static_cast<Ctx<float>*>(c)->out;
+  const size_t nf = (size_t)o.table.cnt * kChannels * 64;
+  if (grid_p2g) o.grid.assign(grid_p2g, grid_p2g + nf);
+  if (grid_v) o.gridv.assign(grid_v, grid_v + nf);
+}
+
+/// GridAngularMomentum of the context's grid after P2G (which = 0) or after
+/// the update (which = 1), accumulated into sum[6].
+void zso_ctx_angular_momentum_f32(void* c, int32_t which, float dx, int32_t mchn, int32_t mvchn, double* sum) {
+  auto& o = static_cast<Ctx<float>*>(c)->out;
+  const auto& g = which ? o.gridv : o.grid;
+  grid_angular_momentum(g.data(), o.table, (i64)(g.size() / (kChannels * 64)), dx, mchn, mvchn, sum);
+}
+
 // ---------------------------------------------------------------- stages
 /// CleanSparsity + ComputeSparsity into the context's table of size T.
 void zso_activate_f32(void* c, int32_t T, int64_t n, const float* x, float dx, int32_t blockLen,

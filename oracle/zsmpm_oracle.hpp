@@ -19,9 +19,13 @@
 //   Table insert/query, activation, enlarge, bucket sort: brute-force
 //   set/first-occurrence/std::stable_sort restatements in the tests, plus the
 //   table invariants (bijection, activeKeys[idx] == key, probe reachability).
-//   P2G / grid update / boundary / G2P: conservation of mass and momentum,
-//   a single-particle known answer, affine-field round trip, and a dense-grid
-//   numpy restatement on tiny inputs.
+//   P2G / grid update / boundary / G2P: the P2G force against a finite-
+//   difference gradient of the discrete elastic energy (sign, dt, D^-1 and
+//   the stress transpose), APIC angular-momentum transfer (C's orientation),
+//   G2P against affine grid fields with a non-symmetric F (x, v, C and the
+//   F-update order), free fall (gravity axis, sign and scale, maxVel),
+//   slip/separate/sticky plane projections, conservation of mass and
+//   momentum, a single-particle known answer.
 //   svd3_jacobi64 (the FP64 twin's SVD): reconstruction / orthogonality /
 //   ordering invariants and agreement with numpy's LAPACK SVD.
 #pragma once
@@ -995,6 +999,29 @@ void grid_boundary(T* grid, const Table& t, i64 nblocks, T dx, const PlaneCollid
         }
       }
   }
+}
+
+/// GridAngularMomentum over Collapse{nBlocks, 64}, simulation/grid/GridOp.hpp:216-259:
+/// for every node with m != 0, x = (blockkey * 4 + coord(cell)) * dx and
+/// x cross mv are formed in T; sum[0..2] += (double)(x cross mv),
+/// sum[3..5] += (double)mv (atomic_add into the caller's six doubles).
+template <typename T>
+void grid_angular_momentum(const T* grid, const Table& t, i64 nblocks, T dx, int mChn, int mvChn,
+                           double sum[6]) {
+  for (i64 b = 0; b < nblocks; ++b)
+    for (int c = 0; c < 64; ++c) {
+      const T mass = grid[gidx(b, mChn, c)];
+      if (mass == (T)0) continue;
+      const i32 cc[3] = {(c >> 4) & 3, (c >> 2) & 3, c & 3};
+      T x[3], mv[3];
+      for (int d = 0; d < 3; ++d) {
+        x[d] = ((T)t.active[3 * b + d] * (T)4 + (T)cc[d]) * dx;
+        mv[d] = grid[gidx(b, mvChn + d, c)];
+      }
+      const T res[3] = {x[1] * mv[2] - x[2] * mv[1], x[2] * mv[0] - x[0] * mv[2], x[0] * mv[1] - x[1] * mv[0]};
+      for (int d = 0; d < 3; ++d) sum[d] += (double)res[d];
+      for (int d = 0; d < 3; ++d) sum[3 + d] += (double)mv[d];
+    }
 }
 
 /// G2PTransfer::operator() over range(N), simulation/transfer/G2P.hpp:43-84.

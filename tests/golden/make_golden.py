@@ -3,7 +3,7 @@
 Runs in the build container only: it loads oracle/_ref/libzpc_refmath.so,
 which oracle/Makefile compiles from /root/reference/include (svd_3d, the two
 stress functions, lame_parameters, the quadratic LocalArena, hash_combine,
-next_2pow). The inputs are seeded and cover the cases that matter for the
+next_2pow, the CFL time step). The inputs are seeded and cover the cases that matter for the
 path: random, near-identity, rotation, rank-deficient, repeated singular
 values, negative determinant; Drucker-Prager inputs in all three return-map
 cases; hash keys that are negative, large and adjacent. Only numbers are
@@ -106,11 +106,18 @@ def main():
     n2p_in = np.array([1, 2, 3, 5, 17, 1000, 36465, 65536, 65537], np.int64)
     n2p = np.array([R.zref_next_2pow(int(v)) for v in n2p_in], np.int64)
 
+    # evaluate_timestep_linear_elasticity (physics/SoundSpeedCfl.hpp:6-15):
+    # the configs' materials and dx values, and a spread of others
+    cfl_in = np.array([[1e4, 0.3, 1e3, 1 / 128, 0.5], [5e4, 0.4, 1e3, 1 / 256, 0.5],
+                       [3.5e5, 0.3, 1e3, 1 / 512, 0.5], [1e6, 0.45, 2e3, 1 / 64, 0.3],
+                       [2e3, 0.1, 5e2, 1 / 32, 0.9], [1e8, 0.2, 8e3, 1 / 1024, 0.6]], np.float32)
+    cfl_dt = np.array([R.zref_cfl_dt(*[float(v) for v in row]) for row in cfl_in], np.float32)
+
     np.savez_compressed(os.path.join(HERE, "refmath.npz"), svd_F=F, svd_U=U, svd_S=S, svd_V=V, E_nu=E_nu,
                         lame=lame, fc_vol=vol, fc_PF=PF_fc, sand_F=Fs, sand_logJp_in=lj_in, sand_logJp_out=lj_out,
                         sand_PF=PF_sand, sand_Fproj=Fp_sand, sand_ys=ys, arena_dx=dx, arena_x=x,
                         arena_corner=corner, arena_w=w, arena_diff=diff, hash_keys=keys, hash_vals=hashes,
-                        n2p_in=n2p_in, n2p_out=n2p)
+                        n2p_in=n2p_in, n2p_out=n2p, cfl_in=cfl_in, cfl_dt=cfl_dt)
     print("wrote", os.path.join(HERE, "refmath.npz"))
 
 

@@ -295,14 +295,18 @@ def test_surrogate_substeps_match_oracle_and_plain(trained):
 
 def test_training_never_ends_worse_than_its_best_checkpoint(pol):
     """At an absurd learning rate (1.0) Adam diverges; the checkpoints restore
-    the best parameters and halve the rate, so the network returned is never
-    worse (on the fixed evaluation sample) than the one training started from,
-    and the reported loss is that network's."""
+    the best parameters, Adam's moments and its step count, and halve the
+    rate. Training must roll back at least once, then recover at a halved
+    rate and keep a checkpoint strictly better than the start; the reported
+    loss is that network's."""
     C = _closure()
     _, model = _sand_model()
     F, lj = C.synthetic_states(1 << 15, seed=19, lo=1e-4, hi=0.25)
     Fd, ljd = _dev(F), _dev(lj)
-    tr = C.train(pol, model, Fd, ljd, steps=600, batch=2048, lr=1.0, seed=2, check_every=100)
+    tr = C.train(pol, model, Fd, ljd, steps=1500, batch=2048, lr=1.0, seed=2, check_every=100)
+    print(f"restores {tr.restores}, kept step {tr.best_step}, eval loss {tr.eval_loss:.3e}", flush=True)
+    assert tr.restores >= 1, "lr 1.0 never diverged: the rollback was not exercised"
+    assert tr.best_step > 0, "no checkpoint after a rollback improved on the start"
     X, R = C.harvest(pol, model, Fd, ljd)
     n = X.shape[0]
     import torch
@@ -311,5 +315,5 @@ def test_training_never_ends_worse_than_its_best_checkpoint(pol):
     Xe, Re = X.index_select(0, ev).contiguous(), R.index_select(0, ev).contiguous()
     start = C.eval_loss(pol, torch.as_tensor(C.init_params(2)).to(X.device), Xe, Re)
     kept = C.eval_loss(pol, tr.params, Xe, Re)
-    assert np.isfinite(kept) and kept <= start * (1 + 1e-6), (start, kept, tr.best_step)
+    assert np.isfinite(kept) and kept < start, (start, kept, tr.best_step)
     assert abs(kept - tr.eval_loss) <= 1e-6 * max(kept, 1e-30), (kept, tr.eval_loss)

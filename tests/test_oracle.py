@@ -83,7 +83,9 @@ def test_next_2pow_golden(gold):
 
 # ------------------------------------------------------------------ against the built reference headers
 
-@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (reference tree absent)")
+# The check loads oracle/_ref only when it runs (never at collection), so a
+# GPU-only test run maps nothing built from the reference.
+@pytest.mark.skipif(not os.path.exists(O.REF_PATH), reason="oracle/_ref not built (reference tree absent)")
 def test_oracle_matches_reference_headers_random():
     R = O.ref()
     rng = np.random.default_rng(1)

@@ -101,6 +101,7 @@ class Trained:
     eval_loss: float = float("nan")  # mean loss of `params` on the evaluation sample
     best_step: int = 0        # Adam steps behind `params`
     eval_samples: int = 0
+    restores: int = 0         # times a diverging checkpoint was rolled back (rate halved each time)
 
 
 def eval_loss(pol: rocm.Policy, params, X, R) -> float:
@@ -120,7 +121,9 @@ def train(pol: rocm.Policy, model: rocm.ModelDesc, F, logJp, steps: int = 2000, 
     segments, and the parameters with the lowest one are returned: one bad
     stretch of minibatches cannot leave a worse network behind. When a
     checkpoint's loss exceeds 1.5 times the best, the parameters and Adam
-    moments go back to the best checkpoint and the learning rate is halved."""
+    moments go back to the best checkpoint and the learning rate is halved.
+    The Adam step count goes back with them: the bias corrections of later
+    steps must match the moments they correct."""
     torch = rocm._torch()
     dev = F.device
     X, R = harvest(pol, model, F, logJp)
@@ -135,6 +138,8 @@ def train(pol: rocm.Policy, model: rocm.ModelDesc, F, logJp, steps: int = 2000, 
     Xe, Re = X.index_select(0, ev).contiguous(), R.index_select(0, ev).contiguous()
     best, best_P, best_step = eval_loss(pol, P, Xe, Re), P.clone(), 0
     best_m, best_v = m.clone(), v.clone()
+    t_adam, best_t = 0, 0  # Adam steps behind (m, v) and behind the best checkpoint's moments
+    restores = 0
     scale = 1.0
     secs = 0.0
     torch.cuda.synchronize()
@@ -145,8 +150,9 @@ def train(pol: rocm.Policy, model: rocm.ModelDesc, F, logJp, steps: int = 2000, 
         rb = R.index_select(0, idx)
         a = 0.5 * (1.0 + np.cos(np.pi * k / max(steps - 1, 1)))
         rate = lr * scale * (0.05 + 0.95 * a)
+        t_adam += 1
         rocm.check(pol.L.zsmpm_mlp_train_step__rocm(pol.h, P.data_ptr(), m.data_ptr(), v.data_ptr(),
-                                                    xb.data_ptr(), rb.data_ptr(), batch, float(rate), k + 1,
+                                                    xb.data_ptr(), rb.data_ptr(), batch, float(rate), t_adam,
                                                     loss[k:k + 1].data_ptr()), "mlp_train_step")
         if (k + 1) % check_every == 0 or k + 1 == steps:
             torch.cuda.synchronize()
@@ -154,17 +160,19 @@ def train(pol: rocm.Policy, model: rocm.ModelDesc, F, logJp, steps: int = 2000, 
             e = eval_loss(pol, P, Xe, Re)
             if e < best:
                 best, best_P, best_step = e, P.clone(), k + 1
-                best_m, best_v = m.clone(), v.clone()
+                best_m, best_v, best_t = m.clone(), v.clone(), t_adam
             elif not (e <= 1.5 * best):  # diverging (or NaN): back to the best checkpoint, half the rate
                 P.copy_(best_P)
                 m.copy_(best_m)
                 v.copy_(best_v)
+                t_adam = best_t
                 scale *= 0.5
+                restores += 1
             torch.cuda.synchronize()
             t0 = time.perf_counter()
     lh = loss.cpu().numpy()
     return Trained(best_P, [float(lh[i]) for i in range(0, steps, log_every)] + [float(lh[-1])], secs, n,
-                   eval_loss=best, best_step=best_step, eval_samples=int(ev.numel()))
+                   eval_loss=best, best_step=best_step, eval_samples=int(ev.numel()), restores=restores)
 
 
 def training_states(sc, sim_steps: int = 0, synthetic: int = 1 << 18, seed: int = 0, device="cuda"):

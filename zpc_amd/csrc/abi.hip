@@ -429,8 +429,19 @@ SortView own_buckets(zsmpm_policy_s* p) {
 }
 
 __global__ void set_maxvel_kernel(uint32_t* counters) { counters[kCtrMaxVel] = 0u; }
+// atomic_max(maxVel, m) (execution/Atomics.hpp:308-350) with float
+// semantics for any caller-initialised value: m = max |v|^2 >= 0 replaces a
+// smaller value, a negative one (sign bit set) and a NaN.
 __global__ void copy_maxvel_kernel(const uint32_t* counters, float* out) {
-  atomicMax(reinterpret_cast<uint32_t*>(out), counters[kCtrMaxVel]);
+  const uint32_t mb = counters[kCtrMaxVel];
+  const float m = __uint_as_float(mb);
+  uint32_t* o = reinterpret_cast<uint32_t*>(out);
+  uint32_t cur = __atomic_load_n(o, __ATOMIC_RELAXED);
+  while (!(__uint_as_float(cur) >= m)) {
+    const uint32_t prev = atomicCAS(o, cur, mb);
+    if (prev == cur) break;
+    cur = prev;
+  }
 }
 
 #define ZS_TRY(expr)              \
@@ -439,6 +450,26 @@ __global__ void copy_maxvel_kernel(const uint32_t* counters, float* out) {
     if (_r != ZSMPM_OK) return _r; \
   } while (0)
 #define ZS_HIP(expr) ZS_TRY(from_hip(expr))
+
+// The first-occurrence bitmap holds one bit per insertion order; an
+// enlargement by nd offsets numbers its candidates up to T * nd, which can
+// exceed what reserve() sized for (particle counts and 8 T). Grows both
+// bitmap arrays (the old ones are released with the rest of the scratch).
+int32_t ensure_bitmap(zsmpm_policy_s* p, uint64_t orders) {
+  const uint64_t words = orders / 32 + 2;
+  if (words <= p->ws.bm_words) return ZSMPM_OK;
+  if (words > 0xffffffffull) return ZSMPM_INVALID_ARGUMENT;
+  ZS_HIP(hipStreamSynchronize(p->stream));
+  uint32_t *bm = nullptr, *pre = nullptr;
+  ZS_HIP(alloc(p, &bm, (size_t)words));
+  ZS_HIP(alloc(p, &pre, (size_t)words));
+  destroy_graph(p);  // a captured graph holds the old addresses
+  ++p->ws_epoch;
+  p->ws.bitmap = bm;
+  p->ws.bm_prefix = pre;
+  p->ws.bm_words = (uint32_t)words;
+  return ZSMPM_OK;
+}
 
 int32_t run_buckets(zsmpm_policy_s* p, const TableView& t, const ParticlesView& ps, float dx, SortView& sv) {
   return from_hip(bucket_sort(t, p->ws, ps.x, ps.n, dx, 4, -2, 0.5f, sv, p->stream));
@@ -920,8 +951,18 @@ int32_t zsmpm_table_enlarge__rocm(zsmpm_policy pol, const zsmpm_table* table, co
   if (!pol || !lo || !hi) return ZSMPM_INVALID_ARGUMENT;
   TableView t;
   ZS_TRY(view_table(table, t));
+  // candidate c = block * nd + offset is its insertion order, a 32-bit value
+  // below T * nd, one bit each in the first-occurrence bitmap
+  int64_t nd = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (hi[d] <= lo[d]) return ZSMPM_OK;  // empty offset range: nothing to insert (EnlargeSparsity loops)
+    if (hi[d] - lo[d] > 1024) return ZSMPM_INVALID_ARGUMENT;
+    nd *= hi[d] - lo[d];
+  }
+  if ((int64_t)t.T * nd >= (int64_t)1 << 32) return ZSMPM_INVALID_ARGUMENT;
   DeviceGuard g(pol->devid);
   ZS_TRY(ensure(pol, pol->reserved_n, t.T));
+  ZS_TRY(ensure_bitmap(pol, (uint64_t)t.T * (uint64_t)nd));
   StageScope sc(pol, ZSMPM_STAGE_ENLARGE);
   return from_hip(table_enlarge(t, pol->ws, lo, hi, t.T, pol->stream));
 }

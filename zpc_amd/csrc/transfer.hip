@@ -640,7 +640,22 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
         s.z += u.z;
         s.w += u.w;
       }
-      if (s.x != 0.f) {
+      float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (!kFold) {
+        f = arena[0][n * kNV + 1];
+#pragma unroll
+        for (int w = 1; w < kP2GWaves; ++w) {
+          const float4 u = arena[w][n * kNV + 1];
+          f.x += u.x;
+          f.y += u.y;
+          f.z += u.z;
+        }
+      }
+      // every node a particle touched, massless particles included (zpc adds
+      // all 27 nodes, P2G.hpp:114-123); untouched nodes hold exact zeros
+      const bool touched = s.x != 0.f || s.y != 0.f || s.z != 0.f || s.w != 0.f ||
+                           (!kFold && (f.x != 0.f || f.y != 0.f || f.z != 0.f));
+      if (touched) {
         int nbi, cell;
         node_home(n / 36 + 1, (n / 6) % 6 + 1, n % 6 + 1, nbi, cell);
         const int32_t gb = nbr[nbi];
@@ -651,14 +666,6 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
           unsafeAtomicAdd(gp + 2 * kBlockCells, s.z);
           unsafeAtomicAdd(gp + 3 * kBlockCells, s.w);
           if (!kFold) {
-            float4 f = arena[0][n * kNV + 1];
-#pragma unroll
-            for (int w = 1; w < kP2GWaves; ++w) {
-              const float4 u = arena[w][n * kNV + 1];
-              f.x += u.x;
-              f.y += u.y;
-              f.z += u.z;
-            }
             unsafeAtomicAdd(gp + 4 * kBlockCells, f.x);
             unsafeAtomicAdd(gp + 5 * kBlockCells, f.y);
             unsafeAtomicAdd(gp + 6 * kBlockCells, f.z);

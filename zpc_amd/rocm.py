@@ -487,6 +487,27 @@ class Simulator:
     def nblocks(self) -> int:
         return int(self.table.cnt.item())
 
+    def evaluated_dt(self, cfl: float = 0.5) -> float:
+        """The builder's time step, cfl * dx / sound speed of the scene's model
+        (MPMSimulator::evaluatedDt, simulation/mpm/Simulator.cpp:53-64)."""
+        return S.cfl_dt(self.sc.material, self.sc.dx, cfl)
+
+    def max_vel_sqr(self) -> float:
+        """max |v|^2 over the grid nodes with mass after the last grid update
+        (MPMSimulator::getMaxVel, simulation/mpm/Simulator.hpp:19-26); syncs."""
+        self.pol.sync()
+        bits = np.array([self.pol.counters()[5]], np.uint32)
+        return float(bits.view(np.float32)[0])
+
+    def adaptive_dt(self, cfl: float = 0.5) -> float:
+        """The next substep's dt: the builder's CFL step, lowered so that the
+        fastest grid node of the last update moves at most cfl * dx."""
+        dt = self.evaluated_dt(cfl)
+        v2 = self.max_vel_sqr()
+        if v2 > 0.0:
+            dt = min(dt, float(np.float32(cfl) * np.float32(self.sc.dx) / np.sqrt(np.float32(v2))))
+        return dt
+
 
 class SceneState:
     """Device buffers of one scene of a batch (no policy of its own)."""

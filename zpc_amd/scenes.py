@@ -109,6 +109,36 @@ def box_walls(dx: float, cells: int = 3, kind: int = STICKY) -> List[Collider]:
     return out
 
 
+def sound_speed(E: float, nu: float, rho: float) -> np.float32:
+    """evaluate_soundspeed_linear_elasticity (physics/SoundSpeedCfl.hpp:6-10) in
+    float32: c = sqrt(E (1 - nu) / ((1 + nu) (1 - 2 nu) rho)), the root taken by
+    math::sqrtNewtonRaphson (math/MathUtils.h:239-251: Newton from 1 until
+    successive iterates differ by at most max(1e-6 n, 128 eps))."""
+    f = np.float32
+    E, nu, rho = f(E), f(nu), f(rho)
+    n = E * (f(1) - nu) / ((f(1) + nu) * (f(1) - f(2) * nu) * rho)
+    eps = f(128) * np.finfo(np.float32).eps
+    if n < -eps:
+        return f("nan")
+    if n < eps:
+        return f(0)
+    xn = f(1)
+    xnp1 = f(0.5) * (xn + n / xn)
+    tol = max(n * f(1e-6), eps)
+    while abs(xnp1 - xn) > tol:
+        xn = xnp1
+        xnp1 = f(0.5) * (xn + n / xn)
+    return xnp1
+
+
+def cfl_dt(material: "Material", dx: float, cfl: float = 0.5) -> float:
+    """evaluate_timestep_linear_elasticity (physics/SoundSpeedCfl.hpp:12-15):
+    cfl * dx / c in float32 -- the time step MPMSimulator's builder evaluates
+    per model (simulation/mpm/Simulator.cpp:53-64)."""
+    f = np.float32
+    return float(f(cfl) * f(dx) / sound_speed(material.E, material.nu, material.rho))
+
+
 def next_2pow(n: int) -> int:
     """math/bit/Bits.h:166-184."""
     return 1 if n <= 1 else 1 << (n - 1).bit_length()
