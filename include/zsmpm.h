@@ -193,6 +193,13 @@ int32_t zsmpm_p2g__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_
 int32_t zsmpm_grid_update__rocm(zsmpm_policy pol, const zsmpm_grid* grid, const zsmpm_table* table, float dt,
                                 const float extf[3], const zsmpm_plane_collider* colliders, int32_t ncolliders,
                                 int32_t fold_rhs, float* max_vel_sqr);
+/* GridAngularMomentum over Collapse{table.size(), 64}, simulation/grid/GridOp.hpp:216-259:
+ * for every node whose channel m_channel is nonzero, with x = (blockkey * 4 +
+ * coord(cell)) * dx, adds (double)(x cross mv) to sum[0..2] and (double)mv to
+ * sum[3..5], mv = channels mv_channel..mv_channel+2. sum: 6 device doubles,
+ * accumulated into (caller-initialised, like zpc's sumAngularMomentum). */
+int32_t zsmpm_grid_angular_momentum__rocm(zsmpm_policy pol, const zsmpm_grid* grid, const zsmpm_table* table,
+                                          int32_t m_channel, int32_t mv_channel, double* sum);
 /* G2PTransfer (APIC) over range(n), simulation/transfer/G2P.hpp:43-84. */
 int32_t zsmpm_g2p__rocm(zsmpm_policy pol, const zsmpm_particles* particles, const zsmpm_table* table,
                         const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt);

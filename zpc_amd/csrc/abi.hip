@@ -967,6 +967,19 @@ int32_t zsmpm_table_enlarge__rocm(zsmpm_policy pol, const zsmpm_table* table, co
   return from_hip(table_enlarge(t, pol->ws, lo, hi, t.T, pol->stream));
 }
 
+int32_t zsmpm_grid_angular_momentum__rocm(zsmpm_policy pol, const zsmpm_grid* grid, const zsmpm_table* table,
+                                          int32_t m_channel, int32_t mv_channel, double* sum) {
+  if (!pol || !sum || m_channel < 0 || m_channel >= 7 || mv_channel < 0 || mv_channel + 3 > 7)
+    return ZSMPM_INVALID_ARGUMENT;
+  TableView t;
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(view_grid(grid));
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, pol->reserved_n, t.T));
+  return from_hip(grid_angular_momentum(grid->data, grid->capacity_blocks, t, grid->dx, m_channel, mv_channel, sum,
+                                        pol->ws, pol->stream));
+}
+
 int32_t zsmpm_table_query__rocm(zsmpm_policy pol, const zsmpm_table* table, const int32_t* keys, int64_t n,
                                 int32_t* out) {
   if (!pol || n < 0 || (n > 0 && (!keys || !out))) return ZSMPM_INVALID_ARGUMENT;

@@ -755,6 +755,56 @@ __global__ __launch_bounds__(kThreads) void grid_update_kernel(float* __restrict
   }
 }
 
+// GridAngularMomentum (GridOp.hpp:216-259): for every node with m != 0,
+// x = (blockkey * 4 + coord(cell)) * dx; sum[0..2] += (double)(x cross mv)
+// formed in float, sum[3..5] += (double)mv. Per-thread double partials, a
+// wave and workgroup reduction, six double atomics per workgroup.
+__global__ __launch_bounds__(kThreads) void angular_momentum_kernel(const float* __restrict__ grid, int32_t grid_cap,
+                                                                    TableView t, float dx, int mchn, int mvchn,
+                                                                    double* sum, uint32_t* err) {
+#pragma clang fp contract(off)
+  int32_t nb = *t.cnt;
+  if (nb > grid_cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, kErrCapacity);
+    nb = grid_cap;
+  }
+  const int64_t nodes = (int64_t)nb * kBlockCells;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nodes; i += stride) {
+    const int32_t b = (int32_t)(i >> 6);
+    const int cell = (int)(i & 63);
+    const float mass = grid[grid_at(b, mchn, cell)];
+    if (mass == 0.f) continue;
+    const float x0 = ((float)t.active[3 * (int64_t)b] * 4.f + (float)((cell >> 4) & 3)) * dx;
+    const float x1 = ((float)t.active[3 * (int64_t)b + 1] * 4.f + (float)((cell >> 2) & 3)) * dx;
+    const float x2 = ((float)t.active[3 * (int64_t)b + 2] * 4.f + (float)(cell & 3)) * dx;
+    const float m0 = grid[grid_at(b, mvchn, cell)], m1 = grid[grid_at(b, mvchn + 1, cell)],
+                m2 = grid[grid_at(b, mvchn + 2, cell)];
+    acc[0] += (double)(x1 * m2 - x2 * m1);
+    acc[1] += (double)(x2 * m0 - x0 * m2);
+    acc[2] += (double)(x0 * m1 - x1 * m0);
+    acc[3] += (double)m0;
+    acc[4] += (double)m1;
+    acc[5] += (double)m2;
+  }
+  __shared__ double part[kWaves][6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, kWave);
+  }
+  const int wave = threadIdx.x / kWave;
+  if (lane_id() == 0)
+    for (int k = 0; k < 6; ++k) part[wave][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double s = 0.0;
+    for (int w = 0; w < kWaves; ++w) s += part[w][threadIdx.x];
+    if (s != 0.0) atomicAdd(&sum[threadIdx.x], s);
+  }
+}
+
 // kNext: -1 plain G2P; 0 / 1 (reorder only): also the next substep's
 // fixed-corotated / Drucker-Prager stress from the updated F (NextStress)
 template <bool kReorder, int kNext>
@@ -1041,6 +1091,12 @@ hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float 
                        const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s) {
   grid_update_kernel<<<persistent_groups(), kThreads, 0, s>>>(grid, grid_cap, t, dt, extf[0], extf[1], extf[2],
                                                                cols, dx, fold_rhs, ws.counters, ws.err);
+  return hipGetLastError();
+}
+
+hipError_t grid_angular_momentum(const float* grid, int32_t grid_cap, const TableView& t, float dx, int mchn,
+                                 int mvchn, double* sum, Workspace& ws, hipStream_t s) {
+  angular_momentum_kernel<<<1024, kThreads, 0, s>>>(grid, grid_cap, t, dx, mchn, mvchn, sum, ws.err);
   return hipGetLastError();
 }
 

@@ -215,6 +215,9 @@ hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortVi
                        bool reset_queue = true);
 hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float dt, const float extf[3],
                        const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s);
+// GridAngularMomentum: sum[0..2] += x cross mv, sum[3..5] += mv over nodes with mass
+hipError_t grid_angular_momentum(const float* grid, int32_t grid_cap, const TableView& t, float dx, int mchn,
+                                 int mvchn, double* sum, Workspace& ws, hipStream_t s);
 // out == nullptr updates ps in place; otherwise the updated particles (and
 // m, logJp, id) are written to out in bucket order (slot = sorted position)
 hipError_t g2p(const ParticlesView& ps, const ParticlesView* out, const TableView& t, const SortView& sv,

@@ -122,6 +122,7 @@ def lib():
         "zsmpm_p2g__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(TableDesc), P(BucketsDesc), P(GridDesc), f32],
         "zsmpm_grid_update__rocm": [vp, P(GridDesc), P(TableDesc), f32, P(C.c_float * 3), P(PlaneCollider), i32,
                                     i32, vp],
+        "zsmpm_grid_angular_momentum__rocm": [vp, P(GridDesc), P(TableDesc), i32, i32, vp],
         "zsmpm_g2p__rocm": [vp, P(ParticlesDesc), P(TableDesc), P(BucketsDesc), P(GridDesc), f32],
         "zsmpm_substep__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(TableDesc), P(BucketsDesc), P(GridDesc), f32,
                                 P(C.c_float * 3), P(PlaneCollider), i32],
@@ -465,6 +466,17 @@ class Simulator:
         check(self.L.zsmpm_grid_update__rocm(self.pol.h, C.byref(self.grid.desc), C.byref(self.table.desc), dt,
                                              C.byref(self.extf), self.cols, self.ncols, fold,
                                              None if max_vel is None else max_vel.data_ptr()), "grid_update")
+
+    def angular_momentum(self, m_channel: int = 0, mv_channel: int = 1) -> np.ndarray:
+        """GridAngularMomentum (GridOp.hpp:216-259) of the grid as it stands:
+        [sum x cross mv (3), sum mv (3)] over nodes with mass, float64."""
+        torch = _torch()
+        out = torch.zeros(6, dtype=torch.float64, device=self.grid.data.device)
+        check(self.L.zsmpm_grid_angular_momentum__rocm(self.pol.h, C.byref(self.grid.desc),
+                                                       C.byref(self.table.desc), m_channel, mv_channel,
+                                                       out.data_ptr()), "grid_angular_momentum")
+        self.pol.sync()
+        return out.cpu().numpy()
 
     def g2p(self, dt=None):
         dt = float(self.sc.dt if dt is None else dt)

@@ -204,12 +204,14 @@ def cube_scene(cells: int = 40, seed: int = 20261201 + 1, dx: float = 1.0 / 128,
 
 
 def spheres_scene(seed: int = 20261201 + 2, dx: float = 1.0 / 256, r: float = 0.15, ppc: int = 8,
-                  speed: float = 2.0, steps: int = 200) -> Scene:
+                  speed: float = 2.0, steps: int = 200, centers=(0.3, 0.7)) -> Scene:
     """Config 2: two elastic spheres r=0.15 at (0.3,0.5,0.5) and (0.7,0.5,0.5)
-    moving at +-2 m/s along x; reference default fixed-corotated E=5e4 nu=0.4."""
+    moving at +-2 m/s along x; reference default fixed-corotated E=5e4 nu=0.4.
+    Particles of the first sphere come first. `centers` moves the spheres
+    along x (smaller scenes that collide within a few hundred substeps)."""
     mat = Material(kind=FIXED_COROTATED, E=5e4, nu=0.4, rho=1e3, volume=dx ** 3 / ppc)
     xs, vs = [], []
-    for ci, (cx, sgn) in enumerate(((0.3, 1.0), (0.7, -1.0))):
+    for ci, (cx, sgn) in enumerate(((centers[0], 1.0), (centers[1], -1.0))):
         c = np.array([cx, 0.5, 0.5])
         lo = np.floor((c - r) / dx).astype(int)
         hi = np.ceil((c + r) / dx).astype(int)
