@@ -153,15 +153,20 @@ def run_path(torch, rocm, sc, args, multi, graph, closure=None):
     sim = rocm.Simulator(sc, reorder=not args.in_place)
     if closure is not None:
         sim.pol.set_closure(closure[0], tol=closure[1], always_network=closure[2])
+    tune = 0
     if multi:
-        # the warm-up substeps also pick the P2G / G2P workgroup counts
-        sim.substeps(args.warmup, autotune=True)
+        # 8 untimed substeps pick the P2G / G2P workgroup counts, whatever
+        # --warmup is; then the warm-up substeps
+        tune = 8
+        sim.substeps(tune, autotune=True)
+        if args.warmup > 0:
+            sim.substeps(args.warmup)
     else:
         for _ in range(args.warmup):
             sim.substep()
     sim.sync()
     b0 = sim.nblocks()
-    print(f"# warm-up done ({args.warmup} substeps), {b0} active blocks", file=sys.stderr, flush=True)
+    print(f"# warm-up done ({tune} tuning + {args.warmup} substeps), {b0} active blocks", file=sys.stderr, flush=True)
     fb0 = int(sim.pol.counters()[4])
     # profiled eager pass: every stage bracketed by HIP events on the stream it
     # runs on (per-stage and per-kernel times, the roofline)
@@ -196,7 +201,7 @@ def run_path(torch, rocm, sc, args, multi, graph, closure=None):
         wall = t1 - t0
         print(f"# graph replays: {wall * 1e3 / args.steps:.3f} ms/substep", file=sys.stderr, flush=True)
     b1 = sim.nblocks()
-    substeps_run = args.warmup + args.steps + (2 + args.steps if graph else 0)
+    substeps_run = tune + args.warmup + args.steps + (2 + args.steps if graph else 0)
     out = {"stages": stages, "blocks": 0.5 * (b0 + b1), "ms": wall * 1e3 / args.steps, "eager_ms": eager_ms,
            "launch_widths": sim.pol.launch_widths(),
            "fallbacks": int(sim.pol.counters()[4]) - fb0, "substeps": substeps_run, "x": x_by_id(torch, sim)}
@@ -230,7 +235,7 @@ def surrogate_leg(torch, rocm, sc, args, graph, plain):
                                  ("zero_start", None, False)):
         r = run_path(torch, rocm, sc, args, True, graph, closure=(params, 1e-4, always))
         dx_err = (r["x"] - plain["x"]).abs().max().item() / sc.dx
-        n_pass = r["substeps"] - args.warmup
+        n_pass = r["substeps"] - args.warmup - 8
         per_launch = {k: (ms / c if c else 0.0) for k, (ms, c) in r["stages"].items()}
         res[name] = {
             "value": sc.n / (r["ms"] * 1e-3),

@@ -263,43 +263,43 @@ int32_t zsmpm_policy_launch_widths__rocm(zsmpm_policy pol, int32_t* p2g_groups, 
 
 /* ------------------------------------------------------------------------
  * Batches of independent scenes (an addition: zpc advances one scene per
- * driver). Every scene has its own model, particles, table, buckets, grid,
- * time step, gravity and colliders; a call advances all of them nsteps
- * substeps.
+ * driver; no reference counterpart). The scenes share one set of particle
+ * arrays, one table, one bucket sort and one grid, and every stage of a
+ * substep is one launch over all of them. Scene s owns the particles
+ * [first, first + count) (the scenes in order, first = the sum of the
+ * earlier counts); its block keys are shifted by s * 1024 along axis 0, so
+ * no block is shared and each block's scene follows from its key. A scene's
+ * own particle block keys must lie in [-512, 511) along axis 0 (ZSMPM_KEY_RANGE
+ * otherwise). Every scene has its own model, dx, dt, gravity and plane
+ * colliders; a scene's particles, grid values and substep results are those
+ * of zsmpm_substeps_sorted__rocm run on that scene alone (its tables list
+ * the same keys, shifted, in the same relative order). The sorted driver
+ * keeps each scene's particles inside its own range.
  * ---------------------------------------------------------------------- */
-#define ZSMPM_MAX_LANES 8
+#define ZSMPM_MAX_SCENES 256
 
-typedef struct zsmpm_scene {
+typedef struct zsmpm_batch_scene {
   zsmpm_abi_header hdr;
-  const zsmpm_model* model;
-  const zsmpm_particles* a;        /* particle state */
-  const zsmpm_particles* b;        /* NULL: substeps in place (zsmpm_substep__rocm); else the sorted
-                                      driver's second set, ping-ponged a -> b -> a (same for every scene) */
-  const zsmpm_table* table;
-  const zsmpm_buckets* buckets;    /* may be NULL (the lane's own) */
-  const zsmpm_grid* grid;
-  float dt;
+  int64_t first, count; /* particle range in the batch's arrays */
+  zsmpm_model model;
+  float dx, dt;
   float extf[3];
-  const zsmpm_plane_collider* colliders;
   int32_t ncolliders;
-} zsmpm_scene;
+  zsmpm_plane_collider colliders[ZSMPM_MAX_COLLIDERS];
+} zsmpm_batch_scene;
 
-/* nsteps substeps of every scene. The scenes are spread over `lanes` lanes
- * (largest first, each to the lane with the fewest particles); a lane is a
- * stream with its own scratch, and one hipGraph holds two substeps of every
- * scene with the lanes forked from and joined to the capture stream. The graph
- * is replayed nsteps / 2 times on the policy's stream (an odd remainder runs
- * as ordinary substeps) and kept on the policy while the scenes stay the same.
- * lanes = 0 autotunes: when nsteps >= 24 the first replays (which are the
- * caller's substeps, not extra ones) time 1, 2, 4 and 8 lanes and the fastest
- * is kept for this scene set; tune_ms (4 floats, may be NULL) receives the
- * milliseconds per replay of each count (-1 where not measured) and lanes_used
- * the count used. final_in_b as zsmpm_substeps_sorted__rocm.
- * Currently returns ZSMPM_UNSUPPORTED_OPERATION after validating its
- * arguments: the path is switched off until a host fault seen in its first
- * run is explained. */
-int32_t zsmpm_substeps_batched__rocm(zsmpm_policy pol, const zsmpm_scene* scenes, int32_t nscenes, int32_t nsteps,
-                                     int32_t lanes, int32_t* lanes_used, float* tune_ms, int32_t* final_in_b);
+/* nsteps sorted substeps of every scene, ping-ponging a -> b -> a like
+ * zsmpm_substeps_sorted__rocm (flags ZSMPM_SUBSTEPS_GRAPH and
+ * ZSMPM_SUBSTEPS_AUTOTUNE as there; the graph holds two substeps of every
+ * scene). a->id (when set) is carried by the reorder. logJp is required
+ * when any scene is Drucker-Prager (other scenes' entries are carried).
+ * grid->dx is not read. Counter 5 (max |v|^2) is the maximum over all
+ * scenes. The learned closure is not applied to batches
+ * (ZSMPM_UNSUPPORTED_OPERATION while one is selected). */
+int32_t zsmpm_substeps_batched__rocm(zsmpm_policy pol, const zsmpm_batch_scene* scenes, int32_t nscenes,
+                                     const zsmpm_particles* a, const zsmpm_particles* b, const zsmpm_table* table,
+                                     const zsmpm_buckets* buckets, const zsmpm_grid* grid, int32_t nsteps,
+                                     int32_t flags, int32_t* final_in_b);
 
 /* ------------------------------------------------------------------------
  * Learned stress closure (an addition: zpc has no learned constitutive path).

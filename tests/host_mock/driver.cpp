@@ -1,6 +1,6 @@
 // Drives the C ABI's host logic against the mock runtime: policy life cycle,
-// graph capture of the sorted driver, closure selection, batched scenes (lane
-// assignment, capture, autotuning bookkeeping) and teardown. Exit status 0
+// graph capture of the sorted driver, closure selection, batched scenes (scene
+// table validation and upload, capture, autotuning) and teardown. Exit status 0
 // and every call returning ZSMPM_OK (or the documented code) is the pass.
 #include <cstdio>
 #include <cstdlib>
@@ -15,7 +15,6 @@ int main(int argc, char** argv) {
   printf("policy %d\n", zsmpm_policy__rocm(0, nullptr, &pol));
   const int ns = 6;
   std::vector<Sc> sc(ns);
-  std::vector<zsmpm_scene> d(ns);
   for (int i = 0; i < ns; ++i) {
     Sc& s = sc[i];
     s.model = mk<zsmpm_model>(); s.model.kind = i & 1; s.model.volume = 1e-9f; s.model.E = 1e4f; s.model.nu = 0.3f; s.model.beta = 1; s.model.yield_surface = 0.3f; s.model.volume_correction = 1;
@@ -32,15 +31,13 @@ int main(int argc, char** argv) {
     s.bk.counts = (int32_t*)calloc(s.bk.capacity_blocks + 1, 4); s.bk.offsets = (int32_t*)calloc(s.bk.capacity_blocks + 1, 4); s.bk.indices = (int32_t*)calloc(n, 4);
     s.g = mk<zsmpm_grid>(); s.g.dx = 1.f / 128; s.g.capacity_blocks = s.t.table_size / 16; s.g.data = (float*)calloc((size_t)s.g.capacity_blocks * 7 * 64, 4);
     for (int k = 0; k < 6; ++k) { s.cols[k] = {}; s.cols[k].normal[k / 2] = 1; }
-    zsmpm_scene& z = d[i]; z = mk<zsmpm_scene>();
-    z.model = &s.model; z.a = &s.a; z.b = &s.b; z.table = &s.t; z.buckets = &s.bk; z.grid = &s.g; z.dt = 1e-4f; z.extf[1] = -9.8f; z.colliders = s.cols; z.ncolliders = 6;
   }
   int fails = 0;
   auto expect = [&](const char* what, int r, int want) {
     printf("%s -> %d\n", what, r);
     if (r != want) { printf("  expected %d\n", want); ++fails; }
   };
-  int32_t used = 0, inb = 0; float tune[4];
+  int32_t inb = 0;
   // the sorted driver: eager, fused, graph (capture + cached replays), odd count
   Sc& s0 = sc[0];
   float g3[3] = {0.f, -9.8f, 0.f};
@@ -61,19 +58,48 @@ int main(int argc, char** argv) {
   cl.tol = 0.f;
   expect("closure tol 0 rejected", zsmpm_policy_closure__rocm(pol, &cl), ZSMPM_INVALID_ARGUMENT);
   expect("closure off", zsmpm_policy_closure__rocm(pol, nullptr), 0);
-  // batched scenes
-  const int want_batch = argc > 1 ? atoi(argv[1]) : 0;
-  for (int lanes : {1, 3}) {
-    char what[64]; snprintf(what, sizeof what, "batched lanes %d", lanes);
-    expect(what, zsmpm_substeps_batched__rocm(pol, d.data(), ns, 4, lanes, &used, tune, &inb), want_batch);
-    if (want_batch == 0) expect("lanes used", used, lanes);
+  // batched scenes: every scene's particles in one set of arrays
+  int64_t ntot = 0;
+  for (int i = 0; i < ns; ++i) ntot += sc[i].a.n;
+  zsmpm_particles ba[2];
+  for (zsmpm_particles& p : ba) {
+    p = mk<zsmpm_particles>(); p.n = ntot;
+    p.x = (float*)calloc(3 * ntot, 4); p.v = (float*)calloc(3 * ntot, 4); p.C = (float*)calloc(9 * ntot, 4); p.F = (float*)calloc(9 * ntot, 4);
+    p.m = (float*)calloc(ntot, 4); p.logJp = (float*)calloc(ntot, 4); p.id = (int32_t*)calloc(ntot, 4);
   }
-  expect("batched autotune", zsmpm_substeps_batched__rocm(pol, d.data(), 5, 25, 0, &used, tune, &inb), want_batch);
-  if (want_batch == 0) {
-    expect("autotune timed 1, 2, 4 lanes", tune[0] > 0 && tune[1] > 0 && tune[2] > 0 && tune[3] < 0, 1);
-    expect("odd batch ends in b", inb, 1);
+  Sc& big = sc[ns - 1];  // its table, buckets and grid serve the batch (buckets sized for it)
+  std::vector<zsmpm_batch_scene> d(ns);
+  int64_t first = 0;
+  for (int i = 0; i < ns; ++i) {
+    zsmpm_batch_scene& z = d[i]; z = mk<zsmpm_batch_scene>();
+    z.first = first; z.count = sc[i].a.n; first += z.count;
+    z.model = sc[i].model; z.dx = (i & 1) ? 1.f / 256 : 1.f / 128; z.dt = 1e-4f; z.extf[1] = -9.8f;
+    z.ncolliders = 6;
+    for (int k = 0; k < 6; ++k) z.colliders[k] = sc[i].cols[k];
   }
-  expect("batched 9 lanes rejected", zsmpm_substeps_batched__rocm(pol, d.data(), ns, 2, 9, &used, tune, &inb), ZSMPM_INVALID_ARGUMENT);
+  zsmpm_buckets bbk = mk<zsmpm_buckets>(); bbk.capacity_blocks = big.bk.capacity_blocks;
+  bbk.counts = big.bk.counts; bbk.offsets = big.bk.offsets; bbk.indices = (int32_t*)calloc(ntot, 4);
+  expect("batched eager", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 3, 0, &inb), 0);
+  expect("batched odd ends in b", inb, 1);
+  expect("batched graph + autotune", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 12, ZSMPM_SUBSTEPS_GRAPH | ZSMPM_SUBSTEPS_AUTOTUNE, &inb), 0);
+  expect("batched graph replayed", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 4, ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
+  d[1].dt = 2e-4f;  // a changed scene table is uploaded again and recaptured
+  expect("batched new scene table", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 2, ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
+  expect("batched fuse rejected", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 2, ZSMPM_SUBSTEPS_FUSE_STRESS, &inb), ZSMPM_INVALID_ARGUMENT);
+  d[2].first += 1;
+  expect("batched gap rejected", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 1, 0, &inb), ZSMPM_INVALID_ARGUMENT);
+  d[2].first -= 1;
+  expect("batched 257 scenes rejected", zsmpm_substeps_batched__rocm(pol, d.data(), 257, &ba[0], &ba[1], &big.t, &bbk, &big.g, 1, 0, &inb), ZSMPM_INVALID_ARGUMENT);
+  d[3].ncolliders = 9;
+  expect("batched 9 colliders rejected", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 1, 0, &inb), ZSMPM_INVALID_ARGUMENT);
+  d[3].ncolliders = 6;
+  cl.tol = 1e-4f;
+  expect("closure on again", zsmpm_policy_closure__rocm(pol, &cl), 0);
+  expect("batched closure unsupported", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 1, 0, &inb), ZSMPM_UNSUPPORTED_OPERATION);
+  expect("closure off again", zsmpm_policy_closure__rocm(pol, nullptr), 0);
+  expect("batched after closure off", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 2, ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
+  for (zsmpm_particles& p : ba) { free(p.x); free(p.v); free(p.C); free(p.F); free(p.m); free(p.logJp); free(p.id); }
+  free(bbk.indices);
   expect("sync", zsmpm_policy_sync(pol), 0);
   expect("del", zsmpm_del_policy__rocm(pol), 0);
   for (Sc& s : sc) {

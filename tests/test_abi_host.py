@@ -6,7 +6,7 @@ graphs as checked handles, stream capture with CUDA's join rules) and no-op
 stand-ins for the kernels' translation units (tests/host_mock/stubs.cpp). The
 driver (tests/host_mock/driver.cpp) runs the policy life cycle, the sorted
 driver eagerly and from a captured graph, the closure selection, the batched
-scenes (lane assignment, capture, autotuning bookkeeping) and teardown; every
+scenes (scene table validation and upload, capture, autotuning) and teardown; every
 call must return the documented code, and AddressSanitizer must report no
 invalid access and no leak. No kernel runs: this checks the host side only.
 """
@@ -23,20 +23,14 @@ MOCK = os.path.join(ROOT, "tests", "host_mock")
 CSRC = os.path.join(ROOT, "zpc_amd", "csrc")
 INC = os.path.join(ROOT, "include")
 
-UNSUPPORTED = -4
-
-
 def _run(cmd, cwd):
     r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
     assert r.returncode == 0, " ".join(cmd) + "\n" + r.stdout[-2000:] + r.stderr[-4000:]
     return r
 
 
-def _build(tmp, batched_on: bool) -> str:
+def _build(tmp) -> str:
     src = open(os.path.join(CSRC, "abi.hip")).read()
-    if batched_on:
-        assert "constexpr bool kBatchedDisabled = true;" in src
-        src = src.replace("constexpr bool kBatchedDisabled = true;", "constexpr bool kBatchedDisabled = false;")
     src = src.replace('"../../include/zsmpm.h"', '"zsmpm.h"')
     abi = os.path.join(tmp, "abi_host.hip")
     open(abi, "w").write(src)
@@ -60,10 +54,8 @@ def _build(tmp, batched_on: bool) -> str:
 
 
 @pytest.mark.skipif(not os.path.exists(HIPCC) or shutil.which("g++") is None, reason="needs hipcc and g++")
-@pytest.mark.parametrize("batched_on", [False, True])
-def test_abi_host_logic_under_asan(tmp_path, batched_on):
-    exe = _build(str(tmp_path), batched_on)
+def test_abi_host_logic_under_asan(tmp_path):
+    exe = _build(str(tmp_path))
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0")
-    want = "0" if batched_on else str(UNSUPPORTED)
-    r = subprocess.run([exe, want], capture_output=True, text=True, env=env, timeout=120)
+    r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=120)
     assert r.returncode == 0 and "PASSED" in r.stdout, r.stdout[-3000:] + r.stderr[-6000:]

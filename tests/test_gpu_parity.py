@@ -442,59 +442,6 @@ def test_graph_substeps_replays(scene, request):
             check_field(f"{name} after {done} substeps", got[name], ps.by_id(name), ps64.by_id(name), tol)
 
 
-BATCH_OFF = ("zsmpm_substeps_batched__rocm is switched off: its first GPU run ended in a host segmentation "
-             "fault inside the call whose cause has not been shown")
-
-
-def _batch_vs_oracle(b, scenes, steps, names, tol, with64=True):
-    for st, sc in zip(b.states, scenes):
-        got = st.parts.host_by_id()
-        ps = _oracle_particles(sc)
-        O.Context().run(O.make_params(sc), ps, steps, reorder=True)
-        ps64 = None
-        if with64:
-            ps64 = _oracle_particles(sc).astype(np.float64)
-            O.Context(np.float64).run(O.make_params(sc), ps64, steps, reorder=True)
-        for name in names:
-            if got[name] is None:
-                continue
-            if ps64 is None:
-                e = rel_err(got[name], ps.by_id(name))
-                assert e < tol, f"{sc.name} {name}: {e:.3e}"
-            else:
-                check_field(f"{sc.name} {name}", got[name], ps.by_id(name), ps64.by_id(name), tol)
-
-
-@pytest.mark.skip(reason=BATCH_OFF)
-@pytest.mark.parametrize("lanes", [1, 3])
-def test_batched_substeps_match_oracle(lanes):
-    """Six independent scenes (boxes and spheres, fixed-corotated and sand,
-    dx 1/128 and 1/256) advanced together, the captured graph spread over
-    `lanes` streams: every scene stays on its own oracle run."""
-    scenes = S.batch_scenes(count=6, lo=2000, hi=20000)
-    rocm = _rocm()
-    b = rocm.Batch(scenes)
-    b.substeps(4, lanes=lanes)
-    b.sync()
-    assert b.lanes_used == lanes
-    _batch_vs_oracle(b, scenes, 4, ("x", "v", "C", "F", "m", "logJp"), 1e-4)
-
-
-@pytest.mark.skip(reason=BATCH_OFF)
-def test_batched_autotune_and_odd_steps():
-    """lanes = 0 with 25 substeps: the first replays time 1, 2 and 4 lanes (8
-    exceeds the 5 scenes), the rest use the fastest, the odd substep runs
-    eagerly; all 25 are real substeps of every scene."""
-    scenes = S.batch_scenes(count=5, lo=2000, hi=10000, seed=20261301)
-    rocm = _rocm()
-    b = rocm.Batch(scenes)
-    b.substeps(25, lanes=0)
-    b.sync()
-    assert b.lanes_used in (1, 2, 4)
-    assert all(t > 0 for t in b.tune_ms[:3]) and b.tune_ms[3] < 0, b.tune_ms
-    _batch_vs_oracle(b, scenes, 25, ("x", "v", "F"), TOL_100STEP, with64=False)
-
-
 @pytest.mark.parametrize("scene", ["cube", "sand"])
 def test_autotuned_substeps(scene, request):
     """ZSMPM_SUBSTEPS_AUTOTUNE: the 8 tuning substeps (four P2G / G2P

@@ -192,9 +192,21 @@ def test_fp64_svd_path(kind):
         ref, lj_ref, _, _ = O.stress_sand64(mat.volume, mu, lam, 0.0, 1.0, mat.yield_surface, 1, lj, F64)
     all64, lj_all, used_all = _stress(pol, mat, 1e30, F, lj)
     assert used_all == n
-    scale = np.maximum(np.max(np.abs(ref), axis=1), 1e-30)
+    # per row, relative to |ref| plus the modulus scale 2 mu vol: a rotation's
+    # stress is zero, and there the float32 output carries the rounding of
+    # terms of size 2 mu vol (sigma - 1) that cancel
+    scale = np.max(np.abs(ref), axis=1) + 2.0 * mu * mat.volume
     err = np.max(np.abs(all64 - ref), axis=1) / scale
-    assert np.all(err < 5e-6), f"FP64 path vs FP64 oracle: max {err.max():.3e} at {int(err.argmax())}"
+    # the constitutive tail after the SVD stays float32 (only the SVD is FP64):
+    # for nearly singular F the sand model divides by the clamped singular
+    # value (>= 1e-4) and multiplies it back, so its rounding grows with the
+    # condition number sigma_max / max(sigma_min, 1e-4)
+    sv = np.linalg.svd(F64.reshape(-1, 3, 3), compute_uv=False)
+    kappa = sv[:, 0] / np.maximum(sv[:, 2], 1e-4)
+    tol = 5e-6 * np.maximum(1.0, kappa)
+    bad = err >= tol
+    assert not np.any(bad), (f"FP64 path vs FP64 oracle: {int(bad.sum())} rows, worst {err.max():.3e} at "
+                             f"{int(err.argmax())} (kappa {kappa[int(err.argmax())]:.3e})")
     if lj is not None:
         assert np.max(np.abs(lj_all - lj_ref)) < 1e-6
     plain, _, used0 = _stress(pol, mat, 0.0, F, lj)

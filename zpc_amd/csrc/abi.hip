@@ -31,16 +31,11 @@ struct zsmpm_policy_s {
   ClosureView closure{};
   float* mlp_partial = nullptr;  // [kMlpGroups * kMlpParams] gradient partials
   float* mlp_loss = nullptr;     // [kMlpGroups]
-  // batched scenes: lanes are child policies with their own stream, side
-  // stream and scratch; a captured graph runs two substeps of every scene
-  bool owns_stream = false;
-  std::vector<zsmpm_policy_s*> lanes;
-  std::vector<hipEvent_t> lane_ev;
-  hipEvent_t bfork_ev = nullptr;
-  hipGraphExec_t bexec = nullptr;
-  std::vector<unsigned char> bkey;
-  std::vector<unsigned char> tune_key;  // scene set the autotuned lane count belongs to
-  int32_t tuned_lanes = 0;
+  // batched scenes (zsmpm_substeps_batched__rocm): the scene table on the
+  // device and the host bytes it was uploaded from
+  SceneParams* scene_p = nullptr;   // [kMaxScenes]
+  int64_t* scene_first = nullptr;   // [kMaxScenes + 1]
+  std::vector<unsigned char> scene_bytes;
   // P2G / G2P workgroup counts picked by ZSMPM_SUBSTEPS_AUTOTUNE (0: default)
   int32_t tuned_p2g = 0, tuned_g2p = 0;
   Workspace ws{};
@@ -486,17 +481,20 @@ struct Chain {
 
 // The explicit driver's substep (stage order of the reconstructed zpc
 // driver); with `particles_out`, G2P also performs the Vector::reorder gather.
+// With ss (a batch, validated by its entry point) every scene's model, dt,
+// gravity and colliders come from the device scene table, and model, dt,
+// extf and colliders are not read.
 int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_particles* particles,
                     const zsmpm_particles* particles_out, const zsmpm_table* table, const zsmpm_buckets* buckets,
                     const zsmpm_grid* grid, float dt, const float extf[3], const zsmpm_plane_collider* colliders,
-                    int32_t ncolliders, const Chain* chain = nullptr) {
-  if (!pol || !extf) return ZSMPM_INVALID_ARGUMENT;
-  Model m;
+                    int32_t ncolliders, const Chain* chain = nullptr, const SceneSet* ss = nullptr) {
+  const bool multi = ss && ss->n > 0;
+  if (!pol || (!multi && !extf)) return ZSMPM_INVALID_ARGUMENT;
+  Model m{};
   ParticlesView ps, po;
   TableView t;
   Colliders cols;
   SortView sv;
-  ZS_TRY(make_model(model, m));
   ZS_TRY(view_particles(particles, ps, true));
   if (particles_out) {
     ZS_TRY(view_particles(particles_out, po, true));
@@ -504,8 +502,15 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
   }
   ZS_TRY(view_table(table, t));
   ZS_TRY(view_grid(grid));
-  ZS_TRY(make_colliders(colliders, ncolliders, cols));
-  if (m.kind == ZSMPM_DRUCKER_PRAGER && !ps.logJp) return ZSMPM_INVALID_ARGUMENT;
+  if (!multi) {
+    ZS_TRY(make_model(model, m));
+    ZS_TRY(make_colliders(colliders, ncolliders, cols));
+    if (m.kind == ZSMPM_DRUCKER_PRAGER && !ps.logJp) return ZSMPM_INVALID_ARGUMENT;
+  } else if (pol->closure_on || (chain && (chain->cached_stress || chain->compute_next))) {
+    return ZSMPM_UNSUPPORTED_OPERATION;
+  }
+  const float no_extf[3] = {0.f, 0.f, 0.f};
+  const float* g3 = multi ? no_extf : extf;
   DeviceGuard g(pol->devid);
   ZS_TRY(ensure(pol, ps.n, t.T));
   if (buckets) {
@@ -528,7 +533,7 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
     ZS_HIP(side.fork());
     StageScope sc(pol, ZSMPM_STAGE_STRESS, pol->side);
     ZS_HIP(zsmpm::stress(m, ps, ws, pol->side, pol->closure_on ? &pol->closure : nullptr, nullptr,
-                         /*beside_sparsity=*/true));
+                         /*beside_sparsity=*/true, ss));
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_TABLE_CLEAR);
@@ -537,7 +542,7 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_ACTIVATE);
-    ZS_HIP(table_activate(t, ws, ps.x, ps.n, dx, 4, -2, 0.5f, s, /*record_slots=*/true));
+    ZS_HIP(table_activate(t, ws, ps.x, ps.n, dx, 4, -2, 0.5f, s, /*record_slots=*/true, ss));
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_ENLARGE);
@@ -561,11 +566,11 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
     // v = (mv + rhs)/m convention), so the update must not add channels 4..6
     StageScope sc(pol, ZSMPM_STAGE_P2G);
     ZS_HIP(zsmpm::p2g_scatter(ps, t, sv, grid->data, grid->capacity_blocks, dt, dx, true, ws, s,
-                              /*reset_queue=*/false));
+                              /*reset_queue=*/false, ss));
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_GRID_UPDATE);
-    ZS_HIP(zsmpm::grid_update(grid->data, grid->capacity_blocks, t, dt, extf, cols, dx, 0, ws, s));
+    ZS_HIP(zsmpm::grid_update(grid->data, grid->capacity_blocks, t, dt, g3, cols, dx, 0, ws, s, ss));
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_G2P);
@@ -579,7 +584,7 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
       }
     }
     ZS_HIP(zsmpm::g2p(ps, particles_out ? &po : nullptr, t, sv, grid->data, grid->capacity_blocks, dt, dx, ws, s,
-                      &ns));
+                      &ns, ss));
   }
   return ZSMPM_OK;
 }
@@ -601,10 +606,13 @@ void key_put(std::vector<unsigned char>& k, const D* d, size_t bytes = sizeof(D)
 int32_t capture_pair(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_particles* a,
                      const zsmpm_particles* b, const zsmpm_table* table, const zsmpm_buckets* buckets,
                      const zsmpm_grid* grid, float dt, const float extf[3], const zsmpm_plane_collider* colliders,
-                     int32_t ncolliders) {
-  if (!model || !table || !grid || !extf || ncolliders < 0 || ncolliders > kMaxColliders)
+                     int32_t ncolliders, const SceneSet* ss = nullptr) {
+  const bool multi = ss && ss->n > 0;
+  if ((!multi && (!model || !extf)) || !table || !grid || ncolliders < 0 || ncolliders > kMaxColliders)
     return ZSMPM_INVALID_ARGUMENT;
   std::vector<unsigned char> key;
+  key_put(key, &multi);
+  if (multi) key.insert(key.end(), pol->scene_bytes.begin(), pol->scene_bytes.end());
   key_put(key, model);
   key_put(key, a);
   key_put(key, b);
@@ -614,7 +622,7 @@ int32_t capture_pair(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_
   key_put(key, &dt);
   key_put(key, extf, 3 * sizeof(float));
   key_put(key, &ncolliders);
-  if (ncolliders > 0) key_put(key, colliders, (size_t)ncolliders * sizeof(zsmpm_plane_collider));
+  if (ncolliders > 0 && colliders) key_put(key, colliders, (size_t)ncolliders * sizeof(zsmpm_plane_collider));
   key_put(key, &pol->ws_epoch);
   key_put(key, &pol->closure_on);
   key_put(key, &pol->closure);
@@ -630,8 +638,10 @@ int32_t capture_pair(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_
   pol->profile = false;
   int32_t r = from_hip(hipStreamBeginCapture(pol->cap, hipStreamCaptureModeThreadLocal));
   const bool begun = r == ZSMPM_OK;
-  if (r == ZSMPM_OK) r = run_substep(pol, model, a, b, table, buckets, grid, dt, extf, colliders, ncolliders);
-  if (r == ZSMPM_OK) r = run_substep(pol, model, b, a, table, buckets, grid, dt, extf, colliders, ncolliders);
+  if (r == ZSMPM_OK)
+    r = run_substep(pol, model, a, b, table, buckets, grid, dt, extf, colliders, ncolliders, nullptr, ss);
+  if (r == ZSMPM_OK)
+    r = run_substep(pol, model, b, a, table, buckets, grid, dt, extf, colliders, ncolliders, nullptr, ss);
   hipGraph_t g = nullptr;
   if (begun) {
     const hipError_t e = hipStreamEndCapture(pol->cap, &g);
@@ -651,173 +661,146 @@ int32_t capture_pair(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_
 
 // ---------------------------------------------------------------- batches
 
-constexpr bool kBatchedDisabled = true;
-
-void destroy_policy(zsmpm_policy_s* p);
-
-int32_t ensure_lanes(zsmpm_policy_s* pol, int32_t k) {
-  if (!pol->bfork_ev) ZS_HIP(hipEventCreateWithFlags(&pol->bfork_ev, hipEventDisableTiming));
-  while ((int32_t)pol->lanes.size() < k) {
-    zsmpm_policy_s* c = new (std::nothrow) zsmpm_policy_s();
-    if (!c) return ZSMPM_ERROR;
-    c->devid = pol->devid;
-    hipEvent_t ev = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+// Validates a batch's scenes against the combined particle arrays and
+// uploads the scene table to the policy (only when its bytes changed; the
+// stream is drained first, since queued or captured substeps read it).
+int32_t upload_scenes(zsmpm_policy_s* pol, const zsmpm_batch_scene* sc, int32_t ns, const ParticlesView& pa,
+                      const ParticlesView& pb, SceneSet& out) {
+  if (ns <= 0 || ns > kMaxScenes) return ZSMPM_INVALID_ARGUMENT;
+  std::vector<SceneParams> params((size_t)ns);
+  std::vector<int64_t> first((size_t)ns + 1);
+  int64_t next = 0;
+  for (int32_t i = 0; i < ns; ++i) {
+    ZS_TRY(check_header(&sc[i]));
+    if (sc[i].first != next || sc[i].count < 0) return ZSMPM_INVALID_ARGUMENT;
+    if (!(sc[i].dx > 0.f) || !isfinite(sc[i].dx) || !isfinite(sc[i].dt)) return ZSMPM_INVALID_ARGUMENT;
+    for (int d = 0; d < 3; ++d)
+      if (!isfinite(sc[i].extf[d])) return ZSMPM_INVALID_ARGUMENT;
+    SceneParams& p = params[(size_t)i];
+    memset(&p, 0, sizeof(p));
+    ZS_TRY(make_model(&sc[i].model, p.model));
+    if (p.model.kind == ZSMPM_DRUCKER_PRAGER && sc[i].count > 0 && (!pa.logJp || !pb.logJp))
+      return ZSMPM_INVALID_ARGUMENT;
+    ZS_TRY(make_colliders(sc[i].colliders, sc[i].ncolliders, p.cols));
+    p.dx = sc[i].dx;
+    p.dx_inv = (float)1.0 / sc[i].dx;  // as the single-scene activation forms it
+    p.dt = sc[i].dt;
+    for (int d = 0; d < 3; ++d) p.g[d] = sc[i].extf[d];
+    first[(size_t)i] = next;
+    next += sc[i].count;
+  }
+  if (next != pa.n) return ZSMPM_INVALID_ARGUMENT;
+  first[(size_t)ns] = next;
+  std::vector<unsigned char> bytes(sizeof(int32_t) + params.size() * sizeof(SceneParams) +
+                                   first.size() * sizeof(int64_t));
+  memcpy(bytes.data(), &ns, sizeof(int32_t));
+  memcpy(bytes.data() + sizeof(int32_t), params.data(), params.size() * sizeof(SceneParams));
+  memcpy(bytes.data() + sizeof(int32_t) + params.size() * sizeof(SceneParams), first.data(),
+         first.size() * sizeof(int64_t));
+  if (!pol->scene_p) {
+    void* a = nullptr;
+    void* b = nullptr;
+    ZS_HIP(hipMalloc(&a, sizeof(SceneParams) * kMaxScenes));
+    const hipError_t e = hipMalloc(&b, sizeof(int64_t) * (kMaxScenes + 1));
     if (e != hipSuccess) {
-      if (c->stream) (void)hipStreamDestroy(c->stream);
-      delete c;
+      (void)hipFree(a);
       return from_hip(e);
     }
-    c->owns_stream = true;
-    pol->lanes.push_back(c);
-    pol->lane_ev.push_back(ev);
+    pol->scene_p = static_cast<SceneParams*>(a);
+    pol->scene_first = static_cast<int64_t*>(b);
+    pol->scene_bytes.clear();
   }
+  if (bytes != pol->scene_bytes) {
+    ZS_HIP(hipStreamSynchronize(pol->stream));
+    ZS_HIP(hipMemcpy(pol->scene_p, params.data(), params.size() * sizeof(SceneParams), hipMemcpyHostToDevice));
+    ZS_HIP(hipMemcpy(pol->scene_first, first.data(), first.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    pol->scene_bytes.swap(bytes);
+  }
+  out.n = ns;
+  out.p = pol->scene_p;
+  out.first = pol->scene_first;
   return ZSMPM_OK;
 }
 
-void destroy_batch_graph(zsmpm_policy_s* p) {
-  if (p->bexec) (void)hipGraphExecDestroy(p->bexec);
-  p->bexec = nullptr;
-  p->bkey.clear();
-}
-
-int32_t check_scene(const zsmpm_scene* sc) {
-  ZS_TRY(check_header(sc));
-  if (!sc->model || !sc->a || !sc->table || !sc->grid) return ZSMPM_INVALID_ARGUMENT;
-  if (sc->ncolliders < 0 || sc->ncolliders > kMaxColliders || (sc->ncolliders > 0 && !sc->colliders))
-    return ZSMPM_INVALID_ARGUMENT;
-  return ZSMPM_OK;
-}
-
-// the bytes every scene's substep depends on (descriptors they point to included)
-std::vector<unsigned char> scenes_key(const zsmpm_scene* sc, int32_t ns) {
-  std::vector<unsigned char> key;
-  for (int32_t i = 0; i < ns; ++i) {
-    key_put(key, sc[i].model);
-    key_put(key, sc[i].a);
-    key_put(key, sc[i].b);
-    key_put(key, sc[i].table);
-    key_put(key, sc[i].buckets);
-    key_put(key, sc[i].grid);
-    key_put(key, &sc[i].dt);
-    key_put(key, sc[i].extf, 3 * sizeof(float));
-    key_put(key, &sc[i].ncolliders);
-    if (sc[i].ncolliders > 0)
-      key_put(key, sc[i].colliders, (size_t)sc[i].ncolliders * sizeof(zsmpm_plane_collider));
-  }
-  return key;
-}
-
-// scenes onto k lanes: largest first, each to the lane with the fewest particles
-std::vector<int32_t> assign_lanes(const zsmpm_scene* sc, int32_t ns, int32_t k) {
-  std::vector<int32_t> order(ns), lane(ns, 0);
-  std::vector<int64_t> load(k, 0);
-  for (int32_t i = 0; i < ns; ++i) order[i] = i;
-  for (int32_t i = 1; i < ns; ++i)  // insertion sort by n, descending; ties by index
-    for (int32_t j = i; j > 0 && sc[order[j]].a->n > sc[order[j - 1]].a->n; --j) std::swap(order[j], order[j - 1]);
-  for (int32_t i : order) {
-    int32_t best = 0;
-    for (int32_t l = 1; l < k; ++l)
-      if (load[l] < load[best]) best = l;
-    lane[i] = best;
-    load[best] += sc[i].a->n;
-  }
-  return lane;
-}
-
-// One scene's substep on a lane: a -> b (sorted driver) or in place.
-int32_t scene_substep(zsmpm_policy_s* p, const zsmpm_scene& sc, bool flip) {
-  const zsmpm_particles* in = sc.b && flip ? sc.b : sc.a;
-  const zsmpm_particles* out = sc.b ? (flip ? sc.a : sc.b) : nullptr;
-  return run_substep(p, sc.model, in, out, sc.table, sc.buckets, sc.grid, sc.dt, sc.extf, sc.colliders,
-                     sc.ncolliders);
-}
-
-// Captures two substeps of every scene into pol->bexec, the scenes spread
-// over k lanes that fork from and join the capture stream.
-int32_t capture_batch(zsmpm_policy_s* pol, const zsmpm_scene* sc, int32_t ns, int32_t k) {
-  std::vector<unsigned char> key = scenes_key(sc, ns);
-  key_put(key, &k);
-  for (int32_t l = 0; l < k; ++l) key_put(key, &pol->lanes[l]->ws_epoch);
-  key_put(key, &pol->closure_on);
-  key_put(key, &pol->closure);
-  if (pol->bexec && key == pol->bkey) return ZSMPM_OK;
-  destroy_batch_graph(pol);
-  const std::vector<int32_t> lane = assign_lanes(sc, ns, k);
-  for (int32_t l = 0; l < k; ++l) {
-    zsmpm_policy_s* c = pol->lanes[l];
-    int64_t nmax = 0;
-    int32_t tmax = 0;
-    for (int32_t i = 0; i < ns; ++i)
-      if (lane[i] == l) {
-        nmax = sc[i].a->n > nmax ? sc[i].a->n : nmax;
-        tmax = sc[i].table->table_size > tmax ? sc[i].table->table_size : tmax;
+// nsteps sorted substeps a -> b -> a ... with the multi-substep flags (the
+// shared body of zsmpm_substeps_sorted__rocm and the batched entry point)
+int32_t substeps_driver(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_particles* a,
+                        const zsmpm_particles* b, const zsmpm_table* table, const zsmpm_buckets* buckets,
+                        const zsmpm_grid* grid, float dt, const float extf[3], const zsmpm_plane_collider* colliders,
+                        int32_t ncolliders, int32_t nsteps, int32_t flags, const SceneSet* ss) {
+  const bool fuse = (flags & ZSMPM_SUBSTEPS_FUSE_STRESS) != 0;
+  const bool graph = (flags & ZSMPM_SUBSTEPS_GRAPH) != 0;
+  // Autotuning: the first 8 substeps (real ones, in pairs a -> b -> a) run
+  // with four P2G / G2P workgroup counts, timed per kernel with the stage
+  // events; each kernel keeps its fastest count.
+  int32_t done = 0;
+  if ((flags & ZSMPM_SUBSTEPS_AUTOTUNE) && nsteps >= 8 && !fuse) {
+    DeviceGuard g(pol->devid);
+    const int32_t cand_p2g[4] = {512, 768, 1024, 2048}, cand_g2p[4] = {8192, 16384, 32768, 65536};
+    const bool prof = pol->profile;
+    ZS_HIP(hipStreamSynchronize(pol->stream));
+    harvest_events(pol);
+    pol->profile = true;
+    double best_p = 0.0, best_g = 0.0;
+    int32_t pick_p = 0, pick_g = 0;
+    int32_t r = ZSMPM_OK;
+    for (int c = 0; c < 4 && r == ZSMPM_OK; ++c) {
+      pol->tuned_p2g = cand_p2g[c];
+      pol->tuned_g2p = cand_g2p[c];
+      const double p0 = pol->stage_ms[ZSMPM_STAGE_P2G], g0 = pol->stage_ms[ZSMPM_STAGE_G2P];
+      for (int k = 0; k < 2 && r == ZSMPM_OK; ++k, ++done)
+        r = run_substep(pol, model, (done & 1) ? b : a, (done & 1) ? a : b, table, buckets, grid, dt, extf,
+                        colliders, ncolliders, nullptr, ss);
+      if (r == ZSMPM_OK) r = from_hip(hipStreamSynchronize(pol->stream));
+      harvest_events(pol);
+      const double dp = pol->stage_ms[ZSMPM_STAGE_P2G] - p0, dg = pol->stage_ms[ZSMPM_STAGE_G2P] - g0;
+      if (c == 0 || dp < best_p) {
+        best_p = dp;
+        pick_p = cand_p2g[c];
       }
-    if (nmax > 0) ZS_TRY(ensure(c, nmax, tmax));
-    ZS_HIP(ensure_side(c));
-    c->closure_on = pol->closure_on;
-    c->closure = pol->closure;
-    c->profile = false;
+      if (c == 0 || dg < best_g) {
+        best_g = dg;
+        pick_g = cand_g2p[c];
+      }
+    }
+    pol->profile = prof;
+    if (r != ZSMPM_OK) return r;
+    pol->tuned_p2g = pick_p;
+    pol->tuned_g2p = pick_g;
   }
-  if (!pol->cap) ZS_HIP(hipStreamCreateWithFlags(&pol->cap, hipStreamNonBlocking));
-  int32_t r = from_hip(hipStreamBeginCapture(pol->cap, hipStreamCaptureModeThreadLocal));
-  const bool begun = r == ZSMPM_OK;
-  if (r == ZSMPM_OK) r = from_hip(hipEventRecord(pol->bfork_ev, pol->cap));
-  for (int32_t l = 0; r == ZSMPM_OK && l < k; ++l) r = from_hip(hipStreamWaitEvent(pol->lanes[l]->stream, pol->bfork_ev, 0));
-  for (int32_t i = 0; r == ZSMPM_OK && i < ns; ++i) {
-    r = scene_substep(pol->lanes[lane[i]], sc[i], false);
-    if (r == ZSMPM_OK) r = scene_substep(pol->lanes[lane[i]], sc[i], true);
+  nsteps -= done;  // an even count: the particles are back in a
+  if (graph) {
+    DeviceGuard g(pol->devid);
+    if (nsteps >= 2) {
+      ZS_TRY(capture_pair(pol, model, a, b, table, buckets, grid, dt, extf, colliders, ncolliders, ss));
+      for (int32_t k = 0; k + 1 < nsteps; k += 2) ZS_HIP(hipGraphLaunch(pol->gexec, pol->stream));
+    }
+    if (nsteps & 1)
+      ZS_TRY(run_substep(pol, model, a, b, table, buckets, grid, dt, extf, colliders, ncolliders, nullptr, ss));
+    return ZSMPM_OK;
   }
-  // every lane joins the capture stream, also after an error
-  for (int32_t l = 0; begun && l < k; ++l) {
-    const hipError_t e1 = hipEventRecord(pol->lane_ev[l], pol->lanes[l]->stream);
-    const hipError_t e2 = hipStreamWaitEvent(pol->cap, pol->lane_ev[l], 0);
-    if (r == ZSMPM_OK) r = from_hip(e1 != hipSuccess ? e1 : e2);
+  for (int32_t k = 0; k < nsteps; ++k) {
+    Chain c;
+    if (fuse) {
+      c.cached_stress = k > 0;
+      c.lj_src = k > 0 ? pol->ws.lj_next[(k - 1) & 1] : nullptr;
+      c.compute_next = k + 1 < nsteps;
+      c.lj_next = pol->ws.lj_next[k & 1];
+    }
+    const zsmpm_particles* in = (k & 1) ? b : a;
+    const zsmpm_particles* out = (k & 1) ? a : b;
+    ZS_TRY(run_substep(pol, model, in, out, table, buckets, grid, dt, extf, colliders, ncolliders, &c, ss));
   }
-  hipGraph_t g = nullptr;
-  if (begun) {
-    const hipError_t e = hipStreamEndCapture(pol->cap, &g);
-    if (r == ZSMPM_OK) r = from_hip(e);
-  }
-  if (r == ZSMPM_OK && g) r = from_hip(hipGraphInstantiate(&pol->bexec, g, nullptr, nullptr, 0));
-  if (g) (void)hipGraphDestroy(g);
-  if (r != ZSMPM_OK) {
-    pol->bexec = nullptr;
-    return r;
-  }
-  pol->bkey.swap(key);
   return ZSMPM_OK;
-}
-
-// milliseconds of `reps` replays of the batch graph on the policy's stream
-int32_t time_batch(zsmpm_policy_s* pol, int32_t reps, float& ms) {
-  hipEvent_t a = take_event(pol), b = take_event(pol);
-  if (!a || !b) return ZSMPM_DEVICE_ERROR;
-  int32_t r = from_hip(hipEventRecord(a, pol->stream));
-  for (int32_t i = 0; r == ZSMPM_OK && i < reps; ++i) r = from_hip(hipGraphLaunch(pol->bexec, pol->stream));
-  if (r == ZSMPM_OK) r = from_hip(hipEventRecord(b, pol->stream));
-  if (r == ZSMPM_OK) r = from_hip(hipEventSynchronize(b));
-  if (r == ZSMPM_OK) r = from_hip(hipEventElapsedTime(&ms, a, b));
-  pol->ev_free.push_back(a);
-  pol->ev_free.push_back(b);
-  return r;
 }
 
 void destroy_policy(zsmpm_policy_s* p) {
   (void)hipStreamSynchronize(p->stream);
-  for (size_t l = 0; l < p->lanes.size(); ++l) {
-    destroy_policy(p->lanes[l]);
-    (void)hipEventDestroy(p->lane_ev[l]);
-  }
-  p->lanes.clear();
-  p->lane_ev.clear();
-  destroy_batch_graph(p);
-  if (p->bfork_ev) (void)hipEventDestroy(p->bfork_ev);
   destroy_events(p);
   free_all(p);
+  if (p->scene_p) (void)hipFree(p->scene_p);
+  if (p->scene_first) (void)hipFree(p->scene_first);
   if (p->cap) (void)hipStreamDestroy(p->cap);
-  if (p->owns_stream && p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
 }
 
@@ -869,15 +852,9 @@ int32_t zsmpm_policy_sync(zsmpm_policy pol) {
   ZS_HIP(hipStreamSynchronize(pol->stream));
   harvest_events(pol);
   uint32_t bits = 0;
-  // the policy's own error bits and those of its batch lanes
-  for (size_t l = 0; l <= pol->lanes.size(); ++l) {
-    zsmpm_policy_s* p = l == 0 ? pol : pol->lanes[l - 1];
-    if (l > 0) ZS_HIP(hipStreamSynchronize(p->stream));
-    if (!p->ws.err) continue;
-    uint32_t b = 0;
-    ZS_HIP(hipMemcpy(&b, p->ws.err, sizeof(uint32_t), hipMemcpyDeviceToHost));
-    if (b) ZS_HIP(hipMemset(p->ws.err, 0, sizeof(uint32_t)));
-    bits |= b;
+  if (pol->ws.err) {
+    ZS_HIP(hipMemcpy(&bits, pol->ws.err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (bits) ZS_HIP(hipMemset(pol->ws.err, 0, sizeof(uint32_t)));
   }
   return from_err_bits(bits);
 }
@@ -889,15 +866,6 @@ int32_t zsmpm_policy_counters(zsmpm_policy pol, uint32_t* out) {
   DeviceGuard g(pol->devid);
   ZS_HIP(hipStreamSynchronize(pol->stream));
   ZS_HIP(hipMemcpy(out, pol->ws.counters, sizeof(uint32_t) * kCtrCount, hipMemcpyDeviceToHost));
-  // the per-particle path counters of batch lanes add to the policy's
-  for (zsmpm_policy_s* c : pol->lanes) {
-    if (!c->ws.counters) continue;
-    uint32_t lc[kCtrCount];
-    ZS_HIP(hipStreamSynchronize(c->stream));
-    ZS_HIP(hipMemcpy(lc, c->ws.counters, sizeof(uint32_t) * kCtrCount, hipMemcpyDeviceToHost));
-    out[kCtrFp64] += lc[kCtrFp64];
-    out[kCtrFallback] += lc[kCtrFallback];
-  }
   return ZSMPM_OK;
 }
 
@@ -1126,67 +1094,8 @@ int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, 
     DeviceGuard g(pol->devid);
     ZS_TRY(ensure(pol, pa.n, table ? table->table_size : 0));
   }
-  // Autotuning: the first 8 substeps (real ones, in pairs a -> b -> a) run
-  // with four P2G / G2P workgroup counts, timed per kernel with the stage
-  // events; each kernel keeps its fastest count.
-  int32_t done = 0;
-  if ((flags & ZSMPM_SUBSTEPS_AUTOTUNE) && nsteps >= 8 && !fuse) {
-    DeviceGuard g(pol->devid);
-    const int32_t cand_p2g[4] = {512, 768, 1024, 2048}, cand_g2p[4] = {8192, 16384, 32768, 65536};
-    const bool prof = pol->profile;
-    ZS_HIP(hipStreamSynchronize(pol->stream));
-    harvest_events(pol);
-    pol->profile = true;
-    double best_p = 0.0, best_g = 0.0;
-    int32_t pick_p = 0, pick_g = 0;
-    int32_t r = ZSMPM_OK;
-    for (int c = 0; c < 4 && r == ZSMPM_OK; ++c) {
-      pol->tuned_p2g = cand_p2g[c];
-      pol->tuned_g2p = cand_g2p[c];
-      const double p0 = pol->stage_ms[ZSMPM_STAGE_P2G], g0 = pol->stage_ms[ZSMPM_STAGE_G2P];
-      for (int k = 0; k < 2 && r == ZSMPM_OK; ++k, ++done)
-        r = run_substep(pol, model, (done & 1) ? b : a, (done & 1) ? a : b, table, buckets, grid, dt, extf,
-                        colliders, ncolliders);
-      if (r == ZSMPM_OK) r = from_hip(hipStreamSynchronize(pol->stream));
-      harvest_events(pol);
-      const double dp = pol->stage_ms[ZSMPM_STAGE_P2G] - p0, dg = pol->stage_ms[ZSMPM_STAGE_G2P] - g0;
-      if (c == 0 || dp < best_p) {
-        best_p = dp;
-        pick_p = cand_p2g[c];
-      }
-      if (c == 0 || dg < best_g) {
-        best_g = dg;
-        pick_g = cand_g2p[c];
-      }
-    }
-    pol->profile = prof;
-    if (r != ZSMPM_OK) return r;
-    pol->tuned_p2g = pick_p;
-    pol->tuned_g2p = pick_g;
-  }
-  nsteps -= done;  // an even count: the particles are back in a
-  if (graph) {
-    DeviceGuard g(pol->devid);
-    if (nsteps >= 2) {
-      ZS_TRY(capture_pair(pol, model, a, b, table, buckets, grid, dt, extf, colliders, ncolliders));
-      for (int32_t k = 0; k + 1 < nsteps; k += 2) ZS_HIP(hipGraphLaunch(pol->gexec, pol->stream));
-    }
-    if (nsteps & 1) ZS_TRY(run_substep(pol, model, a, b, table, buckets, grid, dt, extf, colliders, ncolliders));
-    return ZSMPM_OK;
-  }
-  for (int32_t k = 0; k < nsteps; ++k) {
-    Chain c;
-    if (fuse) {
-      c.cached_stress = k > 0;
-      c.lj_src = k > 0 ? pol->ws.lj_next[(k - 1) & 1] : nullptr;
-      c.compute_next = k + 1 < nsteps;
-      c.lj_next = pol->ws.lj_next[k & 1];
-    }
-    const zsmpm_particles* in = (k & 1) ? b : a;
-    const zsmpm_particles* out = (k & 1) ? a : b;
-    ZS_TRY(run_substep(pol, model, in, out, table, buckets, grid, dt, extf, colliders, ncolliders, &c));
-  }
-  return ZSMPM_OK;
+  return substeps_driver(pol, model, a, b, table, buckets, grid, dt, extf, colliders, ncolliders, nsteps, flags,
+                         nullptr);
 }
 
 int32_t zsmpm_reorder__rocm(zsmpm_policy pol, const zsmpm_buckets* buckets, const zsmpm_particles* in,
@@ -1203,72 +1112,25 @@ int32_t zsmpm_reorder__rocm(zsmpm_policy pol, const zsmpm_buckets* buckets, cons
   return from_hip(reorder_particles(pi, po, sv.indices, pol->ws, pol->stream));
 }
 
-int32_t zsmpm_substeps_batched__rocm(zsmpm_policy pol, const zsmpm_scene* scenes, int32_t nscenes, int32_t nsteps,
-                                     int32_t lanes, int32_t* lanes_used, float* tune_ms, int32_t* final_in_b) {
-  if (!pol || !scenes || nscenes <= 0 || nsteps < 0 || lanes < 0 || lanes > ZSMPM_MAX_LANES)
+int32_t zsmpm_substeps_batched__rocm(zsmpm_policy pol, const zsmpm_batch_scene* scenes, int32_t nscenes,
+                                     const zsmpm_particles* a, const zsmpm_particles* b, const zsmpm_table* table,
+                                     const zsmpm_buckets* buckets, const zsmpm_grid* grid, int32_t nsteps,
+                                     int32_t flags, int32_t* final_in_b) {
+  if (!pol || !scenes || !a || !b || !table || !grid || nsteps < 0 ||
+      (flags & ~(ZSMPM_SUBSTEPS_GRAPH | ZSMPM_SUBSTEPS_AUTOTUNE)))
     return ZSMPM_INVALID_ARGUMENT;
-  for (int32_t i = 0; i < nscenes; ++i) ZS_TRY(check_scene(&scenes[i]));
-  // Switched off: the first run of this path ended in a host segmentation
-  // fault inside this call whose cause has not been shown (the capture forks
-  // lane streams that fork their own side streams, two levels of cross-stream
-  // joins, which no other capture here uses). Nothing below runs until then.
-  if (kBatchedDisabled) return ZSMPM_UNSUPPORTED_OPERATION;
-  if (tune_ms)
-    for (int i = 0; i < 4; ++i) tune_ms[i] = -1.f;
-  bool sorted = scenes[0].b != nullptr;
-  for (int32_t i = 0; i < nscenes; ++i)
-    if ((scenes[i].b != nullptr) != sorted) return ZSMPM_INVALID_ARGUMENT;
-  if (final_in_b) *final_in_b = sorted ? (nsteps & 1) : 0;
+  if (pol->closure_on) return ZSMPM_UNSUPPORTED_OPERATION;  // the learned closure serves single scenes
+  ParticlesView pa, pb;
+  ZS_TRY(view_particles(a, pa, true));
+  ZS_TRY(view_particles(b, pb, true));
+  ZS_TRY(check_out_particles(pa, pb));
+  if (final_in_b) *final_in_b = nsteps & 1;
   DeviceGuard g(pol->devid);
-  int64_t nmax = 0;
-  int32_t tmax = 0;
-  for (int32_t i = 0; i < nscenes; ++i) {
-    nmax = scenes[i].a->n > nmax ? scenes[i].a->n : nmax;
-    tmax = scenes[i].table->table_size > tmax ? scenes[i].table->table_size : tmax;
-  }
-  ZS_TRY(ensure(pol, nmax, tmax));  // the odd remainder runs on the policy itself
-  int32_t pairs = nsteps / 2;
-  const int32_t cand[4] = {1, 2, 4, 8};
-  int32_t k = lanes > 0 ? (lanes < nscenes ? lanes : nscenes) : 0;
-  if (k == 0 && pairs > 0) {
-    const std::vector<unsigned char> sk = scenes_key(scenes, nscenes);
-    if (pol->tuned_lanes > 0 && sk == pol->tune_key) {
-      k = pol->tuned_lanes;
-    } else if (pairs >= 12) {
-      // autotune on the caller's own substeps: every timed replay is a real
-      // pair of substeps of every scene
-      float best = 0.f;
-      for (int c = 0; c < 4; ++c) {
-        const int32_t kk = cand[c];
-        if (kk > nscenes) break;
-        ZS_TRY(ensure_lanes(pol, kk));
-        ZS_TRY(capture_batch(pol, scenes, nscenes, kk));
-        ZS_HIP(hipGraphLaunch(pol->bexec, pol->stream));
-        float ms = 0.f;
-        ZS_TRY(time_batch(pol, 2, ms));
-        pairs -= 3;
-        if (tune_ms) tune_ms[c] = 0.5f * ms;
-        if (k == 0 || ms < best) {
-          best = ms;
-          k = kk;
-        }
-      }
-      pol->tuned_lanes = k;
-      pol->tune_key = sk;
-    } else {
-      k = nscenes < 4 ? nscenes : 4;
-    }
-  }
-  if (k == 0) k = nscenes < 4 ? nscenes : 4;
-  if (lanes_used) *lanes_used = k;
-  if (pairs > 0) {
-    ZS_TRY(ensure_lanes(pol, k));
-    ZS_TRY(capture_batch(pol, scenes, nscenes, k));
-    for (int32_t i = 0; i < pairs; ++i) ZS_HIP(hipGraphLaunch(pol->bexec, pol->stream));
-  }
-  if (nsteps & 1)
-    for (int32_t i = 0; i < nscenes; ++i) ZS_TRY(scene_substep(pol, scenes[i], false));
-  return ZSMPM_OK;
+  SceneSet ss;
+  ZS_TRY(upload_scenes(pol, scenes, nscenes, pa, pb, ss));
+  if (nsteps == 0) return ZSMPM_OK;
+  ZS_TRY(ensure(pol, pa.n, table->table_size));
+  return substeps_driver(pol, nullptr, a, b, table, buckets, grid, 0.f, nullptr, nullptr, 0, nsteps, flags, &ss);
 }
 
 int32_t zsmpm_policy_launch_widths__rocm(zsmpm_policy pol, int32_t* p2g_groups, int32_t* g2p_groups) {

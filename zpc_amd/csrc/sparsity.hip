@@ -132,13 +132,22 @@ constexpr int kActItems = 4;  // 64-particle groups per wave, their loads and fi
 // bytes. The dedup hash keeps the minimum order per key, so the result does
 // not depend on the order of the inserts. With pslot, every particle's
 // scratch slot (the identity of its key) is recorded for the bucket sort.
+// ss.n > 0 (a batch): particle p takes its scene's 1/dx, and its key is
+// shifted by the scene's offset (a key outside the scene's own range
+// [-512, 511) along axis 0 is a key-range error)
 __global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restrict__ x, int64_t n,
                                                             float dxinv, int blockLen, int offset,
-                                                            float disp, Dedup dd, uint32_t* pslot) {
+                                                            float disp, Dedup dd, uint32_t* pslot,
+                                                            SceneSet ss) {
   uint32_t* err = dd.err;
   const int64_t span = (int64_t)kThreads * kActItems;
   const int64_t stride = (int64_t)gridDim.x * span;
   const int lane = lane_id();
+  __shared__ int64_t sfirst[kMaxScenes + 1];
+  if (ss.n > 0) {
+    for (int i = threadIdx.x; i <= ss.n; i += kThreads) sfirst[i] = ss.first[i];
+    __syncthreads();
+  }
   for (int64_t base = (int64_t)blockIdx.x * span; base < n; base += stride) {
     float cx[kActItems], cy[kActItems], cz[kActItems];
 #pragma unroll
@@ -160,8 +169,17 @@ __global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restr
       pk[r] = kEmpty64;
       if (p < n) {
         int32_t k0, k1, k2;
-        block_key(cx[r], cy[r], cz[r], dxinv, blockLen, offset, disp, k0, k1, k2);
-        if (key_packable(k0, k1, k2)) {
+        bool in_range = true;
+        if (ss.n > 0) {
+          const int32_t sc = scene_of_particle(sfirst, ss.n, p);
+          block_key(cx[r], cy[r], cz[r], ss.p[sc].dx_inv, blockLen, offset, disp, k0, k1, k2);
+          // below 511, so the enlargement's key + 1 stays in the scene's range
+          in_range = k0 >= -kSceneKeyHalf && k0 < kSceneKeyHalf - 1;
+          k0 += sc * kSceneKeyStride;
+        } else {
+          block_key(cx[r], cy[r], cz[r], dxinv, blockLen, offset, disp, k0, k1, k2);
+        }
+        if (in_range && key_packable(k0, k1, k2)) {
           pk[r] = pack_key(k0, k1, k2);
         } else {
           atomicOr(err, kErrKeyRange);
@@ -740,7 +758,8 @@ hipError_t table_clear(const TableView& t, Workspace& ws, hipStream_t s, uint32_
 }
 
 hipError_t table_activate(const TableView& t, Workspace& ws, const float* x, int64_t n, float dx,
-                          int blockLen, int offset, float disp, hipStream_t s, bool record_slots) {
+                          int blockLen, int offset, float disp, hipStream_t s, bool record_slots,
+                          const SceneSet* ss) {
   if (blockLen <= 0) return hipErrorInvalidValue;
   int32_t* slot_block = record_slots ? ws.slot_block : nullptr;
   const uint64_t order_bound = (uint64_t)(n > 0 ? n : 1);
@@ -751,7 +770,8 @@ hipError_t table_activate(const TableView& t, Workspace& ws, const float* x, int
   if (n > 0)
     activate_kernel<<<groups_for(n, kThreads * kActItems), kThreads, 0, s>>>(x, n, dxinv, blockLen, offset, disp,
                                                                               dedup_of(ws),
-                                                                              record_slots ? ws.pslot : nullptr);
+                                                                              record_slots ? ws.pslot : nullptr,
+                                                                              ss ? *ss : SceneSet{});
   return finish_insertion(t, ws, order_bound, s, slot_block);
 }
 

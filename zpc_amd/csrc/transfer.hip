@@ -110,11 +110,13 @@ __device__ __forceinline__ void load_neighbours(const int32_t* __restrict__ nbr_
   }
 }
 
-// Count, offset and block key of bucket b (0 for b >= nb or a bucket whose
-// range is not within the particles), written to m[0..5] by one thread.
+// Count, offset, block key and scene of bucket b (0 for b >= nb or a bucket
+// whose range is not within the particles), written to m[0..6] by one
+// thread. In a batch of nscenes > 0 scenes the key is the scene's own
+// (offset removed).
 __device__ __forceinline__ void bucket_meta(int32_t b, int32_t nb, const SortView& sv, const TableView& t,
-                                            int64_t n, int32_t* m) {
-  int32_t cnt = 0, off = 0, k0 = 0, k1 = 0, k2 = 0;
+                                            int64_t n, int32_t* m, int32_t nscenes = 0) {
+  int32_t cnt = 0, off = 0, k0 = 0, k1 = 0, k2 = 0, sc = 0;
   if (b < nb) {
     cnt = sv.counts[b];
     off = sv.offsets[b];
@@ -122,6 +124,10 @@ __device__ __forceinline__ void bucket_meta(int32_t b, int32_t nb, const SortVie
     k0 = t.active[3 * (int64_t)b];
     k1 = t.active[3 * (int64_t)b + 1];
     k2 = t.active[3 * (int64_t)b + 2];
+    if (nscenes > 0) {
+      sc = scene_of_key(k0, nscenes);
+      k0 -= sc * kSceneKeyStride;
+    }
   }
   m[0] = b;
   m[1] = cnt;
@@ -129,6 +135,7 @@ __device__ __forceinline__ void bucket_meta(int32_t b, int32_t nb, const SortVie
   m[3] = k0;
   m[4] = k1;
   m[5] = k2;
+  m[6] = sc;
 }
 
 // Main base cell (i, j, k) in [0,4)^3 <-> lane of the scatter rounds. Lane
@@ -170,28 +177,44 @@ struct Rec {
 // stored F (fixed-corotated or the Drucker-Prager return mapping, which
 // writes logJp back). Particles in array order, 64 consecutive per wave, so
 // rows always move as 16-byte vectors.
+// kKind: 0 fixed-corotated, 1 Drucker-Prager, 2 each particle its scene's
+// model (a batch, ss)
 template <int kKind>
 __global__ __launch_bounds__(kThreads) void stress_kernel(Model m, ParticlesView ps, float* __restrict__ contrib,
-                                                          uint32_t* counters) {
+                                                          uint32_t* counters, SceneSet ss) {
   __shared__ float4 stage4[kWaves][stage_floats(9) / 4];
+  __shared__ int64_t sfirst[kKind == 2 ? kMaxScenes + 1 : 1];
   const int wave = uniform((int32_t)(threadIdx.x / kWave)), lane = lane_id();
   float* stage = reinterpret_cast<float*>(stage4[wave]);
   const int64_t nwaves = (int64_t)gridDim.x * kWaves;
   uint32_t fp64_used = 0;
+  if constexpr (kKind == 2) {
+    for (int i = threadIdx.x; i <= ss.n; i += kThreads) sfirst[i] = ss.first[i];
+    __syncthreads();
+  }
   for (int64_t w = (int64_t)blockIdx.x * kWaves + wave; w * kWave < ps.n; w += nwaves) {
     const int64_t p0 = w * kWave;
     const int32_t cw = (int32_t)min((int64_t)kWave, ps.n - p0);
     const int64_t p = p0 + lane;
     float F[9], PF[9];
-    float lj = (kKind != 0 && lane < cw) ? ps.logJp[p] : 0.f;
+    float lj = (kKind == 1 && lane < cw) ? ps.logJp[p] : 0.f;
     wave_load_rows<9>(ps.F, p0, cw, stage, F);
     if (lane < cw) {
       bool f64 = false;
       if (kKind == 0) {
         stress_fixedcorotated(m, F, PF, f64);
-      } else {
+      } else if (kKind == 1) {
         stress_sand(m, F, lj, PF, nullptr, f64);
         ps.logJp[p] = lj;
+      } else {
+        const Model ms = ss.p[scene_of_particle(sfirst, ss.n, p)].model;
+        if (ms.kind == 0) {
+          stress_fixedcorotated(ms, F, PF, f64);
+        } else {
+          lj = ps.logJp[p];
+          stress_sand(ms, F, lj, PF, nullptr, f64);
+          ps.logJp[p] = lj;
+        }
       }
       fp64_used += f64 ? 1u : 0u;
     }
@@ -204,7 +227,7 @@ template <bool kFold>
 __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, const float* __restrict__ contrib,
                                                        TableView t, SortView sv, const int32_t* __restrict__ nbr_tab,
                                                        float* __restrict__ grid, int32_t grid_cap, float dt,
-                                                       float dx, uint32_t* err, uint32_t* queue) {
+                                                       float dx, uint32_t* err, uint32_t* queue, SceneSet ss) {
   constexpr int kQ = Rec<kFold>::kQuads;
   constexpr int kChunk = Rec<kFold>::kChunk;
   static_assert(kP2GThreads % kWave == 0 && kChunk % kP2GThreads == 0,
@@ -233,8 +256,10 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
   // bucket's first chunk before the flush, so they are in flight across the
   // flush and the next bucket's set-up instead of starting phase A.
   constexpr int kPre = kChunk / kP2GThreads;
-  __shared__ int32_t bmeta[6];  // bucket, count, offset, key
-  if (threadIdx.x == 0) bucket_meta((int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu), nb, sv, t, ps.n, bmeta);
+  __shared__ int32_t bmeta[7];  // bucket, count, offset, key, scene
+  const bool multi = ss.n > 0;
+  if (threadIdx.x == 0)
+    bucket_meta((int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu), nb, sv, t, ps.n, bmeta, ss.n);
   for (int c = threadIdx.x; c <= kMainCells; c += kP2GThreads) {
     ccount[c] = 0;
     cfill[c] = 0;
@@ -257,10 +282,19 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
     const int32_t o0 = bmeta[3] * 4, o1 = bmeta[4] * 4, o2 = bmeta[5] * 4;
     if (cnt == 0) {  // empty or out-of-range bucket: take the next one now
       __syncthreads();
-      if (threadIdx.x == 0) bucket_meta((int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu), nb, sv, t, ps.n, bmeta);
+      if (threadIdx.x == 0)
+        bucket_meta((int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu), nb, sv, t, ps.n, bmeta, ss.n);
       __syncthreads();
       prefetch();
       continue;
+    }
+    // the bucket's grid spacing and force scale (its scene's, in a batch)
+    float bdx = dx, bscale = scale;
+    if (multi) {
+      const SceneParams& sp = ss.p[bmeta[6]];
+      bdx = sp.dx;
+      const float bdx_inv = (float)1.0 / bdx;
+      bscale = -sp.dt * (4.f * bdx_inv * bdx_inv);
     }
     load_neighbours(nbr_tab, b, nbr, err);
     uint32_t nxt = 0;  // thread 0: the next bucket, popped during the last chunk
@@ -338,7 +372,7 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
           const float xs[3] = {x0, x1, x2};
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            const float X = xs[d] / dx;
+            const float X = xs[d] / bdx;
             base[d] = (int32_t)floorf(X - 0.5f);
             l[d] = X - (float)base[d];
           }
@@ -359,12 +393,12 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
         float Af[9], Bf[3];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-          const float cf = PF[k] * scale;
+          const float cf = PF[k] * bscale;
           if (kFold) {
-            A[k] = (mass * C[k] + cf) * dx;
+            A[k] = (mass * C[k] + cf) * bdx;
           } else {
-            A[k] = mass * C[k] * dx;
-            Af[k] = cf * dx;
+            A[k] = mass * C[k] * bdx;
+            Af[k] = cf * bdx;
           }
         }
 #pragma unroll
@@ -619,7 +653,7 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
               }
             }
       }
-      if (last && threadIdx.x == 0) bucket_meta((int32_t)min(nxt, 0x7fffffffu), nb, sv, t, ps.n, bmeta);
+      if (last && threadIdx.x == 0) bucket_meta((int32_t)min(nxt, 0x7fffffffu), nb, sv, t, ps.n, bmeta, ss.n);
       __syncthreads();
     }
     prefetch();  // the next bucket's first-chunk indices (bmeta is the next bucket's now)
@@ -681,10 +715,56 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
 // then the colliders in order on nodes with m > 0 (GridOp.hpp:112-164,
 // Collider.h:80-107). fold adds the rhs channels first (the explicit
 // driver's convention when P2G kept them separate).
+// Node velocity and colliders of one node (mass != 0) of the block with key
+// k; cols is the kernel argument or a scene's colliders in a batch.
+template <typename Cols>
+__device__ __forceinline__ void node_velocity(float* gp, float mass, int fold, float e0, float e1, float e2,
+                                              const Cols& cols, int32_t k0, int32_t k1, int32_t k2, int cell,
+                                              float dx, float& vmax) {
+  // rounded as written (no FMA contraction): a node on a collider plane has
+  // phi = 0 in exact arithmetic, and the sign of the rounded phi must be the
+  // one the host path computes
+#pragma clang fp contract(off)
+  const float inv = 1.f / mass;
+  float mv0 = gp[1 * kBlockCells], mv1 = gp[2 * kBlockCells], mv2 = gp[3 * kBlockCells];
+  if (fold) {
+    mv0 = mv0 + gp[4 * kBlockCells];
+    mv1 = mv1 + gp[5 * kBlockCells];
+    mv2 = mv2 + gp[6 * kBlockCells];
+  }
+  float v[3] = {mv0 * inv + e0, mv1 * inv + e1, mv2 * inv + e2};
+  vmax = fmaxf(vmax, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (mass > 0.f && cols.n > 0) {
+    const float pos[3] = {((float)k0 * 4.f + (float)((cell >> 4) & 3)) * dx,
+                          ((float)k1 * 4.f + (float)((cell >> 2) & 3)) * dx,
+                          ((float)k2 * 4.f + (float)(cell & 3)) * dx};
+    for (int c = 0; c < cols.n; ++c) {
+      const PlaneCollider& pc = cols.c[c];
+      const float phi = pc.normal[0] * (pos[0] - pc.origin[0]) + pc.normal[1] * (pos[1] - pc.origin[1]) +
+                        pc.normal[2] * (pos[2] - pc.origin[2]);
+      if (phi < 0.f) {
+        if (pc.type == 0) {
+          v[0] = v[1] = v[2] = 0.f;
+        } else {
+          const float proj = pc.normal[0] * v[0] + pc.normal[1] * v[1] + pc.normal[2] * v[2];
+          if ((pc.type == 2 && proj < 0.f) || pc.type == 1) {
+            v[0] -= proj * pc.normal[0];
+            v[1] -= proj * pc.normal[1];
+            v[2] -= proj * pc.normal[2];
+          }
+        }
+      }
+    }
+  }
+  gp[1 * kBlockCells] = v[0];
+  gp[2 * kBlockCells] = v[1];
+  gp[3 * kBlockCells] = v[2];
+}
+
 __global__ __launch_bounds__(kThreads) void grid_update_kernel(float* __restrict__ grid, int32_t grid_cap,
                                                                TableView t, float dt, float g0, float g1,
                                                                float g2, Colliders cols, float dx, int fold,
-                                                               uint32_t* counters, uint32_t* err) {
+                                                               uint32_t* counters, uint32_t* err, SceneSet ss) {
   int32_t nb = *t.cnt;
   if (nb > grid_cap) {
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, kErrCapacity);
@@ -702,40 +782,23 @@ __global__ __launch_bounds__(kThreads) void grid_update_kernel(float* __restrict
       float* gp = grid + grid_at(b, 0, cell);
       const float mass = gp[0];
       if (mass != 0.f) {
-        const float inv = 1.f / mass;
-        float mv0 = gp[1 * kBlockCells], mv1 = gp[2 * kBlockCells], mv2 = gp[3 * kBlockCells];
-        if (fold) {
-          mv0 = mv0 + gp[4 * kBlockCells];
-          mv1 = mv1 + gp[5 * kBlockCells];
-          mv2 = mv2 + gp[6 * kBlockCells];
+        int32_t k0 = t.active[3 * (int64_t)b];
+        const int32_t k1 = t.active[3 * (int64_t)b + 1], k2 = t.active[3 * (int64_t)b + 2];
+        if (ss.n > 0) {
+          // the node's scene (from its block key): its dt, gravity, dx and colliders
+          const int32_t sc = scene_of_key(k0, ss.n);
+          const SceneParams& sp = ss.p[sc];
+          k0 -= sc * kSceneKeyStride;
+          node_velocity(gp, mass, fold, sp.g[0] * sp.dt, sp.g[1] * sp.dt, sp.g[2] * sp.dt, sp.cols, k0, k1, k2,
+                        cell, sp.dx, vmax);
+        } else {
+          node_velocity(gp, mass, fold, e0, e1, e2, cols, k0, k1, k2, cell, dx, vmax);
         }
-        float v[3] = {mv0 * inv + e0, mv1 * inv + e1, mv2 * inv + e2};
-        vmax = fmaxf(vmax, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        if (mass > 0.f && cols.n > 0) {
-          const float pos[3] = {((float)t.active[3 * (int64_t)b] * 4.f + (float)((cell >> 4) & 3)) * dx,
-                                ((float)t.active[3 * (int64_t)b + 1] * 4.f + (float)((cell >> 2) & 3)) * dx,
-                                ((float)t.active[3 * (int64_t)b + 2] * 4.f + (float)(cell & 3)) * dx};
-          for (int c = 0; c < cols.n; ++c) {
-            const PlaneCollider& pc = cols.c[c];
-            const float phi = pc.normal[0] * (pos[0] - pc.origin[0]) + pc.normal[1] * (pos[1] - pc.origin[1]) +
-                              pc.normal[2] * (pos[2] - pc.origin[2]);
-            if (phi < 0.f) {
-              if (pc.type == 0) {
-                v[0] = v[1] = v[2] = 0.f;
-              } else {
-                const float proj = pc.normal[0] * v[0] + pc.normal[1] * v[1] + pc.normal[2] * v[2];
-                if ((pc.type == 2 && proj < 0.f) || pc.type == 1) {
-                  v[0] -= proj * pc.normal[0];
-                  v[1] -= proj * pc.normal[1];
-                  v[2] -= proj * pc.normal[2];
-                }
-              }
-            }
-          }
-        }
-        gp[1 * kBlockCells] = v[0];
-        gp[2 * kBlockCells] = v[1];
-        gp[3 * kBlockCells] = v[2];
+      } else if (fold) {
+        // the fold covers every node, massless ones included (their
+        // momentum channels then hold mv + rhs, as after a separate fold)
+#pragma unroll
+        for (int d = 1; d <= 3; ++d) gp[d * kBlockCells] = gp[d * kBlockCells] + gp[(d + 3) * kBlockCells];
       }
     }
   }
@@ -811,16 +874,14 @@ template <bool kReorder, int kNext>
 __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, ParticlesView out, TableView t,
                                                           SortView sv, const int32_t* __restrict__ nbr_tab,
                                                           const float* __restrict__ grid, int32_t grid_cap,
-                                                          float dt, float dx, NextStress ns, uint32_t* err,
-                                                          uint32_t* counters) {
+                                                          float dt_arg, float dx_arg, NextStress ns, uint32_t* err,
+                                                          uint32_t* counters, SceneSet ss) {
   static_assert(kNext < 0 || kReorder, "the fused stress writes in output slot order");
   uint32_t fp64_used = 0;
   __shared__ float4 vel[kVelNodes];
   __shared__ int32_t nbr[8];
   __shared__ float4 stage4[kG2PThreads / kWave][stage_floats(9) / 4];
   const int32_t nb = bounded_blocks(t, sv, grid_cap, err);
-  const float dx_inv = (float)1 / dx;
-  const float D_inv = 4.f * dx_inv * dx_inv;
   const int wave = uniform((int32_t)(threadIdx.x / kWave)), lane = lane_id();
   float* stage = reinterpret_cast<float*>(stage4[wave]);
   for (int32_t b = blockIdx.x; b < nb; b += gridDim.x) {
@@ -828,8 +889,18 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
     const int32_t off = sv.offsets[b];
     if (cnt <= 0 || off < 0 || (int64_t)off + cnt > ps.n) continue;
     load_neighbours(nbr_tab, b, nbr, err);
-    const int32_t o0 = t.active[3 * (int64_t)b] * 4, o1 = t.active[3 * (int64_t)b + 1] * 4,
-                  o2 = t.active[3 * (int64_t)b + 2] * 4;
+    int32_t k0 = t.active[3 * (int64_t)b];
+    // the bucket's grid spacing and time step (its scene's, in a batch)
+    float dx = dx_arg, dt = dt_arg;
+    if (ss.n > 0) {
+      const int32_t sc = scene_of_key(k0, ss.n);
+      k0 -= sc * kSceneKeyStride;
+      dx = ss.p[sc].dx;
+      dt = ss.p[sc].dt;
+    }
+    const float dx_inv = (float)1 / dx;
+    const float D_inv = 4.f * dx_inv * dx_inv;
+    const int32_t o0 = k0 * 4, o1 = t.active[3 * (int64_t)b + 1] * 4, o2 = t.active[3 * (int64_t)b + 2] * 4;
     // the wave's first 64 particle indices load while the velocities are staged
     int32_t first = -1;
     if (wave * kWave + lane < cnt) first = sv.indices[off + wave * kWave + lane];
@@ -1039,14 +1110,14 @@ hipError_t check_launch_limits(int device) {
   if (e != hipSuccess) return e;
   const void* kernels[] = {
       reinterpret_cast<const void*>(&stress_kernel<0>), reinterpret_cast<const void*>(&stress_kernel<1>),
-      reinterpret_cast<const void*>(&p2g_kernel<true>), reinterpret_cast<const void*>(&p2g_kernel<false>),
+      reinterpret_cast<const void*>(&stress_kernel<2>), reinterpret_cast<const void*>(&p2g_kernel<true>), reinterpret_cast<const void*>(&p2g_kernel<false>),
       reinterpret_cast<const void*>(&g2p_kernel<false, -1>), reinterpret_cast<const void*>(&g2p_kernel<true, -1>),
       reinterpret_cast<const void*>(&g2p_kernel<true, 0>), reinterpret_cast<const void*>(&g2p_kernel<true, 1>)};
   for (int i = 0; i < (int)(sizeof(kernels) / sizeof(kernels[0])); ++i) {
     hipFuncAttributes attr;
     e = hipFuncGetAttributes(&attr, kernels[i]);
     if (e != hipSuccess) return e;
-    const size_t extra = i < 2 ? kStressLdsReserve : 0;  // the stress kernels' reserve
+    const size_t extra = i < 3 ? kStressLdsReserve : 0;  // the stress kernels' reserve
     if (attr.sharedSizeBytes + extra > (size_t)max_lds) return hipErrorInvalidConfiguration;
   }
   return closure_launch_limits();
@@ -1060,37 +1131,47 @@ hipError_t grid_clear(float* grid, const int32_t* nblocks_dev, int32_t cap_block
 }
 
 hipError_t stress(const Model& m, const ParticlesView& ps, Workspace& ws, hipStream_t s, const ClosureView* closure,
-                  float* contrib, bool beside_sparsity) {
+                  float* contrib, bool beside_sparsity, const SceneSet* ss) {
   if (ps.n <= 0) return hipSuccess;
   float* out = contrib ? contrib : ws.contrib;
-  if (closure && m.kind == 1 && m.mu != 0.f) return stress_closure(m, *closure, ps, out, ws, s);
   const int64_t waves = (ps.n + kWave - 1) / kWave;
   const int g = groups_for(waves * kWave, kThreads);
   const size_t lds = beside_sparsity ? kStressLdsReserve : 0;
+  if (ss && ss->n > 0) {
+    if (closure) return hipErrorNotSupported;
+    stress_kernel<2><<<g, kThreads, lds, s>>>(m, ps, out, ws.counters, *ss);
+    return hipGetLastError();
+  }
+  if (closure && m.kind == 1 && m.mu != 0.f) return stress_closure(m, *closure, ps, out, ws, s);
   if (m.kind == 0)
-    stress_kernel<0><<<g, kThreads, lds, s>>>(m, ps, out, ws.counters);
+    stress_kernel<0><<<g, kThreads, lds, s>>>(m, ps, out, ws.counters, SceneSet{});
   else
-    stress_kernel<1><<<g, kThreads, lds, s>>>(m, ps, out, ws.counters);
+    stress_kernel<1><<<g, kThreads, lds, s>>>(m, ps, out, ws.counters, SceneSet{});
   return hipGetLastError();
 }
 
 hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
                        int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s,
-                       bool reset_queue) {
+                       bool reset_queue, const SceneSet* ss) {
   const int g = ws.grid_p2g > 0 ? ws.grid_p2g : persistent_groups();
   if (reset_queue) reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
   uint32_t* q = ws.counters + kCtrQueue;
+  const SceneSet sset = ss ? *ss : SceneSet{};
   if (fold)
-    p2g_kernel<true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q);
+    p2g_kernel<true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q,
+                                               sset);
   else
-    p2g_kernel<false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q);
+    p2g_kernel<false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q,
+                                                sset);
   return hipGetLastError();
 }
 
 hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float dt, const float extf[3],
-                       const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s) {
+                       const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s,
+                       const SceneSet* ss) {
   grid_update_kernel<<<persistent_groups(), kThreads, 0, s>>>(grid, grid_cap, t, dt, extf[0], extf[1], extf[2],
-                                                               cols, dx, fold_rhs, ws.counters, ws.err);
+                                                               cols, dx, fold_rhs, ws.counters, ws.err,
+                                                               ss ? *ss : SceneSet{});
   return hipGetLastError();
 }
 
@@ -1102,24 +1183,25 @@ hipError_t grid_angular_momentum(const float* grid, int32_t grid_cap, const Tabl
 
 hipError_t g2p(const ParticlesView& ps, const ParticlesView* out, const TableView& t, const SortView& sv,
                const float* grid, int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s,
-               const NextStress* next) {
+               const NextStress* next, const SceneSet* ss) {
   const int g = ws.grid_g2p > 0 ? ws.grid_g2p : persistent_groups();
   NextStress ns{};
   if (next) ns = *next;
+  const SceneSet sset = ss ? *ss : SceneSet{};
   if (next && next->contrib) {
-    if (!out) return hipErrorInvalidValue;
+    if (!out || sset.n > 0) return hipErrorInvalidValue;
     if (ns.m.kind == 0)
       g2p_kernel<true, 0><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
-                                                    ws.counters);
+                                                    ws.counters, sset);
     else
       g2p_kernel<true, 1><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
-                                                    ws.counters);
+                                                    ws.counters, sset);
   } else if (out) {
     g2p_kernel<true, -1><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
-                                                   ws.counters);
+                                                   ws.counters, sset);
   } else {
     g2p_kernel<false, -1><<<g, kG2PThreads, 0, s>>>(ps, ps, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
-                                                    ws.counters);
+                                                    ws.counters, sset);
   }
   return hipGetLastError();
 }

@@ -51,6 +51,53 @@ struct Colliders {
   PlaneCollider c[kMaxColliders];
 };
 
+// A batch of independent scenes in one table, one grid and one set of
+// particle arrays. Scene s owns the particles [first[s], first[s + 1]); its
+// block keys are shifted by s * kSceneKeyStride along axis 0, so no block is
+// shared between scenes and a block's scene follows from its key. Each scene
+// has its own model, dx, dt, gravity and colliders. The sorted driver keeps
+// every scene's particles in its own range: blocks are numbered in order of
+// first occurrence, so all of scene s's buckets follow scene s - 1's.
+constexpr int32_t kSceneKeyShift = 10;
+constexpr int32_t kSceneKeyStride = 1 << kSceneKeyShift;
+// a scene's own particle keys lie in [-512, 511) along axis 0, its enlarged
+// keys in [-512, 512)
+constexpr int32_t kSceneKeyHalf = kSceneKeyStride / 2;
+constexpr int32_t kMaxScenes = 256;
+
+struct SceneParams {
+  Model model;
+  float dx, dx_inv, dt;
+  float g[3];
+  Colliders cols;
+};
+
+struct SceneSet {
+  int32_t n = 0;                   // 0: one scene, parameters from the launch arguments
+  const SceneParams* p = nullptr;  // [n] device
+  const int64_t* first = nullptr;  // [n + 1] device: first[0] = 0, first[n] = particle count
+};
+
+// scene of a block from the first component of its key, clamped to the
+// batch's n scenes
+__device__ __forceinline__ int32_t scene_of_key(int32_t k0, int32_t n) {
+  return min(max((k0 + kSceneKeyHalf) >> kSceneKeyShift, 0), n - 1);
+}
+
+// scene of particle p: the last s with first[s] <= p
+__device__ __forceinline__ int32_t scene_of_particle(const int64_t* first, int32_t n, int64_t p) {
+  int32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= p) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return lo;
+}
+
 // counters[] slots in the workspace
 enum : int {
   kCtrList = 0,     // unique new keys found by a dedup pass
@@ -169,8 +216,11 @@ hipError_t table_clear(const TableView& t, Workspace& ws, hipStream_t s, uint32_
                        int64_t nzero = 0);
 // record_slots: also fill ws.pslot / ws.slot_block for bucket_sort_slots
 // (the table must be empty: every key is then new)
+// ss (a batch): each particle's key uses its scene's dx and is shifted by the
+// scene's key offset
 hipError_t table_activate(const TableView& t, Workspace& ws, const float* x, int64_t n, float dx,
-                          int blockLen, int offset, float disp, hipStream_t s, bool record_slots = false);
+                          int blockLen, int offset, float disp, hipStream_t s, bool record_slots = false,
+                          const SceneSet* ss = nullptr);
 hipError_t table_enlarge(const TableView& t, Workspace& ws, const int32_t lo[3], const int32_t hi[3],
                          int32_t n0_cap, hipStream_t s);
 hipError_t table_query(const TableView& t, const int32_t* keys, int64_t n, int32_t* out, hipStream_t s);
@@ -195,8 +245,10 @@ hipError_t grid_clear(float* grid, const int32_t* nblocks_dev, int32_t cap_block
 // beside_sparsity: the pass runs on a second stream next to the sparsity
 // stages, and leaves room on every CU for them (at most three of its
 // workgroups per CU)
+// ss (a batch): each particle uses its scene's model (m is not read; no closure)
 hipError_t stress(const Model& m, const ParticlesView& ps, Workspace& ws, hipStream_t s,
-                  const ClosureView* closure = nullptr, float* contrib = nullptr, bool beside_sparsity = false);
+                  const ClosureView* closure = nullptr, float* contrib = nullptr, bool beside_sparsity = false,
+                  const SceneSet* ss = nullptr);
 
 // ---------------------------------------------------------------- closure
 // stress pass of the learned closure into contrib (SVD fallback queued)
@@ -210,11 +262,14 @@ hipError_t mlp_reduce(float* P, float* m, float* v, const float* partial, const 
 hipError_t mlp_harvest(const Model& m, const float* F, const float* logJp, int64_t n, float* X, float* R,
                        hipStream_t s);
 hipError_t closure_launch_limits();
+// ss (a batch, here and below): dx, dt, gravity and colliders of each
+// bucket's or node's scene, read from its block key
 hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
                        int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s,
-                       bool reset_queue = true);
+                       bool reset_queue = true, const SceneSet* ss = nullptr);
 hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float dt, const float extf[3],
-                       const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s);
+                       const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s,
+                       const SceneSet* ss = nullptr);
 // GridAngularMomentum: sum[0..2] += x cross mv, sum[3..5] += mv over nodes with mass
 hipError_t grid_angular_momentum(const float* grid, int32_t grid_cap, const TableView& t, float dx, int mchn,
                                  int mvchn, double* sum, Workspace& ws, hipStream_t s);
@@ -222,7 +277,7 @@ hipError_t grid_angular_momentum(const float* grid, int32_t grid_cap, const Tabl
 // m, logJp, id) are written to out in bucket order (slot = sorted position)
 hipError_t g2p(const ParticlesView& ps, const ParticlesView* out, const TableView& t, const SortView& sv,
                const float* grid, int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s,
-               const NextStress* next = nullptr);
+               const NextStress* next = nullptr, const SceneSet* ss = nullptr);
 // out.attr[i] = in.attr[map[i]] for every attribute; out.id[i] = in.id ? in.id[map[i]] : map[i]
 hipError_t reorder_particles(const ParticlesView& in, const ParticlesView& out, const int32_t* map,
                              Workspace& ws, hipStream_t s);

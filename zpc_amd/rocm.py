@@ -72,19 +72,22 @@ class ClosureDesc(C.Structure):
     _fields_ = [("hdr", Header), ("params", C.c_void_p), ("tol", C.c_float), ("flags", C.c_int32)]
 
 
-class SceneDesc(C.Structure):
-    _fields_ = [("hdr", Header), ("model", C.POINTER(ModelDesc)), ("a", C.POINTER(ParticlesDesc)),
-                ("b", C.POINTER(ParticlesDesc)), ("table", C.POINTER(TableDesc)),
-                ("buckets", C.POINTER(BucketsDesc)), ("grid", C.POINTER(GridDesc)), ("dt", C.c_float),
-                ("extf", C.c_float * 3), ("colliders", C.c_void_p), ("ncolliders", C.c_int32)]
-
-
 # learned closure network sizes (include/zsmpm.h)
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_RECORD, MLP_PARAMS = 9, 64, 7, 10, 5255
+
+MAX_COLLIDERS = 8
+MAX_SCENES = 256
+SCENE_KEY_STRIDE = 1024  # a batch shifts scene s's block keys by s * 1024 along axis 0
 
 
 class PlaneCollider(C.Structure):
     _fields_ = [("type", C.c_int32), ("origin", C.c_float * 3), ("normal", C.c_float * 3)]
+
+
+class BatchSceneDesc(C.Structure):
+    _fields_ = [("hdr", Header), ("first", C.c_int64), ("count", C.c_int64), ("model", ModelDesc),
+                ("dx", C.c_float), ("dt", C.c_float), ("extf", C.c_float * 3), ("ncolliders", C.c_int32),
+                ("colliders", PlaneCollider * MAX_COLLIDERS)]
 
 
 _lib = None
@@ -134,8 +137,8 @@ def lib():
                                         i32, i32, P(C.c_int32)],
         "zsmpm_policy_closure__rocm": [vp, P(ClosureDesc)],
         "zsmpm_policy_launch_widths__rocm": [vp, P(C.c_int32), P(C.c_int32)],
-        "zsmpm_substeps_batched__rocm": [vp, P(SceneDesc), i32, i32, i32, P(C.c_int32), P(C.c_float),
-                                         P(C.c_int32)],
+        "zsmpm_substeps_batched__rocm": [vp, P(BatchSceneDesc), i32, P(ParticlesDesc), P(ParticlesDesc),
+                                         P(TableDesc), P(BucketsDesc), P(GridDesc), i32, i32, P(C.c_int32)],
         "zsmpm_stress__rocm": [vp, P(ModelDesc), vp, vp, i64, vp],
         "zsmpm_mlp_harvest__rocm": [vp, P(ModelDesc), vp, vp, i64, vp, vp],
         "zsmpm_mlp_infer__rocm": [vp, vp, vp, i64, vp],
@@ -521,72 +524,92 @@ class Simulator:
         return dt
 
 
-class SceneState:
-    """Device buffers of one scene of a batch (no policy of its own)."""
-
-    def __init__(self, sc: S.Scene, sorted_driver: bool = True):
-        self.sc = sc
-        T = int(sc.table_size)
-        self.table = DeviceTable(T)
-        cap = max(T // 16, 64)
-        self.grid = DeviceGrid(sc.dx, cap)
-        self.parts = DeviceParticles.from_scene(sc, ids=sorted_driver)
-        self.spare = self.parts.empty_like() if sorted_driver else None
-        self.buckets = DeviceBuckets(cap, sc.n)
-        self.model = model_desc(sc.material)
-        self.cols = colliders_array(sc.colliders)
-        self.ncols = len(sc.colliders)
-
-    def desc(self) -> SceneDesc:
-        d = SceneDesc()
-        d.hdr = _hdr(SceneDesc)
-        d.model = C.pointer(self.model)
-        d.a = C.pointer(self.parts.desc)
-        d.b = C.pointer(self.spare.desc) if self.spare is not None else C.POINTER(ParticlesDesc)()
-        d.table = C.pointer(self.table.desc)
-        d.buckets = C.pointer(self.buckets.desc)
-        d.grid = C.pointer(self.grid.desc)
-        d.dt = float(self.sc.dt)
-        for k in range(3):
-            d.extf[k] = float(self.sc.gravity[k])
-        d.colliders = C.cast(self.cols, C.c_void_p) if self.ncols else None
-        d.ncolliders = self.ncols
-        return d
-
-
 class Batch:
-    """Independent scenes advanced together by zsmpm_substeps_batched__rocm
-    (lanes of streams, one hipGraph holding two substeps of every scene)."""
+    """Independent scenes advanced together by zsmpm_substeps_batched__rocm:
+    their particles concatenated into one set of arrays (scene s at
+    [first[s], first[s] + n_s), ids = positions in the concatenation), one
+    table, one bucket sort and one grid, every stage one launch over all
+    scenes. Sorted driver (ping-pong buffers)."""
 
-    def __init__(self, scenes: Sequence[S.Scene], sorted_driver: bool = True, device: int = 0):
+    def __init__(self, scenes: Sequence[S.Scene], device: int = 0, table_size: Optional[int] = None,
+                 grid_capacity: Optional[int] = None):
+        if not 0 < len(scenes) <= MAX_SCENES:
+            raise ValueError(f"a batch holds 1..{MAX_SCENES} scenes")
+        self.scenes = list(scenes)
         self.pol = Policy(device)
         self.L = lib()
-        self.states = [SceneState(sc, sorted_driver) for sc in scenes]
-        self._build()
-        self.lanes_used = 0
-        self.tune_ms = [-1.0] * 4
+        counts = [int(sc.n) for sc in self.scenes]
+        self.first = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        n = int(self.first[-1])
+        self.n = n
+        cat = lambda key, shape: np.concatenate([np.asarray(getattr(sc, key), np.float32).reshape(shape)
+                                                 for sc in self.scenes])
+        plastic = any(sc.logJp is not None for sc in self.scenes)
+        lj = None
+        if plastic:
+            lj = np.concatenate([np.zeros(sc.n, np.float32) if sc.logJp is None else
+                                 np.asarray(sc.logJp, np.float32).reshape(-1) for sc in self.scenes])
+        self.parts = DeviceParticles(cat("x", (-1, 3)), cat("v", (-1, 3)), cat("C", (-1, 9)), cat("F", (-1, 9)),
+                                     cat("m", (-1,)), lj, ids=np.arange(n, dtype=np.int32))
+        self.spare = self.parts.empty_like()
+        # the scenes' blocks side by side: the table sized for their sum
+        est = sum(max(int(sc.table_size) // 16, 1) for sc in self.scenes)
+        T = int(table_size) if table_size else 16 * S.next_2pow(est)
+        self.table = DeviceTable(T)
+        cap = int(grid_capacity) if grid_capacity else max(T // 16, 64)
+        self.grid = DeviceGrid(min(sc.dx for sc in self.scenes), cap)
+        self.buckets = DeviceBuckets(cap, n)
+        descs = (BatchSceneDesc * len(self.scenes))()
+        for i, sc in enumerate(self.scenes):
+            d = descs[i]
+            d.hdr = _hdr(BatchSceneDesc)
+            d.first = int(self.first[i])
+            d.count = counts[i]
+            d.model = model_desc(sc.material)
+            d.dx = float(sc.dx)
+            d.dt = float(sc.dt)
+            for k in range(3):
+                d.extf[k] = float(sc.gravity[k])
+            if len(sc.colliders) > MAX_COLLIDERS:
+                raise ValueError("at most 8 colliders per scene")
+            d.ncolliders = len(sc.colliders)
+            for j, c in enumerate(sc.colliders):
+                d.colliders[j].type = int(c.kind)
+                for k in range(3):
+                    d.colliders[j].origin[k] = float(c.origin[k])
+                    d.colliders[j].normal[k] = float(c.normal[k])
+        self._descs = descs
+        self.pol.reserve(n, T)
 
-    def _build(self):
-        self._descs = (SceneDesc * len(self.states))(*[st.desc() for st in self.states])
-
-    def substeps(self, n: int, lanes: int = 0):
-        used = C.c_int32(0)
-        tune = (C.c_float * 4)()
+    def substeps(self, n: int, graph: bool = False, autotune: bool = False):
+        """n sorted substeps of every scene (graph: replays of a captured
+        two-substep hipGraph; autotune: the first 8 pick the P2G / G2P
+        workgroup counts)."""
         in_b = C.c_int32(0)
-        check(self.L.zsmpm_substeps_batched__rocm(self.pol.h, self._descs, len(self.states), int(n), int(lanes),
-                                                  C.byref(used), tune, C.byref(in_b)), "substeps_batched")
-        self.lanes_used = used.value
-        if any(t >= 0 for t in tune):
-            self.tune_ms = [float(t) for t in tune]
+        check(self.L.zsmpm_substeps_batched__rocm(self.pol.h, self._descs, len(self.scenes),
+                                                  C.byref(self.parts.desc), C.byref(self.spare.desc),
+                                                  C.byref(self.table.desc), C.byref(self.buckets.desc),
+                                                  C.byref(self.grid.desc), int(n),
+                                                  (2 if graph else 0) | (4 if autotune else 0), C.byref(in_b)),
+              "substeps_batched")
         if in_b.value:
-            for st in self.states:
-                st.parts, st.spare = st.spare, st.parts
-            self._build()
+            self.parts, self.spare = self.spare, self.parts
 
     def sync(self):
         self.pol.sync()
 
+    def nblocks(self) -> int:
+        return int(self.table.cnt.item())
+
+    def scene_state(self, i: int) -> dict:
+        """Scene i's particles in its own id order (ids relative to the scene)."""
+        h = self.parts.host_by_id()
+        lo, hi = int(self.first[i]), int(self.first[i + 1])
+        return {k: (None if a is None else a[lo:hi]) for k, a in h.items()}
+
+    def ids(self) -> np.ndarray:
+        return self.parts.id.cpu().numpy()
+
     @property
     def particles(self) -> int:
-        return sum(st.sc.n for st in self.states)
-
+        return self.n
