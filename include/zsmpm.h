@@ -166,6 +166,21 @@ int32_t zsmpm_table_clear__rocm(zsmpm_policy pol, const zsmpm_table* table);
  * block keys; new keys are numbered in order of first occurrence (seq semantics). */
 int32_t zsmpm_table_activate__rocm(zsmpm_policy pol, const zsmpm_table* table, const float* x, int64_t n,
                                    float dx, int32_t block_len, int32_t offset, float displacement);
+/* zsmpm_table_activate__rocm without any allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its scratch (dedup hash, new-key lists, first-occurrence bitmap, counters)
+ * is one fixed array of the library's own device memory per GPU (about 46 MB,
+ * held until the process ends), not policy scratch; the call makes no
+ * hipMalloc, hipFree or allocating library call. Limits: table_size <= 2^21
+ * slots and n <= 2^24 particles (ZSMPM_INSUFFICIENT_SIZE beyond). Results
+ * equal zsmpm_table_activate__rocm's. Every call on one GPU shares that GPU's
+ * array, whatever its policy: calls that overlap on one GPU (several streams
+ * or host threads) race; calls ordered on one stream do not. Device-side
+ * errors are reported by the next zsmpm_policy_sync of a policy that made
+ * such a call. */
+int32_t zsmpm_table_activate_noalloc__rocm(zsmpm_policy pol, const zsmpm_table* table, const float* x, int64_t n,
+                                           float dx, int32_t block_len, int32_t offset, float displacement);
 /* EnlargeSparsity over range(table.size()), SparsityOp.hpp:88-112. */
 int32_t zsmpm_table_enlarge__rocm(zsmpm_policy pol, const zsmpm_table* table, const int32_t lo[3],
                                   const int32_t hi[3]);

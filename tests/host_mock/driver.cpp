@@ -98,10 +98,19 @@ int main(int argc, char** argv) {
   expect("batched closure unsupported", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 1, 0, &inb), ZSMPM_UNSUPPORTED_OPERATION);
   expect("closure off again", zsmpm_policy_closure__rocm(pol, nullptr), 0);
   expect("batched after closure off", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 2, ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
-  for (zsmpm_particles& p : ba) { free(p.x); free(p.v); free(p.C); free(p.F); free(p.m); free(p.logJp); free(p.id); }
-  free(bbk.indices);
+  // allocation-free activation: limits, then calls on the fixed scratch
+  expect("noalloc activate", zsmpm_table_activate_noalloc__rocm(pol, &big.t, ba[0].x, ntot, 1.f / 128, 4, -2, 0.5f), 0);
+  expect("noalloc activate smaller", zsmpm_table_activate_noalloc__rocm(pol, &sc[0].t, sc[0].a.x, sc[0].a.n, 1.f / 128, 4, -2, 0.5f), 0);
+  expect("noalloc too many particles", zsmpm_table_activate_noalloc__rocm(pol, &big.t, ba[0].x, (1ll << 24) + 1, 1.f / 128, 4, -2, 0.5f), ZSMPM_INSUFFICIENT_SIZE);
+  {
+    zsmpm_table huge = big.t;
+    huge.table_size = 1 << 22;  // descriptor only: rejected before any access
+    expect("noalloc table too large", zsmpm_table_activate_noalloc__rocm(pol, &huge, ba[0].x, ntot, 1.f / 128, 4, -2, 0.5f), ZSMPM_INSUFFICIENT_SIZE);
+  }
   expect("sync", zsmpm_policy_sync(pol), 0);
   expect("del", zsmpm_del_policy__rocm(pol), 0);
+  for (zsmpm_particles& p : ba) { free(p.x); free(p.v); free(p.C); free(p.F); free(p.m); free(p.logJp); free(p.id); }
+  free(bbk.indices);
   for (Sc& s : sc) {
     for (zsmpm_particles* p : {&s.a, &s.b}) { free(p->x); free(p->v); free(p->C); free(p->F); free(p->m); free(p->logJp); free(p->id); }
     free(s.t.keys); free(s.t.indices); free(s.t.status); free(s.t.cnt); free(s.t.active_keys);

@@ -37,6 +37,8 @@ hipError_t hipMemsetAsync(void* p, int v, size_t n, hipStream_t s) {
   memset(p, v, n); return hipSuccess;
 }
 hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { memcpy(d, s, n); return hipSuccess; }
+// host-only build: the device array's host shadow stands in for its memory
+hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return hipSuccess; }
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned int) {
   MStream* m = new MStream{g_id++}; *s = reinterpret_cast<hipStream_t>(m); g_streams.insert(*s); return hipSuccess;
 }
@@ -108,4 +110,5 @@ hipError_t hipLaunchKernel(const void*, dim3, dim3, void**, size_t, hipStream_t 
 void** __hipRegisterFatBinary(const void*) { static void* h[4]; return h; }
 void __hipRegisterFunction(void**, const void*, char*, const char*, unsigned int, void*, void*, void*, void*, int*) {}
 void __hipUnregisterFatBinary(void**) {}
+void __hipRegisterVar(void**, void*, char*, const char*, int, size_t, int, int) {}
 }

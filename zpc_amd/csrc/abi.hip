@@ -5,6 +5,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -36,6 +37,9 @@ struct zsmpm_policy_s {
   SceneParams* scene_p = nullptr;   // [kMaxScenes]
   int64_t* scene_first = nullptr;   // [kMaxScenes + 1]
   std::vector<unsigned char> scene_bytes;
+  // zsmpm_table_activate_noalloc__rocm ran on this policy: its device-side
+  // errors are in the device's fixed scratch array
+  bool noalloc_used = false;
   // P2G / G2P workgroup counts picked by ZSMPM_SUBSTEPS_AUTOTUNE (0: default)
   int32_t tuned_p2g = 0, tuned_g2p = 0;
   Workspace ws{};
@@ -659,6 +663,55 @@ int32_t capture_pair(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_
   return ZSMPM_OK;
 }
 
+// ------------------------------------------------------ allocation-free
+// zsmpm_table_activate_noalloc__rocm runs the activation on one fixed array
+// of the library's own device memory, `scratch` (each GPU holds its own copy
+// until the process ends), instead of policy scratch that hipMalloc provides.
+// Sub-buffers at 256-byte aligned offsets: the dedup hash (keys, orders), the
+// two new-key lists, the claimed-slot mask, the first-occurrence bitmap and
+// its prefix, the popcount scan's tile sums, the counters and the error word.
+constexpr uint32_t kNoAllocSlots = 1u << 21;      // largest table (and dedup hash)
+constexpr int64_t kNoAllocParticles = 1ll << 24;  // most particles per call
+constexpr size_t na_align(size_t v) { return (v + 255) & ~(size_t)255; }
+constexpr uint32_t kNaBmWords = (uint32_t)(kNoAllocParticles / 32 + 2);
+constexpr uint32_t kNaScanWords = 1024;
+constexpr size_t kNaSkey = 0;
+constexpr size_t kNaSorder = na_align(kNaSkey + 8ull * kNoAllocSlots);
+constexpr size_t kNaLkey = na_align(kNaSorder + 4ull * kNoAllocSlots);
+constexpr size_t kNaLval = na_align(kNaLkey + 4ull * kNoAllocSlots);
+constexpr size_t kNaDmask = na_align(kNaLval + 4ull * kNoAllocSlots);
+constexpr size_t kNaBitmap = na_align(kNaDmask + 4ull * (kNoAllocSlots / 32 + 1));
+constexpr size_t kNaPrefix = na_align(kNaBitmap + 4ull * kNaBmWords);
+constexpr size_t kNaScan = na_align(kNaPrefix + 4ull * kNaBmWords);
+constexpr size_t kNaCounters = na_align(kNaScan + 4ull * kNaScanWords);
+constexpr size_t kNaErr = na_align(kNaCounters + 4ull * kCtrCount);
+constexpr size_t kNoAllocBytes = na_align(kNaErr + 16);
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// The device address of `scratch` on each device, looked up once per device
+// (with that device current) by the first policy created on it. The host-side
+// name `scratch` is not a device address; only these pointers are used.
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+void* g_na_ptr[kNaMaxDevices] = {};
+bool g_na_tried[kNaMaxDevices] = {};
+
+void noalloc_setup(int devid) {
+  if (devid < 0 || devid >= kNaMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (g_na_tried[devid]) return;
+  g_na_tried[devid] = true;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess) g_na_ptr[devid] = ptr;
+}
+
+unsigned char* noalloc_base(int devid) {
+  if (devid < 0 || devid >= kNaMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  return static_cast<unsigned char*>(g_na_ptr[devid]);
+}
+
 // ---------------------------------------------------------------- batches
 
 // Validates a batch's scenes against the combined particle arrays and
@@ -823,6 +876,7 @@ int32_t zsmpm_policy__rocm(int32_t devid, void* stream, zsmpm_policy* out) {
   {
     DeviceGuard g(devid);
     if (check_launch_limits(devid) != hipSuccess) return ZSMPM_UNSUPPORTED_OPERATION;
+    noalloc_setup(devid);  // the device's fixed scratch address, once per device
   }
   zsmpm_policy_s* p = new (std::nothrow) zsmpm_policy_s();
   if (!p) return ZSMPM_ERROR;
@@ -855,6 +909,15 @@ int32_t zsmpm_policy_sync(zsmpm_policy pol) {
   if (pol->ws.err) {
     ZS_HIP(hipMemcpy(&bits, pol->ws.err, sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (bits) ZS_HIP(hipMemset(pol->ws.err, 0, sizeof(uint32_t)));
+  }
+  unsigned char* na = pol->noalloc_used ? noalloc_base(pol->devid) : nullptr;
+  if (na) {
+    // the allocation-free activation's error word (shared by every policy on
+    // this device that called it)
+    uint32_t nb = 0;
+    ZS_HIP(hipMemcpy(&nb, na + kNaErr, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (nb) ZS_HIP(hipMemset(na + kNaErr, 0, sizeof(uint32_t)));
+    bits |= nb;
   }
   return from_err_bits(bits);
 }
@@ -912,6 +975,37 @@ int32_t zsmpm_table_activate__rocm(zsmpm_policy pol, const zsmpm_table* table, c
   ZS_TRY(ensure(pol, n, t.T));
   StageScope sc(pol, ZSMPM_STAGE_ACTIVATE);
   return from_hip(table_activate(t, pol->ws, x, n, dx, block_len, offset, displacement, pol->stream));
+}
+
+int32_t zsmpm_table_activate_noalloc__rocm(zsmpm_policy pol, const zsmpm_table* table, const float* x, int64_t n,
+                                           float dx, int32_t block_len, int32_t offset, float displacement) {
+  if (!pol || n < 0 || (n > 0 && !x) || block_len <= 0 || !(dx > 0.f)) return ZSMPM_INVALID_ARGUMENT;
+  TableView t;
+  ZS_TRY(view_table(table, t));
+  if (n > kNoAllocParticles || (uint32_t)t.T > kNoAllocSlots) return ZSMPM_INSUFFICIENT_SIZE;
+  unsigned char* base = noalloc_base(pol->devid);
+  if (!base) return ZSMPM_DEVICE_ERROR;
+  // views into the device's fixed scratch array; the launches below clear
+  // what they read before it is written (dedup hash, mask, bitmap, counters)
+  Workspace w{};
+  w.n_cap = (uint32_t)(n > 0 ? n : 1);
+  w.ts_cap = (uint32_t)t.T > 1024u ? (uint32_t)t.T : 1024u;
+  w.list_cap = w.ts_cap;
+  w.skey = reinterpret_cast<uint64_t*>(base + kNaSkey);
+  w.sorder = reinterpret_cast<uint32_t*>(base + kNaSorder);
+  w.la_key = reinterpret_cast<uint32_t*>(base + kNaLkey);
+  w.la_val = reinterpret_cast<uint32_t*>(base + kNaLval);
+  w.dmask = reinterpret_cast<uint32_t*>(base + kNaDmask);
+  w.bitmap = reinterpret_cast<uint32_t*>(base + kNaBitmap);
+  w.bm_prefix = reinterpret_cast<uint32_t*>(base + kNaPrefix);
+  w.bm_words = kNaBmWords;
+  w.scan_tmp = reinterpret_cast<uint32_t*>(base + kNaScan);
+  w.scan_words = kNaScanWords;
+  w.counters = reinterpret_cast<uint32_t*>(base + kNaCounters);
+  w.err = reinterpret_cast<uint32_t*>(base + kNaErr);
+  DeviceGuard g(pol->devid);
+  pol->noalloc_used = true;
+  return from_hip(table_activate(t, w, x, n, dx, block_len, offset, displacement, pol->stream));
 }
 
 int32_t zsmpm_table_enlarge__rocm(zsmpm_policy pol, const zsmpm_table* table, const int32_t lo[3],

@@ -118,6 +118,7 @@ def lib():
         "zsmpm_policy_stage_times": [vp, P(C.c_double), P(C.c_uint64), i32, i32],
         "zsmpm_table_clear__rocm": [vp, P(TableDesc)],
         "zsmpm_table_activate__rocm": [vp, P(TableDesc), vp, i64, f32, i32, i32, f32],
+        "zsmpm_table_activate_noalloc__rocm": [vp, P(TableDesc), vp, i64, f32, i32, i32, f32],
         "zsmpm_table_enlarge__rocm": [vp, P(TableDesc), P(C.c_int32 * 3), P(C.c_int32 * 3)],
         "zsmpm_table_query__rocm": [vp, P(TableDesc), vp, i64, vp],
         "zsmpm_bucket_sort__rocm": [vp, P(TableDesc), vp, i64, f32, i32, i32, f32, P(BucketsDesc)],
