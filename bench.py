@@ -262,64 +262,72 @@ def surrogate_leg(torch, rocm, sc, args, graph, plain):
 
 
 def batched_leg(torch, rocm, args) -> dict:
-    """The survey's config 4: 64 independent scenes of 10K..500K particles
-    (boxes and spheres, fixed-corotated and sand, dx 1/128 and 1/256). Each
-    scene has its own policy and buffers and advances by replays of its own
-    captured two-substep graph (zsmpm_substeps_sorted__rocm with
-    ZSMPM_SUBSTEPS_GRAPH). With --batch-lanes k > 1 the policies sit on k
-    streams (default priority; scenes largest first, each to the lane with the
-    fewest particles), so the scenes' replays overlap; with k = 1 they run one
-    after another on the default stream. The entry point that puts every scene
-    into one graph (zsmpm_substeps_batched__rocm) is switched off and not used
-    here."""
+    """The 64-scene workload: independent scenes of 10K..500K particles
+    (boxes and spheres, fixed-corotated and sand, dx 1/128 and 1/256) in one
+    table and one grid, advanced by zsmpm_substeps_batched__rocm with every
+    stage one launch over all scenes: 8 autotuning substeps and 2 warm-up
+    substeps, then the timed substeps as replays of the captured two-substep
+    graph. With --batch-per-scene the same scenes are also timed the way a
+    single-scene driver would run them (each its own policy and captured
+    graph, one scene after another) for comparison."""
     from zpc_amd import scenes as S
 
     scenes = S.batch_scenes(64)
-    k = max(1, int(args.batch_lanes))
-    streams = [torch.cuda.Stream() for _ in range(k)] if k > 1 else [torch.cuda.current_stream()]
-    load = [0] * len(streams)
-    lane_of = {}
-    for i in sorted(range(len(scenes)), key=lambda j: -scenes[j].n):
-        lane = min(range(len(streams)), key=lambda l: load[l])
-        lane_of[i] = lane
-        load[lane] += scenes[i].n
-    sims = []
-    for i, sc in enumerate(scenes):
-        with torch.cuda.stream(streams[lane_of[i]]):
-            sims.append(rocm.Simulator(sc, reorder=True))
-    torch.cuda.synchronize()
     n = sum(sc.n for sc in scenes)
-    for sim in sims:  # warm-up, launch-width autotuning and the graph capture, per scene
-        sim.substeps(10, autotune=True)
-        sim.substeps(2, graph=True)
-    for sim in sims:
-        sim.sync()
+    b = rocm.Batch(scenes)
+    b.substeps(8, autotune=True)
+    b.substeps(2, graph=True)
+    b.sync()
+    blocks = b.nblocks()
     steps = args.batch_steps
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for sim in sims:
-        sim.substeps(steps, graph=True)
+    b.substeps(steps, graph=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for sim in sims:
-        sim.sync()
+    b.sync()
     ms = (t1 - t0) * 1e3 / steps
+    widths = b.pol.launch_widths()
+    print(f"# 64 scenes, {n} particles, {blocks} blocks: {ms:.3f} ms per substep of every scene "
+          f"(one launch per stage)", file=sys.stderr, flush=True)
     largest = max(scenes, key=lambda sc: sc.n)
-    print(f"# 64 scenes, {n} particles: {ms:.3f} ms per substep of every scene on {len(streams)} stream(s)",
-          file=sys.stderr, flush=True)
-    return {
+    out = {
         "workload": "64 scenes, particle counts log-uniform in [10K, 500K] (seeded), jittered boxes and spheres, "
                     "fixed-corotated / Drucker-Prager, dx 1/128 or 1/256, dt 1e-4",
         "particles": n,
         "largest_scene_particles": largest.n,
+        "active_blocks": blocks,
         "value": n / (ms * 1e-3),
         "unit": "particle-substeps/s",
         "ms_per_step": ms,
         "steps": steps,
-        "streams": len(streams),
-        "schedule": ("per-scene graph replays on " + (f"{len(streams)} streams" if len(streams) > 1 else
-                     "one stream, one scene after another") + " (the one-graph batched entry point is switched off)"),
+        "p2g_g2p_workgroups": list(widths),
+        "schedule": "zsmpm_substeps_batched__rocm: one table and grid for all scenes, every stage one launch, "
+                    "replays of a captured two-substep hipGraph",
     }
+    del b
+    torch.cuda.empty_cache()
+    if args.batch_per_scene:
+        sims = [rocm.Simulator(sc, reorder=True) for sc in scenes]
+        for sim in sims:
+            sim.substeps(8, autotune=True)
+            sim.substeps(2, graph=True)
+        for sim in sims:
+            sim.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for sim in sims:
+            sim.substeps(steps, graph=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ms1 = (t1 - t0) * 1e3 / steps
+        print(f"# per-scene graphs: {ms1:.3f} ms per substep of every scene", file=sys.stderr, flush=True)
+        out["per_scene_graphs"] = {"ms_per_step": ms1, "value": n / (ms1 * 1e-3),
+                                   "schedule": "each scene its own policy and captured graph, one after another"}
+        out["speedup_vs_per_scene"] = ms1 / ms
+        del sims
+        torch.cuda.empty_cache()
+    return out
 
 
 def main() -> int:
@@ -339,11 +347,10 @@ def main() -> int:
                     help="time the profiled eager launches instead of hipGraph replays")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-closure leg")
     ap.add_argument("--train-steps", type=int, default=2000, help="Adam steps of the closure network")
-    ap.add_argument("--batch", action="store_true",
-                    help="run the 64-scene leg (per-scene graph replays, one scene after another)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--batch-steps", type=int, default=100, help="timed substeps of the batched leg")
-    ap.add_argument("--batch-lanes", type=int, default=1,
-                    help="streams the 64 scenes are spread over (4 and 8 measured no faster than 1)")
+    ap.add_argument("--batch-per-scene", action="store_true",
+                    help="also time the 64 scenes as separate single-scene graphs, one after another")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (the reference kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -394,7 +401,7 @@ def main() -> int:
         surrogate = surrogate_leg(torch, rocm, sc, args, graph, plain)
     x_plain = plain.pop("x")
     batched = None
-    if args.batch:
+    if not args.no_batch:
         del x_plain
         x_plain = None
         torch.cuda.empty_cache()

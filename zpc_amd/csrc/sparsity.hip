@@ -132,9 +132,10 @@ constexpr int kActItems = 4;  // 64-particle groups per wave, their loads and fi
 // bytes. The dedup hash keeps the minimum order per key, so the result does
 // not depend on the order of the inserts. With pslot, every particle's
 // scratch slot (the identity of its key) is recorded for the bucket sort.
-// ss.n > 0 (a batch): particle p takes its scene's 1/dx, and its key is
+// kMulti (a batch, ss): particle p takes its scene's 1/dx, and its key is
 // shifted by the scene's offset (a key outside the scene's own range
 // [-512, 511) along axis 0 is a key-range error)
+template <bool kMulti>
 __global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restrict__ x, int64_t n,
                                                             float dxinv, int blockLen, int offset,
                                                             float disp, Dedup dd, uint32_t* pslot,
@@ -143,8 +144,8 @@ __global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restr
   const int64_t span = (int64_t)kThreads * kActItems;
   const int64_t stride = (int64_t)gridDim.x * span;
   const int lane = lane_id();
-  __shared__ int64_t sfirst[kMaxScenes + 1];
-  if (ss.n > 0) {
+  __shared__ int64_t sfirst[kMulti ? kMaxScenes + 1 : 1];
+  if constexpr (kMulti) {
     for (int i = threadIdx.x; i <= ss.n; i += kThreads) sfirst[i] = ss.first[i];
     __syncthreads();
   }
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(kThreads) void activate_kernel(const float* __restr
       if (p < n) {
         int32_t k0, k1, k2;
         bool in_range = true;
-        if (ss.n > 0) {
+        if constexpr (kMulti) {
           const int32_t sc = scene_of_particle(sfirst, ss.n, p);
           block_key(cx[r], cy[r], cz[r], ss.p[sc].dx_inv, blockLen, offset, disp, k0, k1, k2);
           // below 511, so the enlargement's key + 1 stays in the scene's range
@@ -767,11 +768,15 @@ hipError_t table_activate(const TableView& t, Workspace& ws, const float* x, int
       ws.skey, ws.sorder, ws.dmask, ws.ts_cap, t.cnt, ws.counters, slot_block, ws.bitmap,
       (int64_t)bitmap_words(ws, order_bound));
   const float dxinv = (float)1.0 / dx;
-  if (n > 0)
-    activate_kernel<<<groups_for(n, kThreads * kActItems), kThreads, 0, s>>>(x, n, dxinv, blockLen, offset, disp,
-                                                                              dedup_of(ws),
-                                                                              record_slots ? ws.pslot : nullptr,
-                                                                              ss ? *ss : SceneSet{});
+  if (n > 0) {
+    const int g = groups_for(n, kThreads * kActItems);
+    uint32_t* ps = record_slots ? ws.pslot : nullptr;
+    if (ss && ss->n > 0)
+      activate_kernel<true><<<g, kThreads, 0, s>>>(x, n, dxinv, blockLen, offset, disp, dedup_of(ws), ps, *ss);
+    else
+      activate_kernel<false><<<g, kThreads, 0, s>>>(x, n, dxinv, blockLen, offset, disp, dedup_of(ws), ps,
+                                                    SceneSet{});
+  }
   return finish_insertion(t, ws, order_bound, s, slot_block);
 }
 

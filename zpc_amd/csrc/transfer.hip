@@ -223,7 +223,8 @@ __global__ __launch_bounds__(kThreads) void stress_kernel(Model m, ParticlesView
   if (fp64_used) atomicAdd(&counters[kCtrFp64], fp64_used);
 }
 
-template <bool kFold>
+// kMulti (a batch, ss): each bucket's dx and dt are its scene's
+template <bool kFold, bool kMulti>
 __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, const float* __restrict__ contrib,
                                                        TableView t, SortView sv, const int32_t* __restrict__ nbr_tab,
                                                        float* __restrict__ grid, int32_t grid_cap, float dt,
@@ -257,9 +258,9 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
   // flush and the next bucket's set-up instead of starting phase A.
   constexpr int kPre = kChunk / kP2GThreads;
   __shared__ int32_t bmeta[7];  // bucket, count, offset, key, scene
-  const bool multi = ss.n > 0;
+  const int32_t nscenes = kMulti ? ss.n : 0;
   if (threadIdx.x == 0)
-    bucket_meta((int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu), nb, sv, t, ps.n, bmeta, ss.n);
+    bucket_meta((int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu), nb, sv, t, ps.n, bmeta, nscenes);
   for (int c = threadIdx.x; c <= kMainCells; c += kP2GThreads) {
     ccount[c] = 0;
     cfill[c] = 0;
@@ -283,14 +284,14 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
     if (cnt == 0) {  // empty or out-of-range bucket: take the next one now
       __syncthreads();
       if (threadIdx.x == 0)
-        bucket_meta((int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu), nb, sv, t, ps.n, bmeta, ss.n);
+        bucket_meta((int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu), nb, sv, t, ps.n, bmeta, nscenes);
       __syncthreads();
       prefetch();
       continue;
     }
     // the bucket's grid spacing and force scale (its scene's, in a batch)
     float bdx = dx, bscale = scale;
-    if (multi) {
+    if constexpr (kMulti) {
       const SceneParams& sp = ss.p[bmeta[6]];
       bdx = sp.dx;
       const float bdx_inv = (float)1.0 / bdx;
@@ -653,7 +654,7 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
               }
             }
       }
-      if (last && threadIdx.x == 0) bucket_meta((int32_t)min(nxt, 0x7fffffffu), nb, sv, t, ps.n, bmeta, ss.n);
+      if (last && threadIdx.x == 0) bucket_meta((int32_t)min(nxt, 0x7fffffffu), nb, sv, t, ps.n, bmeta, nscenes);
       __syncthreads();
     }
     prefetch();  // the next bucket's first-chunk indices (bmeta is the next bucket's now)
@@ -717,10 +718,12 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
 // driver's convention when P2G kept them separate).
 // Node velocity and colliders of one node (mass != 0) of the block with key
 // k; cols is the kernel argument or a scene's colliders in a batch.
-template <typename Cols>
+// kLocal: k0_local is the key's first component with a batch's scene offset
+// removed; otherwise the table's own key is read.
+template <bool kLocal = false, typename Cols>
 __device__ __forceinline__ void node_velocity(float* gp, float mass, int fold, float e0, float e1, float e2,
-                                              const Cols& cols, int32_t k0, int32_t k1, int32_t k2, int cell,
-                                              float dx, float& vmax) {
+                                              const Cols& cols, const TableView& t, int32_t b, int32_t k0_local,
+                                              int cell, float dx, float& vmax) {
   // rounded as written (no FMA contraction): a node on a collider plane has
   // phi = 0 in exact arithmetic, and the sign of the rounded phi must be the
   // one the host path computes
@@ -735,6 +738,8 @@ __device__ __forceinline__ void node_velocity(float* gp, float mass, int fold, f
   float v[3] = {mv0 * inv + e0, mv1 * inv + e1, mv2 * inv + e2};
   vmax = fmaxf(vmax, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
   if (mass > 0.f && cols.n > 0) {
+    const int32_t k0 = kLocal ? k0_local : t.active[3 * (int64_t)b];
+    const int32_t k1 = t.active[3 * (int64_t)b + 1], k2 = t.active[3 * (int64_t)b + 2];
     const float pos[3] = {((float)k0 * 4.f + (float)((cell >> 4) & 3)) * dx,
                           ((float)k1 * 4.f + (float)((cell >> 2) & 3)) * dx,
                           ((float)k2 * 4.f + (float)(cell & 3)) * dx};
@@ -761,6 +766,7 @@ __device__ __forceinline__ void node_velocity(float* gp, float mass, int fold, f
   gp[3 * kBlockCells] = v[2];
 }
 
+template <bool kMulti>
 __global__ __launch_bounds__(kThreads) void grid_update_kernel(float* __restrict__ grid, int32_t grid_cap,
                                                                TableView t, float dt, float g0, float g1,
                                                                float g2, Colliders cols, float dx, int fold,
@@ -782,17 +788,16 @@ __global__ __launch_bounds__(kThreads) void grid_update_kernel(float* __restrict
       float* gp = grid + grid_at(b, 0, cell);
       const float mass = gp[0];
       if (mass != 0.f) {
-        int32_t k0 = t.active[3 * (int64_t)b];
-        const int32_t k1 = t.active[3 * (int64_t)b + 1], k2 = t.active[3 * (int64_t)b + 2];
-        if (ss.n > 0) {
+        if constexpr (kMulti) {
           // the node's scene (from its block key): its dt, gravity, dx and colliders
+          int32_t k0 = t.active[3 * (int64_t)b];
           const int32_t sc = scene_of_key(k0, ss.n);
           const SceneParams& sp = ss.p[sc];
           k0 -= sc * kSceneKeyStride;
-          node_velocity(gp, mass, fold, sp.g[0] * sp.dt, sp.g[1] * sp.dt, sp.g[2] * sp.dt, sp.cols, k0, k1, k2,
-                        cell, sp.dx, vmax);
+          node_velocity<true>(gp, mass, fold, sp.g[0] * sp.dt, sp.g[1] * sp.dt, sp.g[2] * sp.dt, sp.cols, t, b, k0,
+                              cell, sp.dx, vmax);
         } else {
-          node_velocity(gp, mass, fold, e0, e1, e2, cols, k0, k1, k2, cell, dx, vmax);
+          node_velocity(gp, mass, fold, e0, e1, e2, cols, t, b, 0, cell, dx, vmax);
         }
       } else if (fold) {
         // the fold covers every node, massless ones included (their
@@ -870,7 +875,7 @@ __global__ __launch_bounds__(kThreads) void angular_momentum_kernel(const float*
 
 // kNext: -1 plain G2P; 0 / 1 (reorder only): also the next substep's
 // fixed-corotated / Drucker-Prager stress from the updated F (NextStress)
-template <bool kReorder, int kNext>
+template <bool kReorder, int kNext, bool kMulti = false>
 __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, ParticlesView out, TableView t,
                                                           SortView sv, const int32_t* __restrict__ nbr_tab,
                                                           const float* __restrict__ grid, int32_t grid_cap,
@@ -882,6 +887,8 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
   __shared__ int32_t nbr[8];
   __shared__ float4 stage4[kG2PThreads / kWave][stage_floats(9) / 4];
   const int32_t nb = bounded_blocks(t, sv, grid_cap, err);
+  const float dx_inv_arg = (float)1 / dx_arg;
+  const float D_inv_arg = 4.f * dx_inv_arg * dx_inv_arg;
   const int wave = uniform((int32_t)(threadIdx.x / kWave)), lane = lane_id();
   float* stage = reinterpret_cast<float*>(stage4[wave]);
   for (int32_t b = blockIdx.x; b < nb; b += gridDim.x) {
@@ -891,15 +898,15 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
     load_neighbours(nbr_tab, b, nbr, err);
     int32_t k0 = t.active[3 * (int64_t)b];
     // the bucket's grid spacing and time step (its scene's, in a batch)
-    float dx = dx_arg, dt = dt_arg;
-    if (ss.n > 0) {
+    float dx = dx_arg, dt = dt_arg, D_inv = D_inv_arg;
+    if constexpr (kMulti) {
       const int32_t sc = scene_of_key(k0, ss.n);
       k0 -= sc * kSceneKeyStride;
       dx = ss.p[sc].dx;
       dt = ss.p[sc].dt;
+      const float dx_inv = (float)1 / dx;
+      D_inv = 4.f * dx_inv * dx_inv;
     }
-    const float dx_inv = (float)1 / dx;
-    const float D_inv = 4.f * dx_inv * dx_inv;
     const int32_t o0 = k0 * 4, o1 = t.active[3 * (int64_t)b + 1] * 4, o2 = t.active[3 * (int64_t)b + 2] * 4;
     // the wave's first 64 particle indices load while the velocities are staged
     int32_t first = -1;
@@ -1110,9 +1117,13 @@ hipError_t check_launch_limits(int device) {
   if (e != hipSuccess) return e;
   const void* kernels[] = {
       reinterpret_cast<const void*>(&stress_kernel<0>), reinterpret_cast<const void*>(&stress_kernel<1>),
-      reinterpret_cast<const void*>(&stress_kernel<2>), reinterpret_cast<const void*>(&p2g_kernel<true>), reinterpret_cast<const void*>(&p2g_kernel<false>),
+      reinterpret_cast<const void*>(&stress_kernel<2>), reinterpret_cast<const void*>(&p2g_kernel<true, false>),
+      reinterpret_cast<const void*>(&p2g_kernel<false, false>), reinterpret_cast<const void*>(&p2g_kernel<true, true>),
+      reinterpret_cast<const void*>(&p2g_kernel<false, true>),
       reinterpret_cast<const void*>(&g2p_kernel<false, -1>), reinterpret_cast<const void*>(&g2p_kernel<true, -1>),
-      reinterpret_cast<const void*>(&g2p_kernel<true, 0>), reinterpret_cast<const void*>(&g2p_kernel<true, 1>)};
+      reinterpret_cast<const void*>(&g2p_kernel<true, 0>), reinterpret_cast<const void*>(&g2p_kernel<true, 1>),
+      reinterpret_cast<const void*>(&g2p_kernel<true, -1, true>),
+      reinterpret_cast<const void*>(&g2p_kernel<false, -1, true>)};
   for (int i = 0; i < (int)(sizeof(kernels) / sizeof(kernels[0])); ++i) {
     hipFuncAttributes attr;
     e = hipFuncGetAttributes(&attr, kernels[i]);
@@ -1156,22 +1167,34 @@ hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortVi
   const int g = ws.grid_p2g > 0 ? ws.grid_p2g : persistent_groups();
   if (reset_queue) reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
   uint32_t* q = ws.counters + kCtrQueue;
-  const SceneSet sset = ss ? *ss : SceneSet{};
-  if (fold)
-    p2g_kernel<true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q,
-                                               sset);
+  const bool multi = ss && ss->n > 0;
+  const SceneSet sset = multi ? *ss : SceneSet{};
+  if (fold && multi)
+    p2g_kernel<true, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err,
+                                                     q, sset);
+  else if (fold)
+    p2g_kernel<true, false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err,
+                                                      q, sset);
+  else if (multi)
+    p2g_kernel<false, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err,
+                                                      q, sset);
   else
-    p2g_kernel<false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q,
-                                                sset);
+    p2g_kernel<false, false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx,
+                                                       ws.err, q, sset);
   return hipGetLastError();
 }
 
 hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float dt, const float extf[3],
                        const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s,
                        const SceneSet* ss) {
-  grid_update_kernel<<<persistent_groups(), kThreads, 0, s>>>(grid, grid_cap, t, dt, extf[0], extf[1], extf[2],
-                                                               cols, dx, fold_rhs, ws.counters, ws.err,
-                                                               ss ? *ss : SceneSet{});
+  if (ss && ss->n > 0)
+    grid_update_kernel<true><<<persistent_groups(), kThreads, 0, s>>>(grid, grid_cap, t, dt, extf[0], extf[1],
+                                                                     extf[2], cols, dx, fold_rhs, ws.counters,
+                                                                     ws.err, *ss);
+  else
+    grid_update_kernel<false><<<persistent_groups(), kThreads, 0, s>>>(grid, grid_cap, t, dt, extf[0], extf[1],
+                                                                      extf[2], cols, dx, fold_rhs, ws.counters,
+                                                                      ws.err, SceneSet{});
   return hipGetLastError();
 }
 
@@ -1196,9 +1219,15 @@ hipError_t g2p(const ParticlesView& ps, const ParticlesView* out, const TableVie
     else
       g2p_kernel<true, 1><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
                                                     ws.counters, sset);
+  } else if (out && sset.n > 0) {
+    g2p_kernel<true, -1, true><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
+                                                         ws.counters, sset);
   } else if (out) {
     g2p_kernel<true, -1><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
                                                    ws.counters, sset);
+  } else if (sset.n > 0) {
+    g2p_kernel<false, -1, true><<<g, kG2PThreads, 0, s>>>(ps, ps, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
+                                                          ws.counters, sset);
   } else {
     g2p_kernel<false, -1><<<g, kG2PThreads, 0, s>>>(ps, ps, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
                                                     ws.counters, sset);
