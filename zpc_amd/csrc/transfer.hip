@@ -169,8 +169,11 @@ __device__ __forceinline__ void bspline_w(float l, float (&w)[3]) {
 template <bool kFold>
 struct Rec {
   static constexpr int kQuads = kFold ? 4 : 7;
-  // particles staged per pass; keeps the workgroup's LDS under 64 KB
-  static constexpr int kChunk = kFold ? 512 : 256;
+  // particles staged per pass. 576 (nine waves) holds a whole bucket at 8
+  // particles per cell (up to 562 in the 16M sand column; 39% of its buckets
+  // exceed 512 and took a nearly empty second pass, 0.80 -> 0.765 ms) and
+  // keeps the folded workgroup at 53 KB of LDS, three per CU
+  static constexpr int kChunk = kFold ? 576 : 256;
 };
 
 // Per-particle constitutive part of P2G (P2G.hpp:57-100): vol P F^T from the
@@ -231,8 +234,8 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
                                                        float dx, uint32_t* err, uint32_t* queue, SceneSet ss) {
   constexpr int kQ = Rec<kFold>::kQuads;
   constexpr int kChunk = Rec<kFold>::kChunk;
-  static_assert(kP2GThreads % kWave == 0 && kChunk % kP2GThreads == 0,
-                "whole waves; a chunk is a whole number of workgroup passes");
+  static_assert(kP2GThreads % kWave == 0 && kChunk % kWave == 0,
+                "whole waves; a chunk is a whole number of waves");
   constexpr int kNV = kFold ? 1 : 2;  // float4 per arena node
   __shared__ float4 arena[kP2GWaves][kArena6 * kNV];
   // records in component-major order: component k of record i at
@@ -256,7 +259,7 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
   // every thread then issues the loads of its particle indices in the next
   // bucket's first chunk before the flush, so they are in flight across the
   // flush and the next bucket's set-up instead of starting phase A.
-  constexpr int kPre = kChunk / kP2GThreads;
+  constexpr int kPre = (kChunk + kP2GThreads - 1) / kP2GThreads;  // chunk slots per thread
   __shared__ int32_t bmeta[7];  // bucket, count, offset, key, scene
   const int32_t nscenes = kMulti ? ss.n : 0;
   if (threadIdx.x == 0)
@@ -447,17 +450,17 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
       }
       // each thread keeps the base offsets and cells of its own particles
       // across the rebuild of list[]
-      uint16_t keep[kChunk / kP2GThreads];
-      uint8_t keepc[kChunk / kP2GThreads];
+      uint16_t keep[kPre];
+      uint8_t keepc[kPre];
 #pragma unroll
-      for (int k = 0; k < kChunk / kP2GThreads; ++k) {
+      for (int k = 0; k < kPre; ++k) {
         const int32_t i = threadIdx.x + k * kP2GThreads;
         keep[k] = i < nch ? list[i] : (uint16_t)0;
         keepc[k] = i < nch ? cellof[i] : (uint8_t)255;
       }
       __syncthreads();
 #pragma unroll
-      for (int k = 0; k < kChunk / kP2GThreads; ++k) {
+      for (int k = 0; k < kPre; ++k) {
         const int32_t i = threadIdx.x + k * kP2GThreads;
         if (i < nch && keepc[k] <= kMainCells) {
           const int32_t slot = atomicAdd(&cfill[keepc[k]], 1);
@@ -606,7 +609,7 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
       }
       // ---- overflow particles: global atomics straight into the grid
 #pragma unroll
-      for (int k = 0; k < kChunk / kP2GThreads; ++k) {
+      for (int k = 0; k < kPre; ++k) {
         const int32_t i = threadIdx.x + k * kP2GThreads;
         if (i >= nch || keepc[k] != kMainCells) continue;
         const int r0 = (keep[k] >> 6) & 7, r1 = (keep[k] >> 3) & 7, r2 = keep[k] & 7;
