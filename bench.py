@@ -167,7 +167,8 @@ def run_path(torch, rocm, sc, args, multi, graph, closure=None):
     sim.sync()
     b0 = sim.nblocks()
     print(f"# warm-up done ({tune} tuning + {args.warmup} substeps), {b0} active blocks", file=sys.stderr, flush=True)
-    fb0 = int(sim.pol.counters()[4])
+    ctr0 = sim.pol.counters()
+    fb0, nw0 = int(ctr0[4]), int(ctr0[9])
     # profiled eager pass: every stage bracketed by HIP events on the stream it
     # runs on (per-stage and per-kernel times, the roofline)
     sim.pol.stage_times(reset=True)
@@ -202,59 +203,114 @@ def run_path(torch, rocm, sc, args, multi, graph, closure=None):
         print(f"# graph replays: {wall * 1e3 / args.steps:.3f} ms/substep", file=sys.stderr, flush=True)
     b1 = sim.nblocks()
     substeps_run = tune + args.warmup + args.steps + (2 + args.steps if graph else 0)
+    ctr = sim.pol.counters()
     out = {"stages": stages, "blocks": 0.5 * (b0 + b1), "ms": wall * 1e3 / args.steps, "eager_ms": eager_ms,
            "launch_widths": sim.pol.launch_widths(),
-           "fallbacks": int(sim.pol.counters()[4]) - fb0, "substeps": substeps_run, "x": x_by_id(torch, sim)}
+           "fallbacks": int(ctr[4]) - fb0, "net_waves": int(ctr[9]) - nw0, "substeps": substeps_run,
+           "x": x_by_id(torch, sim)}
     del sim
     return out
 
 
+def _states(torch, rocm, scene, substeps):
+    """(F, logJp) device tensors of a scene after `substeps` plain sorted substeps."""
+    sim = rocm.Simulator(scene, reorder=True)
+    if substeps > 0:
+        sim.substeps(substeps)
+    sim.sync()
+    F = sim.parts.F.reshape(-1, 9).clone()
+    lj = sim.parts.logJp.clone()
+    del sim
+    return F, lj
+
+
+def _closure_stats(rocm, r, sc, n_pass):
+    waves = n_pass * ((sc.n + 63) // 64)
+    return {"network_wave_fraction": r["net_waves"] / waves,
+            "fallback_fraction": r["fallbacks"] / (sc.n * n_pass)}
+
+
 def surrogate_leg(torch, rocm, sc, args, graph, plain):
-    """The same run with the learned closure: the network is trained on the
-    GPU (outside the timed regions) on states of the scene after a few plain
-    substeps plus random large-strain states; positions are compared with the
-    plain run particle by particle after the same substeps."""
+    """The same runs with the learned closure. The network is trained on the
+    GPU (outside the timed regions) on a pool of about 50M states harvested
+    from plain runs: the column after the warm-up, the column under a smooth
+    initial strain (scenes.strained) at rest and after 20 substeps, plus
+    random large-strain states. Legs on the column (where one Newton step from
+    X0 = 0 certifies every wave, so the network is skipped) and on the
+    strained column (5% compression and 3% shear at t = 0, where it does not): positions are compared with the plain
+    run particle by particle after the same substeps, and the fraction of
+    waves that evaluated the network and of particles that fell back to the
+    SVD is reported."""
     from zpc_amd import closure as CL
+    from zpc_amd import scenes as S
 
     t0 = time.perf_counter()
-    F, lj = CL.training_states(sc, sim_steps=args.warmup, synthetic=1 << 18, seed=0)
-    gen = torch.Generator(device=F.device)
-    gen.manual_seed(0)
-    keep = torch.randperm(F.shape[0], device=F.device, generator=gen)[: 1 << 20]
+    strained = S.strained(sc, seed=5)
+    parts = [_states(torch, rocm, sc, args.warmup), _states(torch, rocm, strained, 0),
+             _states(torch, rocm, strained, 20)]
+    Fs, ljs = CL.synthetic_states(1 << 18, 0)
+    F = torch.cat([p[0] for p in parts] + [torch.as_tensor(Fs).cuda()]).contiguous()
+    lj = torch.cat([p[1] for p in parts] + [torch.as_tensor(ljs).cuda()]).contiguous()
+    del parts
     model = rocm.model_desc(sc.material)
     pol = rocm.Policy()
-    tr = CL.train(pol, model, F[keep].contiguous(), lj[keep].contiguous(), steps=args.train_steps, batch=8192,
-                  lr=3e-3, seed=0)
-    del F, lj, keep
+    tr = CL.train(pol, model, F, lj, steps=args.train_steps, batch=8192, lr=3e-3, seed=0)
+    pool = int(F.shape[0])
+    del F, lj
+    torch.cuda.empty_cache()
     prep = time.perf_counter() - t0
-    print(f"# closure trained: {args.train_steps} Adam steps in {tr.seconds:.2f} s, batch loss {tr.losses[0]:.3e} -> "
-          f"{tr.losses[-1]:.3e}, kept step {tr.best_step} (mean loss {tr.eval_loss:.3e} on {tr.eval_samples} "
-          f"states)", file=sys.stderr, flush=True)
-    res = {}
-    for name, params, always in (("trained", tr.params, False), ("trained_network_every_wave", tr.params, True),
-                                 ("zero_start", None, False)):
-        r = run_path(torch, rocm, sc, args, True, graph, closure=(params, 1e-4, always))
-        dx_err = (r["x"] - plain["x"]).abs().max().item() / sc.dx
+    print(f"# closure trained on {pool} states: {args.train_steps} Adam steps in {tr.seconds:.2f} s, batch loss "
+          f"{tr.losses[0]:.3e} -> {tr.losses[-1]:.3e}, kept step {tr.best_step} (mean loss {tr.eval_loss:.3e} on "
+          f"{tr.eval_samples} states)", file=sys.stderr, flush=True)
+
+    def leg(scene, base, params, always):
+        r = run_path(torch, rocm, scene, args, True, graph, closure=(params, 1e-4, always))
+        dx_err = (r["x"] - base["x"]).abs().max().item() / scene.dx
         n_pass = r["substeps"] - args.warmup - 8
         per_launch = {k: (ms / c if c else 0.0) for k, (ms, c) in r["stages"].items()}
-        res[name] = {
-            "value": sc.n / (r["ms"] * 1e-3),
+        out = {
+            "value": scene.n / (r["ms"] * 1e-3),
             "ms_per_step": r["ms"],
             "eager_ms_per_step": r["eager_ms"],
-            "speedup_vs_plain": plain["ms"] / r["ms"],
+            "speedup_vs_plain": base["ms"] / r["ms"],
             "stress_ms_per_launch": per_launch.get("stress", 0.0),
-            "plain_stress_ms_per_launch": plain["stages"]["stress"][0] / max(plain["stages"]["stress"][1], 1),
-            "fallback_fraction": r["fallbacks"] / (sc.n * n_pass),
+            "plain_stress_ms_per_launch": base["stages"]["stress"][0] / max(base["stages"]["stress"][1], 1),
             "max_position_deviation_from_plain_dx": dx_err,
             "substeps_compared": r["substeps"],
         }
-        print(f"# surrogate ({name}): {r['ms']:.3f} ms/substep, fallback {res[name]['fallback_fraction']:.2e}, "
-              f"max |x - x_plain| = {dx_err:.2e} dx", file=sys.stderr, flush=True)
+        out.update(_closure_stats(rocm, r, scene, n_pass))
         del r
+        return out
+
+    res = {}
+    for name, params, always in (("trained", tr.params, False), ("trained_network_every_wave", tr.params, True),
+                                 ("zero_start", None, False)):
+        res[name] = leg(sc, plain, params, always)
+        print(f"# surrogate ({name}): {res[name]['ms_per_step']:.3f} ms/substep, network waves "
+              f"{res[name]['network_wave_fraction']:.3f}, fallback {res[name]['fallback_fraction']:.2e}, "
+              f"max |x - x_plain| = {res[name]['max_position_deviation_from_plain_dx']:.2e} dx",
+              file=sys.stderr, flush=True)
+    # the strained column: the network is needed in most waves
+    base = run_path(torch, rocm, strained, args, True, graph)
+    print(f"# strained column, plain: {base['ms']:.3f} ms/substep", file=sys.stderr, flush=True)
+    st = {"workload": "the same column at rest under a smooth initial strain: F = 0.95 I + 0.03 e_x e_y^T "
+                      "+ U(-0.002, 0.002) per entry (seeded)",
+          "plain_ms_per_step": base["ms"], "plain_value": sc.n / (base["ms"] * 1e-3)}
+    for name, params in (("trained", tr.params), ("zero_start", None)):
+        st[name] = leg(strained, base, params, False)
+        print(f"# strained column, surrogate ({name}): {st[name]['ms_per_step']:.3f} ms/substep, network waves "
+              f"{st[name]['network_wave_fraction']:.3f}, fallback {st[name]['fallback_fraction']:.2e}, "
+              f"max |x - x_plain| = {st[name]['max_position_deviation_from_plain_dx']:.2e} dx",
+              file=sys.stderr, flush=True)
+    del base
+    res["strained_column"] = st
     res["network"] = ("MLP 9-64-64-7 GELU, BF16 MFMA / FP32 accumulate, trained on this GPU; 'trained' evaluates "
                       "it only in waves that one Newton step from X0 = 0 cannot certify a priori, "
-                      "'trained_network_every_wave' in every wave, 'zero_start' never")
-    res["training"] = {"adam_steps": args.train_steps, "batch": 8192, "samples": tr.samples,
+                      "'trained_network_every_wave' in every wave, 'zero_start' never (uncertified particles "
+                      "take the SVD fallback)")
+    res["training"] = {"adam_steps": args.train_steps, "batch": 8192, "samples": tr.samples, "pool": pool,
+                       "pool_sources": "column after the warm-up, strained column at t = 0 and after 20 plain "
+                                       "substeps (16,777,216 states each), 262,144 random states",
                        "seconds": tr.seconds, "loss_first": tr.losses[0], "loss_last": tr.losses[-1],
                        "loss_eval": tr.eval_loss, "kept_step": tr.best_step, "eval_samples": tr.eval_samples,
                        "data_seconds": prep - tr.seconds}
@@ -346,7 +402,7 @@ def main() -> int:
     ap.add_argument("--eager", action="store_true",
                     help="time the profiled eager launches instead of hipGraph replays")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-closure leg")
-    ap.add_argument("--train-steps", type=int, default=2000, help="Adam steps of the closure network")
+    ap.add_argument("--train-steps", type=int, default=4000, help="Adam steps of the closure network")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--batch-steps", type=int, default=100, help="timed substeps of the batched leg")
     ap.add_argument("--batch-per-scene", action="store_true",

@@ -136,7 +136,8 @@ int32_t zsmpm_policy_reserve__rocm(zsmpm_policy pol, int64_t max_particles, int3
 int32_t zsmpm_policy_sync(zsmpm_policy pol);
 /* Diagnostic counters after a sync: [0] new keys of the last insertion,
  * [2] particle runs of the last bucket sort, [3] FP64 SVDs, [4] closure fallbacks,
- * [5] max |v|^2 bits of the last grid update. out must hold 16 entries. */
+ * [5] max |v|^2 bits of the last grid update, [9] waves of 64 particles that
+ * evaluated the closure network (all passes). out must hold 16 entries. */
 int32_t zsmpm_policy_counters(zsmpm_policy pol, uint32_t* out);
 
 /* Stages timed when profiling is on. */

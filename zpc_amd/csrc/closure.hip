@@ -901,10 +901,11 @@ __global__ void reset_queue_kernel(uint32_t* queue, uint32_t* queue2) {
   if (queue2) *queue2 = 0u;
 }
 
-// the plain SVD return mapping for the queued particles
+// the plain SVD return mapping for the queued particles; also adds the
+// pass's network waves (the queue length netq, or net_all) to the count
 __global__ __launch_bounds__(256) void fallback_kernel(Model m, ParticlesView ps, float* __restrict__ contrib,
                                                        const int32_t* __restrict__ fb, const uint32_t* queue,
-                                                       uint32_t* counters) {
+                                                       uint32_t* counters, const uint32_t* netq, uint32_t net_all) {
   const uint32_t nq = min(*queue, (uint32_t)ps.n);
   uint32_t fp64_used = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
@@ -920,7 +921,11 @@ __global__ __launch_bounds__(256) void fallback_kernel(Model m, ParticlesView ps
     for (int k = 0; k < 9; ++k) contrib[9 * p + k] = PF[k];
     fp64_used += f64 ? 1u : 0u;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[kCtrFallback], nq);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicAdd(&counters[kCtrFallback], nq);
+    const uint32_t nw = netq ? *netq : net_all;
+    if (nw) atomicAdd(&counters[kCtrNetWaves], nw);
+  }
   if (fp64_used) atomicAdd(&counters[kCtrFp64], fp64_used);
 }
 
@@ -986,7 +991,9 @@ hipError_t stress_closure(const Model& m, const ClosureView& cl, const Particles
     closure_kernel<false><<<g_zero, kCThreads, 0, s>>>(m, ps, contrib, nullptr, cl.tol, 0, ws.fb, queue, nullptr,
                                                        nullptr);
   }
-  fallback_kernel<<<persistent_groups(), 256, 0, s>>>(m, ps, contrib, ws.fb, queue, ws.counters);
+  const uint32_t net_all = (cl.params && !two_pass) ? (uint32_t)waves : 0u;
+  fallback_kernel<<<persistent_groups(), 256, 0, s>>>(m, ps, contrib, ws.fb, queue, ws.counters,
+                                                      two_pass ? wq_n : nullptr, net_all);
   return hipGetLastError();
 }
 

@@ -109,6 +109,7 @@ enum : int {
   kCtrQueue = 6,    // next bucket of the running per-bucket launch
   kCtrFbQueue = 7,  // length of the closure's SVD fallback queue (per pass)
   kCtrNetQueue = 8, // waves queued for the closure's network (per pass)
+  kCtrNetWaves = 9, // waves that evaluated the closure's network (all passes)
   kCtrCount = 16
 };
 
