@@ -237,7 +237,8 @@ def surrogate_leg(torch, rocm, sc, args, graph, plain):
     initial strain (scenes.strained) at rest and after 20 substeps, plus
     random large-strain states. Legs on the column (where one Newton step from
     X0 = 0 certifies every wave, so the network is skipped) and on the
-    strained column (5% compression and 3% shear at t = 0, where it does not): positions are compared with the plain
+    strained column (5% compression and 3% shear, held by sticky walls on all
+    six faces, where X0 = 0 certifies no particle): positions are compared with the plain
     run particle by particle after the same substeps, and the fraction of
     waves that evaluated the network and of particles that fell back to the
     SVD is reported."""
@@ -293,8 +294,8 @@ def surrogate_leg(torch, rocm, sc, args, graph, plain):
     # the strained column: the network is needed in most waves
     base = run_path(torch, rocm, strained, args, True, graph)
     print(f"# strained column, plain: {base['ms']:.3f} ms/substep", file=sys.stderr, flush=True)
-    st = {"workload": "the same column at rest under a smooth initial strain: F = 0.95 I + 0.03 e_x e_y^T "
-                      "+ U(-0.002, 0.002) per entry (seeded)",
+    st = {"workload": "the same column at rest under a smooth initial strain, F = 0.95 I + 0.03 e_x e_y^T "
+                      "+ U(-0.002, 0.002) per entry (seeded), in a closed box of sticky walls on its faces",
           "plain_ms_per_step": base["ms"], "plain_value": sc.n / (base["ms"] * 1e-3)}
     for name, params in (("trained", tr.params), ("zero_start", None)):
         st[name] = leg(strained, base, params, False)

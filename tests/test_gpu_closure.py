@@ -317,3 +317,55 @@ def test_training_never_ends_worse_than_its_best_checkpoint(pol):
     kept = C.eval_loss(pol, tr.params, Xe, Re)
     assert np.isfinite(kept) and kept < start, (start, kept, tr.best_step)
     assert abs(kept - tr.eval_loss) <= 1e-6 * max(kept, 1e-30), (kept, tr.eval_loss)
+
+
+def test_trained_network_100_substeps_near_plain(pol):
+    """The sand column under a smooth initial strain (5% compression, 3% shear)
+    in a closed box of sticky walls, so the strain stays (|C - I| 0.15..0.27
+    over 100 substeps), beyond what one Newton step from X0 = 0 certifies a
+    priori (it certifies none of these particles), with a network
+    trained on the GPU on states harvested from plain runs of it (t = 0, 10,
+    30 and 60 substeps) and random states: the network runs (counter 9 counts
+    its waves), fewer than 1% of particle-substeps take the SVD fallback, and
+    after 100 substeps the positions are within 1e-3 dx of the plain GPU path
+    or within twice the plain path's own run-to-run spread."""
+    import torch
+
+    rocm = _rocm()
+    C = _closure()
+    sc = S.strained(S.sand_scene(dims=(16, 32, 16)), seed=21)
+    model = rocm.model_desc(sc.material)
+    Fs, ljs = [], []
+    for k in (0, 10, 30, 60):
+        sim = rocm.Simulator(sc, reorder=True)
+        if k:
+            sim.substeps(k)
+        sim.sync()
+        Fs.append(sim.parts.F.reshape(-1, 9).clone())
+        ljs.append(sim.parts.logJp.clone())
+        del sim
+    Fr, ljr = C.synthetic_states(1 << 16, seed=23, lo=1e-4, hi=0.25)
+    F = torch.cat(Fs + [_dev(Fr)]).contiguous()
+    lj = torch.cat(ljs + [_dev(ljr)]).contiguous()
+    tr = C.train(pol, model, F, lj, steps=1500, batch=4096, lr=3e-3, seed=3)
+    steps = 100
+    runs = {}
+    for name, closure in (("surrogate", True), ("plain_a", False), ("plain_b", False)):
+        sim = rocm.Simulator(sc, reorder=True)
+        if closure:
+            sim.pol.set_closure(tr.params, tol=1e-4)
+        sim.substeps(steps)
+        sim.sync()
+        runs[name] = sim.parts.host_by_id()["x"]
+        if closure:
+            c = sim.pol.counters()
+            waves = steps * ((sc.n + 63) // 64)
+            net, fb = int(c[9]), int(c[4])
+            print(f"strained column, {steps} substeps: network waves {net / waves:.3f}, fallback "
+                  f"{fb / (sc.n * steps):.2e} of particle-substeps", flush=True)
+            assert net > 0, "the network was never evaluated"
+            assert fb < 0.01 * sc.n * steps, fb
+    d_sur = np.max(np.abs(runs["surrogate"] - runs["plain_a"]))
+    d_plain = np.max(np.abs(runs["plain_b"] - runs["plain_a"]))
+    print(f"|x_closure - x_plain| = {d_sur / sc.dx:.2e} dx, plain run-to-run {d_plain / sc.dx:.2e} dx", flush=True)
+    assert d_sur <= max(1e-3 * sc.dx, 2.0 * d_plain), (d_sur, d_plain, sc.dx)

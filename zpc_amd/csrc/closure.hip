@@ -819,18 +819,25 @@ __global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(4, 8)
   const int wave = uniform((int32_t)(threadIdx.x / kWave)), lane = lane_id();
   float* stage = reinterpret_cast<float*>(stage4[wave]);
   const int64_t total_waves = (ps.n + kWave - 1) / kWave;
-  // kNet with a wave queue: only the queued waves, which the zero-start pass
-  // could not certify; without kNet a queue receives those waves instead
+  // kNet with wave flags: only the flagged waves, which the zero-start pass
+  // could not certify (their count goes to wq_n); without kNet the flags
+  // receive that verdict for every wave. Flags instead of a queue: a queue
+  // needs one same-address atomic per wave, which serialises when most waves
+  // need the network (262K per pass at 16M particles).
   const bool from_queue = kNet && wq != nullptr;
-  const int64_t nw = from_queue ? (int64_t)min(*wq_n, (uint32_t)total_waves) : total_waves;
-  if (nw == 0) return;
+  const int64_t nw = total_waves;
+  uint32_t nproc = 0;
   __shared__ float4 nbias[kNet ? kCWaves : 1][kNet ? kNetBias / 4 : 1];
   __shared__ v8bf nw2[kNet ? kNetW2 : 1];
   NetRegs r;
   if constexpr (kNet) load_net(P, r, reinterpret_cast<float*>(nbias[wave]), nw2);
   const int64_t nwaves = (int64_t)gridDim.x * kCWaves;
   for (int64_t i = (int64_t)blockIdx.x * kCWaves + wave; i < nw; i += nwaves) {
-    const int64_t w = from_queue ? (int64_t)wq[i] : i;
+    const int64_t w = i;
+    if (from_queue) {
+      if (uniform((int32_t)wq[w]) == 0) continue;
+      ++nproc;
+    }
     const int64_t p0 = w * kWave;
     const int32_t cw = (int32_t)min((int64_t)kWave, ps.n - p0);
     const int64_t p = p0 + lane;
@@ -845,13 +852,9 @@ __global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(4, 8)
     if constexpr (kNet) {
       use_net = from_queue || always || ballot(lane < cw && !zero_start_certifies(nrm, tol)) != 0;
     } else if (wq != nullptr) {
-      if (ballot(lane < cw && !zero_start_certifies(nrm, tol)) != 0) {  // the network's wave
-        if (lane == 0) {
-          const uint32_t q = atomicAdd(wq_n, 1u);
-          if (q < (uint32_t)total_waves) wq[q] = (uint32_t)w;
-        }
-        continue;
-      }
+      const bool net_wave = ballot(lane < cw && !zero_start_certifies(nrm, tol)) != 0;
+      if (lane == 0) wq[w] = net_wave ? 1u : 0u;
+      if (net_wave) continue;  // the network pass computes this wave
     }
     if (use_net) {
       // features of particle 16 t + (l & 15) in the B layout: through LDS
@@ -886,14 +889,21 @@ __global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(4, 8)
       if (ok) {
         ps.logJp[p] = lj;
       } else {
-        const uint32_t q = atomicAdd(queue, 1u);
-        fb[q] = (int32_t)p;
 #pragma unroll
         for (int k = 0; k < 9; ++k) PF[k] = 0.f;
       }
     }
+    // the wave's rejected particles go to the fallback queue with one atomic
+    const uint64_t rej = ballot(lane < cw && !ok);
+    if (rej != 0) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(queue, (uint32_t)__popcll(rej));
+      base = (uint32_t)__shfl((int)base, 0, kWave);
+      if ((rej >> lane) & 1ull) fb[base + (uint32_t)__popcll(rej & ((1ull << lane) - 1ull))] = (int32_t)p;
+    }
     wave_store_rows<9>(contrib, p0, cw, stage, PF);
   }
+  if (from_queue && lane == 0 && nproc) atomicAdd(wq_n, nproc);
 }
 
 __global__ void reset_queue_kernel(uint32_t* queue, uint32_t* queue2) {
@@ -969,9 +979,9 @@ hipError_t stress_closure(const Model& m, const ClosureView& cl, const Particles
   uint32_t* queue = ws.counters + kCtrFbQueue;
   uint32_t* wq_n = ws.counters + kCtrNetQueue;
   // With a network and without always_net: the zero-start pass takes every
-  // wave (80 VGPRs, six waves per SIMD) and queues the waves it cannot
-  // certify; the network pass (176 VGPRs) then takes only those. Each wave
-  // gets exactly what the single network pass would give it.
+  // wave (80 VGPRs, six waves per SIMD) and flags the waves it cannot
+  // certify; the network pass then takes only those. Each wave gets exactly
+  // what the single network pass would give it.
   const bool two_pass = cl.params && !cl.always_net;
   reset_queue_kernel<<<1, 1, 0, s>>>(queue, two_pass ? wq_n : nullptr);
   const int64_t waves = (ps.n + kWave - 1) / kWave;
@@ -980,7 +990,7 @@ hipError_t stress_closure(const Model& m, const ClosureView& cl, const Particles
   if (two_pass) {
     closure_kernel<false><<<g_zero, kCThreads, 0, s>>>(m, ps, contrib, nullptr, cl.tol, 0, ws.fb, queue, ws.wq, wq_n);
     // 1024 workgroups of four waves fill every SIMD at the network's four
-    // waves per SIMD; more would only cost dispatch when the queue is short
+    // waves per SIMD; they read every wave's flag and take the flagged ones
     const int g_queue = g_net < 1024 ? g_net : 1024;
     closure_kernel<true><<<g_queue, kCThreads, 0, s>>>(m, ps, contrib, cl.params, cl.tol, 0, ws.fb, queue, ws.wq,
                                                        wq_n);

@@ -108,7 +108,7 @@ enum : int {
   kCtrMaxVel = 5,   // max |v|^2 as float bits
   kCtrQueue = 6,    // next bucket of the running per-bucket launch
   kCtrFbQueue = 7,  // length of the closure's SVD fallback queue (per pass)
-  kCtrNetQueue = 8, // waves queued for the closure's network (per pass)
+  kCtrNetQueue = 8, // waves the closure's network pass took (per pass)
   kCtrNetWaves = 9, // waves that evaluated the closure's network (all passes)
   kCtrCount = 16
 };
@@ -158,7 +158,7 @@ struct Workspace {
   float* contrib = nullptr;      // [9 n_cap] vol P F^T of each particle (column-major)
   int32_t* nbr = nullptr;        // [8 ts_cap] block index of key_b + (d>>2&1, d>>1&1, d&1), or -1
   int32_t* fb = nullptr;         // [n_cap] particles queued for the closure's SVD fallback
-  uint32_t* wq = nullptr;        // [n_cap / 64 + 2] waves queued for the closure's network
+  uint32_t* wq = nullptr;        // [n_cap / 64 + 2] per wave: 1 where the closure's network must run
   int32_t grid_p2g = 0;          // P2G / G2P workgroups (0: persistent_groups()), set by the policy
   int32_t grid_g2p = 0;
   float* lj_next[2] = {nullptr, nullptr};  // [n_cap] each: logJp caches of the fused multi-substep driver

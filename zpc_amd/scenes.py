@@ -289,15 +289,31 @@ def tiny_scene(n_cells=(3, 3, 3), seed: int = 7, dx: float = 1.0 / 32, ppc: int 
 
 
 def strained(scene: Scene, seed: int = 0, compression: float = 0.05, shear: float = 0.03,
-             noise: float = 0.002) -> Scene:
+             noise: float = 0.002, closed: bool = True) -> Scene:
     """Return a copy at rest under a smooth initial strain: every particle's F is
     (1 - compression) I plus `shear` in the x-y entry, plus per-entry noise
     uniform in [-noise, noise] (numpy's seeded PCG64, fast at 16M particles).
     Strains of this size are beyond what one Newton step from X0 = 0
-    certifies a priori, so the learned closure evaluates its network; x, v,
-    C, m and logJp are as in the scene."""
+    certifies a priori, so the learned closure evaluates its network. With
+    `closed` the colliders are six sticky planes on the faces of the
+    particles' bounding box (a lid included): the compressed body stays
+    compressed. Released instead, the sand flows apart (zpc never projects F
+    of sand, so |C - I| passes 1 within tens of substeps). x, v, C, m and
+    logJp are as in the scene."""
     rng = np.random.default_rng(seed)
     s = dataclasses.replace(scene)
+    if closed:
+        lo = scene.x.min(axis=0).astype(np.float64)
+        hi = scene.x.max(axis=0).astype(np.float64)
+        cols = []
+        for d in range(3):
+            nrm = [0.0, 0.0, 0.0]
+            nrm[d] = 1.0
+            cols.append(Collider(tuple(float(v) if i == d else 0.0 for i, v in enumerate(lo)), tuple(nrm), STICKY))
+            nrm = [0.0, 0.0, 0.0]
+            nrm[d] = -1.0
+            cols.append(Collider(tuple(float(v) if i == d else 0.0 for i, v in enumerate(hi)), tuple(nrm), STICKY))
+        s.colliders = cols
     F = rng.uniform(-noise, noise, size=(scene.n, 9)).astype(np.float32)
     F[:, 0] += 1.0 - compression
     F[:, 4] += 1.0 - compression
