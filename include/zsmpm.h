@@ -117,6 +117,34 @@ typedef struct zsmpm_plane_collider {
 
 #define ZSMPM_MAX_COLLIDERS 8
 
+enum zsmpm_geometry {
+  ZSMPM_GEOMETRY_PLANE = 0,
+  ZSMPM_GEOMETRY_CUBOID = 1,
+  ZSMPM_GEOMETRY_SPHERE = 2,
+  ZSMPM_GEOMETRY_CYLINDER = 3
+};
+
+/* Collider<AnalyticLevelSet<geometry>> with its rigid motion
+ * (geometry/Collider.h:10-190; geometry/AnalyticLevelSet.h:11-225): the level
+ * set's material point X sits at x = R s X + b, and the object's velocity at
+ * x is omega x (x - b) + (dsdt / s)(x - b) + dbdt. A node with phi(X) < 0
+ * takes that velocity (sticky) or loses the normal part of its velocity
+ * relative to it (slip; separate only when approaching), the normal being
+ * R n(X) (Collider::resolveCollision, Collider.h:81-107). Geometry in
+ * material space: plane p0 = origin, p1 = normal (used as given); cuboid
+ * p0 = min, p1 = max corner; sphere p0 = center, p1[0] = radius; cylinder
+ * p0 = bottom, p1[0] = radius, p1[1] = length along `axis` (0..2). */
+typedef struct zsmpm_collider {
+  int32_t type;     /* zsmpm_collider_type */
+  int32_t geometry; /* zsmpm_geometry */
+  int32_t axis;     /* cylinder axis */
+  float p0[3], p1[3];
+  float R[9];       /* rotation, column-major */
+  float omega[3];   /* angular velocity */
+  float s, dsdt;    /* scale (> 0) and its rate */
+  float b[3], dbdt[3]; /* translation and its rate */
+} zsmpm_collider;
+
 typedef struct zsmpm_policy_s* zsmpm_policy;
 
 /* ABI version of the loaded library. */
@@ -272,6 +300,15 @@ int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, 
                                     const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt,
                                     const float extf[3], const zsmpm_plane_collider* colliders, int32_t ncolliders,
                                     int32_t nsteps, int32_t flags, int32_t* final_in_b);
+
+/* The general colliders of every later grid update on this policy (the
+ * standalone update and the substep drivers, batches included, in each
+ * scene's own coordinates), applied after the call's plane colliders in list
+ * order (ApplyBoundaryConditionOnGridBlocks, GridOp.hpp:112-164). Copied;
+ * a collider's position is the one given until the next call, which a driver
+ * makes between substeps to move it. NULL or n = 0 removes them; at most
+ * ZSMPM_MAX_COLLIDERS. */
+int32_t zsmpm_policy_colliders__rocm(zsmpm_policy pol, const zsmpm_collider* colliders, int32_t n);
 
 /* The P2G and G2P workgroup counts the policy launches with (autotuned or the
  * defaults). No counterpart in the reference (its launch shapes are fixed). */

@@ -25,6 +25,27 @@ _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 
 
+class GeneralColliderParams(C.Structure):
+    """One general collider of ``zso_params`` (oracle_capi.cpp)."""
+
+    _fields_ = [("type", C.c_int32), ("geometry", C.c_int32), ("axis", C.c_int32),
+                ("p0", C.c_double * 3), ("p1", C.c_double * 3), ("R", C.c_double * 9), ("omega", C.c_double * 3),
+                ("s", C.c_double), ("dsdt", C.c_double), ("b", C.c_double * 3), ("dbdt", C.c_double * 3)]
+
+
+def general_collider_params(c) -> GeneralColliderParams:
+    """GeneralColliderParams of a scenes.GeneralCollider."""
+    g = GeneralColliderParams()
+    g.type, g.geometry, g.axis = int(c.kind), int(c.geometry), int(c.axis)
+    for d in range(3):
+        g.p0[d], g.p1[d] = float(c.p0[d]), float(c.p1[d])
+        g.omega[d], g.b[d], g.dbdt[d] = float(c.omega[d]), float(c.b[d]), float(c.dbdt[d])
+    for k in range(9):
+        g.R[k] = float(c.R[k])
+    g.s, g.dsdt = float(c.s), float(c.dsdt)
+    return g
+
+
 class Params(C.Structure):
     """Mirror of ``zso_params`` (oracle_capi.cpp)."""
 
@@ -35,6 +56,7 @@ class Params(C.Structure):
         ("volume_correction", C.c_int32), ("table_size", C.c_int32), ("ncols", C.c_int32),
         ("col_origin", (C.c_double * 3) * 8), ("col_normal", (C.c_double * 3) * 8),
         ("col_type", C.c_int32 * 8),
+        ("ngcols", C.c_int32), ("gcols", GeneralColliderParams * 8),
     ]
 
 
@@ -107,6 +129,10 @@ def lib():
         L.zso_run_sorted_f64.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int32, C.c_int64, _f64p, _f64p,
                                          _f64p, _f64p, _f64p, C.c_void_p, _i32p]
         L.zso_reorder_f32.argtypes = [C.c_int64, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_void_p, _i32p]
+        for name in ("zso_level_set_sdf_f64", "zso_resolve_general_f64"):
+            getattr(L, name).argtypes = [C.POINTER(Params), C.c_int32, C.c_int64, _f64p, _f64p]
+        L.zso_level_set_normal_f64.argtypes = [C.POINTER(Params), C.c_int32, C.c_int64, _f64p, _f64p]
+        L.zso_resolve_general_f32.argtypes = [C.POINTER(Params), C.c_int32, C.c_int64, _f32p, _f32p]
         _lib = L
     return _lib
 
@@ -180,6 +206,10 @@ def make_params(scene, dtype=np.float32) -> Params:
             p.col_origin[i][d] = float(c.origin[d])
             p.col_normal[i][d] = float(c.normal[d])
         p.col_type[i] = int(c.kind)
+    gen = list(getattr(scene, "general_colliders", None) or [])
+    p.ngcols = len(gen)
+    for i, c in enumerate(gen):
+        p.gcols[i] = general_collider_params(c)
     return p
 
 
@@ -214,6 +244,38 @@ def stress_fc(vol, mu, lam, F):
     PF = np.zeros_like(F)
     lib().zso_stress_fc_f32(F.shape[0], vol, mu, lam, F, PF)
     return PF
+
+
+def _one_collider(c) -> Params:
+    p = Params()
+    p.ngcols = 1
+    p.gcols[0] = general_collider_params(c)
+    return p
+
+
+def level_set_sdf(c, X):
+    """phi of the general collider c (scenes.GeneralCollider) at material-space points X [n, 3], float64."""
+    X = np.ascontiguousarray(X, np.float64).reshape(-1, 3)
+    out = np.zeros(X.shape[0])
+    lib().zso_level_set_sdf_f64(C.byref(_one_collider(c)), 0, X.shape[0], X, out)
+    return out
+
+
+def level_set_normal(c, X):
+    """Material-space normals of the general collider c at X [n, 3], float64."""
+    X = np.ascontiguousarray(X, np.float64).reshape(-1, 3)
+    out = np.zeros_like(X)
+    lib().zso_level_set_normal_f64(C.byref(_one_collider(c)), 0, X.shape[0], X, out)
+    return out
+
+
+def resolve_general(c, x, v, dtype=np.float64):
+    """Collider::resolveCollision of c on world-space nodes x [n, 3] with velocities v [n, 3]; returns the new v."""
+    x = np.ascontiguousarray(x, dtype).reshape(-1, 3)
+    v = np.ascontiguousarray(v, dtype).reshape(-1, 3).copy()
+    fn = lib().zso_resolve_general_f64 if dtype == np.float64 else lib().zso_resolve_general_f32
+    fn(C.byref(_one_collider(c)), 0, x.shape[0], x, v)
+    return v
 
 
 def stress_sand(vol, mu, lam, coh, beta, ys, volcorr, logJp, F):

@@ -25,13 +25,25 @@ typedef struct {
   double col_origin[8][3];
   double col_normal[8][3];
   int32_t col_type[8];
+  int32_t ngcols;  /* general colliders, applied after the planes */
+  struct {
+    int32_t type, geometry, axis;
+    double p0[3], p1[3], R[9], omega[3], s, dsdt, b[3], dbdt[3];
+  } gcols[8];
 } zso_params;
 }
 
 namespace {
 
 template <typename T>
-StepParams<T> to_params(const zso_params* p, std::vector<PlaneCollider<T>>& cols) {
+struct ColStore {
+  std::vector<PlaneCollider<T>> planes;
+  std::vector<GeneralCollider<T>> general;
+};
+
+template <typename T>
+StepParams<T> to_params(const zso_params* p, ColStore<T>& store) {
+  std::vector<PlaneCollider<T>>& cols = store.planes;
   StepParams<T> sp;
   sp.dt = (T)p->dt;
   sp.dx = (T)p->dx;
@@ -55,13 +67,33 @@ StepParams<T> to_params(const zso_params* p, std::vector<PlaneCollider<T>>& cols
   }
   sp.cols = cols.data();
   sp.ncols = p->ncols;
+  std::vector<GeneralCollider<T>>& gc = store.general;
+  gc.resize(p->ngcols);
+  for (int c = 0; c < p->ngcols; ++c) {
+    const auto& q = p->gcols[c];
+    gc[c].type = q.type;
+    gc[c].geometry = q.geometry;
+    gc[c].axis = q.axis;
+    for (int d = 0; d < 3; ++d) {
+      gc[c].p0[d] = (T)q.p0[d];
+      gc[c].p1[d] = (T)q.p1[d];
+      gc[c].omega[d] = (T)q.omega[d];
+      gc[c].b[d] = (T)q.b[d];
+      gc[c].dbdt[d] = (T)q.dbdt[d];
+    }
+    for (int k = 0; k < 9; ++k) gc[c].R[k] = (T)q.R[k];
+    gc[c].s = (T)q.s;
+    gc[c].dsdt = (T)q.dsdt;
+  }
+  sp.gcols = gc.data();
+  sp.ngcols = p->ngcols;
   return sp;
 }
 
 template <typename T>
 struct Ctx {
   StepOutputs<T> out;
-  std::vector<PlaneCollider<T>> cols;
+  ColStore<T> cols;
 };
 
 template <typename T>
@@ -146,6 +178,30 @@ void zso_block_keys_f32(int64_t n, const float* x, float dx, int32_t blockLen, i
 // ---------------------------------------------------------------- contexts
 void* zso_ctx_new_f32() { return new Ctx<float>(); }
 void* zso_ctx_new_f64() { return new Ctx<double>(); }
+/// The general collider `idx` of prm (in T): phi of material-space points
+/// X [n, 3], and resolve_general on world-space nodes x [n, 3] with
+/// velocities v [n, 3] (updated in place).
+void zso_level_set_sdf_f64(const zso_params* prm, int32_t idx, int64_t n, const double* X, double* phi) {
+  ColStore<double> st;
+  auto sp = to_params<double>(prm, st);
+  for (int64_t i = 0; i < n; ++i) phi[i] = level_set_sdf(sp.gcols[idx], X + 3 * i);
+}
+void zso_level_set_normal_f64(const zso_params* prm, int32_t idx, int64_t n, const double* X, double* nrm) {
+  ColStore<double> st;
+  auto sp = to_params<double>(prm, st);
+  for (int64_t i = 0; i < n; ++i) level_set_normal(sp.gcols[idx], X + 3 * i, nrm + 3 * i);
+}
+void zso_resolve_general_f64(const zso_params* prm, int32_t idx, int64_t n, const double* x, double* v) {
+  ColStore<double> st;
+  auto sp = to_params<double>(prm, st);
+  for (int64_t i = 0; i < n; ++i) resolve_general(sp.gcols[idx], x + 3 * i, v + 3 * i);
+}
+void zso_resolve_general_f32(const zso_params* prm, int32_t idx, int64_t n, const float* x, float* v) {
+  ColStore<float> st;
+  auto sp = to_params<float>(prm, st);
+  for (int64_t i = 0; i < n; ++i) resolve_general(sp.gcols[idx], x + 3 * i, v + 3 * i);
+}
+
 void zso_ctx_free_f32(void* c) { delete static_cast<Ctx<float>*>(c); }
 void zso_ctx_free_f64(void* c) { delete static_cast<Ctx<double>*>(c); }
 
@@ -249,7 +305,7 @@ void zso_p2g_f32(void* c, const zso_params* prm, int64_t n, float* x, float* v, 
 void zso_p2g_f64_on(void* c32, const zso_params* prm, int64_t n, double* x, double* v, double* C, double* F,
                     double* m, double* logJp, double* grid_out) {
   auto* ctx = static_cast<Ctx<float>*>(c32);
-  std::vector<PlaneCollider<double>> cols;
+  ColStore<double> cols;
   auto sp = to_params<double>(prm, cols);
   const int64_t nb = ctx->out.table.cnt;
   for (int64_t i = 0; i < nb * kChannels * 64; ++i) grid_out[i] = 0.0;
@@ -267,6 +323,7 @@ float zso_grid_update_f32(void* c, const zso_params* prm) {
   float mv = 0;
   grid_velocity(o.gridv.data(), nb, sp.dt, sp.extf, mv);
   grid_boundary(o.gridv.data(), o.table, nb, sp.dx, sp.cols, sp.ncols);
+  grid_boundary_general(o.gridv.data(), o.table, nb, sp.dx, sp.gcols, sp.ngcols);
   o.maxVel = mv;
   return mv;
 }

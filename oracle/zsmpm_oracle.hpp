@@ -822,6 +822,135 @@ struct PlaneCollider {
   int type = kSticky;
 };
 
+enum Geometry : int { kPlane = 0, kCuboid = 1, kSphere = 2, kCylinder = 3 };
+
+/// Collider<AnalyticLevelSet<geometry>> with its rigid motion: the material
+/// point X of the level set sits at x = R s X + b (geometry/Collider.h:10-190;
+/// the level sets of geometry/AnalyticLevelSet.h:11-225). p0/p1: plane origin
+/// and normal; cuboid min and max; sphere center and p1[0] = radius; cylinder
+/// bottom, p1[0] = radius, p1[1] = length, along `axis`. R column-major.
+template <typename T>
+struct GeneralCollider {
+  int type = kSticky, geometry = kPlane, axis = 1;
+  T p0[3] = {0, 0, 0}, p1[3] = {0, 0, 0};
+  T R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  T omega[3] = {0, 0, 0};
+  T s = 1, dsdt = 0;
+  T b[3] = {0, 0, 0}, dbdt[3] = {0, 0, 0};
+};
+
+/// Signed distances of the analytic level sets in material space
+/// (AnalyticLevelSet.h: Plane :26-29, Cuboid :81-88, Sphere :142-145,
+/// Cylinder :180-207), evaluated in T in the order written there.
+template <typename T>
+T level_set_sdf(const GeneralCollider<T>& c, const T X[3]) {
+  switch (c.geometry) {
+    case kPlane:
+      return c.p1[0] * (X[0] - c.p0[0]) + c.p1[1] * (X[1] - c.p0[1]) + c.p1[2] * (X[2] - c.p0[2]);
+    case kCuboid: {
+      T point[3], mx = 0;
+      for (int d = 0; d < 3; ++d) {
+        const T center = (c.p0[d] + c.p1[d]) / (T)2;
+        point[d] = std::fabs(X[d] - center) - (c.p1[d] - c.p0[d]) / (T)2;
+        mx = d == 0 ? point[d] : std::max(mx, point[d]);
+      }
+      for (int d = 0; d < 3; ++d)
+        if (point[d] < 0) point[d] = 0;
+      const T len = std::sqrt(point[0] * point[0] + point[1] * point[1] + point[2] * point[2]);
+      return (mx < 0 ? mx : (T)0) + len;
+    }
+    case kSphere: {
+      const T o[3] = {X[0] - c.p0[0], X[1] - c.p0[1], X[2] - c.p0[2]};
+      return std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) - c.p1[0];
+    }
+    default: {  // cylinder
+      const int ax = c.axis;
+      T dr[2];
+      for (int k = 0, i = 0; k < 3; ++k)
+        if (k != ax) dr[i++] = X[k] - c.p0[k];
+      const T disR = std::sqrt(dr[0] * dr[0] + dr[1] * dr[1]);
+      const T r = c.p1[0], len = c.p1[1];
+      const bool outside = disR > r;
+      if (X[ax] < c.p0[ax]) {
+        const T disL = c.p0[ax] - X[ax];
+        return outside ? std::sqrt((disR - r) * (disR - r) + disL * disL) : disL;
+      }
+      if (X[ax] > c.p0[ax] + len) {
+        const T disL = X[ax] - (c.p0[ax] + len);
+        return outside ? std::sqrt((disR - r) * (disR - r) + disL * disL) : disL;
+      }
+      if (outside) return disR - r;
+      const T disL = std::min(c.p0[ax] + len - X[ax], X[ax] - c.p0[ax]);
+      return -std::min(disL, r - disR);
+    }
+  }
+}
+
+/// Normals in material space: the plane's normal as given, the sphere's
+/// outward direction (zero within 1e-7 squared of the center), central
+/// differences of step 1e-6, normalised, for the cuboid and the cylinder
+/// (AnalyticLevelSet.h :30-33, :89-103, :146-150, :208-222).
+template <typename T>
+void level_set_normal(const GeneralCollider<T>& c, const T X[3], T n[3]) {
+  if (c.geometry == kPlane) {
+    for (int d = 0; d < 3; ++d) n[d] = c.p1[d];
+    return;
+  }
+  if (c.geometry == kSphere) {
+    const T o[3] = {X[0] - c.p0[0], X[1] - c.p0[1], X[2] - c.p0[2]};
+    const T l2 = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+    if (l2 < (T)1e-7) {
+      n[0] = n[1] = n[2] = 0;
+      return;
+    }
+    const T l = std::sqrt(l2);
+    for (int d = 0; d < 3; ++d) n[d] = o[d] / l;
+    return;
+  }
+  const T eps = (T)1e-6;
+  T diff[3];
+  for (int i = 0; i < 3; ++i) {
+    T v1[3] = {X[0], X[1], X[2]}, v2[3] = {X[0], X[1], X[2]};
+    v1[i] = X[i] + eps;
+    v2[i] = X[i] - eps;
+    diff[i] = (level_set_sdf(c, v1) - level_set_sdf(c, v2)) / (eps + eps);
+  }
+  const T l = std::sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+  for (int d = 0; d < 3; ++d) n[d] = diff[d] / l;
+}
+
+/// Collider::resolveCollision (geometry/Collider.h:81-107) with erosion 0 and
+/// the analytic level sets' zero material velocity: where phi(X) < 0 the node
+/// takes the object velocity omega x (x - b) + (s'/s)(x - b) + b' (sticky), or
+/// loses the normal part of its velocity relative to it (slip; separate only
+/// when approaching).
+template <typename T>
+void resolve_general(const GeneralCollider<T>& c, const T x[3], T v[3]) {
+  const T xmb[3] = {x[0] - c.b[0], x[1] - c.b[1], x[2] - c.b[2]};
+  const T inv_s = (T)1 / c.s;
+  T X[3];
+  for (int i = 0; i < 3; ++i)
+    X[i] = (c.R[0 + 3 * i] * xmb[0] + c.R[1 + 3 * i] * xmb[1] + c.R[2 + 3 * i] * xmb[2]) * inv_s;
+  if (!(level_set_sdf(c, X) < (T)0)) return;
+  const T rate = c.dsdt * inv_s;
+  const T wx[3] = {c.omega[1] * xmb[2] - c.omega[2] * xmb[1], c.omega[2] * xmb[0] - c.omega[0] * xmb[2],
+                   c.omega[0] * xmb[1] - c.omega[1] * xmb[0]};
+  T vo[3];
+  for (int d = 0; d < 3; ++d) vo[d] = wx[d] + rate * xmb[d] + c.dbdt[d];
+  if (c.type == kSticky) {
+    for (int d = 0; d < 3; ++d) v[d] = vo[d];
+    return;
+  }
+  for (int d = 0; d < 3; ++d) v[d] -= vo[d];
+  T nm[3], n[3];
+  level_set_normal(c, X, nm);
+  for (int r = 0; r < 3; ++r) n[r] = c.R[r] * nm[0] + c.R[r + 3] * nm[1] + c.R[r + 6] * nm[2];
+  const T proj = n[0] * v[0] + n[1] * v[1] + n[2] * v[2];
+  if ((c.type == kSeparate && proj < 0) || c.type == kSlip)
+    for (int d = 0; d < 3; ++d) v[d] -= proj * n[d];
+  for (int d = 0; d < 3; ++d) v[d] += vo[d];
+}
+
 // ----------------------------------------------------------------------------
 // transfer operators
 // ----------------------------------------------------------------------------
@@ -1001,6 +1130,25 @@ void grid_boundary(T* grid, const Table& t, i64 nblocks, T dx, const PlaneCollid
   }
 }
 
+/// ApplyBoundaryConditionOnGridBlocks (GridOp.hpp:112-164) for general
+/// colliders, once per collider in list order, on nodes with m > 0.
+template <typename T>
+void grid_boundary_general(T* grid, const Table& t, i64 nblocks, T dx, const GeneralCollider<T>* cols, int ncols) {
+  for (int ci = 0; ci < ncols; ++ci)
+    for (i64 b = 0; b < nblocks; ++b)
+      for (int c = 0; c < 64; ++c) {
+        if (!(grid[gidx(b, 0, c)] > 0)) continue;
+        const i32 cc[3] = {(c >> 4) & 3, (c >> 2) & 3, c & 3};
+        T pos[3], v[3];
+        for (int d = 0; d < 3; ++d) {
+          pos[d] = ((T)t.active[3 * b + d] * (T)4 + (T)cc[d]) * dx;
+          v[d] = grid[gidx(b, 1 + d, c)];
+        }
+        resolve_general(cols[ci], pos, v);
+        for (int d = 0; d < 3; ++d) grid[gidx(b, 1 + d, c)] = v[d];
+      }
+}
+
 /// GridAngularMomentum over Collapse{nBlocks, 64}, simulation/grid/GridOp.hpp:216-259:
 /// for every node with m != 0, x = (blockkey * 4 + coord(cell)) * dx and
 /// x cross mv are formed in T; sum[0..2] += (double)(x cross mv),
@@ -1075,6 +1223,8 @@ struct StepParams {
   Material<T> mat;
   const PlaneCollider<T>* cols = nullptr;
   int ncols = 0;
+  const GeneralCollider<T>* gcols = nullptr;  // applied after the planes, in order
+  int ngcols = 0;
   i32 tableSize = 0;
 };
 
@@ -1142,6 +1292,7 @@ void substep(const StepParams<T>& sp, ParticlesView<T> ps, StepOutputs<T>& out,
   T maxVel = 0;
   grid_velocity(grid.data(), nb, sp.dt, sp.extf, maxVel);
   grid_boundary(grid.data(), t, nb, sp.dx, sp.cols, sp.ncols);
+  grid_boundary_general(grid.data(), t, nb, sp.dx, sp.gcols, sp.ngcols);
   out.maxVel = maxVel;
   g2p(sp.dt, sp.dx, ps, t, grid.data());
   if (keep_grids) out.gridv = std::move(grid);

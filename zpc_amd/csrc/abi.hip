@@ -37,6 +37,8 @@ struct zsmpm_policy_s {
   SceneParams* scene_p = nullptr;   // [kMaxScenes]
   int64_t* scene_first = nullptr;   // [kMaxScenes + 1]
   std::vector<unsigned char> scene_bytes;
+  // general colliders of every grid update (zsmpm_policy_colliders__rocm)
+  GeneralColliders gcols{};
   // zsmpm_table_activate_noalloc__rocm ran on this policy: its device-side
   // errors are in the device's fixed scratch array
   bool noalloc_used = false;
@@ -574,7 +576,7 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_GRID_UPDATE);
-    ZS_HIP(zsmpm::grid_update(grid->data, grid->capacity_blocks, t, dt, g3, cols, dx, 0, ws, s, ss));
+    ZS_HIP(zsmpm::grid_update(grid->data, grid->capacity_blocks, t, dt, g3, cols, dx, 0, ws, s, ss, &pol->gcols));
   }
   {
     StageScope sc(pol, ZSMPM_STAGE_G2P);
@@ -632,6 +634,7 @@ int32_t capture_pair(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_
   key_put(key, &pol->closure);
   key_put(key, &pol->tuned_p2g);
   key_put(key, &pol->tuned_g2p);
+  key_put(key, &pol->gcols);
   if (pol->gexec && key == pol->gkey) return ZSMPM_OK;
   destroy_graph(pol);
   ZS_HIP(ensure_side(pol));
@@ -1120,7 +1123,8 @@ int32_t zsmpm_grid_update__rocm(zsmpm_policy pol, const zsmpm_grid* grid, const 
   ZS_TRY(ensure(pol, pol->reserved_n, t.T));
   StageScope sc(pol, ZSMPM_STAGE_GRID_UPDATE);
   set_maxvel_kernel<<<1, 1, 0, pol->stream>>>(pol->ws.counters);
-  ZS_HIP(zsmpm::grid_update(grid->data, grid->capacity_blocks, t, dt, extf, cols, grid->dx, fold_rhs, pol->ws, pol->stream));
+  ZS_HIP(zsmpm::grid_update(grid->data, grid->capacity_blocks, t, dt, extf, cols, grid->dx, fold_rhs, pol->ws,
+                            pol->stream, nullptr, &pol->gcols));
   if (max_vel_sqr) copy_maxvel_kernel<<<1, 1, 0, pol->stream>>>(pol->ws.counters, max_vel_sqr);
   return from_hip(hipGetLastError());
 }
@@ -1225,6 +1229,39 @@ int32_t zsmpm_substeps_batched__rocm(zsmpm_policy pol, const zsmpm_batch_scene* 
   if (nsteps == 0) return ZSMPM_OK;
   ZS_TRY(ensure(pol, pa.n, table->table_size));
   return substeps_driver(pol, nullptr, a, b, table, buckets, grid, 0.f, nullptr, nullptr, 0, nsteps, flags, &ss);
+}
+
+int32_t zsmpm_policy_colliders__rocm(zsmpm_policy pol, const zsmpm_collider* colliders, int32_t n) {
+  if (!pol || n < 0 || n > kMaxColliders || (n > 0 && !colliders)) return ZSMPM_INVALID_ARGUMENT;
+  GeneralColliders g{};
+  for (int32_t i = 0; i < n; ++i) {
+    const zsmpm_collider& c = colliders[i];
+    if (c.type < ZSMPM_STICKY || c.type > ZSMPM_SEPARATE || c.geometry < ZSMPM_GEOMETRY_PLANE ||
+        c.geometry > ZSMPM_GEOMETRY_CYLINDER || c.axis < 0 || c.axis > 2 || !(c.s > 0.f) || !isfinite(c.s) ||
+        !isfinite(c.dsdt))
+      return ZSMPM_INVALID_ARGUMENT;
+    const float* vals[] = {c.p0, c.p1, c.omega, c.b, c.dbdt};
+    for (const float* v : vals)
+      for (int d = 0; d < 3; ++d)
+        if (!isfinite(v[d])) return ZSMPM_INVALID_ARGUMENT;
+    for (int k = 0; k < 9; ++k)
+      if (!isfinite(c.R[k])) return ZSMPM_INVALID_ARGUMENT;
+    GeneralCollider& o = g.c[i];
+    o.type = c.type;
+    o.geometry = c.geometry;
+    o.axis = c.axis;
+    memcpy(o.p0, c.p0, sizeof o.p0);
+    memcpy(o.p1, c.p1, sizeof o.p1);
+    memcpy(o.R, c.R, sizeof o.R);
+    memcpy(o.omega, c.omega, sizeof o.omega);
+    o.s = c.s;
+    o.dsdt = c.dsdt;
+    memcpy(o.b, c.b, sizeof o.b);
+    memcpy(o.dbdt, c.dbdt, sizeof o.dbdt);
+  }
+  g.n = n;
+  pol->gcols = g;
+  return ZSMPM_OK;
 }
 
 int32_t zsmpm_policy_launch_widths__rocm(zsmpm_policy pol, int32_t* p2g_groups, int32_t* g2p_groups) {

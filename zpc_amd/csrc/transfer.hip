@@ -719,14 +719,135 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
 // then the colliders in order on nodes with m > 0 (GridOp.hpp:112-164,
 // Collider.h:80-107). fold adds the rhs channels first (the explicit
 // driver's convention when P2G kept them separate).
+// Signed distance of a general collider's level set at the material-space
+// point X (AnalyticLevelSet.h: Plane :26-29, Cuboid :81-88, Sphere :142-145,
+// Cylinder :180-207), rounded as written.
+__device__ __forceinline__ float level_set_sdf(const GeneralCollider& c, const float (&X)[3]) {
+#pragma clang fp contract(off)
+  if (c.geometry == 0) {
+    return c.p1[0] * (X[0] - c.p0[0]) + c.p1[1] * (X[1] - c.p0[1]) + c.p1[2] * (X[2] - c.p0[2]);
+  }
+  if (c.geometry == 1) {
+    float point[3], mx = 0.f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float center = (c.p0[d] + c.p1[d]) / 2.f;
+      point[d] = fabsf(X[d] - center) - (c.p1[d] - c.p0[d]) / 2.f;
+      mx = d == 0 ? point[d] : fmaxf(mx, point[d]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (point[d] < 0.f) point[d] = 0.f;
+    const float len = sqrtf(point[0] * point[0] + point[1] * point[1] + point[2] * point[2]);
+    return (mx < 0.f ? mx : 0.f) + len;
+  }
+  if (c.geometry == 2) {
+    const float o0 = X[0] - c.p0[0], o1 = X[1] - c.p0[1], o2 = X[2] - c.p0[2];
+    return sqrtf(o0 * o0 + o1 * o1 + o2 * o2) - c.p1[0];
+  }
+  const int ax = c.axis;
+  const float dr0 = X[ax == 0 ? 1 : 0] - c.p0[ax == 0 ? 1 : 0];
+  const float dr1 = X[ax == 2 ? 1 : 2] - c.p0[ax == 2 ? 1 : 2];
+  const float disR = sqrtf(dr0 * dr0 + dr1 * dr1);
+  const float r = c.p1[0], len = c.p1[1];
+  const bool outside = disR > r;
+  if (X[ax] < c.p0[ax]) {
+    const float disL = c.p0[ax] - X[ax];
+    return outside ? sqrtf((disR - r) * (disR - r) + disL * disL) : disL;
+  }
+  if (X[ax] > c.p0[ax] + len) {
+    const float disL = X[ax] - (c.p0[ax] + len);
+    return outside ? sqrtf((disR - r) * (disR - r) + disL * disL) : disL;
+  }
+  if (outside) return disR - r;
+  const float disL = fminf(c.p0[ax] + len - X[ax], X[ax] - c.p0[ax]);
+  return -fminf(disL, r - disR);
+}
+
+// Material-space normal (AnalyticLevelSet.h :30-33, :89-103, :146-150,
+// :208-222): the plane's normal as given, the sphere's outward direction
+// (zero within 1e-7 squared of the center), normalised central differences
+// of step 1e-6 for the cuboid and the cylinder.
+__device__ __forceinline__ void level_set_normal(const GeneralCollider& c, const float (&X)[3], float (&n)[3]) {
+#pragma clang fp contract(off)
+  if (c.geometry == 0) {
+    n[0] = c.p1[0];
+    n[1] = c.p1[1];
+    n[2] = c.p1[2];
+    return;
+  }
+  if (c.geometry == 2) {
+    const float o[3] = {X[0] - c.p0[0], X[1] - c.p0[1], X[2] - c.p0[2]};
+    const float l2 = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+    if (l2 < 1e-7f) {
+      n[0] = n[1] = n[2] = 0.f;
+      return;
+    }
+    const float l = sqrtf(l2);
+    n[0] = o[0] / l;
+    n[1] = o[1] / l;
+    n[2] = o[2] / l;
+    return;
+  }
+  const float eps = 1e-6f;
+  float diff[3];
+  for (int i = 0; i < 3; ++i) {
+    float v1[3] = {X[0], X[1], X[2]}, v2[3] = {X[0], X[1], X[2]};
+    v1[i] = X[i] + eps;
+    v2[i] = X[i] - eps;
+    diff[i] = (level_set_sdf(c, v1) - level_set_sdf(c, v2)) / (eps + eps);
+  }
+  const float l = sqrtf(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+  n[0] = diff[0] / l;
+  n[1] = diff[1] / l;
+  n[2] = diff[2] / l;
+}
+
+// Collider::resolveCollision (Collider.h:81-107) with erosion 0 and the
+// analytic level sets' zero material velocity.
+__device__ __forceinline__ void resolve_general(const GeneralCollider& c, const float (&x)[3], float (&v)[3]) {
+#pragma clang fp contract(off)
+  const float xmb[3] = {x[0] - c.b[0], x[1] - c.b[1], x[2] - c.b[2]};
+  const float inv_s = 1.f / c.s;
+  float X[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    X[i] = (c.R[0 + 3 * i] * xmb[0] + c.R[1 + 3 * i] * xmb[1] + c.R[2 + 3 * i] * xmb[2]) * inv_s;
+  if (!(level_set_sdf(c, X) < 0.f)) return;
+  const float rate = c.dsdt * inv_s;
+  const float wx[3] = {c.omega[1] * xmb[2] - c.omega[2] * xmb[1], c.omega[2] * xmb[0] - c.omega[0] * xmb[2],
+                       c.omega[0] * xmb[1] - c.omega[1] * xmb[0]};
+  float vo[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) vo[d] = wx[d] + rate * xmb[d] + c.dbdt[d];
+  if (c.type == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[d] = vo[d];
+    return;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) v[d] -= vo[d];
+  float nm[3], n[3];
+  level_set_normal(c, X, nm);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) n[r] = c.R[r] * nm[0] + c.R[r + 3] * nm[1] + c.R[r + 6] * nm[2];
+  const float proj = n[0] * v[0] + n[1] * v[1] + n[2] * v[2];
+  if ((c.type == 2 && proj < 0.f) || c.type == 1) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[d] -= proj * n[d];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) v[d] += vo[d];
+}
+
 // Node velocity and colliders of one node (mass != 0) of the block with key
 // k; cols is the kernel argument or a scene's colliders in a batch.
 // kLocal: k0_local is the key's first component with a batch's scene offset
 // removed; otherwise the table's own key is read.
 template <bool kLocal = false, typename Cols>
 __device__ __forceinline__ void node_velocity(float* gp, float mass, int fold, float e0, float e1, float e2,
-                                              const Cols& cols, const TableView& t, int32_t b, int32_t k0_local,
-                                              int cell, float dx, float& vmax) {
+                                              const Cols& cols, const GeneralColliders& gen, const TableView& t,
+                                              int32_t b, int32_t k0_local, int cell, float dx, float& vmax) {
   // rounded as written (no FMA contraction): a node on a collider plane has
   // phi = 0 in exact arithmetic, and the sign of the rounded phi must be the
   // one the host path computes
@@ -740,7 +861,7 @@ __device__ __forceinline__ void node_velocity(float* gp, float mass, int fold, f
   }
   float v[3] = {mv0 * inv + e0, mv1 * inv + e1, mv2 * inv + e2};
   vmax = fmaxf(vmax, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-  if (mass > 0.f && cols.n > 0) {
+  if (mass > 0.f && (cols.n > 0 || gen.n > 0)) {
     const int32_t k0 = kLocal ? k0_local : t.active[3 * (int64_t)b];
     const int32_t k1 = t.active[3 * (int64_t)b + 1], k2 = t.active[3 * (int64_t)b + 2];
     const float pos[3] = {((float)k0 * 4.f + (float)((cell >> 4) & 3)) * dx,
@@ -763,6 +884,7 @@ __device__ __forceinline__ void node_velocity(float* gp, float mass, int fold, f
         }
       }
     }
+    for (int c = 0; c < gen.n; ++c) resolve_general(gen.c[c], pos, v);
   }
   gp[1 * kBlockCells] = v[0];
   gp[2 * kBlockCells] = v[1];
@@ -773,7 +895,8 @@ template <bool kMulti>
 __global__ __launch_bounds__(kThreads) void grid_update_kernel(float* __restrict__ grid, int32_t grid_cap,
                                                                TableView t, float dt, float g0, float g1,
                                                                float g2, Colliders cols, float dx, int fold,
-                                                               uint32_t* counters, uint32_t* err, SceneSet ss) {
+                                                               uint32_t* counters, uint32_t* err, SceneSet ss,
+                                                               GeneralColliders gen) {
   int32_t nb = *t.cnt;
   if (nb > grid_cap) {
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, kErrCapacity);
@@ -797,10 +920,10 @@ __global__ __launch_bounds__(kThreads) void grid_update_kernel(float* __restrict
           const int32_t sc = scene_of_key(k0, ss.n);
           const SceneParams& sp = ss.p[sc];
           k0 -= sc * kSceneKeyStride;
-          node_velocity<true>(gp, mass, fold, sp.g[0] * sp.dt, sp.g[1] * sp.dt, sp.g[2] * sp.dt, sp.cols, t, b, k0,
-                              cell, sp.dx, vmax);
+          node_velocity<true>(gp, mass, fold, sp.g[0] * sp.dt, sp.g[1] * sp.dt, sp.g[2] * sp.dt, sp.cols, gen, t, b,
+                              k0, cell, sp.dx, vmax);
         } else {
-          node_velocity(gp, mass, fold, e0, e1, e2, cols, t, b, 0, cell, dx, vmax);
+          node_velocity(gp, mass, fold, e0, e1, e2, cols, gen, t, b, 0, cell, dx, vmax);
         }
       } else if (fold) {
         // the fold covers every node, massless ones included (their
@@ -1189,15 +1312,16 @@ hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortVi
 
 hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float dt, const float extf[3],
                        const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s,
-                       const SceneSet* ss) {
+                       const SceneSet* ss, const GeneralColliders* gen) {
+  const GeneralColliders g = gen ? *gen : GeneralColliders{};
   if (ss && ss->n > 0)
     grid_update_kernel<true><<<persistent_groups(), kThreads, 0, s>>>(grid, grid_cap, t, dt, extf[0], extf[1],
                                                                      extf[2], cols, dx, fold_rhs, ws.counters,
-                                                                     ws.err, *ss);
+                                                                     ws.err, *ss, g);
   else
     grid_update_kernel<false><<<persistent_groups(), kThreads, 0, s>>>(grid, grid_cap, t, dt, extf[0], extf[1],
                                                                       extf[2], cols, dx, fold_rhs, ws.counters,
-                                                                      ws.err, SceneSet{});
+                                                                      ws.err, SceneSet{}, g);
   return hipGetLastError();
 }
 

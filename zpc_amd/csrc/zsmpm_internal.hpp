@@ -51,6 +51,20 @@ struct Colliders {
   PlaneCollider c[kMaxColliders];
 };
 
+// Collider<AnalyticLevelSet<geometry>> with its rigid motion (zsmpm_collider).
+struct GeneralCollider {
+  int32_t type, geometry, axis;
+  float p0[3], p1[3];
+  float R[9];
+  float omega[3];
+  float s, dsdt;
+  float b[3], dbdt[3];
+};
+struct GeneralColliders {
+  int32_t n = 0;
+  GeneralCollider c[kMaxColliders];
+};
+
 // A batch of independent scenes in one table, one grid and one set of
 // particle arrays. Scene s owns the particles [first[s], first[s + 1]); its
 // block keys are shifted by s * kSceneKeyStride along axis 0, so no block is
@@ -268,9 +282,10 @@ hipError_t closure_launch_limits();
 hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
                        int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s,
                        bool reset_queue = true, const SceneSet* ss = nullptr);
+// gen: general colliders applied after the plane colliders (nullptr: none)
 hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float dt, const float extf[3],
                        const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s,
-                       const SceneSet* ss = nullptr);
+                       const SceneSet* ss = nullptr, const GeneralColliders* gen = nullptr);
 // GridAngularMomentum: sum[0..2] += x cross mv, sum[3..5] += mv over nodes with mass
 hipError_t grid_angular_momentum(const float* grid, int32_t grid_cap, const TableView& t, float dx, int mchn,
                                  int mvchn, double* sum, Workspace& ws, hipStream_t s);

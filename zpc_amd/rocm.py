@@ -84,6 +84,27 @@ class PlaneCollider(C.Structure):
     _fields_ = [("type", C.c_int32), ("origin", C.c_float * 3), ("normal", C.c_float * 3)]
 
 
+class ColliderDesc(C.Structure):
+    """zsmpm_collider: an analytic level-set collider with its rigid motion."""
+
+    _fields_ = [("type", C.c_int32), ("geometry", C.c_int32), ("axis", C.c_int32), ("p0", C.c_float * 3),
+                ("p1", C.c_float * 3), ("R", C.c_float * 9), ("omega", C.c_float * 3), ("s", C.c_float),
+                ("dsdt", C.c_float), ("b", C.c_float * 3), ("dbdt", C.c_float * 3)]
+
+
+def collider_desc(c) -> ColliderDesc:
+    """ColliderDesc of a scenes.GeneralCollider."""
+    d = ColliderDesc()
+    d.type, d.geometry, d.axis = int(c.kind), int(c.geometry), int(c.axis)
+    for k in range(3):
+        d.p0[k], d.p1[k] = float(c.p0[k]), float(c.p1[k])
+        d.omega[k], d.b[k], d.dbdt[k] = float(c.omega[k]), float(c.b[k]), float(c.dbdt[k])
+    for k in range(9):
+        d.R[k] = float(c.R[k])
+    d.s, d.dsdt = float(c.s), float(c.dsdt)
+    return d
+
+
 class BatchSceneDesc(C.Structure):
     _fields_ = [("hdr", Header), ("first", C.c_int64), ("count", C.c_int64), ("model", ModelDesc),
                 ("dx", C.c_float), ("dt", C.c_float), ("extf", C.c_float * 3), ("ncolliders", C.c_int32),
@@ -138,6 +159,7 @@ def lib():
                                         i32, i32, P(C.c_int32)],
         "zsmpm_policy_closure__rocm": [vp, P(ClosureDesc)],
         "zsmpm_policy_launch_widths__rocm": [vp, P(C.c_int32), P(C.c_int32)],
+        "zsmpm_policy_colliders__rocm": [vp, P(ColliderDesc), i32],
         "zsmpm_substeps_batched__rocm": [vp, P(BatchSceneDesc), i32, P(ParticlesDesc), P(ParticlesDesc),
                                          P(TableDesc), P(BucketsDesc), P(GridDesc), i32, i32, P(C.c_int32)],
         "zsmpm_stress__rocm": [vp, P(ModelDesc), vp, vp, i64, vp],
@@ -227,6 +249,13 @@ class Policy:
         d.flags = 1 if always_network else 0
         check(self.L.zsmpm_policy_closure__rocm(self.h, C.byref(d)), "closure")
         self._closure_params = params
+
+    def set_colliders(self, colliders):
+        """General colliders (scenes.GeneralCollider) of every later grid update
+        on this policy, after the call's plane colliders; [] removes them."""
+        cols = list(colliders or [])
+        arr = (ColliderDesc * max(len(cols), 1))(*[collider_desc(c) for c in cols])
+        check(self.L.zsmpm_policy_colliders__rocm(self.h, arr if cols else None, len(cols)), "colliders")
 
     def launch_widths(self):
         """(P2G, G2P) workgroup counts the policy launches with."""
@@ -396,6 +425,8 @@ class Simulator:
         self.extf = (C.c_float * 3)(*[float(g) for g in sc.gravity])
         self.pol.reserve(sc.n, T)
         self.L = lib()
+        if sc.general_colliders:
+            self.pol.set_colliders(sc.general_colliders)
 
     def substep(self, dt: Optional[float] = None):
         dt = float(self.sc.dt if dt is None else dt)

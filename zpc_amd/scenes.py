@@ -71,6 +71,31 @@ class Collider:
     kind: int = STICKY
 
 
+PLANE, CUBOID, SPHERE, CYLINDER = 0, 1, 2, 3
+
+
+@dataclasses.dataclass
+class GeneralCollider:
+    """Collider<AnalyticLevelSet<geometry>> with a rigid motion
+    (geometry/Collider.h:10-190, geometry/AnalyticLevelSet.h): the level set's
+    material point X sits at x = R s X + b, and its velocity there is
+    omega x (x - b) + (dsdt / s)(x - b) + dbdt. p0 / p1: plane origin and
+    normal, cuboid min and max, sphere center and p1[0] = radius, cylinder
+    bottom with p1[0] = radius and p1[1] = length along `axis`. R column-major."""
+
+    geometry: int
+    p0: Sequence[float]
+    p1: Sequence[float]
+    kind: int = STICKY
+    axis: int = 1
+    R: Sequence[float] = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+    omega: Sequence[float] = (0.0, 0.0, 0.0)
+    s: float = 1.0
+    dsdt: float = 0.0
+    b: Sequence[float] = (0.0, 0.0, 0.0)
+    dbdt: Sequence[float] = (0.0, 0.0, 0.0)
+
+
 @dataclasses.dataclass
 class Scene:
     name: str
@@ -87,6 +112,8 @@ class Scene:
     colliders: List[Collider] = dataclasses.field(default_factory=list)
     table_size: int = 0
     steps: int = 100
+    # applied after the plane colliders, in order (at most 8)
+    general_colliders: List[GeneralCollider] = dataclasses.field(default_factory=list)
 
     @property
     def n(self) -> int:
