@@ -158,7 +158,9 @@ def run_path(torch, rocm, sc, args, multi, graph, closure=None):
         # 8 untimed substeps pick the P2G / G2P workgroup counts, whatever
         # --warmup is; then the warm-up substeps
         tune = 8
-        sim.substeps(tune, autotune=True)
+        if args.workgroups:
+            sim.pol.set_launch_widths(*args.workgroups)
+        sim.substeps(tune, autotune=not args.workgroups)
         if args.warmup > 0:
             sim.substeps(args.warmup)
     else:
@@ -411,7 +413,13 @@ def main() -> int:
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (the reference kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
+    ap.add_argument("--workgroups", default="",
+                    help="P2G,G2P workgroup counts to launch with in place of the autotuned ones (the 8 "
+                         "tuning substeps still run, untuned), e.g. to hold a counter pass to a timed run's widths")
     args = ap.parse_args()
+    args.workgroups = tuple(int(v) for v in args.workgroups.split(",")) if args.workgroups else ()
+    if args.workgroups and len(args.workgroups) != 2:
+        ap.error("--workgroups takes two counts: P2G,G2P")
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "

@@ -314,6 +314,11 @@ int32_t zsmpm_policy_colliders__rocm(zsmpm_policy pol, const zsmpm_collider* col
  * defaults). No counterpart in the reference (its launch shapes are fixed). */
 int32_t zsmpm_policy_launch_widths__rocm(zsmpm_policy pol, int32_t* p2g_groups, int32_t* g2p_groups);
 
+/* Sets the P2G and G2P workgroup counts of later launches, as the autotuner
+ * would (0 restores the default; at most 1 << 20), e.g. to replay a tuned
+ * configuration in another process. No counterpart in the reference. */
+int32_t zsmpm_policy_set_launch_widths__rocm(zsmpm_policy pol, int32_t p2g_groups, int32_t g2p_groups);
+
 /* ------------------------------------------------------------------------
  * Batches of independent scenes (an addition: zpc advances one scene per
  * driver; no reference counterpart). The scenes share one set of particle

@@ -49,6 +49,11 @@ int main(int argc, char** argv) {
   int32_t wp = 0, wg = 0;
   expect("launch widths", zsmpm_policy_launch_widths__rocm(pol, &wp, &wg), 0);
   expect("autotuned widths are candidates", (wp == 512 || wp == 768 || wp == 1024 || wp == 2048) && (wg == 8192 || wg == 16384 || wg == 32768 || wg == 65536), 1);
+  expect("set widths", zsmpm_policy_set_launch_widths__rocm(pol, 300, 5000), 0);
+  expect("widths read back", zsmpm_policy_launch_widths__rocm(pol, &wp, &wg) == 0 && wp == 300 && wg == 5000, 1);
+  expect("negative width rejected", zsmpm_policy_set_launch_widths__rocm(pol, -1, 5000), ZSMPM_INVALID_ARGUMENT);
+  expect("huge width rejected", zsmpm_policy_set_launch_widths__rocm(pol, 1, (1 << 20) + 1), ZSMPM_INVALID_ARGUMENT);
+  expect("substeps with set widths", zsmpm_substeps_sorted__rocm(pol, &s0.model, &s0.a, &s0.b, &s0.t, &s0.bk, &s0.g, 1e-4f, g3, s0.cols, 6, 4, ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
   expect("graph + fuse rejected", zsmpm_substeps_sorted__rocm(pol, &s0.model, &s0.a, &s0.b, &s0.t, &s0.bk, &s0.g, 1e-4f, g3, s0.cols, 6, 2, ZSMPM_SUBSTEPS_GRAPH | ZSMPM_SUBSTEPS_FUSE_STRESS, &inb), ZSMPM_INVALID_ARGUMENT);
   // closure selection
   zsmpm_closure cl = mk<zsmpm_closure>();

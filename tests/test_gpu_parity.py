@@ -464,6 +464,28 @@ def test_autotuned_substeps(scene, request):
         check_field(f"{name} after 10 autotuned substeps", got[name], ps.by_id(name), ps64.by_id(name), 1e-4)
 
 
+def test_set_launch_widths_substeps(sand):
+    """Fixed P2G / G2P workgroup counts (zsmpm_policy_set_launch_widths__rocm),
+    odd ones included: P2G drains its work queue and G2P strides over the
+    buckets whatever the count, so the substeps stay on the oracle."""
+    rocm = _rocm()
+    sim = rocm.Simulator(sand, reorder=True)
+    sim.pol.set_launch_widths(37, 1000)
+    assert sim.pol.launch_widths() == (37, 1000)
+    sim.substeps(4, graph=True)
+    sim.sync()
+    got = sim.parts.host_by_id()
+    ps = _oracle_particles(sand)
+    O.Context().run(O.make_params(sand), ps, 4, reorder=True)
+    ps64 = _oracle_particles(sand).astype(np.float64)
+    O.Context(np.float64).run(O.make_params(sand), ps64, 4, reorder=True)
+    for name in ("x", "v", "F"):
+        check_field(f"{name} after 4 substeps at 37 / 1000 workgroups", got[name], ps.by_id(name),
+                    ps64.by_id(name), 1e-4)
+    with pytest.raises(rocm.ZsmpmError):
+        sim.pol.set_launch_widths(-1, 1000)
+
+
 def _first_occurrence(packed):
     """Distinct values of `packed` in order of first occurrence."""
     u, first = np.unique(packed, return_index=True)

@@ -1264,10 +1264,22 @@ int32_t zsmpm_policy_colliders__rocm(zsmpm_policy pol, const zsmpm_collider* col
   return ZSMPM_OK;
 }
 
+// P2G takes buckets from a work queue and G2P strides over them, so any
+// positive workgroup count is correct; the bound keeps a grid launchable.
+constexpr int32_t kMaxLaunchGroups = 1 << 20;
+
 int32_t zsmpm_policy_launch_widths__rocm(zsmpm_policy pol, int32_t* p2g_groups, int32_t* g2p_groups) {
   if (!pol || !p2g_groups || !g2p_groups) return ZSMPM_INVALID_ARGUMENT;
   *p2g_groups = pol->tuned_p2g > 0 ? pol->tuned_p2g : persistent_groups();
   *g2p_groups = pol->tuned_g2p > 0 ? pol->tuned_g2p : persistent_groups();
+  return ZSMPM_OK;
+}
+
+int32_t zsmpm_policy_set_launch_widths__rocm(zsmpm_policy pol, int32_t p2g_groups, int32_t g2p_groups) {
+  if (!pol || p2g_groups < 0 || g2p_groups < 0 || p2g_groups > kMaxLaunchGroups || g2p_groups > kMaxLaunchGroups)
+    return ZSMPM_INVALID_ARGUMENT;
+  pol->tuned_p2g = p2g_groups;
+  pol->tuned_g2p = g2p_groups;
   return ZSMPM_OK;
 }
 

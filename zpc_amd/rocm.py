@@ -159,6 +159,7 @@ def lib():
                                         i32, i32, P(C.c_int32)],
         "zsmpm_policy_closure__rocm": [vp, P(ClosureDesc)],
         "zsmpm_policy_launch_widths__rocm": [vp, P(C.c_int32), P(C.c_int32)],
+        "zsmpm_policy_set_launch_widths__rocm": [vp, C.c_int32, C.c_int32],
         "zsmpm_policy_colliders__rocm": [vp, P(ColliderDesc), i32],
         "zsmpm_substeps_batched__rocm": [vp, P(BatchSceneDesc), i32, P(ParticlesDesc), P(ParticlesDesc),
                                          P(TableDesc), P(BucketsDesc), P(GridDesc), i32, i32, P(C.c_int32)],
@@ -262,6 +263,10 @@ class Policy:
         p, g = C.c_int32(0), C.c_int32(0)
         check(self.L.zsmpm_policy_launch_widths__rocm(self.h, C.byref(p), C.byref(g)), "launch_widths")
         return p.value, g.value
+
+    def set_launch_widths(self, p2g: int, g2p: int):
+        """Fixes the P2G and G2P workgroup counts of later launches (0: default)."""
+        check(self.L.zsmpm_policy_set_launch_widths__rocm(self.h, int(p2g), int(g2p)), "set_launch_widths")
 
     def stage_times(self, reset: bool = False) -> dict:
         """{stage: (total ms, calls)} accumulated since the last reset."""
