@@ -389,6 +389,53 @@ def batched_leg(torch, rocm, args) -> dict:
     return out
 
 
+def adapters_leg(torch, rocm, sc, reps: int = 20):
+    """The layout adapters at the bench scene's size: the particles into and
+    out of a TileVector<f32, 32> with zpc's particle tags (m, x, v, F, C,
+    logJp, id; 27 channels, F and C row-major), and a random permutation of
+    its tiles (TileVector::reorderTiles). Each is timed with HIP events on the
+    policy's stream over `reps` launches; bytes are every attribute read once
+    and written once (108 B per particle each way, 2 x 108 B per element for
+    the tile reorder)."""
+    pol = rocm.Policy()
+    parts = rocm.DeviceParticles.from_scene(sc, ids=True)
+    tags = [("m", 1), ("x", 3), ("v", 3), ("F", 9), ("C", 9), ("logJp", 1), ("id", 1)]
+    tiles = rocm.DeviceTiles(tags, sc.n, 32)
+    other = rocm.DeviceTiles(tags, sc.n, 32)
+    back = parts.empty_like()
+    perm = torch.randperm(tiles.ntiles, device="cuda", dtype=torch.int64).to(torch.int32)
+    per_particle = 4 * tiles.nch
+
+    def timed(fn):
+        fn()
+        pol.sync()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(pol.stream)
+        for _ in range(reps):
+            fn()
+        b.record(pol.stream)
+        b.synchronize()
+        pol.sync()
+        return a.elapsed_time(b) / reps
+
+    out = {"workload": f"{sc.n} particles of the bench scene in a TileVector<f32, 32> of 27 channels "
+                       "(m, x, v, F, C, logJp, id)"}
+    for name, fn, nbytes in (
+            ("to_tiles", lambda: pol.particles_to_tiles(parts, tiles), 2 * per_particle * sc.n),
+            ("from_tiles", lambda: pol.particles_from_tiles(tiles, back), 2 * per_particle * sc.n),
+            ("tiles_reorder", lambda: pol.tiles_reorder(tiles, other, perm), 2 * 4 * tiles.data.numel())):
+        ms = timed(fn)
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "bytes": nbytes, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS}
+        print(f"# {name}: {ms:.3f} ms, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    h0, h1 = parts.host(), back.host()
+    out["round_trip_exact"] = all(bool((h0[k] == h1[k]).all()) for k in ("x", "v", "C", "F", "m", "logJp", "id"))
+    del parts, tiles, other, back
+    pol.close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -413,6 +460,7 @@ def main() -> int:
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (the reference kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
+    ap.add_argument("--no-adapters", action="store_true", help="skip the layout adapters' leg")
     ap.add_argument("--workgroups", default="",
                     help="P2G,G2P workgroup counts to launch with in place of the autotuned ones (the 8 "
                          "tuning substeps still run, untuned), e.g. to hold a counter pass to a timed run's widths")
@@ -471,6 +519,13 @@ def main() -> int:
         x_plain = None
         torch.cuda.empty_cache()
         batched = batched_leg(torch, rocm, args)
+    adapters = None
+    if not args.no_adapters:
+        if x_plain is not None:
+            del x_plain
+            x_plain = None
+            torch.cuda.empty_cache()
+        adapters = adapters_leg(torch, rocm, sc)
     per_launch = {k: (ms / calls if calls else 0.0) for k, (ms, calls) in stages.items()}
     dom = max(per_launch, key=lambda k: stages[k][0])
     dom_bytes = stage_bytes(dom, sc.n, blocks, plastic, sc.table_size)
@@ -536,6 +591,7 @@ def main() -> int:
         "stage_ms_per_launch": per_launch,
         "surrogate": surrogate,
         "batched": batched,
+        "adapters": adapters,
         "cpu_baseline": None,
     }
     del x_plain

@@ -46,7 +46,8 @@ enum zsmpm_result {
   ZSMPM_KEY_RANGE = -8,        /* a block key component outside [-2^20, 2^20) */
   ZSMPM_STENCIL_OUTSIDE = -9,  /* a particle stencil outside its bucket's 2x2x2 blocks */
   ZSMPM_MISSING_BLOCK = -10,   /* a needed block is not in the table (not activated/enlarged) */
-  ZSMPM_CAPACITY = -11         /* more blocks/particles than the buffers or reservation hold */
+  ZSMPM_CAPACITY = -11,        /* more blocks/particles than the buffers or reservation hold */
+  ZSMPM_IO_ERROR = -12         /* a file could not be opened, read or written, or is not a valid checkpoint */
 };
 
 /* Versioned, size-prefixed struct header (execution/AsyncRuntimeAbi.hpp:12-27). */
@@ -433,6 +434,99 @@ int32_t zsmpm_mlp_grad__rocm(zsmpm_policy pol, const float* params, const float*
 int32_t zsmpm_mlp_train_step__rocm(zsmpm_policy pol, float* params, float* adam_m, float* adam_v,
                                    const float* features, const float* records, int64_t n, float lr, int32_t step,
                                    float* loss);
+
+/* ------------------------------------------------------------------------
+ * Particle layout and IO adapters
+ * ---------------------------------------------------------------------- */
+
+/* Particles held in a TileVector<f32, L> (container/TileVector.hpp:15-1160):
+ * element i, channel c at data[(i / L * num_channels + c) * L + i % L]
+ * (TileVector.hpp:108,397,911-913). Each attribute is named by its first
+ * channel (the property's tag offset, getPropertyOffset), -1 when the tile
+ * set does not hold it; widths are x, v: 3, C, F: 9, m, logJp, id: 1 (id holds
+ * int32 bits). The channel ranges of the named attributes must not overlap.
+ * matrix_order ZSMPM_ROW_MAJOR: the 9 channels of C and F are a vec<f32,3,3>
+ * as TileVector's pack<3,3> / tuple<9> read and write it (channel 3r + c is
+ * entry (r, c)); ZSMPM_COLUMN_MAJOR: in the order of zsmpm_particles' rows. */
+enum zsmpm_matrix_order { ZSMPM_COLUMN_MAJOR = 0, ZSMPM_ROW_MAJOR = 1 };
+
+typedef struct zsmpm_tile_particles {
+  zsmpm_abi_header hdr;
+  int64_t n;            /* elements (TileVector::size()) */
+  int32_t lane_width;   /* L: a power of two in [1, 1024]; zpc's particle tiles use 32 */
+  int32_t num_channels; /* channels per element (TileVector::numChannels()) */
+  float* data;          /* [ceil(n / L) * num_channels * L] */
+  int32_t x, v, C, F, m, logJp, id; /* first channel of each attribute, or -1 */
+  int32_t matrix_order; /* zsmpm_matrix_order of the C and F channels */
+} zsmpm_tile_particles;
+
+/* dst's arrays <- the attributes src holds (every attribute with a channel
+ * >= 0 needs its dst array; dst arrays of absent attributes are left as they
+ * are); dst->n == src->n. One pass over HBM. zpc has no such adapter: its
+ * Particles keep one Vector per attribute (geometry/Structurefree.hpp:23-290)
+ * and its TileVector particle sets (particleBins, :226) are filled attribute
+ * by attribute through TileVector::pack / tuple. */
+int32_t zsmpm_particles_from_tiles__rocm(zsmpm_policy pol, const zsmpm_tile_particles* src,
+                                         const zsmpm_particles* dst);
+/* The reverse: the named channels of dst <- src's arrays; other channels and
+ * the lanes of the last tile past n are left as they are. */
+int32_t zsmpm_particles_to_tiles__rocm(zsmpm_policy pol, const zsmpm_particles* src,
+                                       const zsmpm_tile_particles* dst);
+
+/* TileVector::reorderTiles (container/TileVector.hpp:644-680): whole tiles of
+ * lane_width * num_channels floats; gather (scatter = 0): dst tile t = src
+ * tile map[t]; scatter: dst tile map[t] = src tile t. map holds num_tiles
+ * device int32 in [0, num_tiles) (ZSMPM_CAPACITY at the next sync
+ * otherwise); src and dst must not overlap. */
+int32_t zsmpm_tiles_reorder__rocm(zsmpm_policy pol, const float* src, float* dst, const int32_t* map,
+                                  int64_t num_tiles, int32_t lane_width, int32_t num_channels, int32_t scatter);
+
+/* Checkpoints: the particle arrays a descriptor names and, optionally, the
+ * HashTable, in one binary file for resume. Layout (little-endian): a 48-byte
+ * header (magic "ZSMPMCKP", u32 version 1, u32 field bits ZSMPM_CK_*, i64 n,
+ * i32 table size or 0, i32 0, f64 time, i64 step); the particle arrays in the
+ * order x, v, C, F, m, logJp, id (those present, raw); the table in zpc's
+ * serialisation order (keys, indices, status, table size, cnt, active keys;
+ * container/HashTable.hpp:576-590); a trailing u64 checksum (FNV-1a 64 over
+ * the preceding bytes as little-endian 64-bit words, the tail bytes singly).
+ * Save and load synchronise the policy's stream and return when the file is
+ * written or the device arrays hold its contents. */
+enum zsmpm_checkpoint_fields {
+  ZSMPM_CK_X = 1, ZSMPM_CK_V = 2, ZSMPM_CK_C = 4, ZSMPM_CK_F = 8,
+  ZSMPM_CK_M = 16, ZSMPM_CK_LOGJP = 32, ZSMPM_CK_ID = 64
+};
+
+typedef struct zsmpm_checkpoint_info {
+  int64_t n;
+  int32_t table_size; /* 0: no table in the file */
+  int32_t fields;     /* ZSMPM_CK_* bits of the particle arrays in the file */
+  double time;
+  int64_t step;
+} zsmpm_checkpoint_info;
+
+/* x, v, C, F and m are required; logJp and id are saved when set; table may
+ * be NULL. time and step are the caller's, returned by load. */
+int32_t zsmpm_checkpoint_save__rocm(zsmpm_policy pol, const char* path, const zsmpm_particles* particles,
+                                    const zsmpm_table* table, double time, int64_t step);
+/* The header of a checkpoint, to size the buffers of a load. */
+int32_t zsmpm_checkpoint_info__rocm(const char* path, zsmpm_checkpoint_info* out);
+/* Fills the caller's arrays: particles must name exactly the file's arrays
+ * with n equal to the file's, table (NULL when the file has none) the same
+ * table size (ZSMPM_INVALID_ARGUMENT otherwise). A bad magic, version or
+ * checksum, or a short file, is ZSMPM_IO_ERROR (the arrays may then hold part
+ * of the file). out may be NULL. */
+int32_t zsmpm_checkpoint_load__rocm(zsmpm_policy pol, const char* path, const zsmpm_particles* particles,
+                                    const zsmpm_table* table, zsmpm_checkpoint_info* out);
+
+/* write_partio / write_partio_with_stress / write_partio_with_grid
+ * (io/ParticleIO.hpp:10-24, io/ParticleIO.cpp:7-76): a Houdini bgeo file of n
+ * points at pos (device, 3 floats each) with, when attr is set, one point
+ * attribute `attr_name` of attr_width (1: float, as "stress"; 3: vector, as
+ * "force") floats per point (device). The layout is partio's bgeo writer's
+ * (version 5, big-endian, position as a homogeneous 4-vector), the file
+ * format zpc writes through partio. */
+int32_t zsmpm_write_partio__rocm(zsmpm_policy pol, const char* path, const float* pos, int64_t n,
+                                 const float* attr, int32_t attr_width, const char* attr_name);
 
 #ifdef __cplusplus
 }

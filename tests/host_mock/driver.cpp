@@ -112,6 +112,87 @@ int main(int argc, char** argv) {
     huge.table_size = 1 << 22;  // descriptor only: rejected before any access
     expect("noalloc table too large", zsmpm_table_activate_noalloc__rocm(pol, &huge, ba[0].x, ntot, 1.f / 128, 4, -2, 0.5f), ZSMPM_INSUFFICIENT_SIZE);
   }
+  // layout adapters: argument validation (no kernel runs under the mock)
+  {
+    Sc& s0 = sc[0];
+    const int64_t n = s0.a.n, L = 32, nch = 27;
+    std::vector<float> tiles((size_t)((n + L - 1) / L * nch * L));
+    zsmpm_tile_particles tp = mk<zsmpm_tile_particles>();
+    tp.n = n; tp.lane_width = (int32_t)L; tp.num_channels = (int32_t)nch; tp.data = tiles.data();
+    tp.x = 0; tp.v = 3; tp.C = 6; tp.F = 15; tp.m = 24; tp.logJp = 25; tp.id = 26; tp.matrix_order = ZSMPM_ROW_MAJOR;
+    expect("from tiles", zsmpm_particles_from_tiles__rocm(pol, &tp, &s0.b), 0);
+    expect("to tiles", zsmpm_particles_to_tiles__rocm(pol, &s0.a, &tp), 0);
+    zsmpm_tile_particles bad = tp; bad.F = 14;
+    expect("overlapping channels rejected", zsmpm_particles_from_tiles__rocm(pol, &bad, &s0.b), ZSMPM_INVALID_ARGUMENT);
+    bad = tp; bad.m = 27;
+    expect("channel past the end rejected", zsmpm_particles_from_tiles__rocm(pol, &bad, &s0.b), ZSMPM_INVALID_ARGUMENT);
+    bad = tp; bad.lane_width = 24;
+    expect("lane width not a power of two rejected", zsmpm_particles_from_tiles__rocm(pol, &bad, &s0.b), ZSMPM_INVALID_ARGUMENT);
+    bad = tp; bad.n = n - 1;
+    expect("count mismatch rejected", zsmpm_particles_from_tiles__rocm(pol, &bad, &s0.b), ZSMPM_INVALID_ARGUMENT);
+    bad = tp; bad.x = -1; bad.v = -1;
+    expect("absent attributes skipped", zsmpm_particles_from_tiles__rocm(pol, &bad, &s0.b), 0);
+    const int64_t nt = (n + L - 1) / L;
+    std::vector<float> tiles2(tiles.size());
+    std::vector<int32_t> map((size_t)nt);
+    for (int64_t t = 0; t < nt; ++t) map[t] = (int32_t)(nt - 1 - t);
+    expect("tile reorder", zsmpm_tiles_reorder__rocm(pol, tiles.data(), tiles2.data(), map.data(), nt, 32, 27, 0), 0);
+    expect("tile reorder in place rejected", zsmpm_tiles_reorder__rocm(pol, tiles.data(), tiles.data(), map.data(), nt, 32, 27, 1), ZSMPM_INVALID_ARGUMENT);
+  }
+  // checkpoint round trip through a real file (mock device memory is host memory)
+  {
+    Sc& s0 = sc[0];
+    const int64_t n = s0.a.n;
+    for (int64_t i = 0; i < 3 * n; ++i) { s0.a.x[i] = 0.25f * i; s0.a.v[i] = -1.f * i; }
+    for (int64_t i = 0; i < 9 * n; ++i) { s0.a.C[i] = 0.5f * i; s0.a.F[i] = 1.f + i; }
+    for (int64_t i = 0; i < n; ++i) { s0.a.m[i] = 2.f * i; s0.a.logJp[i] = -0.5f * i; s0.a.id[i] = (int32_t)(n - i); }
+    for (int32_t i = 0; i < 3 * s0.t.table_size; ++i) { s0.t.keys[i] = i - 7; s0.t.active_keys[i] = 3 * i; }
+    for (int32_t i = 0; i < s0.t.table_size; ++i) { s0.t.indices[i] = i ^ 5; s0.t.status[i] = -1; }
+    *s0.t.cnt = 1234;
+    const char* path = "zsmpm_ck_probe.bin";
+    expect("checkpoint save", zsmpm_checkpoint_save__rocm(pol, path, &s0.a, &s0.t, 1.5, 42), 0);
+    zsmpm_checkpoint_info info{};
+    expect("checkpoint info", zsmpm_checkpoint_info__rocm(path, &info), 0);
+    expect("info fields", info.n == n && info.table_size == s0.t.table_size && info.fields == 127 && info.time == 1.5 && info.step == 42, 1);
+    zsmpm_table t2 = s0.t;
+    std::vector<int32_t> k2(3 * s0.t.table_size), i2(s0.t.table_size), st2(s0.t.table_size), c2(1), a2(3 * s0.t.table_size);
+    t2.keys = k2.data(); t2.indices = i2.data(); t2.status = st2.data(); t2.cnt = c2.data(); t2.active_keys = a2.data();
+    zsmpm_checkpoint_info got{};
+    expect("checkpoint load", zsmpm_checkpoint_load__rocm(pol, path, &s0.b, &t2, &got), 0);
+    bool same = got.step == 42 && c2[0] == 1234;
+    for (int64_t i = 0; i < 3 * n; ++i) same = same && s0.b.x[i] == s0.a.x[i] && s0.b.v[i] == s0.a.v[i];
+    for (int64_t i = 0; i < 9 * n; ++i) same = same && s0.b.C[i] == s0.a.C[i] && s0.b.F[i] == s0.a.F[i];
+    for (int64_t i = 0; i < n; ++i) same = same && s0.b.m[i] == s0.a.m[i] && s0.b.logJp[i] == s0.a.logJp[i] && s0.b.id[i] == s0.a.id[i];
+    for (int32_t i = 0; i < 3 * s0.t.table_size; ++i) same = same && k2[i] == s0.t.keys[i] && a2[i] == s0.t.active_keys[i];
+    for (int32_t i = 0; i < s0.t.table_size; ++i) same = same && i2[i] == s0.t.indices[i] && st2[i] == s0.t.status[i];
+    expect("checkpoint contents", same, 1);
+    zsmpm_particles fewer = s0.b; fewer.n = n - 1;
+    expect("load into another count rejected", zsmpm_checkpoint_load__rocm(pol, path, &fewer, &t2, nullptr), ZSMPM_INVALID_ARGUMENT);
+    zsmpm_particles nolj = s0.b; nolj.logJp = nullptr;
+    expect("load without logJp rejected", zsmpm_checkpoint_load__rocm(pol, path, &nolj, &t2, nullptr), ZSMPM_INVALID_ARGUMENT);
+    expect("load without the table rejected", zsmpm_checkpoint_load__rocm(pol, path, &s0.b, nullptr, nullptr), ZSMPM_INVALID_ARGUMENT);
+    if (FILE* f = fopen(path, "r+b")) {  // flip one byte of the body
+      fseek(f, 48 + 100, SEEK_SET);
+      int c = fgetc(f);
+      fseek(f, 48 + 100, SEEK_SET);
+      fputc(c ^ 0x40, f);
+      fclose(f);
+    }
+    expect("corrupted checkpoint rejected", zsmpm_checkpoint_load__rocm(pol, path, &s0.b, &t2, nullptr), ZSMPM_IO_ERROR);
+    remove(path);
+    expect("missing file", zsmpm_checkpoint_info__rocm(path, &info), ZSMPM_IO_ERROR);
+    // bgeo: header + 16 bytes per point (+ the attribute) + 2
+    const char* geo = "zsmpm_probe.bgeo";
+    expect("bgeo positions", zsmpm_write_partio__rocm(pol, geo, s0.a.x, n, nullptr, 0, nullptr), 0);
+    FILE* f = fopen(geo, "rb");
+    fseek(f, 0, SEEK_END);
+    const long sz = ftell(f);
+    fclose(f);
+    expect("bgeo size", sz == 41 + 16 * n + 2, 1);
+    expect("bgeo stress", zsmpm_write_partio__rocm(pol, geo, s0.a.x, n, s0.a.m, 1, "stress"), 0);
+    expect("bgeo bad width", zsmpm_write_partio__rocm(pol, geo, s0.a.x, n, s0.a.m, 2, "stress"), ZSMPM_INVALID_ARGUMENT);
+    remove(geo);
+  }
   expect("sync", zsmpm_policy_sync(pol), 0);
   expect("del", zsmpm_del_policy__rocm(pol), 0);
   for (zsmpm_particles& p : ba) { free(p.x); free(p.v); free(p.C); free(p.F); free(p.m); free(p.logJp); free(p.id); }

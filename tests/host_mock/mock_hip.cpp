@@ -37,6 +37,13 @@ hipError_t hipMemsetAsync(void* p, int v, size_t n, hipStream_t s) {
   memset(p, v, n); return hipSuccess;
 }
 hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { memcpy(d, s, n); return hipSuccess; }
+hipError_t hipMemcpyAsync(void* d, const void* src, size_t n, hipMemcpyKind, hipStream_t s) {
+  chk_stream(s, "memcpyAsync");
+  if (s && S(s)->capture) { fprintf(stderr, "MOCK: memcpyAsync captured\n"); abort(); }
+  memcpy(d, src, n); return hipSuccess;
+}
+hipError_t hipHostMalloc(void** p, size_t n, unsigned int) { *p = malloc(n ? n : 1); return hipSuccess; }
+hipError_t hipHostFree(void* p) { free(p); return hipSuccess; }
 // host-only build: the device array's host shadow stands in for its memory
 hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return hipSuccess; }
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned int) {

@@ -6,7 +6,10 @@ graphs as checked handles, stream capture with CUDA's join rules) and no-op
 stand-ins for the kernels' translation units (tests/host_mock/stubs.cpp). The
 driver (tests/host_mock/driver.cpp) runs the policy life cycle, the sorted
 driver eagerly and from a captured graph, the closure selection, the batched
-scenes (scene table validation and upload, capture, autotuning) and teardown; every
+scenes (scene table validation and upload, capture, autotuning), the layout
+adapters' validation, a checkpoint round trip (io.hip, compiled host-only as
+well: its files are real, its device memory is the mock's host memory) with a
+corrupted and a mismatched load, the bgeo writer, and teardown; every
 call must return the documented code, and AddressSanitizer must report no
 invalid access and no leak. No kernel runs: this checks the host side only.
 """
@@ -38,9 +41,13 @@ def _build(tmp) -> str:
             "-Xarch_host", "-fsanitize=address", "-fno-omit-frame-pointer", "-Wno-unused-command-line-argument",
             f"-I{INC}", f"-I{CSRC}"]
     _run(host + ["-c", abi, "-o", "abi.o"], tmp)
+    io_src = open(os.path.join(CSRC, "io.hip")).read().replace('"../../include/zsmpm.h"', '"zsmpm.h"')
+    io = os.path.join(tmp, "io_host.hip")
+    open(io, "w").write(io_src)
+    _run(host + ["-c", io, "-o", "io.o"], tmp)
     _run(host + ["-x", "hip", "-c", os.path.join(MOCK, "stubs.cpp"), "-o", "stubs.o"], tmp)
     # the host-only object references its (absent) device code object by name
-    syms = _run(["nm", "-u", "abi.o"], tmp).stdout
+    syms = _run(["nm", "-u", "abi.o", "io.o"], tmp).stdout
     fat = sorted(set(re.findall(r"__hip_fatbin_[0-9a-f]+", syms)))
     open(os.path.join(tmp, "fatbin.c"), "w").write("".join(f"char {s}[64];\n" for s in fat))
     gxx = ["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address", "-fno-omit-frame-pointer"]
@@ -49,7 +56,7 @@ def _build(tmp) -> str:
                 "-o", "mock.o"], tmp)
     _run(gxx + [f"-I{INC}", "-c", os.path.join(MOCK, "driver.cpp"), "-o", "driver.o"], tmp)
     exe = os.path.join(tmp, "abi_host_probe")
-    _run(gxx + ["driver.o", "abi.o", "stubs.o", "mock.o", "fatbin.o", "-o", exe], tmp)
+    _run(gxx + ["driver.o", "abi.o", "io.o", "stubs.o", "mock.o", "fatbin.o", "-o", exe], tmp)
     return exe
 
 

@@ -1210,6 +1210,102 @@ int32_t zsmpm_reorder__rocm(zsmpm_policy pol, const zsmpm_buckets* buckets, cons
   return from_hip(reorder_particles(pi, po, sv.indices, pol->ws, pol->stream));
 }
 
+// The attributes a tile set names, checked against the arrays of p.
+static int32_t make_tile_attrs(const zsmpm_tile_particles* t, const zsmpm_particles* p, TileAttrs& a) {
+  ZS_TRY(check_header(t));
+  ZS_TRY(check_header(p));
+  if (t->n < 0 || t->n != p->n || !is_pow2(t->lane_width) || t->lane_width > 1024 || t->num_channels < 1 ||
+      (t->n > 0 && !t->data) || (t->matrix_order != ZSMPM_COLUMN_MAJOR && t->matrix_order != ZSMPM_ROW_MAJOR))
+    return ZSMPM_INVALID_ARGUMENT;
+  const int32_t tr = t->matrix_order == ZSMPM_ROW_MAJOR ? 1 : 0;
+  const TileAttr list[kTileAttrs] = {{t->x, 3, 0, p->x},     {t->v, 3, 0, p->v},
+                                     {t->C, 9, tr, p->C},    {t->F, 9, tr, p->F},
+                                     {t->m, 1, 0, p->m},     {t->logJp, 1, 0, p->logJp},
+                                     {t->id, 1, 0, (float*)p->id}};
+  std::vector<char> used((size_t)t->num_channels, 0);
+  a = TileAttrs{};
+  a.n = t->n;
+  a.nch = t->num_channels;
+  while ((1 << a.lane_shift) < t->lane_width) ++a.lane_shift;
+  for (const TileAttr& x : list) {
+    if (x.chn == -1) continue;
+    if (x.chn < 0 || x.chn > t->num_channels - x.width || (t->n > 0 && !x.soa)) return ZSMPM_INVALID_ARGUMENT;
+    for (int32_t c = x.chn; c < x.chn + x.width; ++c) {
+      if (used[c]) return ZSMPM_INVALID_ARGUMENT;
+      used[c] = 1;
+    }
+    a.attr[a.count++] = x;
+  }
+  return ZSMPM_OK;
+}
+
+int32_t zsmpm_particles_from_tiles__rocm(zsmpm_policy pol, const zsmpm_tile_particles* src,
+                                         const zsmpm_particles* dst) {
+  if (!pol || !src || !dst) return ZSMPM_INVALID_ARGUMENT;
+  TileAttrs a;
+  ZS_TRY(make_tile_attrs(src, dst, a));
+  DeviceGuard g(pol->devid);
+  return from_hip(tiles_convert(a, src->data, false, pol->stream));
+}
+
+int32_t zsmpm_particles_to_tiles__rocm(zsmpm_policy pol, const zsmpm_particles* src,
+                                       const zsmpm_tile_particles* dst) {
+  if (!pol || !src || !dst) return ZSMPM_INVALID_ARGUMENT;
+  TileAttrs a;
+  ZS_TRY(make_tile_attrs(dst, src, a));
+  DeviceGuard g(pol->devid);
+  return from_hip(tiles_convert(a, dst->data, true, pol->stream));
+}
+
+int32_t zsmpm_tiles_reorder__rocm(zsmpm_policy pol, const float* src, float* dst, const int32_t* map,
+                                  int64_t num_tiles, int32_t lane_width, int32_t num_channels, int32_t scatter) {
+  if (!pol || num_tiles < 0 || !is_pow2(lane_width) || lane_width > 1024 || num_channels < 1 ||
+      (num_tiles > 0 && (!src || !dst || !map)))
+    return ZSMPM_INVALID_ARGUMENT;
+  const int64_t tf = (int64_t)lane_width * num_channels;
+  if (num_tiles > 0 && overlaps(src, (size_t)(num_tiles * tf) * 4, dst, (size_t)(num_tiles * tf) * 4))
+    return ZSMPM_INVALID_ARGUMENT;
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, 0, 0));
+  return from_hip(tiles_reorder(src, dst, map, num_tiles, tf, scatter != 0, pol->ws.err, pol->stream));
+}
+
+int32_t zsmpm_checkpoint_save__rocm(zsmpm_policy pol, const char* path, const zsmpm_particles* particles,
+                                    const zsmpm_table* table, double time, int64_t step) {
+  if (!pol || !path) return ZSMPM_INVALID_ARGUMENT;
+  ParticlesView ps;
+  ZS_TRY(view_particles(particles, ps, true));
+  TableView t;
+  if (table) ZS_TRY(view_table(table, t));
+  DeviceGuard g(pol->devid);
+  return checkpoint_save(path, ps, table ? &t : nullptr, time, step, pol->stream);
+}
+
+int32_t zsmpm_checkpoint_info__rocm(const char* path, zsmpm_checkpoint_info* out) {
+  if (!path || !out) return ZSMPM_INVALID_ARGUMENT;
+  return checkpoint_info(path, out);
+}
+
+int32_t zsmpm_checkpoint_load__rocm(zsmpm_policy pol, const char* path, const zsmpm_particles* particles,
+                                    const zsmpm_table* table, zsmpm_checkpoint_info* out) {
+  if (!pol || !path) return ZSMPM_INVALID_ARGUMENT;
+  ParticlesView ps;
+  ZS_TRY(view_particles(particles, ps, true));
+  TableView t;
+  if (table) ZS_TRY(view_table(table, t));
+  DeviceGuard g(pol->devid);
+  return checkpoint_load(path, ps, table ? &t : nullptr, out, pol->stream);
+}
+
+int32_t zsmpm_write_partio__rocm(zsmpm_policy pol, const char* path, const float* pos, int64_t n,
+                                 const float* attr, int32_t attr_width, const char* attr_name) {
+  if (!pol || !path || n < 0 || (n > 0 && !pos)) return ZSMPM_INVALID_ARGUMENT;
+  if (attr && (!attr_name || !attr_name[0] || strlen(attr_name) > 0xffff || (attr_width != 1 && attr_width != 3)))
+    return ZSMPM_INVALID_ARGUMENT;
+  DeviceGuard g(pol->devid);
+  return write_bgeo(path, pos, n, attr, attr_width, attr_name, pol->stream);
+}
+
 int32_t zsmpm_substeps_batched__rocm(zsmpm_policy pol, const zsmpm_batch_scene* scenes, int32_t nscenes,
                                      const zsmpm_particles* a, const zsmpm_particles* b, const zsmpm_table* table,
                                      const zsmpm_buckets* buckets, const zsmpm_grid* grid, int32_t nsteps,

@@ -6,6 +6,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "../../include/zsmpm.h"
 #include "zsmpm_device.hpp"
 
 namespace zsmpm {
@@ -300,5 +301,37 @@ hipError_t reorder_particles(const ParticlesView& in, const ParticlesView& out, 
 
 // grid-stride launch width for kernels that loop over a device-resident count
 inline int persistent_groups() { return 4096; }
+
+// ---- particle layout and IO adapters (io.hip) ----
+/// One attribute of a TileVector<f32, L> particle set: its first channel, its
+/// width (1, 3 or 9) and the matching per-attribute array; transpose: the 9
+/// channels hold a vec<f32,3,3> row-major while the array row is column-major.
+struct TileAttr {
+  int32_t chn = 0;
+  int32_t width = 0;
+  int32_t transpose = 0;
+  float* soa = nullptr;
+};
+constexpr int kTileAttrs = 7;
+struct TileAttrs {
+  int64_t n = 0;
+  int32_t lane_shift = 0;  // log2 of the lane width L
+  int32_t nch = 0;         // channels per element
+  int32_t count = 0;
+  TileAttr attr[kTileAttrs];
+};
+// to_tiles: tiles <- arrays; else arrays <- tiles
+hipError_t tiles_convert(const TileAttrs& a, float* tiles, bool to_tiles, hipStream_t s);
+// gather: dst tile t = src tile map[t]; scatter: dst tile map[t] = src tile t
+hipError_t tiles_reorder(const float* src, float* dst, const int32_t* map, int64_t ntiles, int64_t tile_floats,
+                         bool scatter, uint32_t* err, hipStream_t s);
+// Host-side file formats; each synchronises `s` and returns a zsmpm result code.
+int32_t checkpoint_save(const char* path, const ParticlesView& p, const TableView* t, double time, int64_t step,
+                        hipStream_t s);
+int32_t checkpoint_info(const char* path, zsmpm_checkpoint_info* out);
+int32_t checkpoint_load(const char* path, const ParticlesView& p, const TableView* t, zsmpm_checkpoint_info* out,
+                        hipStream_t s);
+int32_t write_bgeo(const char* path, const float* pos, int64_t n, const float* attr, int32_t width,
+                   const char* name, hipStream_t s);
 
 }  // namespace zsmpm

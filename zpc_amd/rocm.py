@@ -25,7 +25,7 @@ ABI_MINOR = 0
 RESULTS = {
     0: "OK", -1: "ERROR", -2: "INCOMPATIBLE_MAJOR", -3: "INSUFFICIENT_SIZE", -4: "UNSUPPORTED_OPERATION",
     -5: "INVALID_ARGUMENT", -6: "DEVICE_ERROR", -7: "TABLE_FULL", -8: "KEY_RANGE", -9: "STENCIL_OUTSIDE",
-    -10: "MISSING_BLOCK", -11: "CAPACITY",
+    -10: "MISSING_BLOCK", -11: "CAPACITY", -12: "IO_ERROR",
 }
 
 
@@ -111,6 +111,22 @@ class BatchSceneDesc(C.Structure):
                 ("colliders", PlaneCollider * MAX_COLLIDERS)]
 
 
+COLUMN_MAJOR, ROW_MAJOR = 0, 1
+
+
+class TileParticlesDesc(C.Structure):
+    """zsmpm_tile_particles: particles in a TileVector<f32, L>."""
+
+    _fields_ = [("hdr", Header), ("n", C.c_int64), ("lane_width", C.c_int32), ("num_channels", C.c_int32),
+                ("data", C.c_void_p), ("x", C.c_int32), ("v", C.c_int32), ("C", C.c_int32), ("F", C.c_int32),
+                ("m", C.c_int32), ("logJp", C.c_int32), ("id", C.c_int32), ("matrix_order", C.c_int32)]
+
+
+class CheckpointInfo(C.Structure):
+    _fields_ = [("n", C.c_int64), ("table_size", C.c_int32), ("fields", C.c_int32), ("time", C.c_double),
+                ("step", C.c_int64)]
+
+
 _lib = None
 
 # zsmpm_stage order
@@ -163,6 +179,13 @@ def lib():
         "zsmpm_policy_colliders__rocm": [vp, P(ColliderDesc), i32],
         "zsmpm_substeps_batched__rocm": [vp, P(BatchSceneDesc), i32, P(ParticlesDesc), P(ParticlesDesc),
                                          P(TableDesc), P(BucketsDesc), P(GridDesc), i32, i32, P(C.c_int32)],
+        "zsmpm_particles_from_tiles__rocm": [vp, P(TileParticlesDesc), P(ParticlesDesc)],
+        "zsmpm_particles_to_tiles__rocm": [vp, P(ParticlesDesc), P(TileParticlesDesc)],
+        "zsmpm_tiles_reorder__rocm": [vp, vp, vp, vp, i64, i32, i32, i32],
+        "zsmpm_checkpoint_save__rocm": [vp, C.c_char_p, P(ParticlesDesc), P(TableDesc), C.c_double, i64],
+        "zsmpm_checkpoint_info__rocm": [C.c_char_p, P(CheckpointInfo)],
+        "zsmpm_checkpoint_load__rocm": [vp, C.c_char_p, P(ParticlesDesc), P(TableDesc), P(CheckpointInfo)],
+        "zsmpm_write_partio__rocm": [vp, C.c_char_p, vp, i64, vp, i32, C.c_char_p],
         "zsmpm_stress__rocm": [vp, P(ModelDesc), vp, vp, i64, vp],
         "zsmpm_mlp_harvest__rocm": [vp, P(ModelDesc), vp, vp, i64, vp, vp],
         "zsmpm_mlp_infer__rocm": [vp, vp, vp, i64, vp],
@@ -268,6 +291,51 @@ class Policy:
         """Fixes the P2G and G2P workgroup counts of later launches (0: default)."""
         check(self.L.zsmpm_policy_set_launch_widths__rocm(self.h, int(p2g), int(g2p)), "set_launch_widths")
 
+    # ---- layout and IO adapters ----
+    def particles_from_tiles(self, tiles: "DeviceTiles", parts: "DeviceParticles", matrix_order: int = ROW_MAJOR):
+        """parts' arrays <- the attributes the tile set holds (one HBM pass)."""
+        check(self.L.zsmpm_particles_from_tiles__rocm(self.h, C.byref(tiles.desc(parts, matrix_order)),
+                                                      C.byref(parts.desc)), "particles_from_tiles")
+
+    def particles_to_tiles(self, parts: "DeviceParticles", tiles: "DeviceTiles", matrix_order: int = ROW_MAJOR):
+        """The tile set's channels of parts' attributes <- parts."""
+        check(self.L.zsmpm_particles_to_tiles__rocm(self.h, C.byref(parts.desc),
+                                                    C.byref(tiles.desc(parts, matrix_order))), "particles_to_tiles")
+
+    def tiles_reorder(self, src: "DeviceTiles", dst: "DeviceTiles", tile_map, scatter: bool = False):
+        """TileVector::reorderTiles: gather dst tile t = src tile map[t]
+        (scatter: dst tile map[t] = src tile t); map a device int32 tensor."""
+        if (src.L, src.nch, src.n) != (dst.L, dst.nch, dst.n):
+            raise ValueError("tile sets of different shapes")
+        check(self.L.zsmpm_tiles_reorder__rocm(self.h, src.data.data_ptr(), dst.data.data_ptr(), tile_map.data_ptr(),
+                                               src.ntiles, src.L, src.nch, 1 if scatter else 0), "tiles_reorder")
+
+    def checkpoint_save(self, path: str, parts: "DeviceParticles", table: Optional["DeviceTable"] = None,
+                        time: float = 0.0, step: int = 0):
+        """Writes parts (and the table) to one checkpoint file; synchronises."""
+        check(self.L.zsmpm_checkpoint_save__rocm(self.h, os.fsencode(path), C.byref(parts.desc),
+                                                 None if table is None else C.byref(table.desc), float(time),
+                                                 int(step)), "checkpoint_save")
+
+    def checkpoint_load(self, path: str, parts: "DeviceParticles", table: Optional["DeviceTable"] = None):
+        """Fills parts (and the table) from a checkpoint; returns its info."""
+        info = CheckpointInfo()
+        check(self.L.zsmpm_checkpoint_load__rocm(self.h, os.fsencode(path), C.byref(parts.desc),
+                                                 None if table is None else C.byref(table.desc), C.byref(info)),
+              "checkpoint_load")
+        return {"n": info.n, "table_size": info.table_size, "fields": info.fields, "time": info.time,
+                "step": info.step}
+
+    def write_partio(self, path: str, pos, attr=None, attr_name: Optional[str] = None):
+        """write_partio (positions) / _with_stress (attr of n floats, named
+        "stress") / _with_grid (attr of n x 3, named "force"): a bgeo file.
+        pos and attr are device float32 tensors."""
+        n = int(pos.shape[0])
+        width = 0 if attr is None else (1 if attr.dim() == 1 else int(attr.shape[1]))
+        check(self.L.zsmpm_write_partio__rocm(self.h, os.fsencode(path), pos.data_ptr(), n,
+                                              None if attr is None else attr.data_ptr(), width,
+                                              None if attr_name is None else attr_name.encode()), "write_partio")
+
     def stage_times(self, reset: bool = False) -> dict:
         """{stage: (total ms, calls)} accumulated since the last reset."""
         n = len(STAGES)
@@ -359,6 +427,48 @@ class DeviceParticles:
             b[ids] = a
             out[k] = b
         return out
+
+
+def checkpoint_info(path: str) -> dict:
+    """The header of a checkpoint file (sizes its load)."""
+    info = CheckpointInfo()
+    check(lib().zsmpm_checkpoint_info__rocm(os.fsencode(path), C.byref(info)), "checkpoint_info")
+    return {"n": info.n, "table_size": info.table_size, "fields": info.fields, "time": info.time,
+            "step": info.step}
+
+
+class DeviceTiles:
+    """A TileVector<f32, L> of n elements: element i, channel c at
+    (i // L * nch + c) * L + i % L (container/TileVector.hpp:108,397). tags:
+    [(name, width)] in channel order, as zpc's PropertyTag lists; attributes
+    named x, v, C, F, m, logJp and id map to the particle arrays."""
+
+    NAMES = ("x", "v", "C", "F", "m", "logJp", "id")
+
+    def __init__(self, tags, n: int, lane_width: int = 32, device="cuda"):
+        torch = _torch()
+        self.tags = [(str(a), int(w)) for a, w in tags]
+        self.offsets, off = {}, 0
+        for a, w in self.tags:
+            self.offsets[a] = off
+            off += w
+        self.nch, self.L, self.n = off, int(lane_width), int(n)
+        self.ntiles = (self.n + self.L - 1) // self.L
+        self.data = torch.zeros(max(self.ntiles, 1) * self.nch * self.L, dtype=torch.float32, device=device)
+
+    def desc(self, parts=None, matrix_order: int = ROW_MAJOR) -> TileParticlesDesc:
+        d = TileParticlesDesc()
+        d.hdr = _hdr(TileParticlesDesc)
+        d.n, d.lane_width, d.num_channels, d.data = self.n, self.L, self.nch, self.data.data_ptr()
+        for a in self.NAMES:
+            named = a in self.offsets and (parts is None or getattr(parts, a) is not None)
+            setattr(d, a, self.offsets[a] if named else -1)
+        d.matrix_order = int(matrix_order)
+        return d
+
+    def host(self) -> np.ndarray:
+        """(ntiles, nch, L) view of the buffer on the host."""
+        return self.data.cpu().numpy().reshape(max(self.ntiles, 1), self.nch, self.L)
 
 
 class DeviceGrid:
@@ -535,6 +645,15 @@ class Simulator:
 
     def sync(self):
         self.pol.sync()
+
+    def save_checkpoint(self, path: str, time: float = 0.0, step: int = 0, table: bool = True):
+        """The particle state (and the table) as one checkpoint file."""
+        self.pol.checkpoint_save(path, self.parts, self.table if table else None, time, step)
+
+    def load_checkpoint(self, path: str) -> dict:
+        """Resumes from a checkpoint written for a scene of the same size."""
+        info = checkpoint_info(path)
+        return self.pol.checkpoint_load(path, self.parts, self.table if info["table_size"] else None)
 
     def nblocks(self) -> int:
         return int(self.table.cnt.item())
