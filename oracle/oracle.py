@@ -133,6 +133,20 @@ def lib():
             getattr(L, name).argtypes = [C.POINTER(Params), C.c_int32, C.c_int64, _f64p, _f64p]
         L.zso_level_set_normal_f64.argtypes = [C.POINTER(Params), C.c_int32, C.c_int64, _f64p, _f64p]
         L.zso_resolve_general_f32.argtypes = [C.POINTER(Params), C.c_int32, C.c_int64, _f32p, _f32p]
+        _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+        L.zso_mt19937.argtypes = [C.c_uint32, C.c_int64, _u32p]
+        L.zso_bht_hash_params.argtypes = [C.c_uint32, _u32p]
+        L.zso_bht_hash.argtypes = [C.c_uint32, C.c_uint32, C.c_int64, _i32p, _u32p]
+        L.zso_bht_table_size.argtypes = [C.c_int64]
+        L.zso_bht_table_size.restype = C.c_int64
+        L.zso_bht_insert.argtypes = [C.c_int32, _u32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, C.c_int64, _i32p,
+                                     _i32p]
+        L.zso_bht_query.argtypes = [C.c_int32, _u32p, _i32p, _i32p, C.c_int64, _i32p, _i32p]
+        L.zso_spg_block_keys.argtypes = [_f32p, C.c_int64, _f32p, _i32p]
+        spg = [C.c_int32, _u32p, _i32p, _i32p, C.c_int32, C.c_int32, _f32p, _f32p, _f32p, C.c_float]
+        L.zso_spg_wsample.argtypes = spg + [C.c_int32, C.c_int64, _f32p, _f32p]
+        L.zso_spg_normal.argtypes = spg + [C.c_int32, C.c_int64, _f32p, _f32p]
+        L.zso_levelset_resolve.argtypes = spg + [C.c_int32, C.c_int32, C.c_int32, C.c_int64, _f32p, _f32p, _i32p]
         _lib = L
     return _lib
 
@@ -173,6 +187,10 @@ def ref():
         L.zref_mat_diag_matT.argtypes = [_f32p, _f32p, _f32p, _f32p]
         L.zref_cfl_dt.argtypes = [C.c_float] * 5
         L.zref_cfl_dt.restype = C.c_float
+        _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+        if hasattr(L, "zref_universal_hash"):
+            L.zref_universal_hash.argtypes = [C.c_uint32, C.c_uint32, C.c_int64, _i32p, _u32p]
+            L.zref_mt19937.argtypes = [C.c_uint32, C.c_int64, _u32p]
         _ref = L
     return _ref
 
@@ -464,3 +482,107 @@ def omp_run(params, ps: HostParticles, warmup: int, steps: int, threads: int = 0
     secs = L.zso_omp_run_f32(C.byref(params), warmup, steps, nt, ps.n, ps.x, ps.v, ps.C, ps.F, ps.m,
                              _ptr(ps.logJp), C.byref(nb))
     return float(secs), int(nt), int(nb.value)
+
+
+# ----------------------------------------------------------------------------- bht / SparseGrid
+
+BHT_SEED = 2  # std::mt19937 rng(2) in bht's constructor (container/Bht.hpp:160-166)
+BHT_SENTINEL = 0x3F3F3F3F
+BHT_FAILURE = -(2 ** 31)
+
+
+def mt19937(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    lib().zso_mt19937(seed, n, out)
+    return out
+
+
+def bht_hash_params(seed: int = BHT_SEED) -> np.ndarray:
+    out = np.zeros(6, np.uint32)
+    lib().zso_bht_hash_params(seed, out)
+    return out
+
+
+def bht_hash(hx: int, hy: int, keys) -> np.ndarray:
+    k = np.ascontiguousarray(keys, np.int32).reshape(-1, 3)
+    out = np.zeros(len(k), np.uint32)
+    lib().zso_bht_hash(hx, hy, len(k), k, out)
+    return out
+
+
+def bht_table_size(n: int) -> int:
+    return int(lib().zso_bht_table_size(n))
+
+
+class HostBht:
+    """bht<i32, 3, int, 16> on the host with zpc's sequential insert."""
+
+    def __init__(self, table_size: int, hash_params=None):
+        T = int(table_size)
+        self.T = T
+        self.h = np.ascontiguousarray(bht_hash_params() if hash_params is None else hash_params, np.uint32)
+        self.keys = np.full(4 * T, BHT_SENTINEL, np.int32)
+        self.indices = np.full(T, -1, np.int32)
+        self.status = np.full(T, -1, np.int32)
+        self.cnt = np.zeros(1, np.int32)
+        self.active = np.zeros(3 * T, np.int32)
+        self.success = np.ones(1, np.int32)
+
+    def insert(self, keys) -> np.ndarray:
+        k = np.ascontiguousarray(keys, np.int32).reshape(-1)
+        out = np.zeros(len(k) // 3, np.int32)
+        lib().zso_bht_insert(self.T, self.h, self.keys, self.indices, self.status, self.cnt, self.active,
+                             self.success, len(out), k, out)
+        return out
+
+    def query(self, keys) -> np.ndarray:
+        k = np.ascontiguousarray(keys, np.int32).reshape(-1)
+        out = np.zeros(len(k) // 3, np.int32)
+        lib().zso_bht_query(self.T, self.h, self.keys, self.indices, len(out), k, out)
+        return out
+
+
+class HostSparseGrid:
+    """SparseGrid<3, f32, 8>: a HostBht of block origins, (blocks, nch, 512)
+    cell values, the index-to-world transform (4 x 4, row-vector convention)
+    and the background value."""
+
+    def __init__(self, table: HostBht, data, i2w, background: float = 0.0):
+        self.table = table
+        self.data = np.ascontiguousarray(data, np.float32)
+        self.nblocks, self.nch = self.data.shape[0], self.data.shape[1]
+        self.i2w = np.ascontiguousarray(i2w, np.float32).reshape(16)
+        self.w2i = np.linalg.inv(np.asarray(i2w, np.float64).reshape(4, 4)).astype(np.float32).reshape(16)
+        self.background = float(background)
+
+    def _args(self):
+        t = self.table
+        return (t.T, t.h, t.keys, t.indices, self.nblocks, self.nch, self.data.reshape(-1), self.i2w, self.w2i,
+                self.background)
+
+    def wsample(self, chn: int, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1)
+        out = np.zeros(len(x) // 3, np.float32)
+        lib().zso_spg_wsample(*self._args(), chn, len(out), x, out)
+        return out
+
+    def normal(self, sdf: int, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1)
+        out = np.zeros(len(x), np.float32)
+        lib().zso_spg_normal(*self._args(), sdf, len(x) // 3, x, out)
+        return out.reshape(-1, 3)
+
+    def resolve(self, sdf: int, vchn: int, kind: int, x, v):
+        """The level-set boundary on nodes at x with velocities v: (v', inside)."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1)
+        v = np.ascontiguousarray(v, np.float32).reshape(-1).copy()
+        inside = np.zeros(len(x) // 3, np.int32)
+        lib().zso_levelset_resolve(*self._args(), sdf, vchn, kind, len(inside), x, v, inside)
+        return v.reshape(-1, 3), inside.astype(bool)
+
+
+def spg_block_keys(w2i, x) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    out = np.zeros(len(x), np.int32)
+    lib().zso_spg_block_keys(np.ascontiguousarray(w2i, np.float32).reshape(16), len(x) // 3, x, out)
+    return out.reshape(-1, 3)

@@ -6,6 +6,7 @@
 #include <memory>
 
 #include "zsmpm_oracle.hpp"
+#include "bht_oracle.hpp"
 
 using namespace zso;
 
@@ -371,6 +372,103 @@ void zso_run_sorted_f64(void* c, const zso_params* prm, int32_t steps, int64_t n
 void zso_reorder_f32(int64_t n, const int32_t* map, float* x, float* v, float* C, float* F, float* m,
                      float* logJp, int32_t* ids) {
   reorder_particles(view<float>(n, x, v, C, F, m, logJp), map, ids);
+}
+
+// ---- bht, SparseGrid, level-set boundary (bht_oracle.hpp) ----
+
+/// n raw draws of MT19937 seeded with `seed`.
+void zso_mt19937(uint32_t seed, int64_t n, uint32_t* out) {
+  Mt19937 r(seed);
+  for (int64_t i = 0; i < n; ++i) out[i] = r();
+}
+void zso_bht_hash_params(uint32_t seed, uint32_t* out6) { bht_hash_params(seed, out6); }
+void zso_bht_hash(uint32_t hx, uint32_t hy, int64_t n, const int32_t* keys, uint32_t* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = bht_hash(hx, hy, keys + 3 * i);
+}
+int64_t zso_bht_table_size(int64_t n) { return bht_table_size(n); }
+
+namespace {
+// a Bht over caller arrays (copied in; copied back by bht_store)
+Bht bht_load(int32_t T, const uint32_t* h6, const int32_t* keys, const int32_t* indices, const int32_t* status,
+             int32_t cnt, const int32_t* active) {
+  Bht b;
+  b.init(T, h6);
+  if (keys) std::memcpy(b.keys.data(), keys, sizeof(int32_t) * 4 * (size_t)T);
+  if (indices) std::memcpy(b.indices.data(), indices, sizeof(int32_t) * (size_t)T);
+  if (status) std::memcpy(b.status.data(), status, sizeof(int32_t) * (size_t)T);
+  if (active) std::memcpy(b.active.data(), active, sizeof(int32_t) * 3 * (size_t)T);
+  b.cnt = cnt;
+  return b;
+}
+}  // namespace
+
+/// Sequential inserts of n keys into the table held by the arrays (in place):
+/// out[i] = BHTView::insert's return for key i.
+void zso_bht_insert(int32_t T, const uint32_t* h6, int32_t* keys, int32_t* indices, int32_t* status,
+                    int32_t* cnt, int32_t* active, int32_t* success, int64_t n, const int32_t* in,
+                    int32_t* out) {
+  Bht b = bht_load(T, h6, keys, indices, status, *cnt, active);
+  b.success = *success;
+  for (int64_t i = 0; i < n; ++i) out[i] = b.insert(in + 3 * i);
+  std::memcpy(keys, b.keys.data(), sizeof(int32_t) * 4 * (size_t)T);
+  std::memcpy(indices, b.indices.data(), sizeof(int32_t) * (size_t)T);
+  std::memcpy(status, b.status.data(), sizeof(int32_t) * (size_t)T);
+  std::memcpy(active, b.active.data(), sizeof(int32_t) * 3 * (size_t)T);
+  *cnt = b.cnt;
+  *success = b.success;
+}
+
+void zso_bht_query(int32_t T, const uint32_t* h6, const int32_t* keys, const int32_t* indices, int64_t n,
+                   const int32_t* q, int32_t* out) {
+  Bht b = bht_load(T, h6, keys, indices, nullptr, 0, nullptr);
+  for (int64_t i = 0; i < n; ++i) out[i] = b.query(q + 3 * i);
+}
+
+namespace {
+SparseGrid spg_load(int32_t T, const uint32_t* h6, const int32_t* keys, const int32_t* indices, int32_t nblocks,
+                    int32_t nch, const float* data, const float* i2w, const float* w2i, float background) {
+  SparseGrid g;
+  g.table = bht_load(T, h6, keys, indices, nullptr, 0, nullptr);
+  g.nch = nch;
+  g.data.assign(data, data + (size_t)nblocks * nch * 512);
+  std::memcpy(g.i2w, i2w, sizeof g.i2w);
+  std::memcpy(g.w2i, w2i, sizeof g.w2i);
+  g.background = background;
+  return g;
+}
+}  // namespace
+
+/// SparseGridView::insert's block key of each world position.
+void zso_spg_block_keys(const float* w2i, int64_t n, const float* x, int32_t* out) {
+  SparseGrid g;
+  std::memcpy(g.w2i, w2i, sizeof g.w2i);
+  for (int64_t i = 0; i < n; ++i) g.block_key(x + 3 * i, out + 3 * i);
+}
+
+/// wSample of channel chn at n world positions.
+void zso_spg_wsample(int32_t T, const uint32_t* h6, const int32_t* keys, const int32_t* indices, int32_t nblocks,
+                     int32_t nch, const float* data, const float* i2w, const float* w2i, float background,
+                     int32_t chn, int64_t n, const float* x, float* out) {
+  SparseGrid g = spg_load(T, h6, keys, indices, nblocks, nch, data, i2w, w2i, background);
+  for (int64_t i = 0; i < n; ++i) out[i] = g.wsample(chn, x + 3 * i);
+}
+
+/// do_getNormal at n world positions.
+void zso_spg_normal(int32_t T, const uint32_t* h6, const int32_t* keys, const int32_t* indices, int32_t nblocks,
+                    int32_t nch, const float* data, const float* i2w, const float* w2i, float background,
+                    int32_t sdf, int64_t n, const float* x, float* out) {
+  SparseGrid g = spg_load(T, h6, keys, indices, nblocks, nch, data, i2w, w2i, background);
+  for (int64_t i = 0; i < n; ++i) g.normal(sdf, x + 3 * i, out + 3 * i);
+}
+
+/// The level-set boundary on n nodes (positions x, velocities v in place);
+/// inside[i] = 1 where the node was inside.
+void zso_levelset_resolve(int32_t T, const uint32_t* h6, const int32_t* keys, const int32_t* indices,
+                          int32_t nblocks, int32_t nch, const float* data, const float* i2w, const float* w2i,
+                          float background, int32_t sdf, int32_t vchn, int32_t type, int64_t n, const float* x,
+                          float* v, int32_t* inside) {
+  SparseGrid g = spg_load(T, h6, keys, indices, nblocks, nch, data, i2w, w2i, background);
+  for (int64_t i = 0; i < n; ++i) inside[i] = levelset_resolve(g, sdf, vchn, type, x + 3 * i, v + 3 * i) ? 1 : 0;
 }
 
 }  // extern "C"

@@ -3,10 +3,13 @@
 // oracle/_ref/libzpc_refmath.so. It is the checker for the oracle's
 // restatement of the pieces of the hot path that compile on their own:
 // svd_3d, the two stress functions, lame_parameters, the quadratic
-// LocalArena, hash_combine, next_2pow and the 3x3 products. The shim adds no
-// arithmetic of its own; each wrapper only moves arrays in and out.
+// LocalArena, hash_combine, next_2pow, the 3x3 products and bht's universal
+// hash (py_interop/HashUtils.hpp). The shim adds no arithmetic of its own;
+// each wrapper only moves arrays in and out. zref_mt19937 draws from the C++
+// standard library's std::mt19937, the engine bht seeds its hashes with.
 #include <cstddef>
 #include <cstdint>
+#include <random>
 
 #include "zensim/math/Hash.hpp"
 #include "zensim/math/bit/Bits.h"
@@ -15,6 +18,8 @@
 #include "zensim/physics/ConstitutiveModel_Vol_dP.hpp"
 #include "zensim/physics/SoundSpeedCfl.hpp"
 #include "zensim/simulation/Utils.hpp"
+#include "zensim/math/Vec.h"
+#include "zensim/py_interop/HashUtils.hpp"
 
 extern "C" {
 
@@ -99,5 +104,17 @@ void zref_mat_diag_matT(const float* m1, const float* d, const float* m2, float*
 
 float zref_cfl_dt(float E, float nu, float rho, float dx, float cfl) {
   return zs::evaluate_timestep_linear_elasticity(E, nu, rho, dx, cfl);
+}
+
+/// universal_hash_base<vec<i32,3>>{hx, hy} of n keys (py_interop/HashUtils.hpp:22-41)
+void zref_universal_hash(uint32_t hx, uint32_t hy, int64_t n, const int32_t* keys, uint32_t* out) {
+  const zs::universal_hash_base<zs::vec<int, 3>> h{hx, hy};
+  for (int64_t i = 0; i < n; ++i) out[i] = h(zs::vec<int, 3>{keys[3 * i], keys[3 * i + 1], keys[3 * i + 2]});
+}
+
+/// n draws of std::mt19937 seeded with `seed`
+void zref_mt19937(uint32_t seed, int64_t n, uint32_t* out) {
+  std::mt19937 r(seed);
+  for (int64_t i = 0; i < n; ++i) out[i] = (uint32_t)r();
 }
 }
