@@ -166,7 +166,8 @@ int32_t zsmpm_policy_sync(zsmpm_policy pol);
 /* Diagnostic counters after a sync: [0] new keys of the last insertion,
  * [2] particle runs of the last bucket sort, [3] FP64 SVDs, [4] closure fallbacks,
  * [5] max |v|^2 bits of the last grid update, [9] waves of 64 particles that
- * evaluated the closure network (all passes). out must hold 16 entries. */
+ * evaluated the closure network (all passes), [10] placement rounds of the
+ * last zsmpm_bht_insert__rocm. out must hold 16 entries. */
 int32_t zsmpm_policy_counters(zsmpm_policy pol, uint32_t* out);
 
 /* Stages timed when profiling is on. */
@@ -434,6 +435,98 @@ int32_t zsmpm_mlp_grad__rocm(zsmpm_policy pol, const float* params, const float*
 int32_t zsmpm_mlp_train_step__rocm(zsmpm_policy pol, float* params, float* adam_m, float* adam_v,
                                    const float* features, const float* records, int64_t n, float lr, int32_t step,
                                    float* loss);
+
+/* ------------------------------------------------------------------------
+ * Bucketed hash table and SparseGrid
+ * ---------------------------------------------------------------------- */
+
+/* bht<i32, 3, int, 16> (container/Bht.hpp:16-310): buckets of 16 slots, three
+ * universal hashes; a key goes to the first of its buckets (hf0, hf1, hf2)
+ * that holds at most 14 keys, in the bucket's first empty slot. Storage keys
+ * are vec<i32,3> padded to 16 bytes (x, y, z, pad); empty slots hold
+ * 0x3f3f3f3f in every component. table_size is a multiple of 16
+ * (zsmpm_bht_table_size); keys must be 16-byte aligned. hash holds (hashx,
+ * hashy) of hf0, hf1, hf2 (zsmpm_bht_hash_params). Key components must lie
+ * in [-2^20, 2^20) (ZSMPM_KEY_RANGE at the next sync otherwise). */
+typedef struct zsmpm_bht {
+  zsmpm_abi_header hdr;
+  int32_t table_size;
+  int32_t* keys;        /* [4 * table_size] */
+  int32_t* indices;     /* [table_size] */
+  int32_t* status;      /* [table_size] */
+  int32_t* cnt;         /* [1] */
+  int32_t* active_keys; /* [3 * table_size] (_activeKeys) */
+  int32_t* success;     /* [1] (_buildSuccess) */
+  uint32_t hash[6];
+} zsmpm_bht;
+
+#define ZSMPM_BHT_SEED 2    /* std::mt19937 rng(2) of bht's constructor (Bht.hpp:160-166) */
+#define ZSMPM_BHT_SERIAL 1  /* zsmpm_bht_insert__rocm: one thread inserts the keys one by one */
+
+/* evaluateTableSize (Bht.hpp:153-157): 2 next_pow2(n) rounded up to the next
+ * multiple of 16 (by 16 when it is one); 0 for n = 0. Host function. */
+int64_t zsmpm_bht_table_size(int64_t expected_entries);
+/* The hash parameters bht draws from std::mt19937(seed) (Bht.hpp:160-166,
+ * universal_hash(std::mt19937&), container/Bcht.hpp:39-43). Host function. */
+void zsmpm_bht_hash_params(uint32_t seed, uint32_t out[6]);
+/* bht::reset(clearCnt = true) (Bht.hpp:316-320): every key empty, status -1,
+ * count 0, success 1; indices are kept. */
+int32_t zsmpm_bht_clear__rocm(zsmpm_policy pol, const zsmpm_bht* table);
+/* BHTView::insert for n keys (Bht.hpp:426-548), with the result of inserting
+ * them one after another in input order: repeats and keys already present
+ * are skipped, new keys take the next indices in order of first occurrence
+ * and their slots are the host path's. out_index (device, may be NULL)
+ * receives what insert returns for each input: the new index, -1 for a key
+ * found, INT32_MIN where no bucket had room or the index comes within 20 of
+ * the table size (both also raise ZSMPM_TABLE_FULL and clear success; the
+ * table is then unspecified). flags ZSMPM_BHT_SERIAL: the sequential
+ * insertion itself on one thread (the reference mode the default is checked
+ * against). Synchronises the policy's stream. zpc's CUDA insert places keys
+ * by atomics in an order that varies from run to run. */
+int32_t zsmpm_bht_insert__rocm(zsmpm_policy pol, const zsmpm_bht* table, const int32_t* keys, int64_t n,
+                               int32_t* out_index, int32_t flags);
+/* BHTView::query / tile_query (Bht.hpp:551-620): out[i] = the index of keys[i]
+ * or -1. */
+int32_t zsmpm_bht_query__rocm(zsmpm_policy pol, const zsmpm_bht* table, const int32_t* keys, int64_t n,
+                              int32_t* out);
+
+/* SparseGrid<3, f32, 8> (geometry/SparseGrid.hpp:16-190): block origins
+ * (multiples of 8) in the bht; cell values in a TileVector<f32, 512>: block b,
+ * channel c, cell k at data[(b * num_channels + c) * 512 + k], k = (x * 8 +
+ * y) * 8 + z; the index-to-world transform i2w (4 x 4, row-major, row-vector
+ * convention: world = [X, 1] i2w, translation in row 3) and its inverse w2i;
+ * the background value of inactive cells. */
+typedef struct zsmpm_sparse_grid {
+  zsmpm_abi_header hdr;
+  zsmpm_bht table;
+  float* data;             /* [capacity_blocks * num_channels * 512] */
+  int32_t num_channels;
+  int32_t capacity_blocks;
+  float i2w[16];
+  float w2i[16];
+  float background;
+} zsmpm_sparse_grid;
+
+/* SparseGridView::insert(x) for n world positions (SparseGrid.hpp:439-446):
+ * the block of floor(w2i x + 1/2), with zsmpm_bht_insert__rocm's semantics. */
+int32_t zsmpm_spg_activate__rocm(zsmpm_policy pol, const zsmpm_sparse_grid* grid, const float* x, int64_t n);
+/* wSample (width 1) / wPack (width 2 or 3) of channels [channel, channel +
+ * width) at n world positions (SparseGrid.hpp:458-518, kernel_e::linear):
+ * trilinear over the 2^3 cells around w2i x, the background where a block is
+ * inactive. out: [width * n]. */
+int32_t zsmpm_spg_sample__rocm(zsmpm_policy pol, const zsmpm_sparse_grid* grid, int32_t channel, int32_t width,
+                               const float* x, int64_t n, float* out);
+/* LevelSetBoundary<SparseLevelSet> (geometry/Collider.h:204-235) applied by
+ * every later grid update of the policy after its plane and analytic
+ * colliders (ApplyBoundaryConditionOnGridBlocks, GridOp.hpp:127-164): where
+ * the grid's channel sdf_channel, sampled at the node, is negative, the node
+ * takes the sampled velocity channels v_channel..+2 (zero for -1) when
+ * sticky, or loses the normal part of its velocity relative to it (slip;
+ * separate only when approaching), the normal from central differences of
+ * the signed distance with eps = voxelSize(0) / 4 (SparseGrid.hpp:524-538).
+ * The grid's buffers are read by every update, not copied. NULL removes it. */
+int32_t zsmpm_policy_levelset_boundary__rocm(zsmpm_policy pol, const zsmpm_sparse_grid* grid, int32_t sdf_channel,
+                                             int32_t v_channel, int32_t type);
 
 /* ------------------------------------------------------------------------
  * Particle layout and IO adapters

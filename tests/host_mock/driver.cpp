@@ -139,6 +139,53 @@ int main(int argc, char** argv) {
     expect("tile reorder", zsmpm_tiles_reorder__rocm(pol, tiles.data(), tiles2.data(), map.data(), nt, 32, 27, 0), 0);
     expect("tile reorder in place rejected", zsmpm_tiles_reorder__rocm(pol, tiles.data(), tiles.data(), map.data(), nt, 32, 27, 1), ZSMPM_INVALID_ARGUMENT);
   }
+  // bht and SparseGrid: argument validation and the policy's scratch
+  {
+    expect("bht table size", zsmpm_bht_table_size(1000) == 2064 && zsmpm_bht_table_size(0) == 0, 1);
+    uint32_t hp[6];
+    zsmpm_bht_hash_params(ZSMPM_BHT_SEED, hp);
+    expect("bht hash params", hp[0] == 1872583848u && hp[1] == 794921487u, 1);
+    const int32_t T = 2064;
+    zsmpm_bht bt = mk<zsmpm_bht>();
+    bt.table_size = T;
+    std::vector<int32_t> bk(4 * T + 4), bi(T), bs(T), bc(1), ba(3 * T), bsu(1);
+    int32_t* kal = bk.data();
+    while ((uintptr_t)kal % 16) ++kal;
+    bt.keys = kal; bt.indices = bi.data(); bt.status = bs.data(); bt.cnt = bc.data(); bt.active_keys = ba.data(); bt.success = bsu.data();
+    for (int i = 0; i < 6; ++i) bt.hash[i] = hp[i];
+    std::vector<int32_t> keys(3 * 500, 1), outi(500);
+    expect("bht clear", zsmpm_bht_clear__rocm(pol, &bt), 0);
+    expect("bht insert", zsmpm_bht_insert__rocm(pol, &bt, keys.data(), 500, outi.data(), 0), 0);
+    expect("bht insert serial", zsmpm_bht_insert__rocm(pol, &bt, keys.data(), 500, nullptr, ZSMPM_BHT_SERIAL), 0);
+    expect("bht insert bad flags", zsmpm_bht_insert__rocm(pol, &bt, keys.data(), 500, nullptr, 4), ZSMPM_INVALID_ARGUMENT);
+    expect("bht insert grows the scratch", zsmpm_bht_insert__rocm(pol, &bt, keys.data(), 400, nullptr, 0), 0);
+    std::vector<int32_t> big_keys(3 * 100000, 0);
+    expect("bht insert larger", zsmpm_bht_insert__rocm(pol, &bt, big_keys.data(), 100000, nullptr, 0), 0);
+    expect("bht query", zsmpm_bht_query__rocm(pol, &bt, keys.data(), 500, outi.data()), 0);
+    zsmpm_bht bad = bt; bad.table_size = 2060;
+    expect("bht size not a multiple of 16", zsmpm_bht_query__rocm(pol, &bad, keys.data(), 500, outi.data()), ZSMPM_INVALID_ARGUMENT);
+    bad = bt; bad.keys = kal + 1;
+    expect("bht keys misaligned", zsmpm_bht_clear__rocm(pol, &bad), ZSMPM_INVALID_ARGUMENT);
+    zsmpm_sparse_grid sg = mk<zsmpm_sparse_grid>();
+    sg.table = bt; sg.num_channels = 4; sg.capacity_blocks = 8;
+    std::vector<float> sd(8 * 4 * 512), xs(3 * 300), so(3 * 300);
+    sg.data = sd.data();
+    for (int i = 0; i < 16; ++i) sg.i2w[i] = sg.w2i[i] = (i % 5 == 0) ? 1.f : 0.f;
+    expect("spg activate", zsmpm_spg_activate__rocm(pol, &sg, xs.data(), 300), 0);
+    expect("spg sample", zsmpm_spg_sample__rocm(pol, &sg, 1, 3, xs.data(), 300, so.data()), 0);
+    expect("spg sample past the channels", zsmpm_spg_sample__rocm(pol, &sg, 2, 3, xs.data(), 300, so.data()), ZSMPM_INVALID_ARGUMENT);
+    expect("levelset boundary", zsmpm_policy_levelset_boundary__rocm(pol, &sg, 0, 1, ZSMPM_SLIP), 0);
+    expect("levelset boundary bad channel", zsmpm_policy_levelset_boundary__rocm(pol, &sg, 0, 2, ZSMPM_SLIP), ZSMPM_INVALID_ARGUMENT);
+    Sc& s0 = sc[0];
+    float g3[3] = {0.f, -9.8f, 0.f};
+    expect("substeps with the levelset boundary", zsmpm_substeps_sorted__rocm(pol, &s0.model, &s0.a, &s0.b, &s0.t, &s0.bk, &s0.g, 1e-4f, g3, s0.cols, 6, 4, ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
+    expect("general colliders keep the boundary", zsmpm_policy_colliders__rocm(pol, nullptr, 0), 0);
+    expect("levelset boundary off", zsmpm_policy_levelset_boundary__rocm(pol, nullptr, 0, -1, 0), 0);
+    expect("substeps after", zsmpm_substeps_sorted__rocm(pol, &s0.model, &s0.a, &s0.b, &s0.t, &s0.bk, &s0.g, 1e-4f, g3, s0.cols, 6, 4, ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
+    uint32_t ctr[16];
+    expect("counters", zsmpm_policy_counters(pol, ctr), 0);
+    expect("bht rounds counter", ctr[10] == 1u, 1);
+  }
   // checkpoint round trip through a real file (mock device memory is host memory)
   {
     Sc& s0 = sc[0];

@@ -36,6 +36,18 @@ hipError_t grid_update(float*, int32_t, const TableView&, float, const float*, c
 hipError_t g2p(const ParticlesView&, const ParticlesView*, const TableView&, const SortView&, const float*, int32_t, float, float, Workspace&, hipStream_t s, const NextStress*, const SceneSet* ss) { return scenes_ok(ss, s); }
 hipError_t grid_angular_momentum(const float*, int32_t, const TableView&, float, int, int, double*, Workspace&, hipStream_t s) { return touch(s); }
 hipError_t reorder_particles(const ParticlesView&, const ParticlesView&, const int32_t*, Workspace&, hipStream_t s) { return touch(s); }
+size_t bht_scratch_bytes(int64_t n, int32_t nb) { return 1024 + 64 * (size_t)(n > 0 ? n : 1) + 16 * (size_t)nb; }
+hipError_t bht_clear(const BhtView&, hipStream_t s) { return touch(s); }
+int32_t bht_insert(const BhtView& t, const int32_t*, int64_t, int32_t*, bool, void* scratch, uint32_t*, hipStream_t s, int32_t* rounds) {
+  if (!scratch || t.nb <= 0 || t.T != 16 * t.nb) return -5;
+  if (rounds) *rounds = 1;
+  return touch(s) == hipSuccess ? 0 : -6;
+}
+hipError_t bht_query_many(const BhtView&, const int32_t*, int64_t, int32_t*, hipStream_t s) { return touch(s); }
+int32_t spg_activate(const SpgView& g, const float*, int64_t, void* scratch, uint32_t*, hipStream_t s) {
+  return !scratch || g.nch < 1 ? -5 : (touch(s) == hipSuccess ? 0 : -6);
+}
+hipError_t spg_sample(const SpgView&, int, int, const float*, int64_t, float*, hipStream_t s) { return touch(s); }
 }
 namespace zsmpm {
 size_t radix_sort_scratch_words(uint32_t) { return 1024; }

@@ -7,6 +7,7 @@
 
 #include <mutex>
 #include <new>
+#include <random>
 #include <vector>
 
 #include "../../include/zsmpm.h"
@@ -42,6 +43,10 @@ struct zsmpm_policy_s {
   // zsmpm_table_activate_noalloc__rocm ran on this policy: its device-side
   // errors are in the device's fixed scratch array
   bool noalloc_used = false;
+  // scratch of bht insertions (grown on demand) and the rounds the last took
+  void* bht_buf = nullptr;
+  size_t bht_bytes = 0;
+  int32_t bht_rounds = 0;
   // P2G / G2P workgroup counts picked by ZSMPM_SUBSTEPS_AUTOTUNE (0: default)
   int32_t tuned_p2g = 0, tuned_g2p = 0;
   Workspace ws{};
@@ -856,6 +861,7 @@ void destroy_policy(zsmpm_policy_s* p) {
   free_all(p);
   if (p->scene_p) (void)hipFree(p->scene_p);
   if (p->scene_first) (void)hipFree(p->scene_first);
+  if (p->bht_buf) (void)hipFree(p->bht_buf);
   if (p->cap) (void)hipStreamDestroy(p->cap);
   delete p;
 }
@@ -932,6 +938,7 @@ int32_t zsmpm_policy_counters(zsmpm_policy pol, uint32_t* out) {
   DeviceGuard g(pol->devid);
   ZS_HIP(hipStreamSynchronize(pol->stream));
   ZS_HIP(hipMemcpy(out, pol->ws.counters, sizeof(uint32_t) * kCtrCount, hipMemcpyDeviceToHost));
+  out[kCtrBhtRounds] = (uint32_t)pol->bht_rounds;
   return ZSMPM_OK;
 }
 
@@ -1210,6 +1217,146 @@ int32_t zsmpm_reorder__rocm(zsmpm_policy pol, const zsmpm_buckets* buckets, cons
   return from_hip(reorder_particles(pi, po, sv.indices, pol->ws, pol->stream));
 }
 
+static int32_t view_bht(const zsmpm_bht* d, BhtView& t) {
+  ZS_TRY(check_header(d));
+  if (d->table_size < kBhtBucket || d->table_size % kBhtBucket != 0 || !d->keys || !d->indices || !d->status ||
+      !d->cnt || !d->active_keys || !d->success || ((uintptr_t)d->keys % 16) != 0)
+    return ZSMPM_INVALID_ARGUMENT;
+  t.T = d->table_size;
+  t.nb = d->table_size / kBhtBucket;
+  t.keys = reinterpret_cast<int4*>(d->keys);
+  t.indices = d->indices;
+  t.status = d->status;
+  t.cnt = d->cnt;
+  t.active = d->active_keys;
+  t.success = d->success;
+  for (int i = 0; i < 6; ++i) t.h[i] = d->hash[i];
+  return ZSMPM_OK;
+}
+
+static int32_t view_spg(const zsmpm_sparse_grid* d, SpgView& g) {
+  ZS_TRY(check_header(d));
+  ZS_TRY(view_bht(&d->table, g.t));
+  if (d->num_channels < 1 || d->capacity_blocks < 0 || (d->capacity_blocks > 0 && !d->data))
+    return ZSMPM_INVALID_ARGUMENT;
+  g.data = d->data;
+  g.nch = d->num_channels;
+  memcpy(g.i2w, d->i2w, sizeof g.i2w);
+  memcpy(g.w2i, d->w2i, sizeof g.w2i);
+  g.background = d->background;
+  return ZSMPM_OK;
+}
+
+// the policy's bht scratch, grown (after draining the stream) when too small
+static int32_t bht_scratch(zsmpm_policy_s* pol, int64_t n, int32_t nb, void** out) {
+  const size_t need = bht_scratch_bytes(n, nb);
+  if (need > pol->bht_bytes) {
+    ZS_HIP(hipStreamSynchronize(pol->stream));
+    if (pol->bht_buf) (void)hipFree(pol->bht_buf);
+    pol->bht_buf = nullptr;
+    pol->bht_bytes = 0;
+    ZS_HIP(hipMalloc(&pol->bht_buf, need));
+    pol->bht_bytes = need;
+  }
+  *out = pol->bht_buf;
+  return ZSMPM_OK;
+}
+
+int64_t zsmpm_bht_table_size(int64_t n) {
+  if (n <= 0) return 0;
+  int64_t p = 1;
+  while (p < n) p <<= 1;
+  const int64_t m = 2 * p;
+  return m + (kBhtBucket - m % kBhtBucket);
+}
+
+void zsmpm_bht_hash_params(uint32_t seed, uint32_t out[6]) {
+  if (!out) return;
+  std::mt19937 rng(seed);
+  for (int h = 0; h < 3; ++h) {
+    uint32_t x = (uint32_t)rng() % kBhtPrime;
+    if (x < 1) x = 1;
+    out[2 * h] = x;
+    out[2 * h + 1] = (uint32_t)rng() % kBhtPrime;
+  }
+}
+
+int32_t zsmpm_bht_clear__rocm(zsmpm_policy pol, const zsmpm_bht* table) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
+  BhtView t;
+  ZS_TRY(view_bht(table, t));
+  DeviceGuard g(pol->devid);
+  return from_hip(bht_clear(t, pol->stream));
+}
+
+int32_t zsmpm_bht_insert__rocm(zsmpm_policy pol, const zsmpm_bht* table, const int32_t* keys, int64_t n,
+                               int32_t* out_index, int32_t flags) {
+  if (!pol || n < 0 || (n > 0 && !keys) || (flags & ~ZSMPM_BHT_SERIAL)) return ZSMPM_INVALID_ARGUMENT;
+  if (n > (int64_t)0x7fffffff) return ZSMPM_CAPACITY;
+  BhtView t;
+  ZS_TRY(view_bht(table, t));
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, 0, 0));
+  void* sc = nullptr;
+  ZS_TRY(bht_scratch(pol, n, t.nb, &sc));
+  return bht_insert(t, keys, n, out_index, (flags & ZSMPM_BHT_SERIAL) != 0, sc, pol->ws.err, pol->stream,
+                    &pol->bht_rounds);
+}
+
+int32_t zsmpm_bht_query__rocm(zsmpm_policy pol, const zsmpm_bht* table, const int32_t* keys, int64_t n,
+                              int32_t* out) {
+  if (!pol || n < 0 || (n > 0 && (!keys || !out))) return ZSMPM_INVALID_ARGUMENT;
+  BhtView t;
+  ZS_TRY(view_bht(table, t));
+  DeviceGuard g(pol->devid);
+  return from_hip(bht_query_many(t, keys, n, out, pol->stream));
+}
+
+int32_t zsmpm_spg_activate__rocm(zsmpm_policy pol, const zsmpm_sparse_grid* grid, const float* x, int64_t n) {
+  if (!pol || n < 0 || (n > 0 && !x)) return ZSMPM_INVALID_ARGUMENT;
+  if (n > (int64_t)0x7fffffff) return ZSMPM_CAPACITY;
+  SpgView gv;
+  ZS_TRY(view_spg(grid, gv));
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, 0, 0));
+  void* sc = nullptr;
+  ZS_TRY(bht_scratch(pol, n, gv.t.nb, &sc));
+  return spg_activate(gv, x, n, sc, pol->ws.err, pol->stream);
+}
+
+int32_t zsmpm_spg_sample__rocm(zsmpm_policy pol, const zsmpm_sparse_grid* grid, int32_t channel, int32_t width,
+                               const float* x, int64_t n, float* out) {
+  if (!pol || n < 0 || (n > 0 && (!x || !out)) || width < 1 || width > 3) return ZSMPM_INVALID_ARGUMENT;
+  SpgView gv;
+  ZS_TRY(view_spg(grid, gv));
+  if (channel < 0 || channel + width > gv.nch) return ZSMPM_INVALID_ARGUMENT;
+  DeviceGuard g(pol->devid);
+  return from_hip(spg_sample(gv, channel, width, x, n, out, pol->stream));
+}
+
+int32_t zsmpm_policy_levelset_boundary__rocm(zsmpm_policy pol, const zsmpm_sparse_grid* grid, int32_t sdf_channel,
+                                             int32_t v_channel, int32_t type) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
+  if (!grid) {
+    pol->gcols.ls = LevelSetBoundary{};
+    return ZSMPM_OK;
+  }
+  LevelSetBoundary ls;
+  ZS_TRY(view_spg(grid, ls.g));
+  if (sdf_channel < 0 || sdf_channel >= ls.g.nch || v_channel < -1 || (v_channel >= 0 && v_channel + 3 > ls.g.nch) ||
+      type < ZSMPM_STICKY || type > ZSMPM_SEPARATE)
+    return ZSMPM_INVALID_ARGUMENT;
+  ls.on = 1;
+  ls.sdf = sdf_channel;
+  ls.vchn = v_channel;
+  ls.type = type;
+  // voxelSize(0): the length of row 0 of the transform's 3 x 3 part (SparseGrid.hpp:389-397)
+  const float v0 = sqrtf(grid->i2w[0] * grid->i2w[0] + grid->i2w[1] * grid->i2w[1] + grid->i2w[2] * grid->i2w[2]);
+  ls.eps = v0 / 4.f;
+  pol->gcols.ls = ls;
+  return ZSMPM_OK;
+}
+
 // The attributes a tile set names, checked against the arrays of p.
 static int32_t make_tile_attrs(const zsmpm_tile_particles* t, const zsmpm_particles* p, TileAttrs& a) {
   ZS_TRY(check_header(t));
@@ -1356,6 +1503,7 @@ int32_t zsmpm_policy_colliders__rocm(zsmpm_policy pol, const zsmpm_collider* col
     memcpy(o.dbdt, c.dbdt, sizeof o.dbdt);
   }
   g.n = n;
+  g.ls = pol->gcols.ls;  // the level-set boundary stays
   pol->gcols = g;
   return ZSMPM_OK;
 }

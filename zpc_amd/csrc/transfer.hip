@@ -861,7 +861,7 @@ __device__ __forceinline__ void node_velocity(float* gp, float mass, int fold, f
   }
   float v[3] = {mv0 * inv + e0, mv1 * inv + e1, mv2 * inv + e2};
   vmax = fmaxf(vmax, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-  if (mass > 0.f && (cols.n > 0 || gen.n > 0)) {
+  if (mass > 0.f && (cols.n > 0 || gen.n > 0 || gen.ls.on)) {
     const int32_t k0 = kLocal ? k0_local : t.active[3 * (int64_t)b];
     const int32_t k1 = t.active[3 * (int64_t)b + 1], k2 = t.active[3 * (int64_t)b + 2];
     const float pos[3] = {((float)k0 * 4.f + (float)((cell >> 4) & 3)) * dx,
@@ -885,6 +885,7 @@ __device__ __forceinline__ void node_velocity(float* gp, float mass, int fold, f
       }
     }
     for (int c = 0; c < gen.n; ++c) resolve_general(gen.c[c], pos, v);
+    if (gen.ls.on) levelset_resolve(gen.ls, pos, v);
   }
   gp[1 * kBlockCells] = v[0];
   gp[2 * kBlockCells] = v[1];

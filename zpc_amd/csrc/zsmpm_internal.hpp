@@ -8,6 +8,7 @@
 
 #include "../../include/zsmpm.h"
 #include "zsmpm_device.hpp"
+#include "bht_device.hpp"
 
 namespace zsmpm {
 
@@ -64,6 +65,7 @@ struct GeneralCollider {
 struct GeneralColliders {
   int32_t n = 0;
   GeneralCollider c[kMaxColliders];
+  LevelSetBoundary ls;  // a SparseGrid signed-distance boundary, applied after the analytic ones
 };
 
 // A batch of independent scenes in one table, one grid and one set of
@@ -125,6 +127,7 @@ enum : int {
   kCtrFbQueue = 7,  // length of the closure's SVD fallback queue (per pass)
   kCtrNetQueue = 8, // waves the closure's network pass took (per pass)
   kCtrNetWaves = 9, // waves that evaluated the closure's network (all passes)
+  kCtrBhtRounds = 10, // placement rounds of the last bht insertion (host-side, filled by the counters call)
   kCtrCount = 16
 };
 
@@ -301,6 +304,19 @@ hipError_t reorder_particles(const ParticlesView& in, const ParticlesView& out, 
 
 // grid-stride launch width for kernels that loop over a device-resident count
 inline int persistent_groups() { return 4096; }
+
+// ---- bht and SparseGrid (bht.hip) ----
+size_t bht_scratch_bytes(int64_t n, int32_t nb);
+hipError_t bht_clear(const BhtView& t, hipStream_t s);
+// Inserts n keys with the sequential semantics (serial: one thread inserts
+// them one after another); out[i] (may be null) as BHTView::insert returns
+// it. Synchronises `s` (once, and once per placement round). rounds: the
+// placement rounds taken. Returns a zsmpm result code.
+int32_t bht_insert(const BhtView& t, const int32_t* keys, int64_t n, int32_t* out, bool serial, void* scratch,
+                   uint32_t* err, hipStream_t s, int32_t* rounds);
+hipError_t bht_query_many(const BhtView& t, const int32_t* keys, int64_t n, int32_t* out, hipStream_t s);
+int32_t spg_activate(const SpgView& g, const float* x, int64_t n, void* scratch, uint32_t* err, hipStream_t s);
+hipError_t spg_sample(const SpgView& g, int chn, int width, const float* x, int64_t n, float* out, hipStream_t s);
 
 // ---- particle layout and IO adapters (io.hip) ----
 /// One attribute of a TileVector<f32, L> particle set: its first channel, its

@@ -114,6 +114,26 @@ class BatchSceneDesc(C.Structure):
 COLUMN_MAJOR, ROW_MAJOR = 0, 1
 
 
+class BhtDesc(C.Structure):
+    """zsmpm_bht: bht<i32, 3, int, 16>."""
+
+    _fields_ = [("hdr", Header), ("table_size", C.c_int32), ("keys", C.c_void_p), ("indices", C.c_void_p),
+                ("status", C.c_void_p), ("cnt", C.c_void_p), ("active_keys", C.c_void_p), ("success", C.c_void_p),
+                ("hash", C.c_uint32 * 6)]
+
+
+class SparseGridDesc(C.Structure):
+    """zsmpm_sparse_grid: SparseGrid<3, f32, 8>."""
+
+    _fields_ = [("hdr", Header), ("table", BhtDesc), ("data", C.c_void_p), ("num_channels", C.c_int32),
+                ("capacity_blocks", C.c_int32), ("i2w", C.c_float * 16), ("w2i", C.c_float * 16),
+                ("background", C.c_float)]
+
+
+BHT_SEED = 2
+BHT_SERIAL = 1
+
+
 class TileParticlesDesc(C.Structure):
     """zsmpm_tile_particles: particles in a TileVector<f32, L>."""
 
@@ -179,6 +199,12 @@ def lib():
         "zsmpm_policy_colliders__rocm": [vp, P(ColliderDesc), i32],
         "zsmpm_substeps_batched__rocm": [vp, P(BatchSceneDesc), i32, P(ParticlesDesc), P(ParticlesDesc),
                                          P(TableDesc), P(BucketsDesc), P(GridDesc), i32, i32, P(C.c_int32)],
+        "zsmpm_bht_clear__rocm": [vp, P(BhtDesc)],
+        "zsmpm_bht_insert__rocm": [vp, P(BhtDesc), vp, i64, vp, i32],
+        "zsmpm_bht_query__rocm": [vp, P(BhtDesc), vp, i64, vp],
+        "zsmpm_spg_activate__rocm": [vp, P(SparseGridDesc), vp, i64],
+        "zsmpm_spg_sample__rocm": [vp, P(SparseGridDesc), i32, i32, vp, i64, vp],
+        "zsmpm_policy_levelset_boundary__rocm": [vp, P(SparseGridDesc), i32, i32, i32],
         "zsmpm_particles_from_tiles__rocm": [vp, P(TileParticlesDesc), P(ParticlesDesc)],
         "zsmpm_particles_to_tiles__rocm": [vp, P(ParticlesDesc), P(TileParticlesDesc)],
         "zsmpm_tiles_reorder__rocm": [vp, vp, vp, vp, i64, i32, i32, i32],
@@ -196,6 +222,10 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = i32
+    L.zsmpm_bht_table_size.argtypes = [i64]
+    L.zsmpm_bht_table_size.restype = i64
+    L.zsmpm_bht_hash_params.argtypes = [C.c_uint32, P(C.c_uint32 * 6)]
+    L.zsmpm_bht_hash_params.restype = None
     _lib = L
     return L
 
@@ -336,6 +366,18 @@ class Policy:
                                               None if attr is None else attr.data_ptr(), width,
                                               None if attr_name is None else attr_name.encode()), "write_partio")
 
+    def set_levelset_boundary(self, grid: Optional["DeviceSparseGrid"], sdf_channel: int = 0, v_channel: int = -1,
+                              kind: int = 0):
+        """LevelSetBoundary<SparseLevelSet> of every later grid update (None
+        removes it); the grid's buffers are read by every update."""
+        if grid is None:
+            check(self.L.zsmpm_policy_levelset_boundary__rocm(self.h, None, 0, -1, 0), "levelset_boundary")
+            self._ls_grid = None
+            return
+        check(self.L.zsmpm_policy_levelset_boundary__rocm(self.h, C.byref(grid.desc()), int(sdf_channel),
+                                                          int(v_channel), int(kind)), "levelset_boundary")
+        self._ls_grid = grid
+
     def stage_times(self, reset: bool = False) -> dict:
         """{stage: (total ms, calls)} accumulated since the last reset."""
         n = len(STAGES)
@@ -469,6 +511,115 @@ class DeviceTiles:
     def host(self) -> np.ndarray:
         """(ntiles, nch, L) view of the buffer on the host."""
         return self.data.cpu().numpy().reshape(max(self.ntiles, 1), self.nch, self.L)
+
+
+def bht_table_size(n: int) -> int:
+    return int(lib().zsmpm_bht_table_size(int(n)))
+
+
+def bht_hash_params(seed: int = BHT_SEED):
+    out = (C.c_uint32 * 6)()
+    lib().zsmpm_bht_hash_params(seed, C.byref(out))
+    return [int(v) for v in out]
+
+
+class DeviceBht:
+    """bht<i32, 3, int, 16> buffers on the device, cleared."""
+
+    def __init__(self, table_size: int, pol: Optional[Policy] = None, hash_params=None, device="cuda"):
+        torch = _torch()
+        T = int(table_size)
+        if T < 16 or T % 16:
+            raise ValueError("bht table size: a positive multiple of 16 (bht_table_size)")
+        self.T = T
+        self.keys = torch.full((T, 4), 0x3F3F3F3F, dtype=torch.int32, device=device)
+        self.indices = torch.full((T,), -1, dtype=torch.int32, device=device)
+        self.status = torch.full((T,), -1, dtype=torch.int32, device=device)
+        self.cnt = torch.zeros(1, dtype=torch.int32, device=device)
+        self.active = torch.zeros((T, 3), dtype=torch.int32, device=device)
+        self.success = torch.ones(1, dtype=torch.int32, device=device)
+        self.hash = list(bht_hash_params() if hash_params is None else hash_params)
+        self.pol = pol if pol is not None else Policy()
+        self.desc = BhtDesc(_hdr(BhtDesc), T, _ptr(self.keys), _ptr(self.indices), _ptr(self.status),
+                            _ptr(self.cnt), _ptr(self.active), _ptr(self.success), (C.c_uint32 * 6)(*self.hash))
+
+    def clear(self):
+        check(self.pol.L.zsmpm_bht_clear__rocm(self.pol.h, C.byref(self.desc)), "bht_clear")
+
+    def insert(self, keys, serial: bool = False, out=None):
+        """Inserts device int32 keys (n, 3); returns the (n,) insert results."""
+        torch = _torch()
+        n = int(keys.shape[0])
+        if out is None:
+            out = torch.empty(max(n, 1), dtype=torch.int32, device=keys.device)
+        check(self.pol.L.zsmpm_bht_insert__rocm(self.pol.h, C.byref(self.desc), keys.data_ptr(), n, out.data_ptr(),
+                                                BHT_SERIAL if serial else 0), "bht_insert")
+        return out[:n]
+
+    def query(self, keys, out=None):
+        torch = _torch()
+        n = int(keys.shape[0])
+        if out is None:
+            out = torch.empty(max(n, 1), dtype=torch.int32, device=keys.device)
+        check(self.pol.L.zsmpm_bht_query__rocm(self.pol.h, C.byref(self.desc), keys.data_ptr(), n, out.data_ptr()),
+              "bht_query")
+        return out[:n]
+
+    def size(self) -> int:
+        return int(self.cnt.item())
+
+
+class DeviceSparseGrid:
+    """SparseGrid<3, f32, 8>: a DeviceBht of block origins, a TileVector<f32,
+    512> of (capacity, nch, 512) cell values, the index-to-world transform (4
+    x 4, row-vector convention) and the background value."""
+
+    def __init__(self, capacity_blocks: int, num_channels: int = 1, i2w=None, background: float = 0.0,
+                 pol: Optional[Policy] = None, device="cuda"):
+        torch = _torch()
+        self.cap = int(capacity_blocks)
+        self.nch = int(num_channels)
+        self.table = DeviceBht(bht_table_size(self.cap), pol)
+        self.pol = self.table.pol
+        self.data = torch.zeros((max(self.cap, 1), self.nch, 512), dtype=torch.float32, device=device)
+        M = np.eye(4) if i2w is None else np.asarray(i2w, np.float64).reshape(4, 4)
+        self.i2w = M.astype(np.float32)
+        self.w2i = np.linalg.inv(M).astype(np.float32)
+        self.background = float(background)
+
+    def desc(self) -> SparseGridDesc:
+        d = SparseGridDesc()
+        d.hdr = _hdr(SparseGridDesc)
+        d.table = self.table.desc
+        d.data, d.num_channels, d.capacity_blocks = self.data.data_ptr(), self.nch, self.cap
+        for i in range(16):
+            d.i2w[i] = float(self.i2w.reshape(16)[i])
+            d.w2i[i] = float(self.w2i.reshape(16)[i])
+        d.background = self.background
+        return d
+
+    def activate(self, x):
+        """SparseGridView::insert of every world position of x (device, (n, 3))."""
+        check(self.pol.L.zsmpm_spg_activate__rocm(self.pol.h, C.byref(self.desc()), x.data_ptr(), int(x.shape[0])),
+              "spg_activate")
+
+    def sample(self, channel: int, x, width: int = 1):
+        """wSample / wPack of [channel, channel + width) at world positions x."""
+        torch = _torch()
+        n = int(x.shape[0])
+        out = torch.empty((max(n, 1), width), dtype=torch.float32, device=x.device)
+        check(self.pol.L.zsmpm_spg_sample__rocm(self.pol.h, C.byref(self.desc()), int(channel), int(width),
+                                                x.data_ptr(), n, out.data_ptr()), "spg_sample")
+        return out[:n, 0] if width == 1 else out[:n]
+
+    def cell_coords(self):
+        """(nblocks, 512, 3) integer coordinates of every active cell (the
+        block origin plus the cell's (x, y, z), k = (x 8 + y) 8 + z)."""
+        nb = self.table.size()
+        org = self.table.active[:nb].cpu().numpy().astype(np.int64)
+        k = np.arange(512)
+        off = np.stack([k >> 6, (k >> 3) & 7, k & 7], -1)
+        return org[:, None, :] + off[None, :, :]
 
 
 class DeviceGrid:
