@@ -436,6 +436,125 @@ def adapters_leg(torch, rocm, sc, reps: int = 20):
     return out
 
 
+def sparse_grid_leg(torch, rocm, sc, reps: int = 10):
+    """The SparseGrid on zpc's bucketed hash table at the bench scene's size:
+    - activation of SparseGrid<3, f32, 8> blocks from all particle positions
+      (one key per particle, a few thousand distinct blocks);
+    - insertion of 16M random keys with 4M distinct ones into a fresh table,
+      and 16M queries of present keys (16-lane bucket compares);
+    - trilinear samples of a signed-distance channel at every particle;
+    - the grid update of the bench scene with and without a level-set
+      boundary (a planar signed distance under the column: slip floor).
+    Times are HIP events on the policy's stream (insertions synchronise once
+    per placement round, so their wall time is what a caller sees)."""
+    import numpy as np
+
+    pol = rocm.Policy()
+    out = {"workload": f"{sc.n} particles of the bench scene; 2^24 random keys with 2^22 distinct"}
+    x = torch.as_tensor(sc.x).cuda()
+    dx8 = sc.dx  # one SparseGrid voxel per MPM cell
+    i2w = np.eye(4)
+    i2w[:3, :3] *= dx8
+
+    def ev_time(fn, n=reps):
+        fn()
+        pol.sync()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(pol.stream)
+        for _ in range(n):
+            fn()
+        b.record(pol.stream)
+        b.synchronize()
+        pol.sync()
+        return a.elapsed_time(b) / n
+
+    g = rocm.DeviceSparseGrid(8192, 1, i2w, background=1.0, pol=pol)
+
+    def activate():
+        g.table.clear()
+        g.activate(x)
+
+    ms = ev_time(activate)
+    blocks = g.table.size()
+    out["activate"] = {"ms": ms, "keys_per_s": sc.n / (ms * 1e-3), "blocks": blocks}
+    print(f"# spg activate: {ms:.3f} ms for {sc.n} positions, {blocks} blocks", file=sys.stderr, flush=True)
+    # a planar signed distance y - y_floor on every cell
+    coords = g.cell_coords()
+    y_floor = float(sc.x[:, 1].min()) + 0.5 * sc.dx
+    vals = ((coords[..., 1].astype(np.float32) * np.float32(dx8)) - np.float32(y_floor)).reshape(blocks, 1, 512)
+    g.data[:blocks].copy_(torch.as_tensor(vals).cuda())
+    samp = torch.empty(sc.n, dtype=torch.float32, device="cuda")
+
+    def sample():
+        check_ = rocm.check
+        check_(pol.L.zsmpm_spg_sample__rocm(pol.h, rocm.C.byref(g.desc()), 0, 1, x.data_ptr(), sc.n,
+                                            samp.data_ptr()), "spg_sample")
+
+    ms = ev_time(sample)
+    out["sample"] = {"ms": ms, "samples_per_s": sc.n / (ms * 1e-3)}
+    print(f"# spg sample: {ms:.3f} ms", file=sys.stderr, flush=True)
+
+    n_keys, n_distinct = 1 << 24, 1 << 22
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    distinct = torch.randint(-(1 << 20), 1 << 20, (n_distinct, 3), dtype=torch.int32, device="cuda", generator=gen)
+    keys = distinct[torch.randint(0, n_distinct, (n_keys,), device="cuda", generator=gen)].contiguous()
+    t = rocm.DeviceBht(rocm.bht_table_size(n_distinct), pol)
+    res = torch.empty(n_keys, dtype=torch.int32, device="cuda")
+
+    def insert():
+        t.clear()
+        t.insert(keys, out=res)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    insert()
+    torch.cuda.synchronize()
+    first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(3):
+        insert()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / 3
+    pol.sync()
+    rounds = int(pol.counters()[10])
+    out["bht_insert"] = {"ms": ms, "keys_per_s": n_keys / (ms * 1e-3), "distinct": t.size(), "rounds": rounds,
+                         "table_size": t.T, "first_call_ms": first * 1e3}
+    print(f"# bht insert: {ms:.3f} ms for {n_keys} keys ({t.size()} distinct, {rounds} rounds)", file=sys.stderr,
+          flush=True)
+    q = keys
+
+    def query():
+        t.query(q, out=res)
+
+    ms = ev_time(query)
+    bytes_q = n_keys * (12 + 4 + 256)  # key, result, one bucket of 16 storage keys
+    out["bht_query"] = {"ms": ms, "keys_per_s": n_keys / (ms * 1e-3), "GB/s_one_bucket": bytes_q / (ms * 1e-3) / 1e9}
+    print(f"# bht query: {ms:.3f} ms", file=sys.stderr, flush=True)
+    del keys, distinct, res, t
+
+    sim = rocm.Simulator(sc, reorder=True)
+    sim.substeps(2)
+    sim.sync()
+
+    def update():
+        sim.grid_update(fold=0)
+
+    ms0 = ev_time(update, 20)
+    sim.pol.set_levelset_boundary(g, 0, -1, 1)
+    ms1 = ev_time(update, 20)
+    sim.pol.set_levelset_boundary(None)
+    out["grid_update_levelset"] = {"ms_without": ms0, "ms_with": ms1,
+                                   "boundary": "planar signed distance y - y_floor in a SparseGrid over the column, "
+                                               "slip"}
+    print(f"# grid update: {ms0:.3f} ms, with the level-set boundary {ms1:.3f} ms", file=sys.stderr, flush=True)
+    del sim, g, x, samp
+    pol.close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -461,6 +580,7 @@ def main() -> int:
                     help="first run one device-to-device copy of this many bytes (the reference kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
     ap.add_argument("--no-adapters", action="store_true", help="skip the layout adapters' leg")
+    ap.add_argument("--no-sparse-grid", action="store_true", help="skip the SparseGrid / bht leg")
     ap.add_argument("--workgroups", default="",
                     help="P2G,G2P workgroup counts to launch with in place of the autotuned ones (the 8 "
                          "tuning substeps still run, untuned), e.g. to hold a counter pass to a timed run's widths")
@@ -526,6 +646,13 @@ def main() -> int:
             x_plain = None
             torch.cuda.empty_cache()
         adapters = adapters_leg(torch, rocm, sc)
+    sparse_grid = None
+    if not args.no_sparse_grid:
+        if x_plain is not None:
+            del x_plain
+            x_plain = None
+            torch.cuda.empty_cache()
+        sparse_grid = sparse_grid_leg(torch, rocm, sc)
     per_launch = {k: (ms / calls if calls else 0.0) for k, (ms, calls) in stages.items()}
     dom = max(per_launch, key=lambda k: stages[k][0])
     dom_bytes = stage_bytes(dom, sc.n, blocks, plastic, sc.table_size)
@@ -592,6 +719,7 @@ def main() -> int:
         "surrogate": surrogate,
         "batched": batched,
         "adapters": adapters,
+        "sparse_grid": sparse_grid,
         "cpu_baseline": None,
     }
     del x_plain

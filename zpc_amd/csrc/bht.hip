@@ -25,7 +25,8 @@ namespace {
 
 constexpr int kBT = 256;
 constexpr uint64_t kEmpty64 = ~0ull;
-constexpr uint32_t kNone = 0xffffffffu;
+constexpr uint32_t kNone = 0xffffffffu;    // an input key out of range
+constexpr uint32_t kRepeat = 0xfffffffeu;  // an input equal to the one before it
 constexpr int kMaxRounds = 64;
 
 int groups_n(int64_t n, int per = kBT) {
@@ -75,14 +76,34 @@ __global__ __launch_bounds__(kBT) void dedup_insert_kernel(const int32_t* __rest
                                                            uint64_t* skey, uint32_t* smin, int64_t S,
                                                            uint32_t* slot_of, uint32_t* err) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
-    const int32_t k0 = keys[3 * i], k1 = keys[3 * i + 1], k2 = keys[3 * i + 2];
-    if (!key_in_range(k0) || !key_in_range(k1) || !key_in_range(k2)) {
+  // whole waves iterate together, so the lanes can compare keys with their
+  // neighbour before touching the hash
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < n; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    const bool live = i < n;
+    int32_t k0 = 0, k1 = 0, k2 = 0;
+    if (live) {
+      k0 = keys[3 * i];
+      k1 = keys[3 * i + 1];
+      k2 = keys[3 * i + 2];
+    }
+    const bool valid = live && key_in_range(k0) && key_in_range(k1) && key_in_range(k2);
+    const uint64_t p = valid ? pack_key(k0, k1, k2) : kEmpty64;
+    // a lane whose left neighbour holds the same key is not its first
+    // occurrence: only the first lane of each run goes to the hash (runs are
+    // common: neighbouring particles share a block)
+    const uint64_t left = __shfl_up(p, 1, kWave);
+    const bool repeat = lane_id() > 0 && valid && left == p;
+    if (!live) continue;
+    if (!valid) {
       atomicOr(err, kErrKeyRange);
       slot_of[i] = kNone;
       continue;
     }
-    const uint64_t p = pack_key(k0, k1, k2);
+    if (repeat) {
+      slot_of[i] = kRepeat;
+      continue;
+    }
     uint64_t h = mix64(p) & (uint64_t)(S - 1);
     for (;;) {
       const uint64_t cur = skey[h];
@@ -107,7 +128,7 @@ __global__ __launch_bounds__(kBT) void first_kernel(const int32_t* __restrict__ 
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint32_t s = slot_of[i];
     uint32_t f = 0;
-    if (s != kNone && smin[s] == (uint32_t)i)
+    if (s < kRepeat && smin[s] == (uint32_t)i)
       f = bht_query(t, keys[3 * i], keys[3 * i + 1], keys[3 * i + 2]) < 0 ? 1u : 0u;
     flag[i] = f;
   }
