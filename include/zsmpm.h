@@ -437,6 +437,51 @@ int32_t zsmpm_mlp_train_step__rocm(zsmpm_policy pol, float* params, float* adam_
                                    float* loss);
 
 /* ------------------------------------------------------------------------
+ * Implicit step: G2P2GTransfer and ImplicitMPMSystem with the conjugate
+ * gradient of math/linear/ConjugateGradient.hpp
+ * (simulation/transfer/G2P2G.hpp:25-150; simulation/mpm/ImplicitMPM.hpp:13-159).
+ * Grid vectors hold 3 floats per node, node = block * 64 + cell (zpc's dof
+ * views over the grid); callers size them grid->capacity_blocks * 64 * 3; the
+ * blocks the table holds are used. The grid's mass channel weights them.
+ * ---------------------------------------------------------------------- */
+
+/* G2P2GTransfer: per particle, C from the node velocities v, F = (I + dt C)
+ * F_old (not stored), the model's vol P F^T times D = 4 / dx^2 (logJp read,
+ * not written back), scattered into r: r[3 node + d] += W (PF[d] xixp0 +
+ * PF[3 + d] xixp1 + PF[6 + d] xixp2). */
+int32_t zsmpm_g2p2g__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
+                          const zsmpm_table* table, const zsmpm_grid* grid, float dt, const float* v, float* r);
+/* ImplicitMPMSystem::multiply: out = G2P2G(in) from zero, then out_i =
+ * (out_i dt^2 + m_i) in_i on nodes with mass (ForceDtSqrPlusMass); other
+ * nodes keep the force. in and out must differ. */
+int32_t zsmpm_implicit_multiply__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
+                                      const zsmpm_table* table, const zsmpm_grid* grid, float dt, const float* in,
+                                      float* out);
+/* ImplicitMPMSystem::project: with any boundary (the call's plane colliders,
+ * the policy's general colliders and level-set boundary), nodes with mass
+ * resolve their vector against each in turn and the others are cleared;
+ * without one nothing changes. */
+int32_t zsmpm_implicit_project__rocm(zsmpm_policy pol, const zsmpm_table* table, const zsmpm_grid* grid,
+                                     const zsmpm_plane_collider* colliders, int32_t ncolliders, float* dof);
+/* ImplicitMPMSystem::precondition: out_i = in_i / m_i on nodes with mass;
+ * other entries of out are left alone. */
+int32_t zsmpm_implicit_precondition__rocm(zsmpm_policy pol, const zsmpm_table* table, const zsmpm_grid* grid,
+                                          const float* in, float* out);
+/* ConjugateGradient::solve on that system: r = project(b - A x), q = M r,
+ * p = q; while sqrt(r.q) > min(rel_tol sqrt(r0.q0), tol) and fewer than
+ * max_iters iterations: temp = project(A p), alpha = r.q / temp.p,
+ * x += alpha p, r -= alpha temp, q = M r, beta = r.q / previous, p = q +
+ * beta p. zpc's defaults: tol 1e-6, rel_tol 0.5, max_iters 1000. x (device)
+ * holds the start and receives the result; iters and residual (host, may be
+ * NULL) the iterations taken and sqrt(r.q). Dot products are summed in
+ * double and read back each iteration (the reference sums in float).
+ * Synchronises the policy's stream. */
+int32_t zsmpm_implicit_solve__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
+                                   const zsmpm_table* table, const zsmpm_grid* grid, float dt,
+                                   const zsmpm_plane_collider* colliders, int32_t ncolliders, float* x, const float* b,
+                                   int32_t max_iters, float tol, float rel_tol, int32_t* iters, float* residual);
+
+/* ------------------------------------------------------------------------
  * Bucketed hash table and SparseGrid
  * ---------------------------------------------------------------------- */
 

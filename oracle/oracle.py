@@ -133,6 +133,15 @@ def lib():
             getattr(L, name).argtypes = [C.POINTER(Params), C.c_int32, C.c_int64, _f64p, _f64p]
         L.zso_level_set_normal_f64.argtypes = [C.POINTER(Params), C.c_int32, C.c_int64, _f64p, _f64p]
         L.zso_resolve_general_f32.argtypes = [C.POINTER(Params), C.c_int32, C.c_int64, _f32p, _f32p]
+        L.zso_g2p2g_f32.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int64, _f32p, _f32p, C.c_void_p, _f32p,
+                                    _f32p]
+        L.zso_implicit_multiply_f32.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int64, _f32p, _f32p, C.c_void_p,
+                                                _f32p, _f32p]
+        L.zso_implicit_project_f32.argtypes = [C.c_void_p, C.POINTER(Params), _f32p]
+        L.zso_implicit_precondition_f32.argtypes = [C.c_void_p, _f32p, _f32p]
+        L.zso_implicit_solve_f32.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int64, _f32p, _f32p, C.c_void_p,
+                                             _f32p, _f32p, C.c_int32, C.c_float, C.c_float, C.POINTER(C.c_float)]
+        L.zso_implicit_solve_f32.restype = C.c_int32
         _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
         L.zso_mt19937.argtypes = [C.c_uint32, C.c_int64, _u32p]
         L.zso_bht_hash_params.argtypes = [C.c_uint32, _u32p]
@@ -425,6 +434,37 @@ class Context:
 
     def grid_update(self, params):
         return self.L.zso_grid_update_f32(self.h, C.byref(params))
+
+    # implicit step on this context's table and P2G grid (vectors: 3 floats per node)
+    def g2p2g(self, params, ps, v):
+        r = np.zeros(self.sizes()["cnt"] * 64 * 3, np.float32)
+        self.L.zso_g2p2g_f32(self.h, C.byref(params), ps.n, ps.x, ps.F, _ptr(ps.logJp),
+                             np.ascontiguousarray(v, np.float32), r)
+        return r
+
+    def implicit_multiply(self, params, ps, vin):
+        out = np.zeros(self.sizes()["cnt"] * 64 * 3, np.float32)
+        self.L.zso_implicit_multiply_f32(self.h, C.byref(params), ps.n, ps.x, ps.F, _ptr(ps.logJp),
+                                         np.ascontiguousarray(vin, np.float32), out)
+        return out
+
+    def implicit_project(self, params, dof):
+        d = np.ascontiguousarray(dof, np.float32).copy()
+        self.L.zso_implicit_project_f32(self.h, C.byref(params), d)
+        return d
+
+    def implicit_precondition(self, vin, vout):
+        o = np.ascontiguousarray(vout, np.float32).copy()
+        self.L.zso_implicit_precondition_f32(self.h, np.ascontiguousarray(vin, np.float32), o)
+        return o
+
+    def implicit_solve(self, params, ps, x0, b, max_iters=1000, tol=1e-6, rel_tol=0.5):
+        x = np.ascontiguousarray(x0, np.float32).copy()
+        res = C.c_float(0)
+        it = self.L.zso_implicit_solve_f32(self.h, C.byref(params), ps.n, ps.x, ps.F, _ptr(ps.logJp), x,
+                                           np.ascontiguousarray(b, np.float32), max_iters, tol, rel_tol,
+                                           C.byref(res))
+        return x, int(it), float(res.value)
 
     def g2p(self, params, ps):
         self.L.zso_g2p_f32(self.h, C.byref(params), ps.n, ps.x, ps.v, ps.C, ps.F)

@@ -374,6 +374,40 @@ void zso_reorder_f32(int64_t n, const int32_t* map, float* x, float* v, float* C
   reorder_particles(view<float>(n, x, v, C, F, m, logJp), map, ids);
 }
 
+// ---- implicit step (zsmpm_oracle.hpp) on the context's table and P2G grid ----
+
+void zso_g2p2g_f32(void* c, const zso_params* prm, int64_t n, float* x, float* F, float* logJp, const float* v,
+                   float* r) {
+  auto* ctx = static_cast<Ctx<float>*>(c);
+  auto sp = to_params<float>(prm, ctx->cols);
+  g2p2g(sp.mat, sp.dt, sp.dx, view<float>(n, x, nullptr, nullptr, F, nullptr, logJp), ctx->out.table, v, r);
+}
+void zso_implicit_multiply_f32(void* c, const zso_params* prm, int64_t n, float* x, float* F, float* logJp,
+                               const float* in, float* out) {
+  auto* ctx = static_cast<Ctx<float>*>(c);
+  auto sp = to_params<float>(prm, ctx->cols);
+  implicit_multiply(sp.mat, sp.dt, sp.dx, view<float>(n, x, nullptr, nullptr, F, nullptr, logJp), ctx->out.table,
+                    ctx->out.grid.data(), (i64)ctx->out.table.cnt, in, out);
+}
+void zso_implicit_project_f32(void* c, const zso_params* prm, float* dof) {
+  auto* ctx = static_cast<Ctx<float>*>(c);
+  auto sp = to_params<float>(prm, ctx->cols);
+  implicit_project(ctx->out.grid.data(), ctx->out.table, (i64)ctx->out.table.cnt, sp.dx, sp.cols, sp.ncols,
+                   sp.gcols, sp.ngcols, dof);
+}
+void zso_implicit_precondition_f32(void* c, const float* in, float* out) {
+  auto* ctx = static_cast<Ctx<float>*>(c);
+  implicit_precondition(ctx->out.grid.data(), (i64)ctx->out.table.cnt, in, out);
+}
+int32_t zso_implicit_solve_f32(void* c, const zso_params* prm, int64_t n, float* x, float* F, float* logJp,
+                               float* xdof, const float* b, int32_t max_iters, float tol, float rel_tol,
+                               float* residual) {
+  auto* ctx = static_cast<Ctx<float>*>(c);
+  auto sp = to_params<float>(prm, ctx->cols);
+  return implicit_solve(sp, view<float>(n, x, nullptr, nullptr, F, nullptr, logJp), ctx->out.table,
+                        ctx->out.grid.data(), (i64)ctx->out.table.cnt, xdof, b, max_iters, tol, rel_tol, residual);
+}
+
 // ---- bht, SparseGrid, level-set boundary (bht_oracle.hpp) ----
 
 /// n raw draws of MT19937 seeded with `seed`.

@@ -1238,6 +1238,176 @@ struct StepOutputs {
   T maxVel = 0;
 };
 
+// ----------------------------------------------------------------------------
+// implicit step: G2P2G and the ImplicitMPMSystem operators, the PCG solve
+// ----------------------------------------------------------------------------
+
+/// G2P2GTransfer::operator() over range(N), simulation/transfer/G2P2G.hpp:49-146
+/// (APIC, quadratic): C from the node vector v (3 floats per node, node =
+/// block * 64 + cell), F = (I + dt C) F_old (not stored), contrib = the
+/// model's vol P F^T times D_inv (logJp read, not written back), and
+/// r[3 node + d] += W (contrib[d] xixp0 + contrib[3+d] xixp1 + contrib[6+d] xixp2).
+template <typename T>
+void g2p2g(const Material<T>& mat, T dt, T dx, ParticlesView<T> ps, Table& t, const T* v, T* r) {
+  const T dx_inv = (T)1 / dx;
+  const T D_inv = (T)4.f * dx_inv * dx_inv;
+  T mu, lambda;
+  lame(mat.E, mat.nu, mu, lambda);
+  for (i64 p = 0; p < ps.n; ++p) {
+    Arena<T> ar;
+    ar.init(dx, ps.x + 3 * p);
+    T C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) {
+          i32 coord[3] = {ar.corner[0] + i, ar.corner[1] + j, ar.corner[2] + k};
+          i32 b;
+          int cell;
+          unpack_coord(t, coord, b, cell);
+          T xixp[3];
+          ar.diff(i, j, k, xixp);
+          const T W = ar.weight(i, j, k);
+          const T* vi = v + 3 * ((i64)b * 64 + cell);
+          for (int d = 0; d < 9; ++d) C[d] += W * vi[d % 3] * xixp[d / 3] * D_inv;
+        }
+    T tmp[9], F[9], contrib[9];
+    for (int d = 0; d < 9; ++d) tmp[d] = C[d] * dt + ((d & 0x3) ? (T)0.f : (T)1.f);
+    matmul3(tmp, ps.F + 9 * p, F);
+    if (mat.kind == kFixedCorotated) {
+      stress_fixedcorotated(mat.volume, mu, lambda, F, contrib);
+    } else {
+      T logJp = ps.logJp[p];
+      stress_sand(mat.volume, mu, lambda, mat.cohesion, mat.beta, mat.yieldSurface, mat.volumeCorrection, logJp,
+                  F, contrib);
+    }
+    for (int d = 0; d < 9; ++d) contrib[d] = contrib[d] * D_inv;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) {
+          i32 coord[3] = {ar.corner[0] + i, ar.corner[1] + j, ar.corner[2] + k};
+          i32 b;
+          int cell;
+          unpack_coord(t, coord, b, cell);
+          T xixp[3];
+          ar.diff(i, j, k, xixp);
+          const T W = ar.weight(i, j, k);
+          T* ri = r + 3 * ((i64)b * 64 + cell);
+          for (int d = 0; d < 3; ++d)
+            ri[d] += W * (contrib[d] * xixp[0] + contrib[3 + d] * xixp[1] + contrib[6 + d] * xixp[2]);
+        }
+  }
+}
+
+/// ImplicitMPMSystem::multiply (simulation/mpm/ImplicitMPM.hpp:35-63): out = 0,
+/// out += G2P2G(in), then out_i = (out_i dt dt + m) in_i on nodes with m > 0
+/// (ForceDtSqrPlusMass, :17-33); other nodes keep the force.
+template <typename T>
+void implicit_multiply(const Material<T>& mat, T dt, T dx, ParticlesView<T> ps, Table& t, const T* grid,
+                       i64 nblocks, const T* in, T* out) {
+  const i64 n = nblocks * 64 * 3;
+  for (i64 i = 0; i < n; ++i) out[i] = 0;
+  g2p2g(mat, dt, dx, ps, t, in, out);
+  for (i64 i = 0; i < n; ++i) {
+    const i64 node = i / 3;
+    const T mass = grid[gidx(node / 64, 0, (int)(node % 64))];
+    if (mass > 0) out[i] = (out[i] * dt * dt + mass) * in[i];
+  }
+}
+
+/// ImplicitMPMSystem::project (:103-127) with its Projector (:65-101), once per
+/// boundary in list order (planes, then general colliders): nodes with m > 0
+/// resolve the collision on their vector, other nodes are cleared. Without
+/// boundaries nothing changes.
+template <typename T>
+void implicit_project(const T* grid, const Table& t, i64 nblocks, T dx, const PlaneCollider<T>* cols, int ncols,
+                      const GeneralCollider<T>* gcols, int ngcols, T* dof) {
+  for (int ci = 0; ci < ncols + ngcols; ++ci)
+    for (i64 b = 0; b < nblocks; ++b)
+      for (int c = 0; c < 64; ++c) {
+        T* v = dof + 3 * (b * 64 + c);
+        if (!(grid[gidx(b, 0, c)] > 0)) {
+          v[0] = v[1] = v[2] = 0;
+          continue;
+        }
+        const i32 cc[3] = {(c >> 4) & 3, (c >> 2) & 3, c & 3};
+        T pos[3];
+        for (int d = 0; d < 3; ++d) pos[d] = ((T)t.active[3 * b + d] * (T)4 + (T)cc[d]) * dx;
+        if (ci < ncols) {
+          const PlaneCollider<T>& col = cols[ci];
+          const T phi = col.normal[0] * (pos[0] - col.origin[0]) + col.normal[1] * (pos[1] - col.origin[1]) +
+                        col.normal[2] * (pos[2] - col.origin[2]);
+          if (phi < (T)0) {
+            if (col.type == kSticky) {
+              v[0] = v[1] = v[2] = 0;
+            } else {
+              const T proj = col.normal[0] * v[0] + col.normal[1] * v[1] + col.normal[2] * v[2];
+              if ((col.type == kSeparate && proj < 0) || col.type == kSlip)
+                for (int d = 0; d < 3; ++d) v[d] -= proj * col.normal[d];
+            }
+          }
+        } else {
+          resolve_general(gcols[ci - ncols], pos, v);
+        }
+      }
+}
+
+/// ImplicitMPMSystem::precondition (:129-155, DivPernodeMass): out_i = in_i / m
+/// on nodes with m > 0; other entries of out are left alone.
+template <typename T>
+void implicit_precondition(const T* grid, i64 nblocks, const T* in, T* out) {
+  const i64 n = nblocks * 64 * 3;
+  for (i64 i = 0; i < n; ++i) {
+    const i64 node = i / 3;
+    const T mass = grid[gidx(node / 64, 0, (int)(node % 64))];
+    if (mass > 0) out[i] = in[i] / mass;
+  }
+}
+
+/// ConjugateGradient::solve (math/linear/ConjugateGradient.hpp:70-175) on the
+/// system above: r = project(b - A x), q = M r, p = q, zTrk = r.q,
+/// tolerance min(relTol sqrt(zTrk), tol); per iteration temp = project(A p),
+/// alpha = zTrk / temp.p, x += alpha p, r -= alpha temp, q = M r,
+/// beta = (q.r) / zTrk, p = q + beta p. Dot products are summed in T in index
+/// order. Returns the iterations taken.
+template <typename T>
+int implicit_solve(const StepParams<T>& sp, ParticlesView<T> ps, Table& t, const T* grid, i64 nblocks, T* x,
+                   const T* b, int maxIters, T tol, T relTol, T* residual) {
+  const i64 n = nblocks * 64 * 3;
+  std::vector<T> r(n), p(n), q(n, 0), temp(n);
+  auto dot = [n](const T* a, const T* c) {
+    T s = 0;
+    for (i64 i = 0; i < n; ++i) s += a[i] * c[i];
+    return s;
+  };
+  auto A = [&](const T* in, T* out) { implicit_multiply(sp.mat, sp.dt, sp.dx, ps, t, grid, nblocks, in, out); };
+  auto project = [&](T* v) { implicit_project(grid, t, nblocks, sp.dx, sp.cols, sp.ncols, sp.gcols, sp.ngcols, v); };
+  A(x, temp.data());
+  for (i64 i = 0; i < n; ++i) r[i] = b[i] - temp[i];
+  project(r.data());
+  implicit_precondition(grid, nblocks, r.data(), q.data());
+  p = q;
+  T zTrk = dot(r.data(), q.data());
+  T norm = std::sqrt(zTrk);
+  const T localTol = std::min(relTol * norm, tol);
+  int iter = 0;
+  for (; iter != maxIters; ++iter) {
+    if (norm <= localTol) break;
+    A(p.data(), temp.data());
+    project(temp.data());
+    const T alpha = zTrk / dot(temp.data(), p.data());
+    for (i64 i = 0; i < n; ++i) x[i] = alpha * p[i] + x[i];
+    for (i64 i = 0; i < n; ++i) r[i] = -alpha * temp[i] + r[i];
+    implicit_precondition(grid, nblocks, r.data(), q.data());
+    const T zLast = zTrk;
+    zTrk = dot(q.data(), r.data());
+    const T beta = zTrk / zLast;
+    for (i64 i = 0; i < n; ++i) p[i] = beta * p[i] + q[i];
+    norm = std::sqrt(zTrk);
+  }
+  if (residual) *residual = norm;
+  return iter;
+}
+
 /// Vector<T>::reorder(pol, map, false_c) applied to every attribute
 /// (container/Vector.hpp:617-646, the gather form ordered[i] = vs[map[i]]),
 /// with an optional particle-id array carried along.

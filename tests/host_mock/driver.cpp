@@ -139,6 +139,22 @@ int main(int argc, char** argv) {
     expect("tile reorder", zsmpm_tiles_reorder__rocm(pol, tiles.data(), tiles2.data(), map.data(), nt, 32, 27, 0), 0);
     expect("tile reorder in place rejected", zsmpm_tiles_reorder__rocm(pol, tiles.data(), tiles.data(), map.data(), nt, 32, 27, 1), ZSMPM_INVALID_ARGUMENT);
   }
+  // implicit step: argument validation, the solver's vectors
+  {
+    Sc& s0 = sc[0];
+    const size_t nv = (size_t)s0.g.capacity_blocks * 64 * 3;
+    std::vector<float> xv(nv), bv(nv), ov(nv);
+    expect("g2p2g", zsmpm_g2p2g__rocm(pol, &s0.model, &s0.a, &s0.t, &s0.g, 1e-4f, xv.data(), ov.data()), 0);
+    expect("implicit multiply", zsmpm_implicit_multiply__rocm(pol, &s0.model, &s0.a, &s0.t, &s0.g, 1e-4f, xv.data(), ov.data()), 0);
+    expect("implicit multiply in place rejected", zsmpm_implicit_multiply__rocm(pol, &s0.model, &s0.a, &s0.t, &s0.g, 1e-4f, xv.data(), xv.data()), ZSMPM_INVALID_ARGUMENT);
+    expect("implicit project", zsmpm_implicit_project__rocm(pol, &s0.t, &s0.g, s0.cols, 6, ov.data()), 0);
+    expect("implicit precondition", zsmpm_implicit_precondition__rocm(pol, &s0.t, &s0.g, ov.data(), xv.data()), 0);
+    int32_t its = -1;
+    float res = -1.f;
+    expect("implicit solve", zsmpm_implicit_solve__rocm(pol, &s0.model, &s0.a, &s0.t, &s0.g, 1e-4f, s0.cols, 6, xv.data(), bv.data(), 10, 1e-6f, 0.5f, &its, &res), 0);
+    expect("implicit solve reports", its >= 0 && res >= 0.f, 1);
+    expect("implicit solve bad tol", zsmpm_implicit_solve__rocm(pol, &s0.model, &s0.a, &s0.t, &s0.g, 1e-4f, s0.cols, 6, xv.data(), bv.data(), 10, -1.f, 0.5f, &its, &res), ZSMPM_INVALID_ARGUMENT);
+  }
   // bht and SparseGrid: argument validation and the policy's scratch
   {
     expect("bht table size", zsmpm_bht_table_size(1000) == 2064 && zsmpm_bht_table_size(0) == 0, 1);

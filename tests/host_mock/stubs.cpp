@@ -36,6 +36,13 @@ hipError_t grid_update(float*, int32_t, const TableView&, float, const float*, c
 hipError_t g2p(const ParticlesView&, const ParticlesView*, const TableView&, const SortView&, const float*, int32_t, float, float, Workspace&, hipStream_t s, const NextStress*, const SceneSet* ss) { return scenes_ok(ss, s); }
 hipError_t grid_angular_momentum(const float*, int32_t, const TableView&, float, int, int, double*, Workspace&, hipStream_t s) { return touch(s); }
 hipError_t reorder_particles(const ParticlesView&, const ParticlesView&, const int32_t*, Workspace&, hipStream_t s) { return touch(s); }
+hipError_t g2p2g_force(const ParticlesView&, const TableView&, const Model&, float, float, const float*, float*, uint32_t*, hipStream_t s) { return touch(s); }
+hipError_t implicit_multiply(const ParticlesView&, const TableView&, const Model&, const float*, int32_t, float, float, const float* in, float* out, uint32_t*, hipStream_t s) { return in == out ? hipErrorInvalidValue : touch(s); }
+hipError_t implicit_project(const float*, int32_t, const TableView&, float, const Colliders&, const GeneralColliders&, float*, hipStream_t s) { return touch(s); }
+hipError_t implicit_precondition(const float*, int32_t, const TableView&, const float*, float*, hipStream_t s) { return touch(s); }
+hipError_t implicit_lincomb(const TableView&, int32_t, float, const float*, float, const float*, float*, hipStream_t s) { return touch(s); }
+hipError_t implicit_dot(const TableView&, int32_t, const float*, const float*, double*, hipStream_t s) { return touch(s); }
+hipError_t implicit_fill(const TableView&, int32_t, float*, float, hipStream_t s) { return touch(s); }
 size_t bht_scratch_bytes(int64_t n, int32_t nb) { return 1024 + 64 * (size_t)(n > 0 ? n : 1) + 16 * (size_t)nb; }
 hipError_t bht_clear(const BhtView&, hipStream_t s) { return touch(s); }
 int32_t bht_insert(const BhtView& t, const int32_t*, int64_t, int32_t*, bool, void* scratch, uint32_t*, hipStream_t s, int32_t* rounds) {

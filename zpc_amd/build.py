@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libzsmpm_rocm.so")
-SOURCES = ["primitives.hip", "sparsity.hip", "transfer.hip", "closure.hip", "io.hip", "bht.hip", "abi.hip"]
+SOURCES = ["primitives.hip", "sparsity.hip", "transfer.hip", "closure.hip", "io.hip", "bht.hip", "implicit.hip", "abi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

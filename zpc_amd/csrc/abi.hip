@@ -43,6 +43,9 @@ struct zsmpm_policy_s {
   // zsmpm_table_activate_noalloc__rocm ran on this policy: its device-side
   // errors are in the device's fixed scratch array
   bool noalloc_used = false;
+  // vectors of the implicit solve (grown on demand)
+  void* cg_buf = nullptr;
+  size_t cg_bytes = 0;
   // scratch of bht insertions (grown on demand) and the rounds the last took
   void* bht_buf = nullptr;
   size_t bht_bytes = 0;
@@ -862,6 +865,7 @@ void destroy_policy(zsmpm_policy_s* p) {
   if (p->scene_p) (void)hipFree(p->scene_p);
   if (p->scene_first) (void)hipFree(p->scene_first);
   if (p->bht_buf) (void)hipFree(p->bht_buf);
+  if (p->cg_buf) (void)hipFree(p->cg_buf);
   if (p->cap) (void)hipStreamDestroy(p->cap);
   delete p;
 }
@@ -1215,6 +1219,147 @@ int32_t zsmpm_reorder__rocm(zsmpm_policy pol, const zsmpm_buckets* buckets, cons
   DeviceGuard g(pol->devid);
   ZS_TRY(ensure(pol, pi.n, pol->reserved_T));
   return from_hip(reorder_particles(pi, po, sv.indices, pol->ws, pol->stream));
+}
+
+// ---- implicit step ----
+
+int32_t zsmpm_g2p2g__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
+                          const zsmpm_table* table, const zsmpm_grid* grid, float dt, const float* v, float* r) {
+  if (!pol || !v || !r) return ZSMPM_INVALID_ARGUMENT;
+  Model m;
+  ParticlesView ps;
+  TableView t;
+  ZS_TRY(make_model(model, m));
+  ZS_TRY(view_particles(particles, ps, true));
+  if (m.kind == 1 && !ps.logJp) return ZSMPM_INVALID_ARGUMENT;
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(view_grid(grid));
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, 0, 0));
+  return from_hip(g2p2g_force(ps, t, m, dt, grid->dx, v, r, pol->ws.err, pol->stream));
+}
+
+int32_t zsmpm_implicit_multiply__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
+                                      const zsmpm_table* table, const zsmpm_grid* grid, float dt, const float* in,
+                                      float* out) {
+  if (!pol || !in || !out || in == out) return ZSMPM_INVALID_ARGUMENT;
+  Model m;
+  ParticlesView ps;
+  TableView t;
+  ZS_TRY(make_model(model, m));
+  ZS_TRY(view_particles(particles, ps, true));
+  if (m.kind == 1 && !ps.logJp) return ZSMPM_INVALID_ARGUMENT;
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(view_grid(grid));
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, 0, 0));
+  return from_hip(implicit_multiply(ps, t, m, grid->data, grid->capacity_blocks, dt, grid->dx, in, out, pol->ws.err,
+                                    pol->stream));
+}
+
+int32_t zsmpm_implicit_project__rocm(zsmpm_policy pol, const zsmpm_table* table, const zsmpm_grid* grid,
+                                     const zsmpm_plane_collider* colliders, int32_t ncolliders, float* dof) {
+  if (!pol || !dof) return ZSMPM_INVALID_ARGUMENT;
+  TableView t;
+  Colliders cols;
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(view_grid(grid));
+  ZS_TRY(make_colliders(colliders, ncolliders, cols));
+  DeviceGuard g(pol->devid);
+  return from_hip(implicit_project(grid->data, grid->capacity_blocks, t, grid->dx, cols, pol->gcols, dof, pol->stream));
+}
+
+int32_t zsmpm_implicit_precondition__rocm(zsmpm_policy pol, const zsmpm_table* table, const zsmpm_grid* grid,
+                                          const float* in, float* out) {
+  if (!pol || !in || !out) return ZSMPM_INVALID_ARGUMENT;
+  TableView t;
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(view_grid(grid));
+  DeviceGuard g(pol->devid);
+  return from_hip(implicit_precondition(grid->data, grid->capacity_blocks, t, in, out, pol->stream));
+}
+
+int32_t zsmpm_implicit_solve__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
+                                   const zsmpm_table* table, const zsmpm_grid* grid, float dt,
+                                   const zsmpm_plane_collider* colliders, int32_t ncolliders, float* x, const float* b,
+                                   int32_t max_iters, float tol, float rel_tol, int32_t* iters, float* residual) {
+  if (!pol || !x || !b || max_iters < 0 || !(tol >= 0.f) || !(rel_tol >= 0.f)) return ZSMPM_INVALID_ARGUMENT;
+  Model m;
+  ParticlesView ps;
+  TableView t;
+  Colliders cols;
+  ZS_TRY(make_model(model, m));
+  ZS_TRY(view_particles(particles, ps, true));
+  if (m.kind == 1 && !ps.logJp) return ZSMPM_INVALID_ARGUMENT;
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(view_grid(grid));
+  ZS_TRY(make_colliders(colliders, ncolliders, cols));
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, 0, 0));
+  hipStream_t s = pol->stream;
+  // the solver's four vectors and a dot-product cell, sized by the grid
+  const size_t nv = (size_t)grid->capacity_blocks * kBlockCells * 3;
+  const size_t need = 4 * nv * sizeof(float) + 256;
+  if (need > pol->cg_bytes) {
+    ZS_HIP(hipStreamSynchronize(s));
+    if (pol->cg_buf) (void)hipFree(pol->cg_buf);
+    pol->cg_buf = nullptr;
+    pol->cg_bytes = 0;
+    ZS_HIP(hipMalloc(&pol->cg_buf, need));
+    pol->cg_bytes = need;
+  }
+  float* r = (float*)pol->cg_buf;
+  float* p = r + nv;
+  float* q = p + nv;
+  float* tmp = q + nv;
+  double* dd = (double*)(tmp + nv);
+  const float* gd = grid->data;
+  const int32_t cap = grid->capacity_blocks;
+  auto dot = [&](const float* a, const float* c, float* outv) -> int32_t {
+    ZS_HIP(implicit_dot(t, cap, a, c, dd, s));
+    double h = 0.0;
+    ZS_HIP(hipMemcpyAsync(&h, dd, sizeof h, hipMemcpyDeviceToHost, s));
+    ZS_HIP(hipStreamSynchronize(s));
+    *outv = (float)h;
+    return ZSMPM_OK;
+  };
+  auto A = [&](const float* in, float* outv) -> int32_t {
+    return from_hip(implicit_multiply(ps, t, m, gd, cap, dt, grid->dx, in, outv, pol->ws.err, s));
+  };
+  auto project = [&](float* v) -> int32_t {
+    return from_hip(implicit_project(gd, cap, t, grid->dx, cols, pol->gcols, v, s));
+  };
+  // ConjugateGradient::solve (math/linear/ConjugateGradient.hpp:70-175)
+  ZS_TRY(A(x, tmp));
+  ZS_HIP(implicit_lincomb(t, cap, 1.f, b, -1.f, tmp, r, s));  // r = b - A x
+  ZS_TRY(project(r));
+  ZS_HIP(implicit_fill(t, cap, q, 0.f, s));                   // q = 0 where the preconditioner leaves it
+  ZS_HIP(implicit_precondition(gd, cap, t, r, q, s));
+  ZS_HIP(implicit_lincomb(t, cap, 1.f, q, 0.f, q, p, s));     // p = q
+  float zTrk = 0.f;
+  ZS_TRY(dot(r, q, &zTrk));
+  float norm = sqrtf(zTrk);
+  const float localTol = fminf(rel_tol * norm, tol);
+  int32_t it = 0;
+  for (; it != max_iters; ++it) {
+    if (norm <= localTol) break;
+    ZS_TRY(A(p, tmp));
+    ZS_TRY(project(tmp));
+    float tp = 0.f;
+    ZS_TRY(dot(tmp, p, &tp));
+    const float alpha = zTrk / tp;
+    ZS_HIP(implicit_lincomb(t, cap, alpha, p, 1.f, x, x, s));
+    ZS_HIP(implicit_lincomb(t, cap, -alpha, tmp, 1.f, r, r, s));
+    ZS_HIP(implicit_precondition(gd, cap, t, r, q, s));
+    const float zLast = zTrk;
+    ZS_TRY(dot(q, r, &zTrk));
+    const float beta = zTrk / zLast;
+    ZS_HIP(implicit_lincomb(t, cap, beta, p, 1.f, q, p, s));
+    norm = sqrtf(zTrk);
+  }
+  if (iters) *iters = it;
+  if (residual) *residual = norm;
+  return ZSMPM_OK;
 }
 
 static int32_t view_bht(const zsmpm_bht* d, BhtView& t) {

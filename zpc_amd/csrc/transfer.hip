@@ -1228,7 +1228,64 @@ __global__ __launch_bounds__(kThreads) void reorder_kernel(ParticlesView in, Par
   }
 }
 
+// ImplicitMPMSystem::Projector (simulation/mpm/ImplicitMPM.hpp:65-101) for
+// every boundary at once: a node with mass resolves its vector against the
+// plane colliders, the general colliders and the level-set boundary in that
+// order; a node without mass is cleared.
+__global__ __launch_bounds__(kThreads) void implicit_project_kernel(const float* __restrict__ grid, int32_t grid_cap,
+                                                                    TableView t, float dx, Colliders cols,
+                                                                    GeneralColliders gen, float* dof) {
+#pragma clang fp contract(off)
+  int32_t nb = *t.cnt;
+  if (nb > grid_cap) nb = grid_cap;
+  const int64_t nodes = (int64_t)nb * kBlockCells;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nodes; i += stride) {
+    const int32_t b = (int32_t)(i >> 6);
+    const int cell = (int)(i & 63);
+    float* dv = dof + 3 * i;
+    if (!(grid[grid_at(b, 0, cell)] > 0.f)) {
+      dv[0] = dv[1] = dv[2] = 0.f;
+      continue;
+    }
+    const int32_t k0 = t.active[3 * (int64_t)b], k1 = t.active[3 * (int64_t)b + 1], k2 = t.active[3 * (int64_t)b + 2];
+    const float pos[3] = {((float)k0 * 4.f + (float)((cell >> 4) & 3)) * dx,
+                          ((float)k1 * 4.f + (float)((cell >> 2) & 3)) * dx,
+                          ((float)k2 * 4.f + (float)(cell & 3)) * dx};
+    float v[3] = {dv[0], dv[1], dv[2]};
+    for (int c = 0; c < cols.n; ++c) {
+      const PlaneCollider& pc = cols.c[c];
+      const float phi = pc.normal[0] * (pos[0] - pc.origin[0]) + pc.normal[1] * (pos[1] - pc.origin[1]) +
+                        pc.normal[2] * (pos[2] - pc.origin[2]);
+      if (phi < 0.f) {
+        if (pc.type == 0) {
+          v[0] = v[1] = v[2] = 0.f;
+        } else {
+          const float proj = pc.normal[0] * v[0] + pc.normal[1] * v[1] + pc.normal[2] * v[2];
+          if ((pc.type == 2 && proj < 0.f) || pc.type == 1) {
+            v[0] -= proj * pc.normal[0];
+            v[1] -= proj * pc.normal[1];
+            v[2] -= proj * pc.normal[2];
+          }
+        }
+      }
+    }
+    for (int c = 0; c < gen.n; ++c) resolve_general(gen.c[c], pos, v);
+    if (gen.ls.on) levelset_resolve(gen.ls, pos, v);
+    dv[0] = v[0];
+    dv[1] = v[1];
+    dv[2] = v[2];
+  }
+}
+
 }  // namespace
+
+hipError_t implicit_project(const float* grid, int32_t grid_cap, const TableView& t, float dx, const Colliders& cols,
+                            const GeneralColliders& gen, float* dof, hipStream_t s) {
+  if (cols.n == 0 && gen.n == 0 && !gen.ls.on) return hipSuccess;  // no boundary: nothing changes
+  implicit_project_kernel<<<persistent_groups(), kThreads, 0, s>>>(grid, grid_cap, t, dx, cols, gen, dof);
+  return hipGetLastError();
+}
 
 // Dynamic LDS the stress pass reserves (and does not use) when it runs beside
 // the sparsity stages: with its 9,280 bytes of static LDS, three workgroups

@@ -305,6 +305,24 @@ hipError_t reorder_particles(const ParticlesView& in, const ParticlesView& out, 
 // grid-stride launch width for kernels that loop over a device-resident count
 inline int persistent_groups() { return 4096; }
 
+// ---- implicit step (implicit.hip; the projection in transfer.hip) ----
+// Vectors hold 3 floats per node (node = block * 64 + cell) for the blocks the
+// table holds (read on the device), at most grid_cap.
+hipError_t g2p2g_force(const ParticlesView& ps, const TableView& t, const Model& m, float dt, float dx,
+                       const float* v, float* r, uint32_t* err, hipStream_t s);
+hipError_t implicit_multiply(const ParticlesView& ps, const TableView& t, const Model& m, const float* grid,
+                             int32_t grid_cap, float dt, float dx, const float* in, float* out, uint32_t* err,
+                             hipStream_t s);
+hipError_t implicit_project(const float* grid, int32_t grid_cap, const TableView& t, float dx, const Colliders& cols,
+                            const GeneralColliders& gen, float* dof, hipStream_t s);
+hipError_t implicit_precondition(const float* grid, int32_t grid_cap, const TableView& t, const float* in, float* out,
+                                 hipStream_t s);
+hipError_t implicit_lincomb(const TableView& t, int32_t grid_cap, float ca, const float* a, float cb, const float* b,
+                            float* c, hipStream_t s);
+hipError_t implicit_dot(const TableView& t, int32_t grid_cap, const float* a, const float* b, double* out,
+                        hipStream_t s);
+hipError_t implicit_fill(const TableView& t, int32_t grid_cap, float* a, float value, hipStream_t s);
+
 // ---- bht and SparseGrid (bht.hip) ----
 size_t bht_scratch_bytes(int64_t n, int32_t nb);
 hipError_t bht_clear(const BhtView& t, hipStream_t s);

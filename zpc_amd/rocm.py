@@ -199,6 +199,13 @@ def lib():
         "zsmpm_policy_colliders__rocm": [vp, P(ColliderDesc), i32],
         "zsmpm_substeps_batched__rocm": [vp, P(BatchSceneDesc), i32, P(ParticlesDesc), P(ParticlesDesc),
                                          P(TableDesc), P(BucketsDesc), P(GridDesc), i32, i32, P(C.c_int32)],
+        "zsmpm_g2p2g__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(TableDesc), P(GridDesc), f32, vp, vp],
+        "zsmpm_implicit_multiply__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(TableDesc), P(GridDesc), f32, vp,
+                                          vp],
+        "zsmpm_implicit_project__rocm": [vp, P(TableDesc), P(GridDesc), P(PlaneCollider), i32, vp],
+        "zsmpm_implicit_precondition__rocm": [vp, P(TableDesc), P(GridDesc), vp, vp],
+        "zsmpm_implicit_solve__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(TableDesc), P(GridDesc), f32,
+                                       P(PlaneCollider), i32, vp, vp, i32, f32, f32, P(C.c_int32), P(C.c_float)],
         "zsmpm_bht_clear__rocm": [vp, P(BhtDesc)],
         "zsmpm_bht_insert__rocm": [vp, P(BhtDesc), vp, i64, vp, i32],
         "zsmpm_bht_query__rocm": [vp, P(BhtDesc), vp, i64, vp],
@@ -796,6 +803,43 @@ class Simulator:
 
     def sync(self):
         self.pol.sync()
+
+    # ---- implicit step (G2P2G, ImplicitMPMSystem, the conjugate gradient) ----
+    def dof_vector(self, fill: float = 0.0):
+        """A grid vector: 3 floats per node of the grid's capacity."""
+        torch = _torch()
+        return torch.full((self.grid.cap * 64 * 3,), float(fill), dtype=torch.float32, device=self.grid.data.device)
+
+    def g2p2g(self, v, r, dt=None):
+        dt = float(self.sc.dt if dt is None else dt)
+        check(self.L.zsmpm_g2p2g__rocm(self.pol.h, C.byref(self.model), C.byref(self.parts.desc),
+                                       C.byref(self.table.desc), C.byref(self.grid.desc), dt, v.data_ptr(),
+                                       r.data_ptr()), "g2p2g")
+
+    def implicit_multiply(self, inp, out, dt=None):
+        dt = float(self.sc.dt if dt is None else dt)
+        check(self.L.zsmpm_implicit_multiply__rocm(self.pol.h, C.byref(self.model), C.byref(self.parts.desc),
+                                                   C.byref(self.table.desc), C.byref(self.grid.desc), dt,
+                                                   inp.data_ptr(), out.data_ptr()), "implicit_multiply")
+
+    def implicit_project(self, dof):
+        check(self.L.zsmpm_implicit_project__rocm(self.pol.h, C.byref(self.table.desc), C.byref(self.grid.desc),
+                                                  self.cols, self.ncols, dof.data_ptr()), "implicit_project")
+
+    def implicit_precondition(self, inp, out):
+        check(self.L.zsmpm_implicit_precondition__rocm(self.pol.h, C.byref(self.table.desc), C.byref(self.grid.desc),
+                                                       inp.data_ptr(), out.data_ptr()), "implicit_precondition")
+
+    def implicit_solve(self, x, b, dt=None, max_iters: int = 1000, tol: float = 1e-6, rel_tol: float = 0.5):
+        """ConjugateGradient::solve of the implicit system; x in place.
+        Returns (iterations, residual)."""
+        dt = float(self.sc.dt if dt is None else dt)
+        it, res = C.c_int32(0), C.c_float(0)
+        check(self.L.zsmpm_implicit_solve__rocm(self.pol.h, C.byref(self.model), C.byref(self.parts.desc),
+                                                C.byref(self.table.desc), C.byref(self.grid.desc), dt, self.cols,
+                                                self.ncols, x.data_ptr(), b.data_ptr(), int(max_iters), float(tol),
+                                                float(rel_tol), C.byref(it), C.byref(res)), "implicit_solve")
+        return it.value, res.value
 
     def save_checkpoint(self, path: str, time: float = 0.0, step: int = 0, table: bool = True):
         """The particle state (and the table) as one checkpoint file."""
