@@ -555,6 +555,64 @@ def sparse_grid_leg(torch, rocm, sc, reps: int = 10):
     return out
 
 
+def implicit_leg(torch, rocm, sc, reps: int = 5):
+    """The implicit step's operators on the bench scene after two substeps:
+    G2P2G of the grid velocity (one launch over all particles: 27-node
+    gather, F update, stress, 81 float atomics), the system's multiply, and a
+    conjugate-gradient solve of (f dt^2 + m) v = m v_explicit to zpc's
+    tolerances (tol 1e-6, rel_tol 0.5)."""
+    import numpy as np
+
+    sim = rocm.Simulator(sc)
+    sim.substeps(2)
+    sim.clear_table()
+    sim.activate()
+    sim.enlarge()
+    sim.bucket_sort()
+    sim.clear_grid()
+    sim.p2g()
+    sim.sync()
+    nb = sim.nblocks()
+    n = nb * 64 * 3
+    g = sim.grid.data[:nb]
+    m = g[:, 0, :].reshape(-1).repeat_interleave(3)
+    mv = g[:, 1:4, :].permute(0, 2, 1).reshape(-1)
+    v = sim.dof_vector()
+    v[:n] = torch.where(m > 0, mv / torch.where(m > 0, m, torch.ones_like(m)), torch.zeros_like(m))
+    b = sim.dof_vector()
+    b[:n] = m * v[:n]
+    r = sim.dof_vector()
+
+    def ev_time(fn):
+        fn()
+        sim.sync()
+        a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(sim.pol.stream)
+        for _ in range(reps):
+            fn()
+        e.record(sim.pol.stream)
+        e.synchronize()
+        sim.sync()
+        return a.elapsed_time(e) / reps
+
+    ms_g2p2g = ev_time(lambda: sim.g2p2g(v, r))
+    ms_mult = ev_time(lambda: sim.implicit_multiply(v, r))
+    x = sim.dof_vector()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it, res = sim.implicit_solve(x, b)
+    torch.cuda.synchronize()
+    ms_solve = (time.perf_counter() - t0) * 1e3
+    out = {"workload": f"{sc.n} particles of the bench scene after two substeps, {nb} blocks",
+           "g2p2g_ms": ms_g2p2g, "g2p2g_particles_per_s": sc.n / (ms_g2p2g * 1e-3), "multiply_ms": ms_mult,
+           "solve_ms": ms_solve, "solve_iterations": it, "solve_residual": res}
+    print(f"# implicit: g2p2g {ms_g2p2g:.3f} ms, multiply {ms_mult:.3f} ms, solve {ms_solve:.2f} ms "
+          f"({it} iterations, residual {res:.3e})", file=sys.stderr, flush=True)
+    del sim, v, b, r, x
+    torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -581,6 +639,7 @@ def main() -> int:
                          "that scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
     ap.add_argument("--no-adapters", action="store_true", help="skip the layout adapters' leg")
     ap.add_argument("--no-sparse-grid", action="store_true", help="skip the SparseGrid / bht leg")
+    ap.add_argument("--no-implicit", action="store_true", help="skip the implicit step's leg")
     ap.add_argument("--workgroups", default="",
                     help="P2G,G2P workgroup counts to launch with in place of the autotuned ones (the 8 "
                          "tuning substeps still run, untuned), e.g. to hold a counter pass to a timed run's widths")
@@ -653,6 +712,13 @@ def main() -> int:
             x_plain = None
             torch.cuda.empty_cache()
         sparse_grid = sparse_grid_leg(torch, rocm, sc)
+    implicit = None
+    if not args.no_implicit:
+        if x_plain is not None:
+            del x_plain
+            x_plain = None
+            torch.cuda.empty_cache()
+        implicit = implicit_leg(torch, rocm, sc)
     per_launch = {k: (ms / calls if calls else 0.0) for k, (ms, calls) in stages.items()}
     dom = max(per_launch, key=lambda k: stages[k][0])
     dom_bytes = stage_bytes(dom, sc.n, blocks, plastic, sc.table_size)
@@ -720,6 +786,7 @@ def main() -> int:
         "batched": batched,
         "adapters": adapters,
         "sparse_grid": sparse_grid,
+        "implicit": implicit,
         "cpu_baseline": None,
     }
     del x_plain

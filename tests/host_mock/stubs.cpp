@@ -36,8 +36,9 @@ hipError_t grid_update(float*, int32_t, const TableView&, float, const float*, c
 hipError_t g2p(const ParticlesView&, const ParticlesView*, const TableView&, const SortView&, const float*, int32_t, float, float, Workspace&, hipStream_t s, const NextStress*, const SceneSet* ss) { return scenes_ok(ss, s); }
 hipError_t grid_angular_momentum(const float*, int32_t, const TableView&, float, int, int, double*, Workspace&, hipStream_t s) { return touch(s); }
 hipError_t reorder_particles(const ParticlesView&, const ParticlesView&, const int32_t*, Workspace&, hipStream_t s) { return touch(s); }
-hipError_t g2p2g_force(const ParticlesView&, const TableView&, const Model&, float, float, const float*, float*, uint32_t*, hipStream_t s) { return touch(s); }
-hipError_t implicit_multiply(const ParticlesView&, const TableView&, const Model&, const float*, int32_t, float, float, const float* in, float* out, uint32_t*, hipStream_t s) { return in == out ? hipErrorInvalidValue : touch(s); }
+hipError_t g2p2g_gather(const ParticlesView&, const TableView&, const Model&, float, float, const float*, float*, uint32_t*, hipStream_t s) { return touch(s); }
+hipError_t implicit_add_force(const float*, int32_t, const TableView&, float*, hipStream_t s) { return touch(s); }
+hipError_t implicit_force_dt2_mass(const float*, int32_t, const TableView&, float, const float* in, float* out, hipStream_t s) { return in == out ? hipErrorInvalidValue : touch(s); }
 hipError_t implicit_project(const float*, int32_t, const TableView&, float, const Colliders&, const GeneralColliders&, float*, hipStream_t s) { return touch(s); }
 hipError_t implicit_precondition(const float*, int32_t, const TableView&, const float*, float*, hipStream_t s) { return touch(s); }
 hipError_t implicit_lincomb(const TableView&, int32_t, float, const float*, float, const float*, float*, hipStream_t s) { return touch(s); }

@@ -43,6 +43,9 @@ struct zsmpm_policy_s {
   // zsmpm_table_activate_noalloc__rocm ran on this policy: its device-side
   // errors are in the device's fixed scratch array
   bool noalloc_used = false;
+  // scratch grid and zero masses of G2P2G's force scatter (grown on demand)
+  void* g2p2g_buf = nullptr;
+  size_t g2p2g_bytes = 0;
   // vectors of the implicit solve (grown on demand)
   void* cg_buf = nullptr;
   size_t cg_bytes = 0;
@@ -866,6 +869,7 @@ void destroy_policy(zsmpm_policy_s* p) {
   if (p->scene_first) (void)hipFree(p->scene_first);
   if (p->bht_buf) (void)hipFree(p->bht_buf);
   if (p->cg_buf) (void)hipFree(p->cg_buf);
+  if (p->g2p2g_buf) (void)hipFree(p->g2p2g_buf);
   if (p->cap) (void)hipStreamDestroy(p->cap);
   delete p;
 }
@@ -1223,6 +1227,49 @@ int32_t zsmpm_reorder__rocm(zsmpm_policy pol, const zsmpm_buckets* buckets, cons
 
 // ---- implicit step ----
 
+// G2P2GTransfer in two passes: the gather half writes each particle's
+// vol P F^T (at F = (I + dt C(v)) F_old) into the workspace's stress array;
+// P2G's force scatter -- one atomic per node and channel after binning in LDS,
+// in place of zpc's 81 per particle -- with zero masses and dt = -1 (its force
+// scale -dt D becomes G2P2G's D) accumulates the force into a scratch grid,
+// which is added to r node by node.
+static int32_t run_g2p2g(zsmpm_policy_s* pol, const Model& m, const ParticlesView& ps, const TableView& t,
+                         const zsmpm_grid* grid, float dt, const float* v, float* r) {
+  hipStream_t s = pol->stream;
+  ZS_TRY(ensure(pol, ps.n, t.T));
+  const size_t gfloats = (size_t)grid->capacity_blocks * kChannels * kBlockCells;
+  const size_t need = 4 * (gfloats + (size_t)ps.n) + 512;
+  if (need > pol->g2p2g_bytes) {
+    ZS_HIP(hipStreamSynchronize(s));
+    if (pol->g2p2g_buf) (void)hipFree(pol->g2p2g_buf);
+    pol->g2p2g_buf = nullptr;
+    pol->g2p2g_bytes = 0;
+    ZS_HIP(hipMalloc(&pol->g2p2g_buf, need));
+    ZS_HIP(hipMemset(pol->g2p2g_buf, 0, need));  // the zero masses stay zero
+    pol->g2p2g_bytes = need;
+  }
+  float* tg = (float*)pol->g2p2g_buf;
+  float* zm = tg + ((gfloats + 63) & ~(size_t)63);
+  SortView sv = own_buckets(pol);
+  ZS_TRY(run_buckets(pol, t, ps, grid->dx, sv));
+  ZS_HIP(block_neighbours(t, pol->ws, s));
+  ZS_HIP(g2p2g_gather(ps, t, m, dt, grid->dx, v, pol->ws.contrib, pol->ws.err, s));
+  ZS_HIP(grid_clear(tg, t.cnt, grid->capacity_blocks, s, nullptr));
+  ParticlesView pz = ps;
+  pz.m = zm;
+  ZS_HIP(p2g_scatter(pz, t, sv, tg, grid->capacity_blocks, -1.f, grid->dx, true, pol->ws, s));
+  ZS_HIP(implicit_add_force(tg, grid->capacity_blocks, t, r, s));
+  return ZSMPM_OK;
+}
+
+// ImplicitMPMSystem::multiply: out = 0, out += G2P2G(in), the mass term
+static int32_t run_implicit_multiply(zsmpm_policy_s* pol, const Model& m, const ParticlesView& ps, const TableView& t,
+                                     const zsmpm_grid* grid, float dt, const float* in, float* out) {
+  ZS_HIP(implicit_fill(t, grid->capacity_blocks, out, 0.f, pol->stream));
+  ZS_TRY(run_g2p2g(pol, m, ps, t, grid, dt, in, out));
+  return from_hip(implicit_force_dt2_mass(grid->data, grid->capacity_blocks, t, dt, in, out, pol->stream));
+}
+
 int32_t zsmpm_g2p2g__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
                           const zsmpm_table* table, const zsmpm_grid* grid, float dt, const float* v, float* r) {
   if (!pol || !v || !r) return ZSMPM_INVALID_ARGUMENT;
@@ -1235,8 +1282,7 @@ int32_t zsmpm_g2p2g__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmp
   ZS_TRY(view_table(table, t));
   ZS_TRY(view_grid(grid));
   DeviceGuard g(pol->devid);
-  ZS_TRY(ensure(pol, 0, 0));
-  return from_hip(g2p2g_force(ps, t, m, dt, grid->dx, v, r, pol->ws.err, pol->stream));
+  return run_g2p2g(pol, m, ps, t, grid, dt, v, r);
 }
 
 int32_t zsmpm_implicit_multiply__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
@@ -1252,9 +1298,7 @@ int32_t zsmpm_implicit_multiply__rocm(zsmpm_policy pol, const zsmpm_model* model
   ZS_TRY(view_table(table, t));
   ZS_TRY(view_grid(grid));
   DeviceGuard g(pol->devid);
-  ZS_TRY(ensure(pol, 0, 0));
-  return from_hip(implicit_multiply(ps, t, m, grid->data, grid->capacity_blocks, dt, grid->dx, in, out, pol->ws.err,
-                                    pol->stream));
+  return run_implicit_multiply(pol, m, ps, t, grid, dt, in, out);
 }
 
 int32_t zsmpm_implicit_project__rocm(zsmpm_policy pol, const zsmpm_table* table, const zsmpm_grid* grid,
@@ -1324,7 +1368,7 @@ int32_t zsmpm_implicit_solve__rocm(zsmpm_policy pol, const zsmpm_model* model, c
     return ZSMPM_OK;
   };
   auto A = [&](const float* in, float* outv) -> int32_t {
-    return from_hip(implicit_multiply(ps, t, m, gd, cap, dt, grid->dx, in, outv, pol->ws.err, s));
+    return run_implicit_multiply(pol, m, ps, t, grid, dt, in, outv);
   };
   auto project = [&](float* v) -> int32_t {
     return from_hip(implicit_project(gd, cap, t, grid->dx, cols, pol->gcols, v, s));

@@ -1,8 +1,8 @@
-// The implicit step's operators on gfx950: G2P2GTransfer (the force of the
-// stress at F = (I + dt C(v)) F_old for a grid velocity vector v), zpc's
-// ImplicitMPMSystem multiply / precondition (the projection lives beside the
-// colliders in transfer.hip) and the vector work of its preconditioned
-// conjugate gradient. Vectors hold 3 floats per grid node, node = block * 64 +
+// The implicit step's operators on gfx950: the gather half of G2P2GTransfer
+// (the stress at F = (I + dt C(v)) F_old for a grid velocity vector v; its
+// force scatter is P2G's), zpc's ImplicitMPMSystem mass term and
+// preconditioner (the projection lives beside the colliders in transfer.hip)
+// and the vector work of its preconditioned conjugate gradient. Vectors hold 3 floats per grid node, node = block * 64 +
 // cell, for the blocks the table holds (the count is read on the device).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -24,11 +24,14 @@ __device__ __forceinline__ float node_mass(const float* grid, int64_t node) {
   return grid[((node >> 6) * kChannels) * kBlockCells + (node & 63)];
 }
 
-// G2P2GTransfer::operator() (simulation/transfer/G2P2G.hpp:49-146), one thread
-// per particle; the 27 stencil nodes lie in at most 8 blocks, each looked up
-// once. Rounded as written, as the oracle is.
-__global__ __launch_bounds__(kIT) void g2p2g_kernel(ParticlesView ps, TableView t, Model m, float dt, float dx,
-                                                    const float* __restrict__ v, float* r, uint32_t* err) {
+// The gather half of G2P2GTransfer::operator() (simulation/transfer/G2P2G.hpp:49-110),
+// one thread per particle: C from the node velocities v (the 27 stencil nodes
+// lie in at most 8 blocks, each looked up once), F = (I + dt C) F_old (not
+// stored), and the model's vol P F^T (logJp read, not written back) into
+// contrib. The scatter half is P2G's force scatter, driven by the C ABI layer.
+// Rounded as written, as the oracle is.
+__global__ __launch_bounds__(kIT) void g2p2g_gather_kernel(ParticlesView ps, TableView t, Model m, float dt, float dx,
+                                                           const float* __restrict__ v, float* contrib, uint32_t* err) {
 #pragma clang fp contract(off)
   const float dx_inv = 1.f / dx;
   const float D_inv = 4.f * dx_inv * dx_inv;
@@ -36,7 +39,6 @@ __global__ __launch_bounds__(kIT) void g2p2g_kernel(ParticlesView ps, TableView 
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < ps.n; p += stride) {
     Arena ar;
     make_arena(ps.x[3 * p], ps.x[3 * p + 1], ps.x[3 * p + 2], dx, ar);
-    // the stencil's blocks: base >> 2 and base >> 2 + 1 per axis
     int32_t blk[2][2][2];
     const int32_t b0 = ar.base[0] >> 2, b1 = ar.base[1] >> 2, b2 = ar.base[2] >> 2;
 #pragma unroll
@@ -85,24 +87,20 @@ __global__ __launch_bounds__(kIT) void g2p2g_kernel(ParticlesView ps, TableView 
       stress_sand(m, F, lj, PF, nullptr, f64);
     }
 #pragma unroll
-    for (int d = 0; d < 9; ++d) PF[d] = PF[d] * D_inv;
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j)
-        for (int k = 0; k < 3; ++k) {
-          const int32_t c0 = ar.base[0] + i, c1 = ar.base[1] + j, c2 = ar.base[2] + k;
-          const int32_t bn = blk[(c0 >> 2) - b0][(c1 >> 2) - b1][(c2 >> 2) - b2];
-          if (bn < 0) continue;
-          const int cell = ((c0 & 3) << 4) | ((c1 & 3) << 2) | (c2 & 3);
-          const float xixp[3] = {(float)i * dx - ar.lp[0], (float)j * dx - ar.lp[1], (float)k * dx - ar.lp[2]};
-          float W = 1.f;
-          W *= ar.w[0][i];
-          W *= ar.w[1][j];
-          W *= ar.w[2][k];
-          float* ri = r + 3 * ((int64_t)bn * kBlockCells + cell);
+    for (int d = 0; d < 9; ++d) contrib[9 * p + d] = PF[d];
+  }
+}
+
+// grid channels 1..3 (the folded P2G force) -> r, 3 floats per node
+__global__ __launch_bounds__(kIT) void add_force_kernel(const float* __restrict__ grid, TableView t, int32_t grid_cap,
+                                                        float* r) {
+  const int64_t n = node_count(t, grid_cap);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t b = i >> 6;
+    const int64_t cell = i & 63;
 #pragma unroll
-          for (int d = 0; d < 3; ++d)
-            atomicAdd(&ri[d], W * (PF[d] * xixp[0] + PF[3 + d] * xixp[1] + PF[6 + d] * xixp[2]));
-        }
+    for (int d = 0; d < 3; ++d) r[3 * i + d] += grid[(b * kChannels + 1 + d) * kBlockCells + cell];
   }
 }
 
@@ -174,20 +172,21 @@ int groups_for_nodes() { return 2048; }
 
 }  // namespace
 
-hipError_t g2p2g_force(const ParticlesView& ps, const TableView& t, const Model& m, float dt, float dx,
-                       const float* v, float* r, uint32_t* err, hipStream_t s) {
+hipError_t g2p2g_gather(const ParticlesView& ps, const TableView& t, const Model& m, float dt, float dx,
+                        const float* v, float* contrib, uint32_t* err, hipStream_t s) {
   if (ps.n == 0) return hipSuccess;
   const int64_t g = (ps.n + kIT - 1) / kIT;
-  g2p2g_kernel<<<(int)(g > 65536 ? 65536 : g), kIT, 0, s>>>(ps, t, m, dt, dx, v, r, err);
+  g2p2g_gather_kernel<<<(int)(g > 65536 ? 65536 : g), kIT, 0, s>>>(ps, t, m, dt, dx, v, contrib, err);
   return hipGetLastError();
 }
 
-hipError_t implicit_multiply(const ParticlesView& ps, const TableView& t, const Model& m, const float* grid,
-                             int32_t grid_cap, float dt, float dx, const float* in, float* out, uint32_t* err,
-                             hipStream_t s) {
-  fill_kernel<<<groups_for_nodes(), kIT, 0, s>>>(out, t, grid_cap, 0.f);
-  hipError_t e = g2p2g_force(ps, t, m, dt, dx, in, out, err, s);
-  if (e != hipSuccess) return e;
+hipError_t implicit_add_force(const float* grid, int32_t grid_cap, const TableView& t, float* r, hipStream_t s) {
+  add_force_kernel<<<groups_for_nodes(), kIT, 0, s>>>(grid, t, grid_cap, r);
+  return hipGetLastError();
+}
+
+hipError_t implicit_force_dt2_mass(const float* grid, int32_t grid_cap, const TableView& t, float dt, const float* in,
+                                   float* out, hipStream_t s) {
   force_dt2_mass_kernel<<<groups_for_nodes(), kIT, 0, s>>>(grid, t, grid_cap, dt, in, out);
   return hipGetLastError();
 }
