@@ -557,8 +557,9 @@ def sparse_grid_leg(torch, rocm, sc, reps: int = 10):
 
 def implicit_leg(torch, rocm, sc, reps: int = 5):
     """The implicit step's operators on the bench scene after two substeps:
-    G2P2G of the grid velocity (one launch over all particles: 27-node
-    gather, F update, stress, 81 float atomics), the system's multiply, and a
+    G2P2G of the grid velocity (a per-particle gather of the 27-node stencil,
+    F update and stress, then P2G's LDS-binned scatter of the force), the
+    system's multiply, and a
     conjugate-gradient solve of (f dt^2 + m) v = m v_explicit to zpc's
     tolerances (tol 1e-6, rel_tol 0.5)."""
     import numpy as np
@@ -613,6 +614,52 @@ def implicit_leg(torch, rocm, sc, reps: int = 5):
     return out
 
 
+def transfers_leg(torch, rocm, sc, reps: int = 5):
+    """The particle -> cell -> grid transfers beside the APIC ones on the bench
+    scene after two substeps (particles in bucket order): P2C2G (stress pass +
+    cell-kernel scatter) against P2G (stress pass + quadratic scatter), and
+    G2C2P against G2P, each a whole C-ABI call on the policy's stream."""
+    sim = rocm.Simulator(sc)
+    sim.substeps(2)
+    sim.clear_table()
+    sim.activate()
+    sim.enlarge()
+    sim.bucket_sort()
+    sim.sync()
+
+    def ev_time(fn):
+        fn()
+        sim.sync()
+        a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(sim.pol.stream)
+        for _ in range(reps):
+            fn()
+        e.record(sim.pol.stream)
+        e.synchronize()
+        sim.sync()
+        return a.elapsed_time(e) / reps
+
+    def p2(fn):
+        def run():
+            sim.clear_grid()
+            fn()
+        return run
+
+    ms_p2g = ev_time(p2(sim.p2g))
+    ms_p2c2g = ev_time(p2(sim.p2c2g))
+    sim.grid_update(fold=0)
+    ms_g2p = ev_time(sim.g2p)
+    ms_g2c2p = ev_time(sim.g2c2p)
+    out = {"workload": f"{sc.n} particles of the bench scene after two substeps, {sim.nblocks()} blocks",
+           "p2c2g_ms": ms_p2c2g, "p2g_ms": ms_p2g, "g2c2p_ms": ms_g2c2p, "g2p_ms": ms_g2p,
+           "p2c2g_particles_per_s": sc.n / (ms_p2c2g * 1e-3), "g2c2p_particles_per_s": sc.n / (ms_g2c2p * 1e-3)}
+    print(f"# transfers: p2c2g {ms_p2c2g:.3f} ms (p2g {ms_p2g:.3f}), g2c2p {ms_g2c2p:.3f} ms (g2p {ms_g2p:.3f})",
+          file=sys.stderr, flush=True)
+    del sim
+    torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -640,6 +687,7 @@ def main() -> int:
     ap.add_argument("--no-adapters", action="store_true", help="skip the layout adapters' leg")
     ap.add_argument("--no-sparse-grid", action="store_true", help="skip the SparseGrid / bht leg")
     ap.add_argument("--no-implicit", action="store_true", help="skip the implicit step's leg")
+    ap.add_argument("--no-transfers", action="store_true", help="skip the P2C2G / G2C2P leg")
     ap.add_argument("--workgroups", default="",
                     help="P2G,G2P workgroup counts to launch with in place of the autotuned ones (the 8 "
                          "tuning substeps still run, untuned), e.g. to hold a counter pass to a timed run's widths")
@@ -719,6 +767,13 @@ def main() -> int:
             x_plain = None
             torch.cuda.empty_cache()
         implicit = implicit_leg(torch, rocm, sc)
+    transfers = None
+    if not args.no_transfers:
+        if x_plain is not None:
+            del x_plain
+            x_plain = None
+            torch.cuda.empty_cache()
+        transfers = transfers_leg(torch, rocm, sc)
     per_launch = {k: (ms / calls if calls else 0.0) for k, (ms, calls) in stages.items()}
     dom = max(per_launch, key=lambda k: stages[k][0])
     dom_bytes = stage_bytes(dom, sc.n, blocks, plastic, sc.table_size)
@@ -787,6 +842,7 @@ def main() -> int:
         "adapters": adapters,
         "sparse_grid": sparse_grid,
         "implicit": implicit,
+        "transfers": transfers,
         "cpu_baseline": None,
     }
     del x_plain

@@ -249,6 +249,27 @@ int32_t zsmpm_grid_angular_momentum__rocm(zsmpm_policy pol, const zsmpm_grid* gr
 /* G2PTransfer (APIC) over range(n), simulation/transfer/G2P.hpp:43-84. */
 int32_t zsmpm_g2p__rocm(zsmpm_policy pol, const zsmpm_particles* particles, const zsmpm_table* table,
                         const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt);
+/* P2C2GTransfer (simulation/transfer/P2C2G.hpp:53-181), fixed-corotated and
+ * Drucker-Prager: the particle -> cell -> node transfer. Each cell gathers the
+ * particles within dx of its centre on every axis with the tent weight, C and
+ * vol P F^T scaled per column by Dinv = 2 / (dx^2 - 2 Xrel^2) (Xrel: the
+ * offset from the nearest node), the stress by -dt, summed with m C; each
+ * cell then adds its moments to its 8 corner nodes with weight 1/8. Writes m
+ * and mv (channels 0..3) into the cleared grid; the force channels are not
+ * touched. The reference's cell-level IndexBuckets (index_buckets_for_particles,
+ * simulation/particle/Query.tpp:9-56) are not needed: `buckets` is the same
+ * block sort as for zsmpm_p2g__rocm (NULL computes it), and each particle's
+ * stencil is scattered once. Drucker-Prager's logJp is written back once
+ * (the reference writes it from every cell that gathers the particle). */
+int32_t zsmpm_p2c2g__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
+                          const zsmpm_table* table, const zsmpm_buckets* buckets, const zsmpm_grid* grid,
+                          float dt);
+/* PreG2C2PTransfer + G2C2PTransfer + PostG2C2PTransfer, G2C2P.hpp:58-271: v and
+ * B = sum W v_i (x_i - x_p)^T gathered through the cells from the node
+ * velocities (channels 1..3), B stored in C, F = (I + dt B Dinv) F at the old
+ * position, x += v dt. */
+int32_t zsmpm_g2c2p__rocm(zsmpm_policy pol, const zsmpm_particles* particles, const zsmpm_table* table,
+                          const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt);
 /* One explicit substep in the reconstructed driver's stage order: table clear,
  * activation (blockLen 4, offset -2, displacement 0.5), enlargement {0,0,0}..{2,2,2},
  * bucket sort, grid clear, P2G, rhs fold + grid velocity + colliders, G2P.

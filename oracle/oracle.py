@@ -117,6 +117,13 @@ def lib():
                                   _f32p, C.c_void_p]
         L.zso_p2g_f64_on.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int64, _f64p, _f64p, _f64p, _f64p,
                                      _f64p, C.c_void_p, _f64p]
+        L.zso_p2c2g_f32.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                    C.c_void_p]
+        L.zso_p2c2g_f64_on.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int64, _f64p, _f64p, _f64p, _f64p,
+                                       _f64p, C.c_void_p, _f64p]
+        L.zso_g2c2p_f32.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int64, _f32p, _f32p, _f32p, _f32p]
+        L.zso_g2c2p_f64_on.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int64, _f64p, _f64p, _f64p, _f64p,
+                                       _f64p]
         L.zso_grid_update_f32.argtypes = [C.c_void_p, C.POINTER(Params)]
         L.zso_grid_update_f32.restype = C.c_float
         L.zso_g2p_f32.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int64, _f32p, _f32p, _f32p, _f32p]
@@ -431,6 +438,28 @@ class Context:
         self.L.zso_p2g_f64_on(self.h, C.byref(params), n, ps64.x, ps64.v, ps64.C, ps64.F, ps64.m,
                               _ptr(ps64.logJp), g)
         return g.reshape(-1, 7, 64)
+
+    # particle -> cell -> grid transfers (P2C2G / G2C2P) on this context's table
+    def p2c2g(self, params, ps):
+        """P2C2G into a cleaned grid, kept as this context's P2G grid."""
+        self.L.zso_p2c2g_f32(self.h, C.byref(params), ps.n, ps.x, ps.v, ps.C, ps.F, ps.m, _ptr(ps.logJp))
+
+    def p2c2g_f64(self, params, ps64):
+        """FP64 P2C2G of double particles on this (f32) context's table."""
+        g = np.zeros(self.sizes()["cnt"] * 7 * 64)
+        self.L.zso_p2c2g_f64_on(self.h, C.byref(params), ps64.n, ps64.x, ps64.v, ps64.C, ps64.F, ps64.m,
+                                _ptr(ps64.logJp), g)
+        return g.reshape(-1, 7, 64)
+
+    def g2c2p(self, params, ps):
+        """G2C2P (pre, per-cell gather, post) on this context's updated grid."""
+        self.L.zso_g2c2p_f32(self.h, C.byref(params), ps.n, ps.x, ps.v, ps.C, ps.F)
+
+    def g2c2p_f64(self, params, ps64, gridv):
+        """FP64 G2C2P of double particles with a double node-velocity grid."""
+        g = np.ascontiguousarray(gridv, np.float64).reshape(-1)
+        assert g.size == self.sizes()["cnt"] * 7 * 64
+        self.L.zso_g2c2p_f64_on(self.h, C.byref(params), ps64.n, ps64.x, ps64.v, ps64.C, ps64.F, g)
 
     def grid_update(self, params):
         return self.L.zso_grid_update_f32(self.h, C.byref(params))

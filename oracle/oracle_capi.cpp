@@ -334,6 +334,51 @@ void zso_g2p_f32(void* c, const zso_params* prm, int64_t n, float* x, float* v, 
   g2p(sp.dt, sp.dx, view<float>(n, x, v, C, F, nullptr, nullptr), ctx->out.table, ctx->out.gridv.data());
 }
 
+/// P2C2G (zsmpm_oracle.hpp p2c2g) into a cleaned grid on the context's table,
+/// kept as the context's P2G grid; the cell buckets are built from x.
+void zso_p2c2g_f32(void* c, const zso_params* prm, int64_t n, float* x, float* v, float* C, float* F, float* m,
+                   float* logJp) {
+  auto* ctx = static_cast<Ctx<float>*>(c);
+  auto sp = to_params<float>(prm, ctx->cols);
+  auto& o = ctx->out;
+  o.grid.assign((size_t)o.table.cnt * kChannels * 64, 0.f);
+  CellBuckets ib;
+  ib.build(x, n, sp.dx);
+  p2c2g(sp.mat, sp.dt, sp.dx, view<float>(n, x, v, C, F, m, logJp), o.table, ib, o.grid.data());
+}
+/// FP64 P2C2G of double particles on the f32 context's table; grid_out holds
+/// table.cnt * 7 * 64 doubles.
+void zso_p2c2g_f64_on(void* c32, const zso_params* prm, int64_t n, double* x, double* v, double* C, double* F,
+                      double* m, double* logJp, double* grid_out) {
+  auto* ctx = static_cast<Ctx<float>*>(c32);
+  ColStore<double> cols;
+  auto sp = to_params<double>(prm, cols);
+  const int64_t nb = ctx->out.table.cnt;
+  for (int64_t i = 0; i < nb * kChannels * 64; ++i) grid_out[i] = 0.0;
+  CellBuckets ib;
+  ib.build(x, n, sp.dx);
+  p2c2g(sp.mat, sp.dt, sp.dx, view<double>(n, x, v, C, F, m, logJp), ctx->out.table, ib, grid_out);
+}
+/// G2C2P on the context's updated grid (zso_grid_update_f32 or set_grids).
+void zso_g2c2p_f32(void* c, const zso_params* prm, int64_t n, float* x, float* v, float* C, float* F) {
+  auto* ctx = static_cast<Ctx<float>*>(c);
+  auto sp = to_params<float>(prm, ctx->cols);
+  CellBuckets ib;
+  ib.build(x, n, sp.dx);
+  g2c2p(sp.dt, sp.dx, view<float>(n, x, v, C, F, nullptr, nullptr), ctx->out.table, ib, ctx->out.gridv.data());
+}
+/// FP64 G2C2P of double particles on the f32 context's table with a double
+/// node-velocity grid (table.cnt * 7 * 64).
+void zso_g2c2p_f64_on(void* c32, const zso_params* prm, int64_t n, double* x, double* v, double* C, double* F,
+                      const double* gridv) {
+  auto* ctx = static_cast<Ctx<float>*>(c32);
+  ColStore<double> cols;
+  auto sp = to_params<double>(prm, cols);
+  CellBuckets ib;
+  ib.build(x, n, sp.dx);
+  g2c2p(sp.dt, sp.dx, view<double>(n, x, v, C, F, nullptr, nullptr), ctx->out.table, ib, gridv);
+}
+
 /// Run `steps` substeps in place; the context keeps the last substep's
 /// table, sort and grids.
 void zso_run_f32(void* c, const zso_params* prm, int32_t steps, int64_t n, float* x, float* v,

@@ -1209,6 +1209,201 @@ void g2p(T dt, T dx, ParticlesView<T> ps, Table& t, const T* grid) {
 }
 
 // ----------------------------------------------------------------------------
+// P2C2G / G2C2P: particle -> cell -> node transfers over cell-level buckets
+// ----------------------------------------------------------------------------
+
+/// The IndexBuckets of index_buckets_for_particles(pol, particles, dx, 0.5)
+/// (simulation/particle/Query.tpp:9-56): a table of cells built with
+/// blockLen 1, offset 0 and displacement 0.5 (the cell of x is
+/// floor(x / dx + 0.5), the collocated bucketCoord of
+/// container/IndexBuckets.hpp:96-104) and each cell's particles by
+/// increasing index.
+struct CellBuckets {
+  Table t;
+  std::vector<i32> counts, offsets, indices;
+  template <typename T>
+  void build(const T* x, i64 n, T dx) {
+    t.resize((i32)evaluate_table_size(n > 0 ? n : 1));
+    compute_sparsity(t, x, n, dx, 1, 0, (T)0.5);
+    counts.assign((size_t)t.cnt + 1, 0);
+    offsets.assign((size_t)t.cnt + 1, 0);
+    indices.assign((size_t)n, 0);
+    bucket_sort(t, x, n, dx, 1, 0, (T)0.5, counts.data(), offsets.data(), indices.data());
+  }
+};
+
+/// The per-axis factor of P2C2G / G2C2P (P2C2G.hpp:86-90, G2C2P.hpp:235-238):
+/// Xrel = x - lower_trunc(x dx_inv + 0.5) dx, the offset from the nearest
+/// node, and Dinv = 2 / (dx^2 - 2 Xrel^2).
+template <typename T>
+inline void cell_dinv(const T* pos, T dx, T dx_inv, T Dinv[3]) {
+  for (int d = 0; d < 3; ++d) {
+    const T xr = pos[d] - (T)(i32)std::floor(pos[d] * dx_inv + (T)0.5) * dx;
+    Dinv[d] = (T)2 / (dx * dx - (T)2 * xr * xr);
+  }
+}
+
+/// P2C2GTransfer::operator()(blockid, cellid) over Collapse{table.cnt, 64}
+/// (simulation/transfer/P2C2G.hpp:53-181), fixed-corotated and Drucker-Prager
+/// models. Stage 1 gathers, for the cell at coord (centre (coord + 0.5) dx),
+/// the particles of the 27 buckets coord - 1 .. coord + 1 that lie within dx
+/// on every axis, with C scaled per column by Dinv, the model's vol P F^T
+/// scaled by Dinv (-dt) plus m C, and the tent weight prod(1 - |xc - xp| / dx)
+/// into m_c, mv_c, QDinv_c and QDinvXp_c; stage 2 adds (m_c, mv_c + QDinv_c
+/// x_i - QDinvXp_c) / 8 to the 2x2x2 nodes x_i = (coord + {0,1}^3) dx of the
+/// cell that have a block. Writes channels 0..3 of the grid.
+///
+/// logJp: the reference's functor reads a particle's logJp, runs the return
+/// mapping and writes it back from every cell that gathers the particle (up
+/// to eight, concurrently on the GPU), so the stored value depends on the
+/// interleaving. This restatement takes the interleaving in which every
+/// gather reads the logJp the transfer started with: the stress is the same
+/// from every cell and the value written is the once-updated one.
+template <typename T>
+void p2c2g(const Material<T>& mat, T dt, T dx, ParticlesView<T> ps, Table& t, CellBuckets& ib, T* grid) {
+  const T dx_inv = (T)1.0 / dx;
+  T mu, lambda;
+  lame(mat.E, mat.nu, mu, lambda);
+  std::vector<T> lj0;
+  if (ps.logJp) lj0.assign(ps.logJp, ps.logJp + ps.n);
+  for (i32 b = 0; b < t.cnt; ++b)
+    for (int cellid = 0; cellid < 64; ++cellid) {
+      T m_c = 0, mv_c[3] = {0, 0, 0}, QDinv_c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, QDinvXp_c[3] = {0, 0, 0};
+      const i32 coord[3] = {t.active[3 * b] * 4 + (cellid >> 4), t.active[3 * b + 1] * 4 + ((cellid >> 2) & 3),
+                            t.active[3 * b + 2] * 4 + (cellid & 3)};
+      T posc[3];
+      for (int d = 0; d < 3; ++d) posc[d] = ((T)coord[d] + (T)0.5) * dx;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          for (int k = 0; k < 3; ++k) {
+            const i32 bc[3] = {coord[0] - 1 + i, coord[1] - 1 + j, coord[2] - 1 + k};
+            const i32 bno = ib.t.query(bc);
+            if (bno < 0) continue;
+            for (i32 st = ib.offsets[bno], ed = ib.offsets[bno + 1]; st != ed; ++st) {
+              const i64 p = ib.indices[st];
+              const T* posp = ps.x + 3 * p;
+              bool in = true;
+              for (int d = 0; d < 3; ++d)
+                if (std::abs(posp[d] - posc[d]) > dx) in = false;
+              if (!in) continue;
+              T Dinv[3];
+              cell_dinv(posp, dx, dx_inv, Dinv);
+              const T* vel = ps.v + 3 * p;
+              const T mass = ps.m[p];
+              T C[9], contrib[9];
+              for (int d = 0; d < 9; ++d) C[d] = ps.C[9 * p + d] * Dinv[d / 3];
+              if (mat.kind == kFixedCorotated) {
+                stress_fixedcorotated(mat.volume, mu, lambda, ps.F + 9 * p, contrib);
+              } else {
+                T logJp = lj0[p];
+                stress_sand(mat.volume, mu, lambda, mat.cohesion, mat.beta, mat.yieldSurface,
+                            mat.volumeCorrection, logJp, ps.F + 9 * p, contrib);
+                ps.logJp[p] = logJp;
+              }
+              for (int d = 0; d < 9; ++d) contrib[d] = contrib[d] * (Dinv[d / 3] * -dt);
+              for (int d = 0; d < 9; ++d) contrib[d] = contrib[d] + C[d] * mass;
+              T Wpc = 1;
+              for (int d = 0; d < 3; ++d) Wpc *= (T)1 - std::abs((posc[d] - posp[d]) * dx_inv);
+              m_c += mass * Wpc;
+              for (int d = 0; d < 3; ++d) {
+                mv_c[d] += mass * vel[d] * Wpc;
+                QDinvXp_c[d] += (contrib[d] * posp[0] + contrib[3 + d] * posp[1] + contrib[6 + d] * posp[2]) * Wpc;
+              }
+              for (int d = 0; d < 9; ++d) QDinv_c[d] += contrib[d] * Wpc;
+            }
+          }
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+          for (int k = 0; k < 2; ++k) {
+            const i32 ci[3] = {coord[0] + i, coord[1] + j, coord[2] + k};
+            const T posi[3] = {(T)ci[0] * dx, (T)ci[1] * dx, (T)ci[2] * dx};
+            i32 nb;
+            int cell;
+            unpack_coord(t, ci, nb, cell);
+            if (nb < 0) continue;
+            const T Wci = (T)(1. / 8);
+            grid[gidx(nb, 0, cell)] += m_c * Wci;
+            for (int d = 0; d < 3; ++d)
+              grid[gidx(nb, 1 + d, cell)] +=
+                  (mv_c[d] + ((QDinv_c[d] * posi[0] + QDinv_c[3 + d] * posi[1] + QDinv_c[6 + d] * posi[2]) -
+                              QDinvXp_c[d])) *
+                  Wci;
+          }
+    }
+}
+
+/// PreG2C2PTransfer (G2C2P.hpp:197-212: v = 0, B = 0), G2C2PTransfer over
+/// Collapse{table.cnt, 64} (:58-151: the cell's v_c and v_c x^T from its
+/// 2x2x2 nodes with weight 1/8, then for each particle of the 27 buckets
+/// around it within dx on every axis, v_p += W v_c and
+/// B_p += W (v_c x^T - v_c x_p^T), W the tent weight, zero beyond one cell)
+/// and PostG2C2PTransfer (:214-271: C = B scaled per column by Dinv at the
+/// old position, F = (I + dt C) F, x += v dt; B stays in the C attribute).
+/// The grid's channels 1..3 hold the node velocities.
+template <typename T>
+void g2c2p(T dt, T dx, ParticlesView<T> ps, Table& t, CellBuckets& ib, const T* grid) {
+  const T dx_inv = (T)1 / dx;
+  for (i64 p = 0; p < ps.n; ++p) {
+    for (int d = 0; d < 3; ++d) ps.v[3 * p + d] = 0;
+    for (int d = 0; d < 9; ++d) ps.C[9 * p + d] = 0;
+  }
+  for (i32 b = 0; b < t.cnt; ++b)
+    for (int cellid = 0; cellid < 64; ++cellid) {
+      const i32 coord[3] = {t.active[3 * b] * 4 + (cellid >> 4), t.active[3 * b + 1] * 4 + ((cellid >> 2) & 3),
+                            t.active[3 * b + 2] * 4 + (cellid & 3)};
+      T v_c[3] = {0, 0, 0}, vx_c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+          for (int k = 0; k < 2; ++k) {
+            const i32 ci[3] = {coord[0] + i, coord[1] + j, coord[2] + k};
+            const T posi[3] = {(T)ci[0] * dx, (T)ci[1] * dx, (T)ci[2] * dx};
+            i32 nb;
+            int cell;
+            unpack_coord(t, ci, nb, cell);
+            if (nb < 0) continue;
+            const T W = (T)(1. / 8);
+            const T vi[3] = {grid[gidx(nb, 1, cell)], grid[gidx(nb, 2, cell)], grid[gidx(nb, 3, cell)]};
+            for (int d = 0; d < 3; ++d) v_c[d] += vi[d] * W;
+            for (int d = 0; d < 9; ++d) vx_c[d] += W * vi[d % 3] * posi[d / 3];
+          }
+      T posc[3];
+      for (int d = 0; d < 3; ++d) posc[d] = ((T)coord[d] + (T)0.5) * dx;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          for (int k = 0; k < 3; ++k) {
+            const i32 bc[3] = {coord[0] - 1 + i, coord[1] - 1 + j, coord[2] - 1 + k};
+            const i32 bno = ib.t.query(bc);
+            if (bno < 0) continue;
+            for (i32 st = ib.offsets[bno], ed = ib.offsets[bno + 1]; st != ed; ++st) {
+              const i64 p = ib.indices[st];
+              const T* posp = ps.x + 3 * p;
+              bool in = true;
+              for (int d = 0; d < 3; ++d)
+                if (std::abs(posp[d] - posc[d]) > dx) in = false;
+              if (!in) continue;
+              T W = 1;
+              for (int d = 0; d < 3; ++d) {
+                const T xabs = std::abs((posc[d] - posp[d]) * dx_inv);
+                W *= xabs <= (T)1 ? (T)1 - xabs : (T)0;
+              }
+              for (int d = 0; d < 3; ++d) ps.v[3 * p + d] += v_c[d] * W;
+              for (int d = 0; d < 9; ++d) ps.C[9 * p + d] += W * (vx_c[d] - v_c[d % 3] * posp[d / 3]);
+            }
+          }
+    }
+  for (i64 p = 0; p < ps.n; ++p) {
+    T* pos = ps.x + 3 * p;
+    T Dinv[3];
+    cell_dinv(pos, dx, dx_inv, Dinv);
+    T tmp[9], F[9];
+    for (int d = 0; d < 9; ++d) tmp[d] = ps.C[9 * p + d] * Dinv[d / 3] * dt + ((d & 0x3) ? (T)0.f : (T)1.f);
+    matmul3(tmp, ps.F + 9 * p, F);
+    for (int d = 0; d < 9; ++d) ps.F[9 * p + d] = F[d];
+    for (int d = 0; d < 3; ++d) pos[d] += ps.v[3 * p + d] * dt;
+  }
+}
+
+// ----------------------------------------------------------------------------
 // one explicit substep, stage order of the reconstructed driver
 // ----------------------------------------------------------------------------
 

@@ -47,6 +47,17 @@ int main(int argc, char** argv) {
   expect("odd count ends in b", inb, 1);
   expect("autotune", zsmpm_substeps_sorted__rocm(pol, &s0.model, &s0.a, &s0.b, &s0.t, &s0.bk, &s0.g, 1e-4f, g3, s0.cols, 6, 10, ZSMPM_SUBSTEPS_AUTOTUNE | ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
   int32_t wp = 0, wg = 0;
+  // the particle -> cell -> grid transfers: argument checks, own bucket sort
+  expect("p2c2g", zsmpm_p2c2g__rocm(pol, &s0.model, &s0.a, &s0.t, &s0.bk, &s0.g, 1e-4f), 0);
+  expect("p2c2g own buckets", zsmpm_p2c2g__rocm(pol, &s0.model, &s0.a, &s0.t, nullptr, &s0.g, 1e-4f), 0);
+  expect("p2c2g null grid", zsmpm_p2c2g__rocm(pol, &s0.model, &s0.a, &s0.t, &s0.bk, nullptr, 1e-4f), ZSMPM_INVALID_ARGUMENT);
+  expect("g2c2p", zsmpm_g2c2p__rocm(pol, &s0.a, &s0.t, &s0.bk, &s0.g, 1e-4f), 0);
+  expect("g2c2p null particles", zsmpm_g2c2p__rocm(pol, nullptr, &s0.t, &s0.bk, &s0.g, 1e-4f), ZSMPM_INVALID_ARGUMENT);
+  {
+    zsmpm_model dp = s0.model; dp.kind = ZSMPM_DRUCKER_PRAGER;
+    zsmpm_particles nolj = s0.a; nolj.logJp = nullptr;
+    expect("p2c2g sand without logJp", zsmpm_p2c2g__rocm(pol, &dp, &nolj, &s0.t, &s0.bk, &s0.g, 1e-4f), ZSMPM_INVALID_ARGUMENT);
+  }
   expect("launch widths", zsmpm_policy_launch_widths__rocm(pol, &wp, &wg), 0);
   expect("autotuned widths are candidates", (wp == 512 || wp == 768 || wp == 1024 || wp == 2048) && (wg == 8192 || wg == 16384 || wg == 32768 || wg == 65536), 1);
   expect("set widths", zsmpm_policy_set_launch_widths__rocm(pol, 300, 5000), 0);

@@ -35,6 +35,8 @@ hipError_t p2g_scatter(const ParticlesView&, const TableView&, const SortView&, 
 hipError_t grid_update(float*, int32_t, const TableView&, float, const float*, const Colliders&, float, int, Workspace&, hipStream_t s, const SceneSet* ss, const GeneralColliders* gen) { return gen && gen->n > kMaxColliders ? hipErrorInvalidValue : scenes_ok(ss, s); }
 hipError_t g2p(const ParticlesView&, const ParticlesView*, const TableView&, const SortView&, const float*, int32_t, float, float, Workspace&, hipStream_t s, const NextStress*, const SceneSet* ss) { return scenes_ok(ss, s); }
 hipError_t grid_angular_momentum(const float*, int32_t, const TableView&, float, int, int, double*, Workspace&, hipStream_t s) { return touch(s); }
+hipError_t p2c2g_scatter(const ParticlesView&, const TableView&, const SortView&, float*, int32_t, float, float, Workspace&, hipStream_t s) { return touch(s); }
+hipError_t g2c2p(const ParticlesView&, const TableView&, const SortView&, const float*, int32_t, float, float, Workspace&, hipStream_t s) { return touch(s); }
 hipError_t reorder_particles(const ParticlesView&, const ParticlesView&, const int32_t*, Workspace&, hipStream_t s) { return touch(s); }
 hipError_t g2p2g_gather(const ParticlesView&, const TableView&, const Model&, float, float, const float*, float*, uint32_t*, hipStream_t s) { return touch(s); }
 hipError_t implicit_add_force(const float*, int32_t, const TableView&, float*, hipStream_t s) { return touch(s); }

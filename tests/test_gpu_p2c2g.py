@@ -1,0 +1,173 @@
+"""The particle -> cell -> grid transfers on the GPU (zsmpm_p2c2g__rocm,
+zsmpm_g2c2p__rocm; the cell-kernel variants of the P2G scatter and G2P
+kernels in transfer.hip) through the C ABI against the oracle's per-cell
+restatement (zsmpm_oracle.hpp p2c2g / g2c2p), stage by stage and chained,
+for the fixed-corotated cube and the Drucker-Prager column; and, at half a
+million particles, the size-independent property of the kernel: an affine
+velocity field with B = A D survives P2C2G, the grid velocity and G2C2P.
+
+A field matches when it is within tol of the float32 oracle, or -- where the
+order of float sums decides (the oracle adds per cell with absolute
+positions, the GPU per particle with relative ones) -- at least as close to
+the float64 oracle as the float32 oracle is, within a factor 2.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from zpc_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-30))
+
+
+def _check(name, gpu, ref32, ref64, tol=TOL):
+    e32 = _rel(gpu, ref32)
+    e64 = _rel(gpu, ref64)
+    e_ref = _rel(ref32, ref64)
+    print(f"{name}: gpu-fp32 {e32:.2e} gpu-fp64 {e64:.2e} fp32-fp64 {e_ref:.2e}", flush=True)
+    if e32 < tol:
+        return
+    assert e64 <= max(tol, 2.0 * e_ref), name
+
+
+def _scene(kind):
+    if kind == "sand":
+        return S.perturb(S.sand_scene(dims=(8, 16, 8)), seed=3, f_scale=0.02, c_scale=1.0, v_scale=0.2)
+    return S.perturb(S.cube_scene(cells=12), seed=3, f_scale=0.02, c_scale=1.0, v_scale=0.2)
+
+
+def _gpu_p2c2g(sc):
+    from zpc_amd import rocm
+
+    sim = rocm.Simulator(sc)
+    sim.clear_table()
+    sim.activate()
+    sim.enlarge()
+    sim.bucket_sort()
+    sim.clear_grid()
+    sim.p2c2g()
+    sim.sync()
+    return sim
+
+
+def _oracle_ctx(sc):
+    ctx = O.Context()
+    ctx.activate(sc.table_size, sc.x, sc.dx)
+    ctx.enlarge()
+    return ctx
+
+
+def _parts(sc, dtype=np.float32):
+    return O.HostParticles(sc.x, sc.v, sc.C, sc.F, sc.m, sc.logJp, dtype=dtype)
+
+
+@pytest.mark.parametrize("kind", ["cube", "sand"])
+def test_p2c2g_grid_matches_oracle(kind):
+    sc = _scene(kind)
+    print(f"p2c2g {kind}: {sc.n} particles", flush=True)
+    sim = _gpu_p2c2g(sc)
+    nb = sim.nblocks()
+    g = sim.grid.data[:nb].cpu().numpy()
+    ctx = _oracle_ctx(sc)
+    assert ctx.sizes()["cnt"] == nb
+    prm = O.make_params(sc)
+    ps64 = _parts(sc, np.float64)
+    g64 = ctx.p2c2g_f64(prm, ps64)
+    ps32 = _parts(sc)
+    ctx.p2c2g(prm, ps32)
+    g32, _, _ = ctx.grids()
+    _check(f"{kind} m", g[:, 0], g32[:, 0], g64[:, 0])
+    _check(f"{kind} mv", g[:, 1:4], g32[:, 1:4], g64[:, 1:4])
+    assert np.all(g[:, 4:7] == 0.0), "P2C2G writes m and mv only"
+    if sc.logJp is not None:
+        lj = sim.parts.logJp.cpu().numpy()
+        _check(f"{kind} logJp", lj, ps32.logJp, ps64.logJp)
+
+
+@pytest.mark.parametrize("kind", ["cube", "sand"])
+def test_g2c2p_matches_oracle_on_gpu_grid(kind):
+    """G2C2P alone: the oracle reads the GPU's own node velocities."""
+    sc = _scene(kind)
+    sim = _gpu_p2c2g(sc)
+    sim.grid_update(fold=0)
+    sim.sync()
+    nb = sim.nblocks()
+    gv = sim.grid.data[:nb].cpu().numpy()
+    sim.g2c2p()
+    sim.sync()
+    got = sim.parts.host()
+    ctx = _oracle_ctx(sc)
+    prm = O.make_params(sc)
+    ps64 = _parts(sc, np.float64)
+    ctx.g2c2p_f64(prm, ps64, gv.astype(np.float64))
+    ps32 = _parts(sc)
+    ctx.set_grids(grid_v=gv)
+    ctx.g2c2p(prm, ps32)
+    for name in ("v", "C", "F", "x"):
+        _check(f"{kind} {name}", got[name], getattr(ps32, name), getattr(ps64, name))
+
+
+@pytest.mark.parametrize("kind", ["cube", "sand"])
+def test_transfer_chain_matches_oracle(kind):
+    """P2C2G, grid velocity with gravity and walls, G2C2P: the particle state."""
+    sc = _scene(kind)
+    sim = _gpu_p2c2g(sc)
+    sim.grid_update(fold=0)
+    sim.g2c2p()
+    sim.sync()
+    got = sim.parts.host()
+    ctx = _oracle_ctx(sc)
+    prm = O.make_params(sc)
+    ps32 = _parts(sc)
+    ctx.p2c2g(prm, ps32)
+    ctx.grid_update(prm)
+    ctx.g2c2p(prm, ps32)
+    ps64 = _parts(sc, np.float64)
+    g64 = ctx.p2c2g_f64(prm, ps64)
+    # the float64 grid velocity of the same update, from the float64 grid
+    ctx.set_grids(grid_p2g=g64.astype(np.float32))
+    ctx.grid_update(prm)
+    _, gv, _ = ctx.grids()
+    ctx.g2c2p_f64(prm, ps64, gv.astype(np.float64))
+    for name in ("v", "C", "F", "x"):
+        _check(f"{kind} chain {name}", got[name], getattr(ps32, name), getattr(ps64, name), tol=1e-4)
+
+
+def test_affine_field_round_trip_half_million():
+    sc = S.cube_scene(cells=40)
+    sc.material.E = 0.0
+    sc.colliders = []
+    sc.gravity = (0.0, 0.0, 0.0)
+    print(f"affine round trip: {sc.n} particles", flush=True)
+    A = np.array([[0.3, -1.1, 0.4], [0.9, 0.2, -0.5], [-0.7, 0.6, 0.1]])
+    b = np.array([0.2, -0.3, 0.05])
+    x = sc.x.astype(np.float64).reshape(-1, 3)
+    dx = float(np.float32(sc.dx))
+    xr = x - np.floor(x / dx + 0.5) * dx
+    D = dx * dx / 2 - xr * xr  # the kernel's second moment per axis
+    B = A[None] * D[:, None, :]  # column j scaled by D_j: C = B Dinv = A
+    sc.v = (x @ A.T + b).astype(np.float32)
+    sc.C = B.transpose(0, 2, 1).reshape(-1, 9).astype(np.float32)
+    F0 = sc.F.reshape(-1, 3, 3).transpose(0, 2, 1).astype(np.float64)
+    sim = _gpu_p2c2g(sc)
+    sim.grid_update(fold=0)
+    sim.g2c2p()
+    sim.sync()
+    got = sim.parts.host()
+    vw = x @ A.T + b
+    ev = np.abs(got["v"].reshape(-1, 3) - vw).max() / np.abs(vw).max()
+    Bg = got["C"].reshape(-1, 3, 3).transpose(0, 2, 1)
+    eb = np.abs(Bg - B).max() / np.abs(B).max()
+    Fg = got["F"].reshape(-1, 3, 3).transpose(0, 2, 1)
+    ef = np.abs(Fg - (np.eye(3) + sc.dt * A)[None] @ F0).max()
+    ex = np.abs(got["x"].reshape(-1, 3) - (x + vw * sc.dt)).max()
+    print(f"affine: v {ev:.2e} B {eb:.2e} F {ef:.2e} x {ex:.2e}", flush=True)
+    assert ev < 1e-5 and eb < 1e-4 and ef < 1e-6 and ex < 1e-6

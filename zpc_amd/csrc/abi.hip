@@ -1168,6 +1168,61 @@ int32_t zsmpm_g2p__rocm(zsmpm_policy pol, const zsmpm_particles* particles, cons
       zsmpm::g2p(ps, nullptr, t, sv, grid->data, grid->capacity_blocks, dt, grid->dx, pol->ws, pol->stream));
 }
 
+int32_t zsmpm_p2c2g__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
+                          const zsmpm_table* table, const zsmpm_buckets* buckets, const zsmpm_grid* grid,
+                          float dt) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
+  Model m;
+  ParticlesView ps;
+  TableView t;
+  ZS_TRY(make_model(model, m));
+  ZS_TRY(view_particles(particles, ps, true));
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(view_grid(grid));
+  if (m.kind == ZSMPM_DRUCKER_PRAGER && !ps.logJp) return ZSMPM_INVALID_ARGUMENT;
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, ps.n, t.T));
+  SortView sv;
+  if (buckets) {
+    ZS_TRY(view_buckets(buckets, sv));
+  } else {
+    sv = own_buckets(pol);
+    StageScope sc(pol, ZSMPM_STAGE_BUCKET_SORT);
+    ZS_TRY(run_buckets(pol, t, ps, grid->dx, sv));
+  }
+  {
+    StageScope sc(pol, ZSMPM_STAGE_STRESS);
+    ZS_HIP(zsmpm::stress(m, ps, pol->ws, pol->stream));
+  }
+  StageScope sc(pol, ZSMPM_STAGE_P2G);
+  ZS_HIP(block_neighbours(t, pol->ws, pol->stream));
+  return from_hip(
+      zsmpm::p2c2g_scatter(ps, t, sv, grid->data, grid->capacity_blocks, dt, grid->dx, pol->ws, pol->stream));
+}
+
+int32_t zsmpm_g2c2p__rocm(zsmpm_policy pol, const zsmpm_particles* particles, const zsmpm_table* table,
+                          const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt) {
+  if (!pol) return ZSMPM_INVALID_ARGUMENT;
+  ParticlesView ps;
+  TableView t;
+  ZS_TRY(view_particles(particles, ps, true));
+  ZS_TRY(view_table(table, t));
+  ZS_TRY(view_grid(grid));
+  DeviceGuard g(pol->devid);
+  ZS_TRY(ensure(pol, ps.n, t.T));
+  SortView sv;
+  if (buckets) {
+    ZS_TRY(view_buckets(buckets, sv));
+  } else {
+    sv = own_buckets(pol);
+    StageScope sc(pol, ZSMPM_STAGE_BUCKET_SORT);
+    ZS_TRY(run_buckets(pol, t, ps, grid->dx, sv));
+  }
+  StageScope sc(pol, ZSMPM_STAGE_G2P);
+  ZS_HIP(block_neighbours(t, pol->ws, pol->stream));
+  return from_hip(zsmpm::g2c2p(ps, t, sv, grid->data, grid->capacity_blocks, dt, grid->dx, pol->ws, pol->stream));
+}
+
 int32_t zsmpm_substep__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
                             const zsmpm_table* table, const zsmpm_buckets* buckets, const zsmpm_grid* grid,
                             float dt, const float extf[3], const zsmpm_plane_collider* colliders,

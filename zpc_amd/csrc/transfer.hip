@@ -163,6 +163,40 @@ __device__ __forceinline__ void bspline_w(float l, float (&w)[3]) {
   w[2] = 0.5f * zz * zz;
 }
 
+// Node weights of the particle -> cell -> node kernel (P2C2GTransfer,
+// simulation/transfer/P2C2G.hpp:143-147 and the 1/8 of :164; G2C2PTransfer,
+// G2C2P.hpp:81-84, :112-119): the tent weights 1 - t and t to the two cell
+// centres within reach (t = l - 0.5 from the local position l in [0.5, 1.5)),
+// each spread evenly over its cell's corners, per axis (1 - t) / 2, 1 / 2 and
+// t / 2 on the nodes base .. base + 2.
+__device__ __forceinline__ void cell_w(float l, float (&w)[3]) {
+#pragma clang fp contract(off)
+  const float t = l - 0.5f;
+  w[0] = (1.f - t) * 0.5f;
+  w[1] = 0.5f;
+  w[2] = t * 0.5f;
+}
+
+template <bool kCell>
+__device__ __forceinline__ void stencil_w(float l, float (&w)[3]) {
+  if constexpr (kCell)
+    cell_w(l, w);
+  else
+    bspline_w(l, w);
+}
+
+// Per-axis factor of the cell kernel (P2C2G.hpp:86-90, G2C2P.hpp:235-238):
+// Dinv = 2 / (dx^2 - 2 Xrel^2), Xrel = x - floor(x dx_inv + 0.5) dx the offset
+// from the nearest node; 1 / Dinv is the kernel's second moment on that axis.
+__device__ __forceinline__ void cell_dinv(const float (&x)[3], float dx, float dx_inv, float (&Dinv)[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float xr = x[d] - (float)(int32_t)floorf(x[d] * dx_inv + 0.5f) * dx;
+    Dinv[d] = 2.f / (dx * dx - 2.f * xr * xr);
+  }
+}
+
 // Record layout (float4 units): q0 = (m, B0, B1, B2), q1 = (l0, l1, l2, A8),
 // q2 = A0..A3, q3 = A4..A7 with A column-major (A[d + 3e]) already times dx;
 // the unfolded layout appends the force part q4 = (Bf0..2, Af8), q5, q6.
@@ -226,8 +260,11 @@ __global__ __launch_bounds__(kThreads) void stress_kernel(Model m, ParticlesView
   if (fp64_used) atomicAdd(&counters[kCtrFp64], fp64_used);
 }
 
-// kMulti (a batch, ss): each bucket's dx and dt are its scene's
-template <bool kFold, bool kMulti>
+// kMulti (a batch, ss): each bucket's dx and dt are its scene's.
+// kCell (folded, one scene): P2C2GTransfer's particle -> cell -> node kernel
+// (cell_w) in place of the quadratic B-spline, with C and the stress scaled
+// per column by the cell kernel's Dinv.
+template <bool kFold, bool kMulti, bool kCell = false>
 __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, const float* __restrict__ contrib,
                                                        TableView t, SortView sv, const int32_t* __restrict__ nbr_tab,
                                                        float* __restrict__ grid, int32_t grid_cap, float dt,
@@ -395,14 +432,24 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
         float A[9], Bv[3];
         const float mv[3] = {mass * v0, mass * v1, mass * v2};
         float Af[9], Bf[3];
+        if constexpr (kCell) {
+          // P2C2G.hpp:86-96, 138-140: C and vol P F^T scaled per column by
+          // Dinv, the stress by -dt, summed with m C
+#pragma clang fp contract(off)
+          float Dinv[3];
+          cell_dinv(xv, bdx, dx_inv, Dinv);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-          const float cf = PF[k] * bscale;
-          if (kFold) {
-            A[k] = (mass * C[k] + cf) * bdx;
-          } else {
-            A[k] = mass * C[k] * bdx;
-            Af[k] = cf * bdx;
+          for (int k = 0; k < 9; ++k) A[k] = (PF[k] * (Dinv[k / 3] * -dt) + (C[k] * Dinv[k / 3]) * mass) * bdx;
+        } else {
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            const float cf = PF[k] * bscale;
+            if (kFold) {
+              A[k] = (mass * C[k] + cf) * bdx;
+            } else {
+              A[k] = mass * C[k] * bdx;
+              Af[k] = cf * bdx;
+            }
           }
         }
 #pragma unroll
@@ -494,9 +541,9 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
               const int32_t i = list[lstart + r];
               const float4 q0 = rec[i], q1 = rec[kChunk + i], q2 = rec[2 * kChunk + i], q3 = rec[3 * kChunk + i];
               float w0[3], w1[3], w2[3];
-              bspline_w(q1.x, w0);
-              bspline_w(q1.y, w1);
-              bspline_w(q1.z, w2);
+              stencil_w<kCell>(q1.x, w0);
+              stencil_w<kCell>(q1.y, w1);
+              stencil_w<kCell>(q1.z, w2);
               const float fa = (float)a;
 #pragma unroll
               for (int bb = 0; bb < 3; ++bb) {
@@ -560,9 +607,9 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
               Af[8] = qf0.w;
             }
             float w0[3], w1[3], w2[3];
-            bspline_w(q1.x, w0);
-            bspline_w(q1.y, w1);
-            bspline_w(q1.z, w2);
+            stencil_w<kCell>(q1.x, w0);
+            stencil_w<kCell>(q1.y, w1);
+            stencil_w<kCell>(q1.z, w2);
 #pragma unroll
             for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -633,9 +680,9 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
           Af[8] = qf0.w;
         }
         float w0[3], w1[3], w2[3];
-        bspline_w(q1.x, w0);
-        bspline_w(q1.y, w1);
-        bspline_w(q1.z, w2);
+        stencil_w<kCell>(q1.x, w0);
+        stencil_w<kCell>(q1.y, w1);
+        stencil_w<kCell>(q1.z, w2);
         for (int a = 0; a < 3; ++a)
           for (int bb = 0; bb < 3; ++bb)
             for (int c = 0; c < 3; ++c) {
@@ -1001,8 +1048,12 @@ __global__ __launch_bounds__(kThreads) void angular_momentum_kernel(const float*
 }
 
 // kNext: -1 plain G2P; 0 / 1 (reorder only): also the next substep's
-// fixed-corotated / Drucker-Prager stress from the updated F (NextStress)
-template <bool kReorder, int kNext, bool kMulti = false>
+// fixed-corotated / Drucker-Prager stress from the updated F (NextStress).
+// kCell (in place, one scene): G2C2PTransfer with its pre and post passes
+// (G2C2P.hpp:58-151, 197-271): the cell kernel's weights (cell_w), B =
+// sum W v_i (x_i - x_p)^T stored as C, and F updated with B scaled per column
+// by Dinv at the old position.
+template <bool kReorder, int kNext, bool kMulti = false, bool kCell = false>
 __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, ParticlesView out, TableView t,
                                                           SortView sv, const int32_t* __restrict__ nbr_tab,
                                                           const float* __restrict__ grid, int32_t grid_cap,
@@ -1088,6 +1139,11 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
       }
       Arena ar;
       make_arena(x[0], x[1], x[2], dx, ar);
+      if constexpr (kCell) {
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cell_w(x[d] / dx - (float)ar.base[d], ar.w[d]);
+      }
       const int lb0 = ar.base[0] - o0, lb1 = ar.base[1] - o1, lb2 = ar.base[2] - o2;
       const bool outside = live && (lb0 < 0 || lb0 > 5 || lb1 < 0 || lb1 > 5 || lb2 < 0 || lb2 > 5);
       if (outside) atomicOr(err, kErrArena);
@@ -1111,7 +1167,10 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
               v[0] += vi.x * W;
               v[1] += vi.y * W;
               v[2] += vi.z * W;
-              const float w0 = W * vi.x * D_inv, w1 = W * vi.y * D_inv, w2 = W * vi.z * D_inv;
+              // the cell kernel keeps B (no D_inv): C = B Dinv is formed below
+              const float w0 = kCell ? W * vi.x : W * vi.x * D_inv;
+              const float w1 = kCell ? W * vi.y : W * vi.y * D_inv;
+              const float w2 = kCell ? W * vi.z : W * vi.z * D_inv;
               C[0] += w0 * xa;
               C[1] += w1 * xa;
               C[2] += w2 * xa;
@@ -1124,14 +1183,26 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
             }
           }
         }
-        // F <- (I + dt C) F, column-major (G2P.hpp:75-78)
+        // F <- (I + dt C) F, column-major (G2P.hpp:75-78); the cell kernel's
+        // C is B scaled per column by Dinv (G2C2P.hpp:231-253)
+        float Cs[9];
+        if constexpr (kCell) {
+#pragma clang fp contract(off)
+          float Dinv[3];
+          cell_dinv(x, dx, (float)1 / dx, Dinv);
+#pragma unroll
+          for (int k = 0; k < 9; ++k) Cs[k] = C[k] * Dinv[k / 3];
+        } else {
+#pragma unroll
+          for (int k = 0; k < 9; ++k) Cs[k] = C[k];
+        }
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
-            const float a0 = C[r] * dt + (r == 0 ? 1.f : 0.f);
-            const float a1 = C[r + 3] * dt + (r == 1 ? 1.f : 0.f);
-            const float a2 = C[r + 6] * dt + (r == 2 ? 1.f : 0.f);
+            const float a0 = Cs[r] * dt + (r == 0 ? 1.f : 0.f);
+            const float a1 = Cs[r + 3] * dt + (r == 1 ? 1.f : 0.f);
+            const float a2 = Cs[r + 6] * dt + (r == 2 ? 1.f : 0.f);
             Fn[r + 3 * j] = a0 * F[3 * j] + a1 * F[3 * j + 1] + a2 * F[3 * j + 2];
           }
 #pragma unroll
@@ -1307,7 +1378,9 @@ hipError_t check_launch_limits(int device) {
       reinterpret_cast<const void*>(&g2p_kernel<false, -1>), reinterpret_cast<const void*>(&g2p_kernel<true, -1>),
       reinterpret_cast<const void*>(&g2p_kernel<true, 0>), reinterpret_cast<const void*>(&g2p_kernel<true, 1>),
       reinterpret_cast<const void*>(&g2p_kernel<true, -1, true>),
-      reinterpret_cast<const void*>(&g2p_kernel<false, -1, true>)};
+      reinterpret_cast<const void*>(&g2p_kernel<false, -1, true>),
+      reinterpret_cast<const void*>(&p2g_kernel<true, false, true>),
+      reinterpret_cast<const void*>(&g2p_kernel<false, -1, false, true>)};
   for (int i = 0; i < (int)(sizeof(kernels) / sizeof(kernels[0])); ++i) {
     hipFuncAttributes attr;
     e = hipFuncGetAttributes(&attr, kernels[i]);
@@ -1365,6 +1438,23 @@ hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortVi
   else
     p2g_kernel<false, false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx,
                                                        ws.err, q, sset);
+  return hipGetLastError();
+}
+
+hipError_t p2c2g_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
+                         int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s) {
+  const int g = ws.grid_p2g > 0 ? ws.grid_p2g : persistent_groups();
+  reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
+  p2g_kernel<true, false, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx,
+                                                          ws.err, ws.counters + kCtrQueue, SceneSet{});
+  return hipGetLastError();
+}
+
+hipError_t g2c2p(const ParticlesView& ps, const TableView& t, const SortView& sv, const float* grid, int32_t grid_cap,
+                 float dt, float dx, Workspace& ws, hipStream_t s) {
+  const int g = ws.grid_g2p > 0 ? ws.grid_g2p : persistent_groups();
+  g2p_kernel<false, -1, false, true><<<g, kG2PThreads, 0, s>>>(ps, ps, t, sv, ws.nbr, grid, grid_cap, dt, dx,
+                                                               NextStress{}, ws.err, ws.counters, SceneSet{});
   return hipGetLastError();
 }
 

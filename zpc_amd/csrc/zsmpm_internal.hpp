@@ -286,6 +286,13 @@ hipError_t closure_launch_limits();
 hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
                        int32_t grid_cap, float dt, float dx, bool fold, Workspace& ws, hipStream_t s,
                        bool reset_queue = true, const SceneSet* ss = nullptr);
+// P2C2GTransfer as a scatter of each particle's cell-kernel stencil (folded
+// into m and mv, channels 0..3) with the stress already in ws.contrib
+hipError_t p2c2g_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
+                         int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s);
+// G2C2PTransfer with its pre and post passes, in place: v, B (in C), F, x
+hipError_t g2c2p(const ParticlesView& ps, const TableView& t, const SortView& sv, const float* grid, int32_t grid_cap,
+                 float dt, float dx, Workspace& ws, hipStream_t s);
 // gen: general colliders applied after the plane colliders (nullptr: none)
 hipError_t grid_update(float* grid, int32_t grid_cap, const TableView& t, float dt, const float extf[3],
                        const Colliders& cols, float dx, int fold_rhs, Workspace& ws, hipStream_t s,

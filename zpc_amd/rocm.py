@@ -185,6 +185,8 @@ def lib():
                                     i32, vp],
         "zsmpm_grid_angular_momentum__rocm": [vp, P(GridDesc), P(TableDesc), i32, i32, vp],
         "zsmpm_g2p__rocm": [vp, P(ParticlesDesc), P(TableDesc), P(BucketsDesc), P(GridDesc), f32],
+        "zsmpm_p2c2g__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(TableDesc), P(BucketsDesc), P(GridDesc), f32],
+        "zsmpm_g2c2p__rocm": [vp, P(ParticlesDesc), P(TableDesc), P(BucketsDesc), P(GridDesc), f32],
         "zsmpm_substep__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(TableDesc), P(BucketsDesc), P(GridDesc), f32,
                                 P(C.c_float * 3), P(PlaneCollider), i32],
         "zsmpm_substep_sorted__rocm": [vp, P(ModelDesc), P(ParticlesDesc), P(ParticlesDesc), P(TableDesc),
@@ -790,6 +792,19 @@ class Simulator:
         dt = float(self.sc.dt if dt is None else dt)
         check(self.L.zsmpm_g2p__rocm(self.pol.h, C.byref(self.parts.desc), C.byref(self.table.desc),
                                      C.byref(self.buckets.desc), C.byref(self.grid.desc), dt), "g2p")
+
+    def p2c2g(self, dt=None):
+        """P2C2GTransfer into the (cleared) grid: m and mv through the cells."""
+        dt = float(self.sc.dt if dt is None else dt)
+        check(self.L.zsmpm_p2c2g__rocm(self.pol.h, C.byref(self.model), C.byref(self.parts.desc),
+                                       C.byref(self.table.desc), C.byref(self.buckets.desc),
+                                       C.byref(self.grid.desc), dt), "p2c2g")
+
+    def g2c2p(self, dt=None):
+        """G2C2PTransfer with its pre and post passes on the particles, in place."""
+        dt = float(self.sc.dt if dt is None else dt)
+        check(self.L.zsmpm_g2c2p__rocm(self.pol.h, C.byref(self.parts.desc), C.byref(self.table.desc),
+                                       C.byref(self.buckets.desc), C.byref(self.grid.desc), dt), "g2c2p")
 
     def reorder_particles(self):
         """Vector::reorder of every attribute by the current bucket sort."""
