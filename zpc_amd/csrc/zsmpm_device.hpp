@@ -613,6 +613,37 @@ __device__ __forceinline__ int stress_sand(const Model& m, const float* F, float
       which = 3;
     }
   }
+  if (which != 0 && !Fproj) {
+    // With the projected F = U diag(ns) V^T and P = U diag(Ph) V^T,
+    // P F^T = U diag(Ph) V^T V diag(ns) U^T = U diag(tau) U^T (V^T V = I to
+    // rounding), tau_i = Ph_i ns_i = 2 mu ln ns_i + lambda sum_j ln ns_j, the
+    // Kirchhoff stress: no division, and ln ns_i is the exponent the
+    // projection chose (no exp / log round trip). Symmetric: 6 entries. It
+    // agrees with the composed form to 1e-6 of the largest entry.
+    float H[3];
+    if (which == 2) {
+      H[0] = H[1] = H[2] = m.cohesion;
+    } else if (which == 1) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) H[i] = eps[i] + m.cohesion;
+    } else {
+      const float r = (ehn + (3.f * m.lambda + smu) / smu * tr * m.yield_surface) / ehn;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) H[i] = eps[i] - r * eh[i] + m.cohesion;
+    }
+    const float trl = H[0] + H[1] + H[2];
+    const float tau[3] = {(smu * H[0] + m.lambda * trl) * m.volume, (smu * H[1] + m.lambda * trl) * m.volume,
+                          (smu * H[2] + m.lambda * trl) * m.volume};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = i; j < 3; ++j) {
+        const float e = U[i][0] * tau[0] * U[j][0] + U[i][1] * tau[1] * U[j][1] + U[i][2] * tau[2] * U[j][2];
+        PF[i + 3 * j] = e;
+        PF[j + 3 * i] = e;
+      }
+    return which;
+  }
   // projected F = U diag(ns) V^T (column-major)
   float Fp[9];
   if (which != 0) {
