@@ -556,7 +556,8 @@ def sparse_grid_leg(torch, rocm, sc, reps: int = 10):
 
 
 def implicit_leg(torch, rocm, sc, reps: int = 5):
-    """The implicit step's operators on the bench scene after two substeps:
+    """The implicit step's operators on the bench scene after two sorted
+    substeps (particles in bucket order):
     G2P2G of the grid velocity (a per-particle gather of the 27-node stencil,
     F update and stress, then P2G's LDS-binned scatter of the force), the
     system's multiply, and a
@@ -564,7 +565,7 @@ def implicit_leg(torch, rocm, sc, reps: int = 5):
     tolerances (tol 1e-6, rel_tol 0.5)."""
     import numpy as np
 
-    sim = rocm.Simulator(sc)
+    sim = rocm.Simulator(sc, reorder=True)  # particles in bucket order, as the sorted driver leaves them
     sim.substeps(2)
     sim.clear_table()
     sim.activate()
@@ -616,10 +617,10 @@ def implicit_leg(torch, rocm, sc, reps: int = 5):
 
 def transfers_leg(torch, rocm, sc, reps: int = 5):
     """The particle -> cell -> grid transfers beside the APIC ones on the bench
-    scene after two substeps (particles in bucket order): P2C2G (stress pass +
+    scene after two sorted substeps (particles in bucket order): P2C2G (stress pass +
     cell-kernel scatter) against P2G (stress pass + quadratic scatter), and
     G2C2P against G2P, each a whole C-ABI call on the policy's stream."""
-    sim = rocm.Simulator(sc)
+    sim = rocm.Simulator(sc, reorder=True)  # particles in bucket order, as the sorted driver leaves them
     sim.substeps(2)
     sim.clear_table()
     sim.activate()
