@@ -15,10 +15,11 @@
 //      node value(o) = W(o) (B + dx A o) -- kept in LDS, and a base-cell id
 //      (one of the 64 main cells [4b+1, 4b+5)^3, or "overflow");
 //   B  a counting sort of the chunk by base cell (LDS integer atomics);
-//   C  rounds: in round r lane l of a wave takes the r-th particle of main
-//      cell l, so the 64 lanes of one instruction address 64 different nodes
-//      and scatter with plain float4 read-modify-writes into the wave's own
-//      6^3 node arena; overflow particles scatter with global atomics;
+//   C  lane l of a wave sums its share of main cell l's particles in
+//      registers, one x-slab of the 27-node stencil at a time, and adds the
+//      node sums to the wave's own 6^3 node arena with plain float4
+//      read-modify-writes (the 64 lanes address 64 different nodes);
+//      overflow particles scatter with global atomics;
 //   then the four wave arenas are summed and flushed with one global float
 //   atomic per touched node and channel (zpc's CUDA P2G issues 189 global
 //   atomics per particle, cuda/simulation/transfer/P2G.hpp:98-115).
@@ -30,6 +31,8 @@
 // gathers from there.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "zsmpm_device.hpp"
 #include "zsmpm_internal.hpp"
@@ -287,7 +290,6 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
   __shared__ uint8_t cellof[kChunk];
   __shared__ int32_t ccount[kMainCells + 1], cstart[kMainCells + 1], cfill[kMainCells + 1];
   __shared__ int32_t nbr[8];
-  __shared__ int32_t maxc;
 
   const int32_t nb = bounded_blocks(t, sv, grid_cap, err);
   const float dx_inv = (float)1.0 / dx;
@@ -491,13 +493,7 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
           if (lane >= o) inc += u;
         }
         cstart[threadIdx.x] = inc - c;
-        int32_t mx = c;
-#pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, kWave));
-        if (threadIdx.x == kWave - 1) {
-          cstart[kMainCells] = inc;  // overflow particles follow the main cells
-          maxc = mx;
-        }
+        if (threadIdx.x == kWave - 1) cstart[kMainCells] = inc;  // overflow particles follow the main cells
       }
       // each thread keeps the base offsets and cells of its own particles
       // across the rebuild of list[]
@@ -528,14 +524,17 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
         int lr0, lr1, lr2;  // base - (4b+1) of the lane's main cell
         lane_cell(lane, lr0, lr1, lr2);
         float4* ar = arena[wave];
-        const int32_t rounds = maxc;
-        if constexpr (kFold) {
-          // All particles of one base cell touch the same 27 nodes: the lane
-          // sums its share of its cell's particles (every kP2GWaves-th, starting
-          // at the wave index) in registers and adds the node sums to the
-          // wave's arena once per cell, instead of 27 LDS read-modify-writes
-          // per particle. One pass per x-slab of the stencil keeps the
-          // accumulators at 9 nodes x 4 channels.
+        // All particles of one base cell touch the same 27 nodes: the lane
+        // sums its share of its cell's particles (every kP2GWaves-th, starting
+        // at the wave index) in registers and adds the node sums to the
+        // wave's arena once per cell, instead of 27 LDS read-modify-writes
+        // per particle. One pass per x-slab of the stencil keeps the
+        // accumulators at 9 nodes x 4 channels. Part 0 is the mass and
+        // momentum (with the force folded in under kFold); without kFold a
+        // second pass over the same particles sums the force channels into
+        // the node's second float4.
+        auto accumulate = [&](auto part) {
+          constexpr int kPart = decltype(part)::value;
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
             float acc[9][4];
@@ -543,7 +542,14 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
             for (int k = 0; k < 9; ++k) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.f;
             for (int32_t r = wave; r < lcount; r += kP2GWaves) {
               const int32_t i = list[lstart + r];
-              const float4 q0 = rec[i], q1 = rec[kChunk + i], q2 = rec[2 * kChunk + i], q3 = rec[3 * kChunk + i];
+              const float4 q1 = rec[kChunk + i];
+              // part 0: (m, B) in q0, A0..A3 in q2, A4..A7 in q3, A8 in q1.w;
+              // part 1: (Bf, Af8) in q0, Af0..Af3 in q2, Af4..Af7 in q3
+              const float4 q0 = rec[(kPart == 0 ? 0 : 4 % kQ) * kChunk + i];
+              const float4 q2 = rec[(kPart == 0 ? 2 : 5 % kQ) * kChunk + i];
+              const float4 q3 = rec[(kPart == 0 ? 3 : 6 % kQ) * kChunk + i];
+              const float b0 = kPart == 0 ? q0.y : q0.x, b1 = kPart == 0 ? q0.z : q0.y, b2 = kPart == 0 ? q0.w : q0.z;
+              const float a8 = kPart == 0 ? q1.w : q0.w;
               float w0[3], w1[3], w2[3];
               stencil_w<kCell>(q1.x, w0);
               stencil_w<kCell>(q1.y, w1);
@@ -553,18 +559,18 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
               for (int bb = 0; bb < 3; ++bb) {
                 const float w01 = w0[a] * w1[bb];
                 const float fb = (float)bb;
-                const float s0 = q0.y + q2.x * fa + q2.w * fb;
-                const float s1 = q0.z + q2.y * fa + q3.x * fb;
-                const float s2 = q0.w + q2.z * fa + q3.y * fb;
+                const float s0 = b0 + q2.x * fa + q2.w * fb;
+                const float s1 = b1 + q2.y * fa + q3.x * fb;
+                const float s2 = b2 + q2.z * fa + q3.y * fb;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                   const float W = w01 * w2[c];
                   const float fc = (float)c;
                   float* e = acc[bb * 3 + c];
-                  e[0] += q0.x * W;
+                  if (kPart == 0) e[0] += q0.x * W;
                   e[1] += W * (s0 + q3.z * fc);
                   e[2] += W * (s1 + q3.w * fc);
-                  e[3] += W * (s2 + q1.w * fc);
+                  e[3] += W * (s2 + a8 * fc);
                 }
               }
             }
@@ -575,88 +581,30 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
                 for (int c = 0; c < 3; ++c) {
                   const int n = (lr0 + a) * 36 + (lr1 + bb) * 6 + (lr2 + c);
                   const float* e = acc[bb * 3 + c];
-                  // lanes of one step add to distinct nodes; steps stay in
-                  // issue order (see the unfolded path below)
-                  asm volatile("" ::: "memory");
-                  float4 v = ar[n];
-                  v.x += e[0];
-                  v.y += e[1];
-                  v.z += e[2];
-                  v.w += e[3];
-                  ar[n] = v;
-                }
-            }
-          }
-        }
-        for (int32_t r = wave; !kFold && r < rounds; r += kP2GWaves) {
-          if (r < lcount) {
-            const int32_t i = list[lstart + r];
-            const float4 q0 = rec[i], q1 = rec[kChunk + i], q2 = rec[2 * kChunk + i], q3 = rec[3 * kChunk + i];
-            const float Am[9] = {q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w, q1.w};
-            float Af[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, Bf[3] = {0.f, 0.f, 0.f};
-            if (!kFold) {
-              const float4 qf0 = rec[(4 % kQ) * kChunk + i], qf1 = rec[(5 % kQ) * kChunk + i],
-                         qf2 = rec[(6 % kQ) * kChunk + i];
-              Bf[0] = qf0.x;
-              Bf[1] = qf0.y;
-              Bf[2] = qf0.z;
-              Af[0] = qf1.x;
-              Af[1] = qf1.y;
-              Af[2] = qf1.z;
-              Af[3] = qf1.w;
-              Af[4] = qf2.x;
-              Af[5] = qf2.y;
-              Af[6] = qf2.z;
-              Af[7] = qf2.w;
-              Af[8] = qf0.w;
-            }
-            float w0[3], w1[3], w2[3];
-            stencil_w<kCell>(q1.x, w0);
-            stencil_w<kCell>(q1.y, w1);
-            stencil_w<kCell>(q1.z, w2);
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-              for (int bb = 0; bb < 3; ++bb) {
-                const float w01 = w0[a] * w1[bb];
-                const float fa = (float)a, fb = (float)bb;
-                const float s0 = q0.y + Am[0] * fa + Am[3] * fb;
-                const float s1 = q0.z + Am[1] * fa + Am[4] * fb;
-                const float s2 = q0.w + Am[2] * fa + Am[5] * fb;
-                float f0 = 0.f, f1 = 0.f, f2 = 0.f;
-                if (!kFold) {
-                  f0 = Bf[0] + Af[0] * fa + Af[3] * fb;
-                  f1 = Bf[1] + Af[1] * fa + Af[4] * fb;
-                  f2 = Bf[2] + Af[2] * fa + Af[5] * fb;
-                }
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                  const float W = w01 * w2[c];
-                  const float fc = (float)c;
-                  const int n = (lr0 + a) * 36 + (lr1 + bb) * 6 + (lr2 + c);
                   // Lanes of one step touch distinct nodes, but a node a lane
                   // updates in this step may be one another lane updated in an
-                  // earlier step. The LDS unit serves a wave's accesses in issue
-                  // order; this barrier keeps the compiler from hoisting this
-                  // step's read above earlier steps' writes.
+                  // earlier step. The LDS unit serves a wave's accesses in
+                  // issue order; this barrier keeps the compiler from hoisting
+                  // this step's read above earlier steps' writes.
                   asm volatile("" ::: "memory");
-                  float4 acc = ar[n * kNV];
-                  acc.x += q0.x * W;
-                  acc.y += W * (s0 + Am[6] * fc);
-                  acc.z += W * (s1 + Am[7] * fc);
-                  acc.w += W * (s2 + Am[8] * fc);
-                  ar[n * kNV] = acc;
-                  if (!kFold) {
-                    float4 accf = ar[n * kNV + 1];
-                    accf.x += W * (f0 + Af[6] * fc);
-                    accf.y += W * (f1 + Af[7] * fc);
-                    accf.z += W * (f2 + Af[8] * fc);
-                    ar[n * kNV + 1] = accf;
+                  float4 v = ar[n * kNV + kPart];
+                  if (kPart == 0) {
+                    v.x += e[0];
+                    v.y += e[1];
+                    v.z += e[2];
+                    v.w += e[3];
+                  } else {
+                    v.x += e[1];
+                    v.y += e[2];
+                    v.z += e[3];
                   }
+                  ar[n * kNV + kPart] = v;
                 }
-              }
+            }
           }
-        }
+        };
+        accumulate(std::integral_constant<int, 0>{});
+        if constexpr (!kFold) accumulate(std::integral_constant<int, 1>{});
       }
       // ---- overflow particles: global atomics straight into the grid
 #pragma unroll
