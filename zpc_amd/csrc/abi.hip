@@ -1288,8 +1288,10 @@ int32_t zsmpm_reorder__rocm(zsmpm_policy pol, const zsmpm_buckets* buckets, cons
 // in place of zpc's 81 per particle -- with zero masses and dt = -1 (its force
 // scale -dt D becomes G2P2G's D) accumulates the force into a scratch grid,
 // which is added to r node by node.
+// sorted: the policy's bucket sort and neighbour table already hold this
+// table and these particles (the solver sorts once for all its products)
 static int32_t run_g2p2g(zsmpm_policy_s* pol, const Model& m, const ParticlesView& ps, const TableView& t,
-                         const zsmpm_grid* grid, float dt, const float* v, float* r) {
+                         const zsmpm_grid* grid, float dt, const float* v, float* r, bool sorted = false) {
   hipStream_t s = pol->stream;
   ZS_TRY(ensure(pol, ps.n, t.T));
   const size_t gfloats = (size_t)grid->capacity_blocks * kChannels * kBlockCells;
@@ -1306,9 +1308,11 @@ static int32_t run_g2p2g(zsmpm_policy_s* pol, const Model& m, const ParticlesVie
   float* tg = (float*)pol->g2p2g_buf;
   float* zm = tg + ((gfloats + 63) & ~(size_t)63);
   SortView sv = own_buckets(pol);
-  ZS_TRY(run_buckets(pol, t, ps, grid->dx, sv));
-  ZS_HIP(block_neighbours(t, pol->ws, s));
-  ZS_HIP(g2p2g_gather(ps, t, m, dt, grid->dx, v, pol->ws.contrib, pol->ws.err, s));
+  if (!sorted) {
+    ZS_TRY(run_buckets(pol, t, ps, grid->dx, sv));
+    ZS_HIP(block_neighbours(t, pol->ws, s));
+  }
+  ZS_HIP(g2p2g_gather(ps, t, sv, v, grid->capacity_blocks, m, dt, grid->dx, pol->ws, s));
   ZS_HIP(grid_clear(tg, t.cnt, grid->capacity_blocks, s, nullptr));
   ParticlesView pz = ps;
   pz.m = zm;
@@ -1319,9 +1323,10 @@ static int32_t run_g2p2g(zsmpm_policy_s* pol, const Model& m, const ParticlesVie
 
 // ImplicitMPMSystem::multiply: out = 0, out += G2P2G(in), the mass term
 static int32_t run_implicit_multiply(zsmpm_policy_s* pol, const Model& m, const ParticlesView& ps, const TableView& t,
-                                     const zsmpm_grid* grid, float dt, const float* in, float* out) {
+                                     const zsmpm_grid* grid, float dt, const float* in, float* out,
+                                     bool sorted = false) {
   ZS_HIP(implicit_fill(t, grid->capacity_blocks, out, 0.f, pol->stream));
-  ZS_TRY(run_g2p2g(pol, m, ps, t, grid, dt, in, out));
+  ZS_TRY(run_g2p2g(pol, m, ps, t, grid, dt, in, out, sorted));
   return from_hip(implicit_force_dt2_mass(grid->data, grid->capacity_blocks, t, dt, in, out, pol->stream));
 }
 
@@ -1422,8 +1427,14 @@ int32_t zsmpm_implicit_solve__rocm(zsmpm_policy pol, const zsmpm_model* model, c
     *outv = (float)h;
     return ZSMPM_OK;
   };
+  // the particles and the table stay put during the solve: one bucket sort
+  // and neighbour table serve every product
+  ZS_TRY(ensure(pol, ps.n, t.T));
+  SortView sv_solve = own_buckets(pol);
+  ZS_TRY(run_buckets(pol, t, ps, grid->dx, sv_solve));
+  ZS_HIP(block_neighbours(t, pol->ws, s));
   auto A = [&](const float* in, float* outv) -> int32_t {
-    return run_implicit_multiply(pol, m, ps, t, grid, dt, in, outv);
+    return run_implicit_multiply(pol, m, ps, t, grid, dt, in, outv, true);
   };
   auto project = [&](float* v) -> int32_t {
     return from_hip(implicit_project(gd, cap, t, grid->dx, cols, pol->gcols, v, s));

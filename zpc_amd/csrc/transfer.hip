@@ -1225,6 +1225,143 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
   if (kNext >= 0 && fp64_used) atomicAdd(&counters[kCtrFp64], fp64_used);
 }
 
+// The gather half of G2PTransfer's implicit twin, G2P2GTransfer::operator()
+// (simulation/transfer/G2P2G.hpp:49-110), per bucket as G2P: the bucket's
+// 8^3 node velocities from the vector v (3 floats per node, node = block * 64
+// + cell) are staged in LDS, and each particle takes C from its 27 staged
+// nodes, F = (I + dt C) F_old (not stored) and the model's vol P F^T (logJp
+// read, not written back) into contrib[9 p]. The force scatter is P2G's.
+__global__ __launch_bounds__(kG2PThreads) void g2p2g_gather_kernel(ParticlesView ps, TableView t, SortView sv,
+                                                                   const int32_t* __restrict__ nbr_tab,
+                                                                   const float* __restrict__ v, int32_t grid_cap,
+                                                                   Model m, float dt, float dx,
+                                                                   float* __restrict__ contrib, uint32_t* err,
+                                                                   uint32_t* counters) {
+  uint32_t fp64_used = 0;
+  __shared__ float4 vel[kVelNodes];
+  __shared__ int32_t nbr[8];
+  __shared__ float4 stage4[kG2PThreads / kWave][stage_floats(9) / 4];
+  const int32_t nb = bounded_blocks(t, sv, grid_cap, err);
+  const float dx_inv = (float)1 / dx;
+  const float D_inv = 4.f * dx_inv * dx_inv;
+  const int wave = uniform((int32_t)(threadIdx.x / kWave)), lane = lane_id();
+  float* stage = reinterpret_cast<float*>(stage4[wave]);
+  for (int32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    const int32_t cnt = sv.counts[b];
+    const int32_t off = sv.offsets[b];
+    if (cnt <= 0 || off < 0 || (int64_t)off + cnt > ps.n) continue;
+    if (threadIdx.x < 8) {
+      const int32_t r = nbr_tab[8 * (int64_t)b + threadIdx.x];
+      nbr[threadIdx.x] = r;
+      if (r < 0) atomicOr(err, kErrMissingBlock);
+    }
+    const int32_t o0 = t.active[3 * (int64_t)b] * 4, o1 = t.active[3 * (int64_t)b + 1] * 4,
+                  o2 = t.active[3 * (int64_t)b + 2] * 4;
+    int32_t first = -1;
+    if (wave * kWave + lane < cnt) first = sv.indices[off + wave * kWave + lane];
+    __syncthreads();
+    for (int s = threadIdx.x; s < kArena8; s += kG2PThreads) {
+      const int nbi = s >> 6, cell = s & 63;
+      const int r0 = ((nbi >> 2) & 1) * 4 + ((cell >> 4) & 3);
+      const int r1 = ((nbi >> 1) & 1) * 4 + ((cell >> 2) & 3);
+      const int r2 = (nbi & 1) * 4 + (cell & 3);
+      const int32_t gb = nbr[nbi];
+      float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (gb >= 0 && gb < grid_cap) {
+        const float* vp = v + 3 * ((int64_t)gb * kBlockCells + cell);
+        w = make_float4(vp[0], vp[1], vp[2], 0.f);
+      }
+      vel[vel_at(r0, r1, r2)] = w;
+    }
+    __syncthreads();
+    for (int32_t i0 = wave * kWave; i0 < cnt; i0 += kG2PThreads) {
+      const int32_t cw = min(kWave, cnt - i0);
+      const bool valid = lane < cw;
+      const int64_t p = valid ? (int64_t)(i0 == wave * kWave ? first : sv.indices[off + i0 + lane]) : -1;
+      const bool bad = valid && (p < 0 || p >= ps.n);
+      if (bad) atomicOr(err, kErrCapacity);
+      const bool live = valid && !bad;
+      int64_t p0;
+      const bool contig = wave_contiguous(p, live, p0) && ballot(bad) == 0;
+      float x[3] = {0.f, 0.f, 0.f}, F[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      float lj = (m.kind == 1 && live) ? ps.logJp[p] : 0.f;
+      if (contig) {
+        RowLoad<3> lx;
+        RowLoad<9> lf;
+        lx.issue(ps.x, p0, cw);
+        lf.issue(ps.F, p0, cw);
+        lx.finish(stage, cw, x);
+        lf.finish(stage, cw, F);
+      } else if (live) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x[d] = ps.x[3 * p + d];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = ps.F[9 * p + k];
+      }
+      Arena ar;
+      make_arena(x[0], x[1], x[2], dx, ar);
+      const int lb0 = ar.base[0] - o0, lb1 = ar.base[1] - o1, lb2 = ar.base[2] - o2;
+      const bool outside = live && (lb0 < 0 || lb0 > 5 || lb1 < 0 || lb1 > 5 || lb2 < 0 || lb2 > 5);
+      if (outside) atomicOr(err, kErrArena);
+      const bool ok = live && !outside;
+      float PF[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (ok) {
+        float C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float xa = (float)a * dx - ar.lp[0];
+#pragma unroll
+          for (int bb = 0; bb < 3; ++bb) {
+            const float xb = (float)bb * dx - ar.lp[1];
+            const float wab = ar.w[0][a] * ar.w[1][bb];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const float xc = (float)c * dx - ar.lp[2];
+              const float W = wab * ar.w[2][c];
+              const float4 vi = vel[vel_at(lb0 + a, lb1 + bb, lb2 + c)];
+              const float w0 = W * vi.x * D_inv, w1 = W * vi.y * D_inv, w2 = W * vi.z * D_inv;
+              C[0] += w0 * xa;
+              C[1] += w1 * xa;
+              C[2] += w2 * xa;
+              C[3] += w0 * xb;
+              C[4] += w1 * xb;
+              C[5] += w2 * xb;
+              C[6] += w0 * xc;
+              C[7] += w1 * xc;
+              C[8] += w2 * xc;
+            }
+          }
+        }
+        // F = (I + dt C) F_old, column-major (G2P2G.hpp:81-84)
+        float Fn[9];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            const float a0 = C[r] * dt + (r == 0 ? 1.f : 0.f);
+            const float a1 = C[r + 3] * dt + (r == 1 ? 1.f : 0.f);
+            const float a2 = C[r + 6] * dt + (r == 2 ? 1.f : 0.f);
+            Fn[r + 3 * j] = a0 * F[3 * j] + a1 * F[3 * j + 1] + a2 * F[3 * j + 2];
+          }
+        bool f64 = false;
+        if (m.kind == 0)
+          stress_fixedcorotated(m, Fn, PF, f64);
+        else
+          stress_sand(m, Fn, lj, PF, nullptr, f64);
+        fp64_used += f64 ? 1u : 0u;
+      }
+      if (contig && ballot(outside) == 0) {
+        wave_store_rows<9>(contrib, p0, cw, stage, PF);
+      } else if (ok) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) contrib[9 * p + k] = PF[k];
+      }
+    }
+    __syncthreads();
+  }
+  if (fp64_used) atomicAdd(&counters[kCtrFp64], fp64_used);
+}
+
 // Vector::reorder gather (container/Vector.hpp:617-646) of every attribute
 __global__ __launch_bounds__(kThreads) void reorder_kernel(ParticlesView in, ParticlesView out,
                                                            const int32_t* __restrict__ map, uint32_t* err) {
@@ -1399,6 +1536,15 @@ hipError_t p2c2g_scatter(const ParticlesView& ps, const TableView& t, const Sort
   reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
   p2g_kernel<true, false, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx,
                                                           ws.err, ws.counters + kCtrQueue, SceneSet{});
+  return hipGetLastError();
+}
+
+hipError_t g2p2g_gather(const ParticlesView& ps, const TableView& t, const SortView& sv, const float* v,
+                        int32_t grid_cap, const Model& m, float dt, float dx, Workspace& ws, hipStream_t s) {
+  if (ps.n == 0) return hipSuccess;
+  const int g = ws.grid_g2p > 0 ? ws.grid_g2p : persistent_groups();
+  g2p2g_gather_kernel<<<g, kG2PThreads, 0, s>>>(ps, t, sv, ws.nbr, v, grid_cap, m, dt, dx, ws.contrib, ws.err,
+                                                 ws.counters);
   return hipGetLastError();
 }
 
