@@ -38,7 +38,7 @@ hipError_t grid_angular_momentum(const float*, int32_t, const TableView&, float,
 hipError_t p2c2g_scatter(const ParticlesView&, const TableView&, const SortView&, float*, int32_t, float, float, Workspace&, hipStream_t s) { return touch(s); }
 hipError_t g2c2p(const ParticlesView&, const TableView&, const SortView&, const float*, int32_t, float, float, Workspace&, hipStream_t s) { return touch(s); }
 hipError_t reorder_particles(const ParticlesView&, const ParticlesView&, const int32_t*, Workspace&, hipStream_t s) { return touch(s); }
-hipError_t g2p2g_gather(const ParticlesView&, const TableView&, const SortView&, const float*, int32_t, const Model&, float, float, Workspace&, hipStream_t s) { return touch(s); }
+hipError_t g2p2g_gather(const ParticlesView&, const TableView&, const SortView&, const float*, int32_t, float, float, float*, Workspace&, hipStream_t s) { return touch(s); }
 hipError_t implicit_add_force(const float*, int32_t, const TableView&, float*, hipStream_t s) { return touch(s); }
 hipError_t implicit_force_dt2_mass(const float*, int32_t, const TableView&, float, const float* in, float* out, hipStream_t s) { return in == out ? hipErrorInvalidValue : touch(s); }
 hipError_t implicit_project(const float*, int32_t, const TableView&, float, const Colliders&, const GeneralColliders&, float*, hipStream_t s) { return touch(s); }

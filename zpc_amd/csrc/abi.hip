@@ -1295,7 +1295,11 @@ static int32_t run_g2p2g(zsmpm_policy_s* pol, const Model& m, const ParticlesVie
   hipStream_t s = pol->stream;
   ZS_TRY(ensure(pol, ps.n, t.T));
   const size_t gfloats = (size_t)grid->capacity_blocks * kChannels * kBlockCells;
-  const size_t need = 4 * (gfloats + (size_t)ps.n) + 512;
+  // scratch grid, zero masses, the updated F and a logJp copy the return
+  // mapping may write (G2P2G reads logJp and leaves it)
+  const size_t galign = (gfloats + 63) & ~(size_t)63;
+  const size_t nalign = ((size_t)ps.n + 63) & ~(size_t)63;
+  const size_t need = 4 * (galign + 11 * nalign) + 512;
   if (need > pol->g2p2g_bytes) {
     ZS_HIP(hipStreamSynchronize(s));
     if (pol->g2p2g_buf) (void)hipFree(pol->g2p2g_buf);
@@ -1306,13 +1310,23 @@ static int32_t run_g2p2g(zsmpm_policy_s* pol, const Model& m, const ParticlesVie
     pol->g2p2g_bytes = need;
   }
   float* tg = (float*)pol->g2p2g_buf;
-  float* zm = tg + ((gfloats + 63) & ~(size_t)63);
+  float* zm = tg + galign;
+  float* Fn = zm + nalign;
+  float* lj = Fn + 9 * nalign;
   SortView sv = own_buckets(pol);
   if (!sorted) {
     ZS_TRY(run_buckets(pol, t, ps, grid->dx, sv));
     ZS_HIP(block_neighbours(t, pol->ws, s));
   }
-  ZS_HIP(g2p2g_gather(ps, t, sv, v, grid->capacity_blocks, m, dt, grid->dx, pol->ws, s));
+  ZS_HIP(g2p2g_gather(ps, t, sv, v, grid->capacity_blocks, dt, grid->dx, Fn, pol->ws, s));
+  // the model's vol P F^T of the updated F into the workspace's stress array
+  ParticlesView pf = ps;
+  pf.F = Fn;
+  if (m.kind == 1) {
+    ZS_HIP(hipMemcpyAsync(lj, ps.logJp, sizeof(float) * (size_t)ps.n, hipMemcpyDeviceToDevice, s));
+    pf.logJp = lj;
+  }
+  ZS_HIP(zsmpm::stress(m, pf, pol->ws, s));
   ZS_HIP(grid_clear(tg, t.cnt, grid->capacity_blocks, s, nullptr));
   ParticlesView pz = ps;
   pz.m = zm;

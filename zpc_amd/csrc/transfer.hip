@@ -1226,18 +1226,17 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
 }
 
 // The gather half of G2PTransfer's implicit twin, G2P2GTransfer::operator()
-// (simulation/transfer/G2P2G.hpp:49-110), per bucket as G2P: the bucket's
+// (simulation/transfer/G2P2G.hpp:49-84), per bucket as G2P: the bucket's
 // 8^3 node velocities from the vector v (3 floats per node, node = block * 64
 // + cell) are staged in LDS, and each particle takes C from its 27 staged
-// nodes, F = (I + dt C) F_old (not stored) and the model's vol P F^T (logJp
-// read, not written back) into contrib[9 p]. The force scatter is P2G's.
+// nodes and writes F = (I + dt C) F_old to Fout[9 p]. The model's stress of
+// that F is the ordinary stress pass (full occupancy; the SVD inside this
+// kernel held it to three waves per SIMD), the force scatter P2G's.
 __global__ __launch_bounds__(kG2PThreads) void g2p2g_gather_kernel(ParticlesView ps, TableView t, SortView sv,
                                                                    const int32_t* __restrict__ nbr_tab,
                                                                    const float* __restrict__ v, int32_t grid_cap,
-                                                                   Model m, float dt, float dx,
-                                                                   float* __restrict__ contrib, uint32_t* err,
-                                                                   uint32_t* counters) {
-  uint32_t fp64_used = 0;
+                                                                   float dt, float dx, float* __restrict__ Fout,
+                                                                   uint32_t* err) {
   __shared__ float4 vel[kVelNodes];
   __shared__ int32_t nbr[8];
   __shared__ float4 stage4[kG2PThreads / kWave][stage_floats(9) / 4];
@@ -1284,7 +1283,6 @@ __global__ __launch_bounds__(kG2PThreads) void g2p2g_gather_kernel(ParticlesView
       int64_t p0;
       const bool contig = wave_contiguous(p, live, p0) && ballot(bad) == 0;
       float x[3] = {0.f, 0.f, 0.f}, F[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      float lj = (m.kind == 1 && live) ? ps.logJp[p] : 0.f;
       if (contig) {
         RowLoad<3> lx;
         RowLoad<9> lf;
@@ -1304,7 +1302,9 @@ __global__ __launch_bounds__(kG2PThreads) void g2p2g_gather_kernel(ParticlesView
       const bool outside = live && (lb0 < 0 || lb0 > 5 || lb1 < 0 || lb1 > 5 || lb2 < 0 || lb2 > 5);
       if (outside) atomicOr(err, kErrArena);
       const bool ok = live && !outside;
-      float PF[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      float Fn[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Fn[k] = F[k];
       if (ok) {
         float C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -1333,7 +1333,6 @@ __global__ __launch_bounds__(kG2PThreads) void g2p2g_gather_kernel(ParticlesView
           }
         }
         // F = (I + dt C) F_old, column-major (G2P2G.hpp:81-84)
-        float Fn[9];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -1343,23 +1342,16 @@ __global__ __launch_bounds__(kG2PThreads) void g2p2g_gather_kernel(ParticlesView
             const float a2 = C[r + 6] * dt + (r == 2 ? 1.f : 0.f);
             Fn[r + 3 * j] = a0 * F[3 * j] + a1 * F[3 * j + 1] + a2 * F[3 * j + 2];
           }
-        bool f64 = false;
-        if (m.kind == 0)
-          stress_fixedcorotated(m, Fn, PF, f64);
-        else
-          stress_sand(m, Fn, lj, PF, nullptr, f64);
-        fp64_used += f64 ? 1u : 0u;
       }
       if (contig && ballot(outside) == 0) {
-        wave_store_rows<9>(contrib, p0, cw, stage, PF);
-      } else if (ok) {
+        wave_store_rows<9>(Fout, p0, cw, stage, Fn);
+      } else if (live) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) contrib[9 * p + k] = PF[k];
+        for (int k = 0; k < 9; ++k) Fout[9 * p + k] = Fn[k];
       }
     }
     __syncthreads();
   }
-  if (fp64_used) atomicAdd(&counters[kCtrFp64], fp64_used);
 }
 
 // Vector::reorder gather (container/Vector.hpp:617-646) of every attribute
@@ -1540,11 +1532,10 @@ hipError_t p2c2g_scatter(const ParticlesView& ps, const TableView& t, const Sort
 }
 
 hipError_t g2p2g_gather(const ParticlesView& ps, const TableView& t, const SortView& sv, const float* v,
-                        int32_t grid_cap, const Model& m, float dt, float dx, Workspace& ws, hipStream_t s) {
+                        int32_t grid_cap, float dt, float dx, float* Fout, Workspace& ws, hipStream_t s) {
   if (ps.n == 0) return hipSuccess;
   const int g = ws.grid_g2p > 0 ? ws.grid_g2p : persistent_groups();
-  g2p2g_gather_kernel<<<g, kG2PThreads, 0, s>>>(ps, t, sv, ws.nbr, v, grid_cap, m, dt, dx, ws.contrib, ws.err,
-                                                 ws.counters);
+  g2p2g_gather_kernel<<<g, kG2PThreads, 0, s>>>(ps, t, sv, ws.nbr, v, grid_cap, dt, dx, Fout, ws.err);
   return hipGetLastError();
 }
 

@@ -315,9 +315,9 @@ inline int persistent_groups() { return 4096; }
 // ---- implicit step (implicit.hip; the projection in transfer.hip) ----
 // Vectors hold 3 floats per node (node = block * 64 + cell) for the blocks the
 // table holds (read on the device), at most grid_cap.
-// G2P2G's gather half: contrib[9 p] = vol P F^T at F = (I + dt C(v)) F_old
+// G2P2G's gather half, per bucket: Fout[9 p] = (I + dt C(v)) F_old
 hipError_t g2p2g_gather(const ParticlesView& ps, const TableView& t, const SortView& sv, const float* v,
-                        int32_t grid_cap, const Model& m, float dt, float dx, Workspace& ws, hipStream_t s);
+                        int32_t grid_cap, float dt, float dx, float* Fout, Workspace& ws, hipStream_t s);
 // r += grid channels 1..3 (a folded P2G force) node by node
 hipError_t implicit_add_force(const float* grid, int32_t grid_cap, const TableView& t, float* r, hipStream_t s);
 // out_i = (out_i dt^2 + m) in_i where the node has mass
