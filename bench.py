@@ -208,6 +208,7 @@ def run_path(torch, rocm, sc, args, multi, graph, closure=None):
     ctr = sim.pol.counters()
     out = {"stages": stages, "blocks": 0.5 * (b0 + b1), "ms": wall * 1e3 / args.steps, "eager_ms": eager_ms,
            "launch_widths": sim.pol.launch_widths(),
+           "launch_tiles": sim.pol.launch_tiles(),
            "fallbacks": int(ctr[4]) - fb0, "net_waves": int(ctr[9]) - nw0, "substeps": substeps_run,
            "x": x_by_id(torch, sim)}
     del sim
@@ -347,6 +348,7 @@ def batched_leg(torch, rocm, args) -> dict:
     b.sync()
     ms = (t1 - t0) * 1e3 / steps
     widths = b.pol.launch_widths()
+    tiles = b.pol.launch_tiles()
     print(f"# 64 scenes, {n} particles, {blocks} blocks: {ms:.3f} ms per substep of every scene "
           f"(one launch per stage)", file=sys.stderr, flush=True)
     largest = max(scenes, key=lambda sc: sc.n)
@@ -361,6 +363,7 @@ def batched_leg(torch, rocm, args) -> dict:
         "ms_per_step": ms,
         "steps": steps,
         "p2g_g2p_workgroups": list(widths),
+        "p2g_chunk_g2p_threads": list(tiles),
         "schedule": "zsmpm_substeps_batched__rocm: one table and grid for all scenes, every stage one launch, "
                     "replays of a captured two-substep hipGraph",
     }
@@ -807,6 +810,7 @@ def main() -> int:
             "particle_order": "original (in place)" if args.in_place else
                               "bucket order, reordered inside G2P every substep",
             "p2g_g2p_workgroups": list(plain["launch_widths"]),
+            "p2g_chunk_g2p_threads": list(plain["launch_tiles"]),
             "driver": ("zsmpm_substeps_sorted__rocm with ZSMPM_SUBSTEPS_GRAPH (a captured two-substep "
                        "hipGraph replayed)" if graph else
                        "zsmpm_substeps_sorted__rocm (one call for all timed substeps)") if multi else

@@ -315,9 +315,10 @@ int32_t zsmpm_substep_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, c
  * captured again when any argument or the policy's scratch changes. Stage
  * profiling does not see graph replays.
  * With ZSMPM_SUBSTEPS_AUTOTUNE (and nsteps >= 8, not with FUSE_STRESS) the
- * first 8 substeps, which are the caller's own, run in pairs with four P2G /
- * G2P workgroup counts timed per kernel; the policy keeps the fastest count
- * of each kernel for all later calls (zsmpm_policy_launch_widths__rocm). */
+ * first 8 substeps, which are the caller's own, run in pairs with four P2G
+ * and four G2P launch configurations (tile and workgroup count) timed per
+ * kernel; the policy keeps the fastest of each kernel for all later calls
+ * (zsmpm_policy_launch_widths__rocm, zsmpm_policy_launch_tiles__rocm). */
 int32_t zsmpm_substeps_sorted__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* a,
                                     const zsmpm_particles* b, const zsmpm_table* table,
                                     const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt,
@@ -341,6 +342,14 @@ int32_t zsmpm_policy_launch_widths__rocm(zsmpm_policy pol, int32_t* p2g_groups, 
  * would (0 restores the default; at most 1 << 20), e.g. to replay a tuned
  * configuration in another process. No counterpart in the reference. */
 int32_t zsmpm_policy_set_launch_widths__rocm(zsmpm_policy pol, int32_t p2g_groups, int32_t g2p_groups);
+/* The launch tiles of the sorted substep's P2G and G2P (no counterpart in zpc,
+ * whose launches have one shape): P2G's chunk (576 particles over four waves,
+ * three workgroups per CU, or 256 over two waves, six per CU) and G2P's
+ * threads per bucket (128 or 256). ZSMPM_SUBSTEPS_AUTOTUNE picks them with the
+ * workgroup counts; set 0 for the default (576, 128). Results do not depend
+ * on them beyond float summation order. Other values: ZSMPM_INVALID_ARGUMENT. */
+int32_t zsmpm_policy_launch_tiles__rocm(zsmpm_policy pol, int32_t* p2g_chunk, int32_t* g2p_threads);
+int32_t zsmpm_policy_set_launch_tiles__rocm(zsmpm_policy pol, int32_t p2g_chunk, int32_t g2p_threads);
 
 /* ------------------------------------------------------------------------
  * Batches of independent scenes (an addition: zpc advances one scene per

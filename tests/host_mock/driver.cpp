@@ -59,7 +59,16 @@ int main(int argc, char** argv) {
     expect("p2c2g sand without logJp", zsmpm_p2c2g__rocm(pol, &dp, &nolj, &s0.t, &s0.bk, &s0.g, 1e-4f), ZSMPM_INVALID_ARGUMENT);
   }
   expect("launch widths", zsmpm_policy_launch_widths__rocm(pol, &wp, &wg), 0);
-  expect("autotuned widths are candidates", (wp == 512 || wp == 768 || wp == 1024 || wp == 2048) && (wg == 8192 || wg == 16384 || wg == 32768 || wg == 65536), 1);
+  expect("autotuned widths are candidates", (wp == 1024 || wp == 2048 || wp == 4096) && (wg == 32768 || wg == 65536), 1);
+  int32_t tc = 0, tt = 0;
+  expect("launch tiles", zsmpm_policy_launch_tiles__rocm(pol, &tc, &tt), 0);
+  expect("autotuned tiles are candidates", (tc == 576 || tc == 256) && (tt == 128 || tt == 256), 1);
+  expect("set tiles", zsmpm_policy_set_launch_tiles__rocm(pol, 256, 256), 0);
+  expect("tiles read back", zsmpm_policy_launch_tiles__rocm(pol, &tc, &tt) == 0 && tc == 256 && tt == 256, 1);
+  expect("substeps with set tiles", zsmpm_substeps_sorted__rocm(pol, &s0.model, &s0.a, &s0.b, &s0.t, &s0.bk, &s0.g, 1e-4f, g3, s0.cols, 6, 4, ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
+  expect("odd chunk rejected", zsmpm_policy_set_launch_tiles__rocm(pol, 300, 128), ZSMPM_INVALID_ARGUMENT);
+  expect("odd threads rejected", zsmpm_policy_set_launch_tiles__rocm(pol, 576, 64), ZSMPM_INVALID_ARGUMENT);
+  expect("default tiles", zsmpm_policy_set_launch_tiles__rocm(pol, 0, 0) == 0 && zsmpm_policy_launch_tiles__rocm(pol, &tc, &tt) == 0 && tc == 576 && tt == 128, 1);
   expect("set widths", zsmpm_policy_set_launch_widths__rocm(pol, 300, 5000), 0);
   expect("widths read back", zsmpm_policy_launch_widths__rocm(pol, &wp, &wg) == 0 && wp == 300 && wg == 5000, 1);
   expect("negative width rejected", zsmpm_policy_set_launch_widths__rocm(pol, -1, 5000), ZSMPM_INVALID_ARGUMENT);

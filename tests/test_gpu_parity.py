@@ -486,6 +486,28 @@ def test_set_launch_widths_substeps(sand):
         sim.pol.set_launch_widths(-1, 1000)
 
 
+@pytest.mark.parametrize("tiles", [(256, 256), (576, 256), (256, 128)])
+def test_launch_tiles_substeps(sand, tiles):
+    """Each P2G chunk / G2P workgroup shape the autotuner chooses between
+    (zsmpm_policy_set_launch_tiles__rocm), with graph replays: the substeps
+    stay on the oracle."""
+    rocm = _rocm()
+    sim = rocm.Simulator(sand, reorder=True)
+    sim.pol.set_launch_tiles(*tiles)
+    assert sim.pol.launch_tiles() == tiles
+    sim.substeps(4, graph=True)
+    sim.sync()
+    got = sim.parts.host_by_id()
+    ps = _oracle_particles(sand)
+    O.Context().run(O.make_params(sand), ps, 4, reorder=True)
+    ps64 = _oracle_particles(sand).astype(np.float64)
+    O.Context(np.float64).run(O.make_params(sand), ps64, 4, reorder=True)
+    for name in ("x", "v", "F"):
+        check_field(f"{name} after 4 substeps at tiles {tiles}", got[name], ps.by_id(name), ps64.by_id(name), 1e-4)
+    with pytest.raises(rocm.ZsmpmError):
+        sim.pol.set_launch_tiles(300, 128)
+
+
 def _first_occurrence(packed):
     """Distinct values of `packed` in order of first occurrence."""
     u, first = np.unique(packed, return_index=True)

@@ -55,6 +55,7 @@ struct zsmpm_policy_s {
   int32_t bht_rounds = 0;
   // P2G / G2P workgroup counts picked by ZSMPM_SUBSTEPS_AUTOTUNE (0: default)
   int32_t tuned_p2g = 0, tuned_g2p = 0;
+  int32_t tuned_p2g_tile = 0, tuned_g2p_threads = 0;  // launch tiles (0: defaults)
   Workspace ws{};
   std::vector<void*> allocs;
   int64_t reserved_n = 0;
@@ -540,6 +541,8 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
   Workspace& ws = pol->ws;
   ws.grid_p2g = pol->tuned_p2g;
   ws.grid_g2p = pol->tuned_g2p;
+  ws.p2g_tile = pol->tuned_p2g_tile;
+  ws.g2p_threads = pol->tuned_g2p_threads;
   const int32_t lo[3] = {0, 0, 0}, hi[3] = {2, 2, 2};
   // The stress pass reads F and logJp and writes ws.contrib; the sparsity
   // stages read x only. It runs on the side stream beside them and P2G waits
@@ -645,6 +648,8 @@ int32_t capture_pair(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_
   key_put(key, &pol->closure);
   key_put(key, &pol->tuned_p2g);
   key_put(key, &pol->tuned_g2p);
+  key_put(key, &pol->tuned_p2g_tile);
+  key_put(key, &pol->tuned_g2p_threads);
   key_put(key, &pol->gcols);
   if (pol->gexec && key == pol->gkey) return ZSMPM_OK;
   destroy_graph(pol);
@@ -798,22 +803,28 @@ int32_t substeps_driver(zsmpm_policy_s* pol, const zsmpm_model* model, const zsm
   const bool fuse = (flags & ZSMPM_SUBSTEPS_FUSE_STRESS) != 0;
   const bool graph = (flags & ZSMPM_SUBSTEPS_GRAPH) != 0;
   // Autotuning: the first 8 substeps (real ones, in pairs a -> b -> a) run
-  // with four P2G / G2P workgroup counts, timed per kernel with the stage
-  // events; each kernel keeps its fastest count.
+  // with four P2G and four G2P launch configurations (tile and workgroup
+  // count), timed per kernel with the stage events; each kernel keeps its
+  // fastest. P2G: four waves with 576-particle chunks at 1,024 and 2,048
+  // workgroups, two waves with 256-particle chunks at 2,048 and 4,096; G2P:
+  // 128 and 256 threads per bucket at 32,768 and 65,536 workgroups.
   int32_t done = 0;
   if ((flags & ZSMPM_SUBSTEPS_AUTOTUNE) && nsteps >= 8 && !fuse) {
     DeviceGuard g(pol->devid);
-    const int32_t cand_p2g[4] = {512, 768, 1024, 2048}, cand_g2p[4] = {8192, 16384, 32768, 65536};
+    const int32_t cand_p2g[4] = {1024, 2048, 2048, 4096}, cand_g2p[4] = {32768, 65536, 32768, 65536};
+    const int32_t cand_p2g_tile[4] = {0, 0, 1, 1}, cand_g2p_threads[4] = {128, 128, 256, 256};
     const bool prof = pol->profile;
     ZS_HIP(hipStreamSynchronize(pol->stream));
     harvest_events(pol);
     pol->profile = true;
     double best_p = 0.0, best_g = 0.0;
-    int32_t pick_p = 0, pick_g = 0;
+    int32_t pick_p = 0, pick_g = 0, pick_pt = 0, pick_gt = 0;
     int32_t r = ZSMPM_OK;
     for (int c = 0; c < 4 && r == ZSMPM_OK; ++c) {
       pol->tuned_p2g = cand_p2g[c];
       pol->tuned_g2p = cand_g2p[c];
+      pol->tuned_p2g_tile = cand_p2g_tile[c];
+      pol->tuned_g2p_threads = cand_g2p_threads[c];
       const double p0 = pol->stage_ms[ZSMPM_STAGE_P2G], g0 = pol->stage_ms[ZSMPM_STAGE_G2P];
       for (int k = 0; k < 2 && r == ZSMPM_OK; ++k, ++done)
         r = run_substep(pol, model, (done & 1) ? b : a, (done & 1) ? a : b, table, buckets, grid, dt, extf,
@@ -824,16 +835,20 @@ int32_t substeps_driver(zsmpm_policy_s* pol, const zsmpm_model* model, const zsm
       if (c == 0 || dp < best_p) {
         best_p = dp;
         pick_p = cand_p2g[c];
+        pick_pt = cand_p2g_tile[c];
       }
       if (c == 0 || dg < best_g) {
         best_g = dg;
         pick_g = cand_g2p[c];
+        pick_gt = cand_g2p_threads[c];
       }
     }
     pol->profile = prof;
     if (r != ZSMPM_OK) return r;
     pol->tuned_p2g = pick_p;
     pol->tuned_g2p = pick_g;
+    pol->tuned_p2g_tile = pick_pt;
+    pol->tuned_g2p_threads = pick_gt;
   }
   nsteps -= done;  // an even count: the particles are back in a
   if (graph) {
@@ -1793,6 +1808,22 @@ int32_t zsmpm_policy_set_launch_widths__rocm(zsmpm_policy pol, int32_t p2g_group
     return ZSMPM_INVALID_ARGUMENT;
   pol->tuned_p2g = p2g_groups;
   pol->tuned_g2p = g2p_groups;
+  return ZSMPM_OK;
+}
+
+int32_t zsmpm_policy_launch_tiles__rocm(zsmpm_policy pol, int32_t* p2g_chunk, int32_t* g2p_threads) {
+  if (!pol || !p2g_chunk || !g2p_threads) return ZSMPM_INVALID_ARGUMENT;
+  *p2g_chunk = pol->tuned_p2g_tile == 1 ? 256 : 576;
+  *g2p_threads = pol->tuned_g2p_threads == 256 ? 256 : 128;
+  return ZSMPM_OK;
+}
+
+int32_t zsmpm_policy_set_launch_tiles__rocm(zsmpm_policy pol, int32_t p2g_chunk, int32_t g2p_threads) {
+  if (!pol || (p2g_chunk != 0 && p2g_chunk != 576 && p2g_chunk != 256) ||
+      (g2p_threads != 0 && g2p_threads != 128 && g2p_threads != 256))
+    return ZSMPM_INVALID_ARGUMENT;
+  pol->tuned_p2g_tile = p2g_chunk == 256 ? 1 : 0;
+  pol->tuned_g2p_threads = g2p_threads == 256 ? 256 : 0;
   return ZSMPM_OK;
 }
 

@@ -57,6 +57,12 @@ __device__ __forceinline__ int vel_at(int r0, int r1, int r2) { return r0 * kVel
 constexpr int kG2PThreads = 128;
 constexpr int kP2GThreads = 256;  // one workgroup per bucket
 constexpr int kP2GWaves = kP2GThreads / kWave;
+// Launch tiles the autotuner chooses between. P2G tile 0: four waves and
+// chunks of Rec::kChunk particles (a whole bucket of up to 576, three
+// workgroups per CU); tile 1: two waves and chunks of 256 (six workgroups per
+// CU, for sparser buckets). G2P: 128 or 256 threads per bucket.
+constexpr int p2g_threads(int tile) { return tile == 0 ? kP2GThreads : 128; }
+constexpr int p2g_chunk(int tile, int fold_chunk) { return tile == 0 ? fold_chunk : 256; }
 
 inline int groups_for(int64_t n, int threads) {
   int64_t g = (n + threads - 1) / threads;
@@ -267,21 +273,24 @@ __global__ __launch_bounds__(kThreads) void stress_kernel(Model m, ParticlesView
   if (fp64_used) atomicAdd(&counters[kCtrFp64], fp64_used);
 }
 
-// kMulti (a batch, ss): each bucket's dx and dt are its scene's.
+// kMulti (a batch, ss): each bucket's dx and dt are its scene's. kTile: the
+// launch tile (p2g_threads, p2g_chunk).
 // kCell (folded, one scene): P2C2GTransfer's particle -> cell -> node kernel
 // (cell_w) in place of the quadratic B-spline, with C and the stress scaled
 // per column by the cell kernel's Dinv.
-template <bool kFold, bool kMulti, bool kCell = false>
-__global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, const float* __restrict__ contrib,
+template <bool kFold, bool kMulti, bool kCell = false, int kTile = 0>
+__global__ __launch_bounds__(p2g_threads(kTile)) void p2g_kernel(ParticlesView ps, const float* __restrict__ contrib,
                                                        TableView t, SortView sv, const int32_t* __restrict__ nbr_tab,
                                                        float* __restrict__ grid, int32_t grid_cap, float dt,
                                                        float dx, uint32_t* err, uint32_t* queue, SceneSet ss) {
   constexpr int kQ = Rec<kFold>::kQuads;
-  constexpr int kChunk = Rec<kFold>::kChunk;
-  static_assert(kP2GThreads % kWave == 0 && kChunk % kWave == 0,
+  constexpr int kT = p2g_threads(kTile);
+  constexpr int kWv = kT / kWave;
+  constexpr int kChunk = p2g_chunk(kTile, Rec<kFold>::kChunk);
+  static_assert(kT % kWave == 0 && kChunk % kWave == 0,
                 "whole waves; a chunk is a whole number of waves");
   constexpr int kNV = kFold ? 1 : 2;  // float4 per arena node
-  __shared__ float4 arena[kP2GWaves][kArena6 * kNV];
+  __shared__ float4 arena[kWv][kArena6 * kNV];
   // records in component-major order: component k of record i at
   // rec[k * kChunk + i], so consecutive records are consecutive float4 (no
   // LDS bank conflicts when lanes write consecutive records)
@@ -302,12 +311,12 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
   // every thread then issues the loads of its particle indices in the next
   // bucket's first chunk before the flush, so they are in flight across the
   // flush and the next bucket's set-up instead of starting phase A.
-  constexpr int kPre = (kChunk + kP2GThreads - 1) / kP2GThreads;  // chunk slots per thread
+  constexpr int kPre = (kChunk + kT - 1) / kT;  // chunk slots per thread
   __shared__ int32_t bmeta[7];  // bucket, count, offset, key, scene
   const int32_t nscenes = kMulti ? ss.n : 0;
   if (threadIdx.x == 0)
     bucket_meta((int32_t)min(atomicAdd(queue, 1u), 0x7fffffffu), nb, sv, t, ps.n, bmeta, nscenes);
-  for (int c = threadIdx.x; c <= kMainCells; c += kP2GThreads) {
+  for (int c = threadIdx.x; c <= kMainCells; c += kT) {
     ccount[c] = 0;
     cfill[c] = 0;
   }
@@ -317,7 +326,7 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
     const int32_t lim = min(bmeta[1], kChunk), po = bmeta[2];
 #pragma unroll
     for (int k = 0; k < kPre; ++k) {
-      const int32_t i = threadIdx.x + k * kP2GThreads;
+      const int32_t i = threadIdx.x + k * kT;
       pre[k] = i < lim ? sv.indices[po + i] : -1;
     }
   };
@@ -351,19 +360,19 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
       if (c0 > 0) {
         // later chunks clear the cell counters here; a bucket's first chunk
         // finds them cleared by the previous bucket's flush (or the prologue)
-        for (int c = threadIdx.x; c <= kMainCells; c += kP2GThreads) {  // 64 main cells + overflow
+        for (int c = threadIdx.x; c <= kMainCells; c += kT) {  // 64 main cells + overflow
           ccount[c] = 0;
           cfill[c] = 0;
         }
         __syncthreads();
       }
       // ---- A: stress + affine record per particle. Each wave takes 64
-      // consecutive chunk slots (i = threadIdx.x + k * kP2GThreads, as phase B
+      // consecutive chunk slots (i = threadIdx.x + k * kT, as phase B
       // expects); a contiguous run of particles is read with 16-byte vectors,
       // staged in the wave's own (not yet written) record slots.
 #pragma unroll
       for (int kk = 0; kk < kPre; ++kk) {
-        const int32_t i0 = wave * kWave + kk * kP2GThreads;
+        const int32_t i0 = wave * kWave + kk * kT;
         if (i0 >= nch) break;
         const int32_t i = i0 + lane;
         const int32_t cw = min(kWave, nch - i0);
@@ -501,14 +510,14 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
       uint8_t keepc[kPre];
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
-        const int32_t i = threadIdx.x + k * kP2GThreads;
+        const int32_t i = threadIdx.x + k * kT;
         keep[k] = i < nch ? list[i] : (uint16_t)0;
         keepc[k] = i < nch ? cellof[i] : (uint8_t)255;
       }
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
-        const int32_t i = threadIdx.x + k * kP2GThreads;
+        const int32_t i = threadIdx.x + k * kT;
         if (i < nch && keepc[k] <= kMainCells) {
           const int32_t slot = atomicAdd(&cfill[keepc[k]], 1);
           list[cstart[keepc[k]] + slot] = (uint16_t)i;
@@ -525,7 +534,7 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
         lane_cell(lane, lr0, lr1, lr2);
         float4* ar = arena[wave];
         // All particles of one base cell touch the same 27 nodes: the lane
-        // sums its share of its cell's particles (every kP2GWaves-th, starting
+        // sums its share of its cell's particles (every kWv-th, starting
         // at the wave index) in registers and adds the node sums to the
         // wave's arena once per cell, instead of 27 LDS read-modify-writes
         // per particle. One pass per x-slab of the stencil keeps the
@@ -540,7 +549,7 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
             float acc[9][4];
 #pragma unroll
             for (int k = 0; k < 9; ++k) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.f;
-            for (int32_t r = wave; r < lcount; r += kP2GWaves) {
+            for (int32_t r = wave; r < lcount; r += kWv) {
               const int32_t i = list[lstart + r];
               const float4 q1 = rec[kChunk + i];
               // part 0: (m, B) in q0, A0..A3 in q2, A4..A7 in q3, A8 in q1.w;
@@ -609,7 +618,7 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
       // ---- overflow particles: global atomics straight into the grid
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
-        const int32_t i = threadIdx.x + k * kP2GThreads;
+        const int32_t i = threadIdx.x + k * kT;
         if (i >= nch || keepc[k] != kMainCells) continue;
         const int r0 = (keep[k] >> 6) & 7, r1 = (keep[k] >> 3) & 7, r2 = keep[k] & 7;
         const float4 q0 = rec[i], q1 = rec[kChunk + i], q2 = rec[2 * kChunk + i], q3 = rec[3 * kChunk + i];
@@ -662,15 +671,15 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
     prefetch();  // the next bucket's first-chunk indices (bmeta is the next bucket's now)
     // the next bucket's first chunk finds its cell counters cleared (they are
     // not read from here on; the barrier after the flush orders the clear)
-    for (int c = threadIdx.x; c <= kMainCells; c += kP2GThreads) {
+    for (int c = threadIdx.x; c <= kMainCells; c += kT) {
       ccount[c] = 0;
       cfill[c] = 0;
     }
     // ---- flush: sum the wave arenas, one global atomic per touched node/channel
-    for (int n = threadIdx.x; n < kArena6; n += kP2GThreads) {
+    for (int n = threadIdx.x; n < kArena6; n += kT) {
       float4 s = arena[0][n * kNV];
 #pragma unroll
-      for (int w = 1; w < kP2GWaves; ++w) {
+      for (int w = 1; w < kWv; ++w) {
         const float4 u = arena[w][n * kNV];
         s.x += u.x;
         s.y += u.y;
@@ -681,7 +690,7 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(ParticlesView ps, cons
       if (!kFold) {
         f = arena[0][n * kNV + 1];
 #pragma unroll
-        for (int w = 1; w < kP2GWaves; ++w) {
+        for (int w = 1; w < kWv; ++w) {
           const float4 u = arena[w][n * kNV + 1];
           f.x += u.x;
           f.y += u.y;
@@ -1005,8 +1014,8 @@ __global__ __launch_bounds__(kThreads) void angular_momentum_kernel(const float*
 // (G2C2P.hpp:58-151, 197-271): the cell kernel's weights (cell_w), B =
 // sum W v_i (x_i - x_p)^T stored as C, and F updated with B scaled per column
 // by Dinv at the old position.
-template <bool kReorder, int kNext, bool kMulti = false, bool kCell = false>
-__global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, ParticlesView out, TableView t,
+template <bool kReorder, int kNext, bool kMulti = false, bool kCell = false, int kT = kG2PThreads>
+__global__ __launch_bounds__(kT) void g2p_kernel(ParticlesView ps, ParticlesView out, TableView t,
                                                           SortView sv, const int32_t* __restrict__ nbr_tab,
                                                           const float* __restrict__ grid, int32_t grid_cap,
                                                           float dt_arg, float dx_arg, NextStress ns, uint32_t* err,
@@ -1015,7 +1024,7 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
   uint32_t fp64_used = 0;
   __shared__ float4 vel[kVelNodes];
   __shared__ int32_t nbr[8];
-  __shared__ float4 stage4[kG2PThreads / kWave][stage_floats(9) / 4];
+  __shared__ float4 stage4[kT / kWave][stage_floats(9) / 4];
   const int32_t nb = bounded_blocks(t, sv, grid_cap, err);
   const float dx_inv_arg = (float)1 / dx_arg;
   const float D_inv_arg = 4.f * dx_inv_arg * dx_inv_arg;
@@ -1043,7 +1052,7 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
     if (wave * kWave + lane < cnt) first = sv.indices[off + wave * kWave + lane];
     __syncthreads();
     // stage node velocities: slot s -> (neighbour block s>>6, cell s&63), stored at vel_at(r0, r1, r2)
-    for (int s = threadIdx.x; s < kArena8; s += kG2PThreads) {
+    for (int s = threadIdx.x; s < kArena8; s += kT) {
       const int nbi = s >> 6, cell = s & 63;
       const int r0 = ((nbi >> 2) & 1) * 4 + ((cell >> 4) & 3);
       const int r1 = ((nbi >> 1) & 1) * 4 + ((cell >> 2) & 3);
@@ -1058,7 +1067,7 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(ParticlesView ps, Part
     }
     __syncthreads();
     // each wave takes 64 consecutive bucket slots at a time (wave-uniform loop)
-    for (int32_t i0 = wave * kWave; i0 < cnt; i0 += kG2PThreads) {
+    for (int32_t i0 = wave * kWave; i0 < cnt; i0 += kT) {
       const int32_t cw = min(kWave, cnt - i0);
       const bool valid = lane < cw;
       const int64_t p = valid ? (int64_t)(i0 == wave * kWave ? first : sv.indices[off + i0 + lane]) : -1;
@@ -1461,7 +1470,12 @@ hipError_t check_launch_limits(int device) {
       reinterpret_cast<const void*>(&g2p_kernel<true, -1, true>),
       reinterpret_cast<const void*>(&g2p_kernel<false, -1, true>),
       reinterpret_cast<const void*>(&p2g_kernel<true, false, true>),
-      reinterpret_cast<const void*>(&g2p_kernel<false, -1, false, true>)};
+      reinterpret_cast<const void*>(&g2p_kernel<false, -1, false, true>),
+      reinterpret_cast<const void*>(&p2g_kernel<true, false, false, 1>),
+      reinterpret_cast<const void*>(&p2g_kernel<true, true, false, 1>),
+      reinterpret_cast<const void*>(&g2p_kernel<true, -1, false, false, 256>),
+      reinterpret_cast<const void*>(&g2p_kernel<true, -1, true, false, 256>),
+      reinterpret_cast<const void*>(&g2p2g_gather_kernel)};
   for (int i = 0; i < (int)(sizeof(kernels) / sizeof(kernels[0])); ++i) {
     hipFuncAttributes attr;
     e = hipFuncGetAttributes(&attr, kernels[i]);
@@ -1507,9 +1521,16 @@ hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortVi
   uint32_t* q = ws.counters + kCtrQueue;
   const bool multi = ss && ss->n > 0;
   const SceneSet sset = multi ? *ss : SceneSet{};
-  if (fold && multi)
+  const bool tile1 = ws.p2g_tile == 1;  // the folded launches take the tuned tile
+  if (fold && multi && tile1)
+    p2g_kernel<true, true, false, 1><<<g, p2g_threads(1), 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt,
+                                                                  dx, ws.err, q, sset);
+  else if (fold && multi)
     p2g_kernel<true, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err,
                                                      q, sset);
+  else if (fold && tile1)
+    p2g_kernel<true, false, false, 1><<<g, p2g_threads(1), 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt,
+                                                                   dx, ws.err, q, sset);
   else if (fold)
     p2g_kernel<true, false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err,
                                                       q, sset);
@@ -1583,9 +1604,15 @@ hipError_t g2p(const ParticlesView& ps, const ParticlesView* out, const TableVie
     else
       g2p_kernel<true, 1><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
                                                     ws.counters, sset);
+  } else if (out && sset.n > 0 && ws.g2p_threads == 256) {
+    g2p_kernel<true, -1, true, false, 256><<<g, 256, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns,
+                                                             ws.err, ws.counters, sset);
   } else if (out && sset.n > 0) {
     g2p_kernel<true, -1, true><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
                                                          ws.counters, sset);
+  } else if (out && ws.g2p_threads == 256) {
+    g2p_kernel<true, -1, false, false, 256><<<g, 256, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns,
+                                                              ws.err, ws.counters, sset);
   } else if (out) {
     g2p_kernel<true, -1><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,
                                                    ws.counters, sset);

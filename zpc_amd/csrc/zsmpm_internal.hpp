@@ -179,6 +179,8 @@ struct Workspace {
   uint32_t* wq = nullptr;        // [n_cap / 64 + 2] per wave: 1 where the closure's network must run
   int32_t grid_p2g = 0;          // P2G / G2P workgroups (0: persistent_groups()), set by the policy
   int32_t grid_g2p = 0;
+  int32_t p2g_tile = 0;     // folded P2G launch tile: 0 four waves / 576-particle chunks, 1 two waves / 256
+  int32_t g2p_threads = 0;  // sorted G2P threads per bucket: 256, else 128
   float* lj_next[2] = {nullptr, nullptr};  // [n_cap] each: logJp caches of the fused multi-substep driver
   uint32_t* bitmap = nullptr;    // [bm_words] one bit per insertion order: the orders of new keys
   uint32_t* bm_prefix = nullptr; // [bm_words] exclusive prefix of the bit counts

@@ -197,6 +197,8 @@ def lib():
                                         i32, i32, P(C.c_int32)],
         "zsmpm_policy_closure__rocm": [vp, P(ClosureDesc)],
         "zsmpm_policy_launch_widths__rocm": [vp, P(C.c_int32), P(C.c_int32)],
+        "zsmpm_policy_launch_tiles__rocm": [vp, P(C.c_int32), P(C.c_int32)],
+        "zsmpm_policy_set_launch_tiles__rocm": [vp, i32, i32],
         "zsmpm_policy_set_launch_widths__rocm": [vp, C.c_int32, C.c_int32],
         "zsmpm_policy_colliders__rocm": [vp, P(ColliderDesc), i32],
         "zsmpm_substeps_batched__rocm": [vp, P(BatchSceneDesc), i32, P(ParticlesDesc), P(ParticlesDesc),
@@ -329,6 +331,17 @@ class Policy:
     def set_launch_widths(self, p2g: int, g2p: int):
         """Fixes the P2G and G2P workgroup counts of later launches (0: default)."""
         check(self.L.zsmpm_policy_set_launch_widths__rocm(self.h, int(p2g), int(g2p)), "set_launch_widths")
+
+    def launch_tiles(self):
+        """(P2G chunk, G2P threads per bucket) of the sorted substep's launches."""
+        a, b = C.c_int32(0), C.c_int32(0)
+        check(self.L.zsmpm_policy_launch_tiles__rocm(self.h, C.byref(a), C.byref(b)), "launch_tiles")
+        return a.value, b.value
+
+    def set_launch_tiles(self, p2g_chunk: int, g2p_threads: int):
+        """Fixes the P2G chunk (576 or 256) and G2P threads (128 or 256); 0: default."""
+        check(self.L.zsmpm_policy_set_launch_tiles__rocm(self.h, int(p2g_chunk), int(g2p_threads)),
+              "set_launch_tiles")
 
     # ---- layout and IO adapters ----
     def particles_from_tiles(self, tiles: "DeviceTiles", parts: "DeviceParticles", matrix_order: int = ROW_MAJOR):
