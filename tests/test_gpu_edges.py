@@ -140,6 +140,34 @@ def test_empty_inputs(sand):
     assert bool((sim.parts.x == x0).all())
 
 
+def test_empty_transfers_and_implicit(sand):
+    """n = 0 on an empty table: P2C2G, G2C2P and G2P2G succeed, touch no
+    particle and add nothing."""
+    rocm = _rocm()
+    sim = rocm.Simulator(sand)
+    L = rocm.lib()
+    sim.clear_table()
+    sim.activate(n=0)
+    sim.enlarge()
+    sim.sync()
+    assert sim.table.host()["cnt"] == 0
+    e = _with_n(sim.parts.desc, 0)
+    x0 = sim.parts.x.clone()
+    sim.grid.data.fill_(0.0)
+    rocm.check(L.zsmpm_p2c2g__rocm(sim.pol.h, C.byref(sim.model), C.byref(e), C.byref(sim.table.desc), None,
+                                   C.byref(sim.grid.desc), float(sand.dt)), "p2c2g")
+    rocm.check(L.zsmpm_g2c2p__rocm(sim.pol.h, C.byref(e), C.byref(sim.table.desc), None, C.byref(sim.grid.desc),
+                                   float(sand.dt)), "g2c2p")
+    v = sim.dof_vector(1.0)
+    r = sim.dof_vector(0.0)
+    rocm.check(L.zsmpm_g2p2g__rocm(sim.pol.h, C.byref(sim.model), C.byref(e), C.byref(sim.table.desc),
+                                   C.byref(sim.grid.desc), float(sand.dt), v.data_ptr(), r.data_ptr()), "g2p2g")
+    sim.sync()
+    assert bool((sim.parts.x == x0).all())
+    assert float(sim.grid.data.abs().sum()) == 0.0
+    assert float(r.abs().sum()) == 0.0
+
+
 def test_key_range_reported(sand):
     """A particle whose block key leaves [-2^20, 2^20) is reported as
     ZSMPM_KEY_RANGE by the activation and by the substep."""

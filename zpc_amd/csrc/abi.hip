@@ -1338,7 +1338,8 @@ static int32_t run_g2p2g(zsmpm_policy_s* pol, const Model& m, const ParticlesVie
   ParticlesView pf = ps;
   pf.F = Fn;
   if (m.kind == 1) {
-    ZS_HIP(hipMemcpyAsync(lj, ps.logJp, sizeof(float) * (size_t)ps.n, hipMemcpyDeviceToDevice, s));
+    if (ps.n > 0)
+      ZS_HIP(hipMemcpyAsync(lj, ps.logJp, sizeof(float) * (size_t)ps.n, hipMemcpyDeviceToDevice, s));
     pf.logJp = lj;
   }
   ZS_HIP(zsmpm::stress(m, pf, pol->ws, s));
