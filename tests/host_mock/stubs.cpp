@@ -36,6 +36,7 @@ hipError_t grid_update(float*, int32_t, const TableView&, float, const float*, c
 hipError_t g2p(const ParticlesView&, const ParticlesView*, const TableView&, const SortView&, const float*, int32_t, float, float, Workspace&, hipStream_t s, const NextStress*, const SceneSet* ss) { return scenes_ok(ss, s); }
 hipError_t grid_angular_momentum(const float*, int32_t, const TableView&, float, int, int, double*, Workspace&, hipStream_t s) { return touch(s); }
 hipError_t p2c2g_scatter(const ParticlesView&, const TableView&, const SortView&, float*, int32_t, float, float, Workspace&, hipStream_t s) { return touch(s); }
+hipError_t p2g_force_scatter(const ParticlesView&, const TableView&, const SortView&, float*, int32_t, float, float, Workspace&, hipStream_t s) { return touch(s); }
 hipError_t g2c2p(const ParticlesView&, const TableView&, const SortView&, const float*, int32_t, float, float, Workspace&, hipStream_t s) { return touch(s); }
 hipError_t reorder_particles(const ParticlesView&, const ParticlesView&, const int32_t*, Workspace&, hipStream_t s) { return touch(s); }
 hipError_t g2p2g_gather(const ParticlesView&, const TableView&, const SortView&, const float*, int32_t, float, float, float*, Workspace&, hipStream_t s) { return touch(s); }

@@ -1310,23 +1310,21 @@ static int32_t run_g2p2g(zsmpm_policy_s* pol, const Model& m, const ParticlesVie
   hipStream_t s = pol->stream;
   ZS_TRY(ensure(pol, ps.n, t.T));
   const size_t gfloats = (size_t)grid->capacity_blocks * kChannels * kBlockCells;
-  // scratch grid, zero masses, the updated F and a logJp copy the return
-  // mapping may write (G2P2G reads logJp and leaves it)
+  // scratch grid, the updated F and a logJp copy the return mapping may
+  // write (G2P2G reads logJp and leaves it)
   const size_t galign = (gfloats + 63) & ~(size_t)63;
   const size_t nalign = ((size_t)ps.n + 63) & ~(size_t)63;
-  const size_t need = 4 * (galign + 11 * nalign) + 512;
+  const size_t need = 4 * (galign + 10 * nalign) + 512;
   if (need > pol->g2p2g_bytes) {
     ZS_HIP(hipStreamSynchronize(s));
     if (pol->g2p2g_buf) (void)hipFree(pol->g2p2g_buf);
     pol->g2p2g_buf = nullptr;
     pol->g2p2g_bytes = 0;
     ZS_HIP(hipMalloc(&pol->g2p2g_buf, need));
-    ZS_HIP(hipMemset(pol->g2p2g_buf, 0, need));  // the zero masses stay zero
     pol->g2p2g_bytes = need;
   }
   float* tg = (float*)pol->g2p2g_buf;
-  float* zm = tg + galign;
-  float* Fn = zm + nalign;
+  float* Fn = tg + galign;
   float* lj = Fn + 9 * nalign;
   SortView sv = own_buckets(pol);
   if (!sorted) {
@@ -1344,9 +1342,7 @@ static int32_t run_g2p2g(zsmpm_policy_s* pol, const Model& m, const ParticlesVie
   }
   ZS_HIP(zsmpm::stress(m, pf, pol->ws, s));
   ZS_HIP(grid_clear(tg, t.cnt, grid->capacity_blocks, s, nullptr));
-  ParticlesView pz = ps;
-  pz.m = zm;
-  ZS_HIP(p2g_scatter(pz, t, sv, tg, grid->capacity_blocks, -1.f, grid->dx, true, pol->ws, s));
+  ZS_HIP(p2g_force_scatter(ps, t, sv, tg, grid->capacity_blocks, -1.f, grid->dx, pol->ws, s));
   ZS_HIP(implicit_add_force(tg, grid->capacity_blocks, t, r, s));
   return ZSMPM_OK;
 }

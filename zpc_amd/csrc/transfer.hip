@@ -274,11 +274,12 @@ __global__ __launch_bounds__(kThreads) void stress_kernel(Model m, ParticlesView
 }
 
 // kMulti (a batch, ss): each bucket's dx and dt are its scene's. kTile: the
-// launch tile (p2g_threads, p2g_chunk).
+// launch tile (p2g_threads, p2g_chunk). kForce (folded): the force only, as
+// if every particle were massless -- v, C and m are not read (G2P2G's scatter).
 // kCell (folded, one scene): P2C2GTransfer's particle -> cell -> node kernel
 // (cell_w) in place of the quadratic B-spline, with C and the stress scaled
 // per column by the cell kernel's Dinv.
-template <bool kFold, bool kMulti, bool kCell = false, int kTile = 0>
+template <bool kFold, bool kMulti, bool kCell = false, int kTile = 0, bool kForce = false>
 __global__ __launch_bounds__(p2g_threads(kTile)) void p2g_kernel(ParticlesView ps, const float* __restrict__ contrib,
                                                        TableView t, SortView sv, const int32_t* __restrict__ nbr_tab,
                                                        float* __restrict__ grid, int32_t grid_cap, float dt,
@@ -393,28 +394,28 @@ __global__ __launch_bounds__(p2g_threads(kTile)) void p2g_kernel(ParticlesView p
         float xv[3] = {0.f, 0.f, 0.f}, vv[3] = {0.f, 0.f, 0.f};
         float PF[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         float C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        const float mass = live ? ps.m[p] : 0.f;
+        const float mass = (live && !kForce) ? ps.m[p] : 0.f;
         if (contig) {
           RowLoad<3> lx, lv;
           RowLoad<9> lp, lc;
           lx.issue(ps.x, p0, cw);
-          lv.issue(ps.v, p0, cw);
+          if (!kForce) lv.issue(ps.v, p0, cw);
           lp.issue(contrib, p0, cw);
-          lc.issue(ps.C, p0, cw);
+          if (!kForce) lc.issue(ps.C, p0, cw);
           lx.finish(stage, cw, xv);
-          lv.finish(stage, cw, vv);
+          if (!kForce) lv.finish(stage, cw, vv);
           lp.finish(stage, cw, PF);
-          lc.finish(stage, cw, C);
+          if (!kForce) lc.finish(stage, cw, C);
         } else if (live) {
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
             xv[d] = ps.x[3 * p + d];
-            vv[d] = ps.v[3 * p + d];
+            if (!kForce) vv[d] = ps.v[3 * p + d];
           }
 #pragma unroll
           for (int k = 0; k < 9; ++k) {
             PF[k] = contrib[9 * p + k];
-            C[k] = ps.C[9 * p + k];
+            if (!kForce) C[k] = ps.C[9 * p + k];
           }
         }
         if (!live) continue;
@@ -1475,7 +1476,8 @@ hipError_t check_launch_limits(int device) {
       reinterpret_cast<const void*>(&p2g_kernel<true, true, false, 1>),
       reinterpret_cast<const void*>(&g2p_kernel<true, -1, false, false, 256>),
       reinterpret_cast<const void*>(&g2p_kernel<true, -1, true, false, 256>),
-      reinterpret_cast<const void*>(&g2p2g_gather_kernel)};
+      reinterpret_cast<const void*>(&g2p2g_gather_kernel),
+      reinterpret_cast<const void*>(&p2g_kernel<true, false, false, 0, true>)};
   for (int i = 0; i < (int)(sizeof(kernels) / sizeof(kernels[0])); ++i) {
     hipFuncAttributes attr;
     e = hipFuncGetAttributes(&attr, kernels[i]);
@@ -1549,6 +1551,16 @@ hipError_t p2c2g_scatter(const ParticlesView& ps, const TableView& t, const Sort
   reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
   p2g_kernel<true, false, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx,
                                                           ws.err, ws.counters + kCtrQueue, SceneSet{});
+  return hipGetLastError();
+}
+
+hipError_t p2g_force_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
+                             int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s) {
+  const int g = ws.grid_p2g > 0 ? ws.grid_p2g : persistent_groups();
+  reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
+  p2g_kernel<true, false, false, 0, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap,
+                                                                    dt, dx, ws.err, ws.counters + kCtrQueue,
+                                                                    SceneSet{});
   return hipGetLastError();
 }
 

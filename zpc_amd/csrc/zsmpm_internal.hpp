@@ -317,6 +317,10 @@ inline int persistent_groups() { return 4096; }
 // ---- implicit step (implicit.hip; the projection in transfer.hip) ----
 // Vectors hold 3 floats per node (node = block * 64 + cell) for the blocks the
 // table holds (read on the device), at most grid_cap.
+// P2G's folded scatter of the stress in ws.contrib alone (as if every
+// particle were massless; v, C and m are not read): G2P2G's force half
+hipError_t p2g_force_scatter(const ParticlesView& ps, const TableView& t, const SortView& sv, float* grid,
+                             int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s);
 // G2P2G's gather half, per bucket: Fout[9 p] = (I + dt C(v)) F_old
 hipError_t g2p2g_gather(const ParticlesView& ps, const TableView& t, const SortView& sv, const float* v,
                         int32_t grid_cap, float dt, float dx, float* Fout, Workspace& ws, hipStream_t s);
