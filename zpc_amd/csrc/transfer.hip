@@ -23,9 +23,10 @@
 //   then the four wave arenas are summed and flushed with one global float
 //   atomic per touched node and channel (zpc's CUDA P2G issues 189 global
 //   atomics per particle, cuda/simulation/transfer/P2G.hpp:98-115).
-// With kFold the force term is added to the momentum channels at P2G (the
+// The substep's P2G adds the force term to the momentum channels (the
 // explicit driver's v = (mv + rhs)/m convention) and the grid carries
-// {m, mv + rhs}; otherwise the rhs channels 4..6 are kept as zpc lays them out.
+// {m, mv + rhs}; the stage-wise P2G keeps the rhs channels 4..6 as zpc lays
+// them out, from two launches: momentum only, then the force only.
 //
 // G2P stages the 8^3 node velocities of a bucket in LDS (float4 per node) and
 // gathers from there.
@@ -211,16 +212,14 @@ __device__ __forceinline__ void cell_dinv(const float (&x)[3], float dx, float d
 }
 
 // Record layout (float4 units): q0 = (m, B0, B1, B2), q1 = (l0, l1, l2, A8),
-// q2 = A0..A3, q3 = A4..A7 with A column-major (A[d + 3e]) already times dx;
-// the unfolded layout appends the force part q4 = (Bf0..2, Af8), q5, q6.
-template <bool kFold>
+// q2 = A0..A3, q3 = A4..A7 with A column-major (A[d + 3e]) already times dx.
 struct Rec {
-  static constexpr int kQuads = kFold ? 4 : 7;
+  static constexpr int kQuads = 4;
   // particles staged per pass. 576 (nine waves) holds a whole bucket at 8
   // particles per cell (up to 562 in the 16M sand column; 39% of its buckets
   // exceed 512 and took a nearly empty second pass, 0.80 -> 0.765 ms) and
-  // keeps the folded workgroup at 53 KB of LDS, three per CU
-  static constexpr int kChunk = kFold ? 576 : 256;
+  // keeps the workgroup at 53 KB of LDS, three per CU
+  static constexpr int kChunk = 576;
 };
 
 // Per-particle constitutive part of P2G (P2G.hpp:57-100): vol P F^T from the
@@ -273,25 +272,30 @@ __global__ __launch_bounds__(kThreads) void stress_kernel(Model m, ParticlesView
   if (fp64_used) atomicAdd(&counters[kCtrFp64], fp64_used);
 }
 
-// kMulti (a batch, ss): each bucket's dx and dt are its scene's. kTile: the
-// launch tile (p2g_threads, p2g_chunk). kForce (folded): the force only, as
-// if every particle were massless -- v, C and m are not read (G2P2G's scatter).
-// kCell (folded, one scene): P2C2GTransfer's particle -> cell -> node kernel
+// kMode 0: mass and momentum with the force folded in (channels 0..3);
+// 1: mass and momentum only (the stress is not read); 2: the force only, as
+// if every particle were massless (v, C and m are not read), into channels
+// fch .. fch + 2. kMulti (a batch, ss): each bucket's dx and dt are its
+// scene's. kTile: the launch tile (p2g_threads, p2g_chunk).
+// kCell (mode 0, one scene): P2C2GTransfer's particle -> cell -> node kernel
 // (cell_w) in place of the quadratic B-spline, with C and the stress scaled
 // per column by the cell kernel's Dinv.
-template <bool kFold, bool kMulti, bool kCell = false, int kTile = 0, bool kForce = false>
+template <int kMode, bool kMulti, bool kCell = false, int kTile = 0>
 __global__ __launch_bounds__(p2g_threads(kTile)) void p2g_kernel(ParticlesView ps, const float* __restrict__ contrib,
                                                        TableView t, SortView sv, const int32_t* __restrict__ nbr_tab,
                                                        float* __restrict__ grid, int32_t grid_cap, float dt,
-                                                       float dx, uint32_t* err, uint32_t* queue, SceneSet ss) {
-  constexpr int kQ = Rec<kFold>::kQuads;
+                                                       float dx, uint32_t* err, uint32_t* queue, SceneSet ss,
+                                                       int32_t fch) {
+  static_assert(kMode >= 0 && kMode <= 2 && (!kCell || kMode == 0), "P2G modes");
+  constexpr bool kMom = kMode != 2;     // mass, v and C are read
+  constexpr bool kStress = kMode != 1;  // the stress is read
+  constexpr int kQ = Rec::kQuads;
   constexpr int kT = p2g_threads(kTile);
   constexpr int kWv = kT / kWave;
-  constexpr int kChunk = p2g_chunk(kTile, Rec<kFold>::kChunk);
+  constexpr int kChunk = p2g_chunk(kTile, Rec::kChunk);
   static_assert(kT % kWave == 0 && kChunk % kWave == 0,
                 "whole waves; a chunk is a whole number of waves");
-  constexpr int kNV = kFold ? 1 : 2;  // float4 per arena node
-  __shared__ float4 arena[kWv][kArena6 * kNV];
+  __shared__ float4 arena[kWv][kArena6];
   // records in component-major order: component k of record i at
   // rec[k * kChunk + i], so consecutive records are consecutive float4 (no
   // LDS bank conflicts when lanes write consecutive records)
@@ -394,28 +398,28 @@ __global__ __launch_bounds__(p2g_threads(kTile)) void p2g_kernel(ParticlesView p
         float xv[3] = {0.f, 0.f, 0.f}, vv[3] = {0.f, 0.f, 0.f};
         float PF[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         float C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        const float mass = (live && !kForce) ? ps.m[p] : 0.f;
+        const float mass = (live && kMom) ? ps.m[p] : 0.f;
         if (contig) {
           RowLoad<3> lx, lv;
           RowLoad<9> lp, lc;
           lx.issue(ps.x, p0, cw);
-          if (!kForce) lv.issue(ps.v, p0, cw);
-          lp.issue(contrib, p0, cw);
-          if (!kForce) lc.issue(ps.C, p0, cw);
+          if (kMom) lv.issue(ps.v, p0, cw);
+          if (kStress) lp.issue(contrib, p0, cw);
+          if (kMom) lc.issue(ps.C, p0, cw);
           lx.finish(stage, cw, xv);
-          if (!kForce) lv.finish(stage, cw, vv);
-          lp.finish(stage, cw, PF);
-          if (!kForce) lc.finish(stage, cw, C);
+          if (kMom) lv.finish(stage, cw, vv);
+          if (kStress) lp.finish(stage, cw, PF);
+          if (kMom) lc.finish(stage, cw, C);
         } else if (live) {
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
             xv[d] = ps.x[3 * p + d];
-            if (!kForce) vv[d] = ps.v[3 * p + d];
+            if (kMom) vv[d] = ps.v[3 * p + d];
           }
 #pragma unroll
           for (int k = 0; k < 9; ++k) {
-            PF[k] = contrib[9 * p + k];
-            if (!kForce) C[k] = ps.C[9 * p + k];
+            if (kStress) PF[k] = contrib[9 * p + k];
+            if (kMom) C[k] = ps.C[9 * p + k];
           }
         }
         if (!live) continue;
@@ -444,10 +448,9 @@ __global__ __launch_bounds__(p2g_threads(kTile)) void p2g_kernel(ParticlesView p
           atomicOr(err, kErrArena);
           cell = 255;
         }
-        // affine record: momentum m v + m C (o - l) dx, force contrib' (o - l) dx
+        // affine record: m v + (m C + contrib') (o - l) dx
         float A[9], Bv[3];
         const float mv[3] = {mass * v0, mass * v1, mass * v2};
-        float Af[9], Bf[3];
         if constexpr (kCell) {
           // P2C2G.hpp:86-96, 138-140: C and vol P F^T scaled per column by
           // Dinv, the stress by -dt, summed with m C
@@ -459,29 +462,17 @@ __global__ __launch_bounds__(p2g_threads(kTile)) void p2g_kernel(ParticlesView p
         } else {
 #pragma unroll
           for (int k = 0; k < 9; ++k) {
-            const float cf = PF[k] * bscale;
-            if (kFold) {
-              A[k] = (mass * C[k] + cf) * bdx;
-            } else {
-              A[k] = mass * C[k] * bdx;
-              Af[k] = cf * bdx;
-            }
+            if (kMode == 0) A[k] = (mass * C[k] + PF[k] * bscale) * bdx;
+            if (kMode == 1) A[k] = mass * C[k] * bdx;
+            if (kMode == 2) A[k] = PF[k] * bscale * bdx;
           }
         }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          Bv[d] = mv[d] - (A[d] * l[0] + A[d + 3] * l[1] + A[d + 6] * l[2]);
-          if (!kFold) Bf[d] = -(Af[d] * l[0] + Af[d + 3] * l[1] + Af[d + 6] * l[2]);
-        }
+        for (int d = 0; d < 3; ++d) Bv[d] = mv[d] - (A[d] * l[0] + A[d + 3] * l[1] + A[d + 6] * l[2]);
         rec[0 * kChunk + i] = make_float4(mass, Bv[0], Bv[1], Bv[2]);
         rec[1 * kChunk + i] = make_float4(l[0], l[1], l[2], A[8]);
         rec[2 * kChunk + i] = make_float4(A[0], A[1], A[2], A[3]);
         rec[3 * kChunk + i] = make_float4(A[4], A[5], A[6], A[7]);
-        if (!kFold) {
-          rec[(4 % kQ) * kChunk + i] = make_float4(Bf[0], Bf[1], Bf[2], Af[8]);
-          rec[(5 % kQ) * kChunk + i] = make_float4(Af[0], Af[1], Af[2], Af[3]);
-          rec[(6 % kQ) * kChunk + i] = make_float4(Af[4], Af[5], Af[6], Af[7]);
-        }
         cellof[i] = (uint8_t)cell;
         if (cell <= kMainCells) atomicAdd(&ccount[cell], 1);
         list[i] = (uint16_t)((r0 << 6) | (r1 << 3) | r2);  // base offsets; the list is rebuilt in B
@@ -490,7 +481,7 @@ __global__ __launch_bounds__(p2g_threads(kTile)) void p2g_kernel(ParticlesView p
         // the first chunk's loads are done with the wave's arena as stage:
         // clear it for the bucket's accumulation
         wave_lds_barrier();
-        for (int k = lane; k < kArena6 * kNV; k += kWave) arena[wave][k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int k = lane; k < kArena6; k += kWave) arena[wave][k] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
       __syncthreads();
       // ---- B: counting sort of the chunk by base cell
@@ -539,82 +530,61 @@ __global__ __launch_bounds__(p2g_threads(kTile)) void p2g_kernel(ParticlesView p
         // at the wave index) in registers and adds the node sums to the
         // wave's arena once per cell, instead of 27 LDS read-modify-writes
         // per particle. One pass per x-slab of the stencil keeps the
-        // accumulators at 9 nodes x 4 channels. Part 0 is the mass and
-        // momentum (with the force folded in under kFold); without kFold a
-        // second pass over the same particles sums the force channels into
-        // the node's second float4.
-        auto accumulate = [&](auto part) {
-          constexpr int kPart = decltype(part)::value;
+        // accumulators at 9 nodes x 4 channels.
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            float acc[9][4];
+        for (int a = 0; a < 3; ++a) {
+          float acc[9][4];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.f;
-            for (int32_t r = wave; r < lcount; r += kWv) {
-              const int32_t i = list[lstart + r];
-              const float4 q1 = rec[kChunk + i];
-              // part 0: (m, B) in q0, A0..A3 in q2, A4..A7 in q3, A8 in q1.w;
-              // part 1: (Bf, Af8) in q0, Af0..Af3 in q2, Af4..Af7 in q3
-              const float4 q0 = rec[(kPart == 0 ? 0 : 4 % kQ) * kChunk + i];
-              const float4 q2 = rec[(kPart == 0 ? 2 : 5 % kQ) * kChunk + i];
-              const float4 q3 = rec[(kPart == 0 ? 3 : 6 % kQ) * kChunk + i];
-              const float b0 = kPart == 0 ? q0.y : q0.x, b1 = kPart == 0 ? q0.z : q0.y, b2 = kPart == 0 ? q0.w : q0.z;
-              const float a8 = kPart == 0 ? q1.w : q0.w;
-              float w0[3], w1[3], w2[3];
-              stencil_w<kCell>(q1.x, w0);
-              stencil_w<kCell>(q1.y, w1);
-              stencil_w<kCell>(q1.z, w2);
-              const float fa = (float)a;
+          for (int k = 0; k < 9; ++k) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.f;
+          for (int32_t r = wave; r < lcount; r += kWv) {
+            const int32_t i = list[lstart + r];
+            const float4 q0 = rec[i], q1 = rec[kChunk + i], q2 = rec[2 * kChunk + i], q3 = rec[3 * kChunk + i];
+            float w0[3], w1[3], w2[3];
+            stencil_w<kCell>(q1.x, w0);
+            stencil_w<kCell>(q1.y, w1);
+            stencil_w<kCell>(q1.z, w2);
+            const float fa = (float)a;
 #pragma unroll
-              for (int bb = 0; bb < 3; ++bb) {
-                const float w01 = w0[a] * w1[bb];
-                const float fb = (float)bb;
-                const float s0 = b0 + q2.x * fa + q2.w * fb;
-                const float s1 = b1 + q2.y * fa + q3.x * fb;
-                const float s2 = b2 + q2.z * fa + q3.y * fb;
+            for (int bb = 0; bb < 3; ++bb) {
+              const float w01 = w0[a] * w1[bb];
+              const float fb = (float)bb;
+              const float s0 = q0.y + q2.x * fa + q2.w * fb;
+              const float s1 = q0.z + q2.y * fa + q3.x * fb;
+              const float s2 = q0.w + q2.z * fa + q3.y * fb;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                  const float W = w01 * w2[c];
-                  const float fc = (float)c;
-                  float* e = acc[bb * 3 + c];
-                  if (kPart == 0) e[0] += q0.x * W;
-                  e[1] += W * (s0 + q3.z * fc);
-                  e[2] += W * (s1 + q3.w * fc);
-                  e[3] += W * (s2 + a8 * fc);
-                }
+              for (int c = 0; c < 3; ++c) {
+                const float W = w01 * w2[c];
+                const float fc = (float)c;
+                float* e = acc[bb * 3 + c];
+                if (kMom) e[0] += q0.x * W;
+                e[1] += W * (s0 + q3.z * fc);
+                e[2] += W * (s1 + q3.w * fc);
+                e[3] += W * (s2 + q1.w * fc);
               }
             }
-            if (wave < lcount) {
-#pragma unroll
-              for (int bb = 0; bb < 3; ++bb)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                  const int n = (lr0 + a) * 36 + (lr1 + bb) * 6 + (lr2 + c);
-                  const float* e = acc[bb * 3 + c];
-                  // Lanes of one step touch distinct nodes, but a node a lane
-                  // updates in this step may be one another lane updated in an
-                  // earlier step. The LDS unit serves a wave's accesses in
-                  // issue order; this barrier keeps the compiler from hoisting
-                  // this step's read above earlier steps' writes.
-                  asm volatile("" ::: "memory");
-                  float4 v = ar[n * kNV + kPart];
-                  if (kPart == 0) {
-                    v.x += e[0];
-                    v.y += e[1];
-                    v.z += e[2];
-                    v.w += e[3];
-                  } else {
-                    v.x += e[1];
-                    v.y += e[2];
-                    v.z += e[3];
-                  }
-                  ar[n * kNV + kPart] = v;
-                }
-            }
           }
-        };
-        accumulate(std::integral_constant<int, 0>{});
-        if constexpr (!kFold) accumulate(std::integral_constant<int, 1>{});
+          if (wave < lcount) {
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                const int n = (lr0 + a) * 36 + (lr1 + bb) * 6 + (lr2 + c);
+                const float* e = acc[bb * 3 + c];
+                // Lanes of one step touch distinct nodes, but a node a lane
+                // updates in this step may be one another lane updated in an
+                // earlier step. The LDS unit serves a wave's accesses in issue
+                // order; this barrier keeps the compiler from hoisting this
+                // step's read above earlier steps' writes.
+                asm volatile("" ::: "memory");
+                float4 v = ar[n];
+                v.x += e[0];
+                v.y += e[1];
+                v.z += e[2];
+                v.w += e[3];
+                ar[n] = v;
+              }
+          }
+        }
       }
       // ---- overflow particles: global atomics straight into the grid
 #pragma unroll
@@ -624,23 +594,6 @@ __global__ __launch_bounds__(p2g_threads(kTile)) void p2g_kernel(ParticlesView p
         const int r0 = (keep[k] >> 6) & 7, r1 = (keep[k] >> 3) & 7, r2 = keep[k] & 7;
         const float4 q0 = rec[i], q1 = rec[kChunk + i], q2 = rec[2 * kChunk + i], q3 = rec[3 * kChunk + i];
         const float Am[9] = {q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w, q1.w};
-        float Af[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, Bf[3] = {0.f, 0.f, 0.f};
-        if (!kFold) {
-          const float4 qf0 = rec[(4 % kQ) * kChunk + i], qf1 = rec[(5 % kQ) * kChunk + i],
-                         qf2 = rec[(6 % kQ) * kChunk + i];
-          Bf[0] = qf0.x;
-          Bf[1] = qf0.y;
-          Bf[2] = qf0.z;
-          Af[0] = qf1.x;
-          Af[1] = qf1.y;
-          Af[2] = qf1.z;
-          Af[3] = qf1.w;
-          Af[4] = qf2.x;
-          Af[5] = qf2.y;
-          Af[6] = qf2.z;
-          Af[7] = qf2.w;
-          Af[8] = qf0.w;
-        }
         float w0[3], w1[3], w2[3];
         stencil_w<kCell>(q1.x, w0);
         stencil_w<kCell>(q1.y, w1);
@@ -655,15 +608,11 @@ __global__ __launch_bounds__(p2g_threads(kTile)) void p2g_kernel(ParticlesView p
               if (gb < 0 || gb >= grid_cap) continue;
               float* gp = grid + grid_at(gb, 0, cell);
               const float fa = (float)a, fb = (float)bb, fc = (float)c;
-              unsafeAtomicAdd(gp, q0.x * W);
-              unsafeAtomicAdd(gp + 1 * kBlockCells, W * (q0.y + Am[0] * fa + Am[3] * fb + Am[6] * fc));
-              unsafeAtomicAdd(gp + 2 * kBlockCells, W * (q0.z + Am[1] * fa + Am[4] * fb + Am[7] * fc));
-              unsafeAtomicAdd(gp + 3 * kBlockCells, W * (q0.w + Am[2] * fa + Am[5] * fb + Am[8] * fc));
-              if (!kFold) {
-                unsafeAtomicAdd(gp + 4 * kBlockCells, W * (Bf[0] + Af[0] * fa + Af[3] * fb + Af[6] * fc));
-                unsafeAtomicAdd(gp + 5 * kBlockCells, W * (Bf[1] + Af[1] * fa + Af[4] * fb + Af[7] * fc));
-                unsafeAtomicAdd(gp + 6 * kBlockCells, W * (Bf[2] + Af[2] * fa + Af[5] * fb + Af[8] * fc));
-              }
+              float* gv = kMom ? gp + kBlockCells : gp + fch * kBlockCells;  // momentum or force channels
+              if (kMom) unsafeAtomicAdd(gp, q0.x * W);
+              unsafeAtomicAdd(gv, W * (q0.y + Am[0] * fa + Am[3] * fb + Am[6] * fc));
+              unsafeAtomicAdd(gv + kBlockCells, W * (q0.z + Am[1] * fa + Am[4] * fb + Am[7] * fc));
+              unsafeAtomicAdd(gv + 2 * kBlockCells, W * (q0.w + Am[2] * fa + Am[5] * fb + Am[8] * fc));
             }
       }
       if (last && threadIdx.x == 0) bucket_meta((int32_t)min(nxt, 0x7fffffffu), nb, sv, t, ps.n, bmeta, nscenes);
@@ -678,45 +627,29 @@ __global__ __launch_bounds__(p2g_threads(kTile)) void p2g_kernel(ParticlesView p
     }
     // ---- flush: sum the wave arenas, one global atomic per touched node/channel
     for (int n = threadIdx.x; n < kArena6; n += kT) {
-      float4 s = arena[0][n * kNV];
+      float4 s = arena[0][n];
 #pragma unroll
       for (int w = 1; w < kWv; ++w) {
-        const float4 u = arena[w][n * kNV];
+        const float4 u = arena[w][n];
         s.x += u.x;
         s.y += u.y;
         s.z += u.z;
         s.w += u.w;
       }
-      float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (!kFold) {
-        f = arena[0][n * kNV + 1];
-#pragma unroll
-        for (int w = 1; w < kWv; ++w) {
-          const float4 u = arena[w][n * kNV + 1];
-          f.x += u.x;
-          f.y += u.y;
-          f.z += u.z;
-        }
-      }
       // every node a particle touched, massless particles included (zpc adds
       // all 27 nodes, P2G.hpp:114-123); untouched nodes hold exact zeros
-      const bool touched = s.x != 0.f || s.y != 0.f || s.z != 0.f || s.w != 0.f ||
-                           (!kFold && (f.x != 0.f || f.y != 0.f || f.z != 0.f));
+      const bool touched = s.x != 0.f || s.y != 0.f || s.z != 0.f || s.w != 0.f;
       if (touched) {
         int nbi, cell;
         node_home(n / 36 + 1, (n / 6) % 6 + 1, n % 6 + 1, nbi, cell);
         const int32_t gb = nbr[nbi];
         if (gb >= 0 && gb < grid_cap) {
           float* gp = grid + grid_at(gb, 0, cell);
-          unsafeAtomicAdd(gp, s.x);
-          unsafeAtomicAdd(gp + 1 * kBlockCells, s.y);
-          unsafeAtomicAdd(gp + 2 * kBlockCells, s.z);
-          unsafeAtomicAdd(gp + 3 * kBlockCells, s.w);
-          if (!kFold) {
-            unsafeAtomicAdd(gp + 4 * kBlockCells, f.x);
-            unsafeAtomicAdd(gp + 5 * kBlockCells, f.y);
-            unsafeAtomicAdd(gp + 6 * kBlockCells, f.z);
-          }
+          float* gv = kMom ? gp + kBlockCells : gp + fch * kBlockCells;  // momentum or force channels
+          if (kMom) unsafeAtomicAdd(gp, s.x);
+          unsafeAtomicAdd(gv, s.y);
+          unsafeAtomicAdd(gv + kBlockCells, s.z);
+          unsafeAtomicAdd(gv + 2 * kBlockCells, s.w);
         }
       }
     }
@@ -1463,21 +1396,22 @@ hipError_t check_launch_limits(int device) {
   if (e != hipSuccess) return e;
   const void* kernels[] = {
       reinterpret_cast<const void*>(&stress_kernel<0>), reinterpret_cast<const void*>(&stress_kernel<1>),
-      reinterpret_cast<const void*>(&stress_kernel<2>), reinterpret_cast<const void*>(&p2g_kernel<true, false>),
-      reinterpret_cast<const void*>(&p2g_kernel<false, false>), reinterpret_cast<const void*>(&p2g_kernel<true, true>),
-      reinterpret_cast<const void*>(&p2g_kernel<false, true>),
+      reinterpret_cast<const void*>(&stress_kernel<2>), reinterpret_cast<const void*>(&p2g_kernel<0, false>),
+      reinterpret_cast<const void*>(&p2g_kernel<1, false>), reinterpret_cast<const void*>(&p2g_kernel<0, true>),
+      reinterpret_cast<const void*>(&p2g_kernel<1, true>),
       reinterpret_cast<const void*>(&g2p_kernel<false, -1>), reinterpret_cast<const void*>(&g2p_kernel<true, -1>),
       reinterpret_cast<const void*>(&g2p_kernel<true, 0>), reinterpret_cast<const void*>(&g2p_kernel<true, 1>),
       reinterpret_cast<const void*>(&g2p_kernel<true, -1, true>),
       reinterpret_cast<const void*>(&g2p_kernel<false, -1, true>),
-      reinterpret_cast<const void*>(&p2g_kernel<true, false, true>),
+      reinterpret_cast<const void*>(&p2g_kernel<0, false, true>),
       reinterpret_cast<const void*>(&g2p_kernel<false, -1, false, true>),
-      reinterpret_cast<const void*>(&p2g_kernel<true, false, false, 1>),
-      reinterpret_cast<const void*>(&p2g_kernel<true, true, false, 1>),
+      reinterpret_cast<const void*>(&p2g_kernel<0, false, false, 1>),
+      reinterpret_cast<const void*>(&p2g_kernel<0, true, false, 1>),
       reinterpret_cast<const void*>(&g2p_kernel<true, -1, false, false, 256>),
       reinterpret_cast<const void*>(&g2p_kernel<true, -1, true, false, 256>),
       reinterpret_cast<const void*>(&g2p2g_gather_kernel),
-      reinterpret_cast<const void*>(&p2g_kernel<true, false, false, 0, true>)};
+      reinterpret_cast<const void*>(&p2g_kernel<2, false>),
+      reinterpret_cast<const void*>(&p2g_kernel<2, true>)};
   for (int i = 0; i < (int)(sizeof(kernels) / sizeof(kernels[0])); ++i) {
     hipFuncAttributes attr;
     e = hipFuncGetAttributes(&attr, kernels[i]);
@@ -1525,23 +1459,31 @@ hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortVi
   const SceneSet sset = multi ? *ss : SceneSet{};
   const bool tile1 = ws.p2g_tile == 1;  // the folded launches take the tuned tile
   if (fold && multi && tile1)
-    p2g_kernel<true, true, false, 1><<<g, p2g_threads(1), 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt,
-                                                                  dx, ws.err, q, sset);
+    p2g_kernel<0, true, false, 1><<<g, p2g_threads(1), 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx,
+                                                               ws.err, q, sset, 1);
   else if (fold && multi)
-    p2g_kernel<true, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err,
-                                                     q, sset);
+    p2g_kernel<0, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q,
+                                                  sset, 1);
   else if (fold && tile1)
-    p2g_kernel<true, false, false, 1><<<g, p2g_threads(1), 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt,
-                                                                   dx, ws.err, q, sset);
+    p2g_kernel<0, false, false, 1><<<g, p2g_threads(1), 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt,
+                                                                dx, ws.err, q, sset, 1);
   else if (fold)
-    p2g_kernel<true, false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err,
-                                                      q, sset);
-  else if (multi)
-    p2g_kernel<false, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err,
-                                                      q, sset);
-  else
-    p2g_kernel<false, false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx,
-                                                       ws.err, q, sset);
+    p2g_kernel<0, false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q,
+                                                   sset, 1);
+  else if (multi) {
+    // the force kept in channels 4..6: momentum, then the force
+    p2g_kernel<1, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q,
+                                                  sset, 1);
+    reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
+    p2g_kernel<2, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q,
+                                                  sset, 4);
+  } else {
+    p2g_kernel<1, false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q,
+                                                   sset, 1);
+    reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
+    p2g_kernel<2, false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err, q,
+                                                   sset, 4);
+  }
   return hipGetLastError();
 }
 
@@ -1549,8 +1491,8 @@ hipError_t p2c2g_scatter(const ParticlesView& ps, const TableView& t, const Sort
                          int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s) {
   const int g = ws.grid_p2g > 0 ? ws.grid_p2g : persistent_groups();
   reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
-  p2g_kernel<true, false, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx,
-                                                          ws.err, ws.counters + kCtrQueue, SceneSet{});
+  p2g_kernel<0, false, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx,
+                                                       ws.err, ws.counters + kCtrQueue, SceneSet{}, 1);
   return hipGetLastError();
 }
 
@@ -1558,9 +1500,8 @@ hipError_t p2g_force_scatter(const ParticlesView& ps, const TableView& t, const 
                              int32_t grid_cap, float dt, float dx, Workspace& ws, hipStream_t s) {
   const int g = ws.grid_p2g > 0 ? ws.grid_p2g : persistent_groups();
   reset_queue_kernel<<<1, 1, 0, s>>>(ws.counters);
-  p2g_kernel<true, false, false, 0, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap,
-                                                                    dt, dx, ws.err, ws.counters + kCtrQueue,
-                                                                    SceneSet{});
+  p2g_kernel<2, false><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx, ws.err,
+                                                 ws.counters + kCtrQueue, SceneSet{}, 1);
   return hipGetLastError();
 }
 
