@@ -402,10 +402,14 @@ __global__ __launch_bounds__(p2g_threads(kTile)) void p2g_kernel(ParticlesView p
         if (contig) {
           RowLoad<3> lx, lv;
           RowLoad<9> lp, lc;
+          // the wave issues its row loads ahead of other waves' scatter
+          // arithmetic (0.763 -> 0.757 ms at 16M particles)
+          __builtin_amdgcn_s_setprio(2);
           lx.issue(ps.x, p0, cw);
           if (kMom) lv.issue(ps.v, p0, cw);
           if (kStress) lp.issue(contrib, p0, cw);
           if (kMom) lc.issue(ps.C, p0, cw);
+          __builtin_amdgcn_s_setprio(0);
           lx.finish(stage, cw, xv);
           if (kMom) lv.finish(stage, cw, vv);
           if (kStress) lp.finish(stage, cw, PF);
