@@ -657,10 +657,26 @@ def transfers_leg(torch, rocm, sc, reps: int = 5):
     out = {"workload": f"{sc.n} particles of the bench scene after two substeps, {sim.nblocks()} blocks",
            "p2c2g_ms": ms_p2c2g, "p2g_ms": ms_p2g, "g2c2p_ms": ms_g2c2p, "g2p_ms": ms_g2p,
            "p2c2g_particles_per_s": sc.n / (ms_p2c2g * 1e-3), "g2c2p_particles_per_s": sc.n / (ms_g2c2p * 1e-3)}
-    print(f"# transfers: p2c2g {ms_p2c2g:.3f} ms (p2g {ms_p2g:.3f}), g2c2p {ms_g2c2p:.3f} ms (g2p {ms_g2p:.3f})",
-          file=sys.stderr, flush=True)
     del sim
     torch.cuda.empty_cache()
+    # whole sorted substeps (hipGraph replays) with each transfer scheme, from
+    # the column at rest (its C = 0 is B = 0 for the cell scheme)
+    for name, scheme in (("apic", rocm.TRANSFER_APIC), ("cell", rocm.TRANSFER_CELL)):
+        sim = rocm.Simulator(sc, reorder=True)
+        sim.pol.set_transfer_scheme(scheme)
+        sim.substeps(8, autotune=True)
+        sim.substeps(4, graph=True)
+        sim.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.substeps(20, graph=True)
+        torch.cuda.synchronize()
+        out[f"substep_{name}_ms"] = (time.perf_counter() - t0) * 1e3 / 20
+        del sim
+        torch.cuda.empty_cache()
+    print(f"# transfers: p2c2g {ms_p2c2g:.3f} ms (p2g {ms_p2g:.3f}), g2c2p {ms_g2c2p:.3f} ms (g2p {ms_g2p:.3f}); "
+          f"sorted substep with the cell transfers {out['substep_cell_ms']:.3f} ms "
+          f"(APIC {out['substep_apic_ms']:.3f})", file=sys.stderr, flush=True)
     return out
 
 

@@ -342,6 +342,18 @@ int32_t zsmpm_policy_launch_widths__rocm(zsmpm_policy pol, int32_t* p2g_groups, 
  * would (0 restores the default; at most 1 << 20), e.g. to replay a tuned
  * configuration in another process. No counterpart in the reference. */
 int32_t zsmpm_policy_set_launch_widths__rocm(zsmpm_policy pol, int32_t p2g_groups, int32_t g2p_groups);
+/* The transfers of every later substep on this policy (zsmpm_substep__rocm,
+ * zsmpm_substep_sorted__rocm, zsmpm_substeps_sorted__rocm, graphs included):
+ * ZSMPM_TRANSFER_APIC (P2GTransfer / G2PTransfer, the default) or
+ * ZSMPM_TRANSFER_CELL (P2C2GTransfer / G2C2PTransfer as zsmpm_p2c2g__rocm and
+ * zsmpm_g2c2p__rocm compute them: the stored C is G2C2P's B). The cell
+ * scheme runs one launch tile, and is not available for batches or with
+ * ZSMPM_SUBSTEPS_FUSE_STRESS (ZSMPM_UNSUPPORTED_OPERATION). No counterpart in
+ * zpc, whose driver calls the transfer functors it was built with. */
+#define ZSMPM_TRANSFER_APIC 0
+#define ZSMPM_TRANSFER_CELL 1
+int32_t zsmpm_policy_transfer_scheme__rocm(zsmpm_policy pol, int32_t scheme);
+
 /* The launch tiles of the sorted substep's P2G and G2P (no counterpart in zpc,
  * whose launches have one shape): P2G's chunk (576 particles over four waves,
  * three workgroups per CU, or 256 over two waves, six per CU) and G2P's

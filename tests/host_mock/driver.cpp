@@ -68,6 +68,11 @@ int main(int argc, char** argv) {
   expect("substeps with set tiles", zsmpm_substeps_sorted__rocm(pol, &s0.model, &s0.a, &s0.b, &s0.t, &s0.bk, &s0.g, 1e-4f, g3, s0.cols, 6, 4, ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
   expect("odd chunk rejected", zsmpm_policy_set_launch_tiles__rocm(pol, 300, 128), ZSMPM_INVALID_ARGUMENT);
   expect("odd threads rejected", zsmpm_policy_set_launch_tiles__rocm(pol, 576, 64), ZSMPM_INVALID_ARGUMENT);
+  expect("bad transfer scheme", zsmpm_policy_transfer_scheme__rocm(pol, 2), ZSMPM_INVALID_ARGUMENT);
+  expect("cell transfer", zsmpm_policy_transfer_scheme__rocm(pol, ZSMPM_TRANSFER_CELL), 0);
+  expect("cell substeps, graph", zsmpm_substeps_sorted__rocm(pol, &s0.model, &s0.a, &s0.b, &s0.t, &s0.bk, &s0.g, 1e-4f, g3, s0.cols, 6, 3, ZSMPM_SUBSTEPS_GRAPH, &inb), 0);
+  expect("cell + fuse unsupported", zsmpm_substeps_sorted__rocm(pol, &s0.model, &s0.a, &s0.b, &s0.t, &s0.bk, &s0.g, 1e-4f, g3, s0.cols, 6, 2, ZSMPM_SUBSTEPS_FUSE_STRESS, &inb), ZSMPM_UNSUPPORTED_OPERATION);
+  expect("apic transfer", zsmpm_policy_transfer_scheme__rocm(pol, ZSMPM_TRANSFER_APIC), 0);
   expect("default tiles", zsmpm_policy_set_launch_tiles__rocm(pol, 0, 0) == 0 && zsmpm_policy_launch_tiles__rocm(pol, &tc, &tt) == 0 && tc == 576 && tt == 128, 1);
   expect("set widths", zsmpm_policy_set_launch_widths__rocm(pol, 300, 5000), 0);
   expect("widths read back", zsmpm_policy_launch_widths__rocm(pol, &wp, &wg) == 0 && wp == 300 && wg == 5000, 1);
@@ -104,6 +109,9 @@ int main(int argc, char** argv) {
   }
   zsmpm_buckets bbk = mk<zsmpm_buckets>(); bbk.capacity_blocks = big.bk.capacity_blocks;
   bbk.counts = big.bk.counts; bbk.offsets = big.bk.offsets; bbk.indices = (int32_t*)calloc(ntot, 4);
+  expect("cell transfer for a batch", zsmpm_policy_transfer_scheme__rocm(pol, ZSMPM_TRANSFER_CELL), 0);
+  expect("batched cell unsupported", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 1, 0, &inb), ZSMPM_UNSUPPORTED_OPERATION);
+  expect("apic transfer back", zsmpm_policy_transfer_scheme__rocm(pol, ZSMPM_TRANSFER_APIC), 0);
   expect("batched eager", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 3, 0, &inb), 0);
   expect("batched odd ends in b", inb, 1);
   expect("batched graph + autotune", zsmpm_substeps_batched__rocm(pol, d.data(), ns, &ba[0], &ba[1], &big.t, &bbk, &big.g, 12, ZSMPM_SUBSTEPS_GRAPH | ZSMPM_SUBSTEPS_AUTOTUNE, &inb), 0);

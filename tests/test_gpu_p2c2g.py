@@ -171,3 +171,42 @@ def test_affine_field_round_trip_half_million():
     ex = np.abs(got["x"].reshape(-1, 3) - (x + vw * sc.dt)).max()
     print(f"affine: v {ev:.2e} B {eb:.2e} F {ef:.2e} x {ex:.2e}", flush=True)
     assert ev < 1e-5 and eb < 1e-4 and ef < 1e-6 and ex < 1e-6
+
+
+@pytest.mark.parametrize("kind", ["cube", "sand"])
+def test_cell_transfer_sorted_substeps(kind):
+    """zsmpm_policy_transfer_scheme__rocm(ZSMPM_TRANSFER_CELL): sorted substeps
+    (a captured pair and an odd one) with P2C2G / G2C2P against the oracle's
+    composition per substep -- activation, bucket sort, P2C2G, grid update,
+    G2C2P, the reorder by the substep's sort -- compared by particle id."""
+    from zpc_amd import rocm
+
+    sc = _scene(kind)
+    # the cell scheme stores B = C D (D ~ dx^2 / 4 per axis), not C: the
+    # perturbed velocity gradients as B, so the state stays physical over
+    # several substeps
+    sc.C = (sc.C.astype(np.float64) * (sc.dx * sc.dx / 4.0)).astype(np.float32)
+    sim = rocm.Simulator(sc, reorder=True)
+    sim.pol.set_transfer_scheme(rocm.TRANSFER_CELL)
+    sim.substeps(3, graph=True)
+    sim.sync()
+    got = sim.parts.host_by_id()
+    prm = O.make_params(sc)
+    ps = _parts(sc)
+    for _ in range(3):
+        ctx = O.Context()
+        ctx.activate(sc.table_size, ps.x, sc.dx)
+        ctx.enlarge()
+        ctx.bucket_sort(ps.x, sc.dx)
+        _, _, order = ctx.sort()
+        ctx.p2c2g(prm, ps)
+        ctx.grid_update(prm)
+        ctx.g2c2p(prm, ps)
+        ps.reorder(order)
+    for name in ("x", "v", "F"):
+        e = _rel(got[name], ps.by_id(name))
+        print(f"{kind} cell substeps {name}: {e:.2e}", flush=True)
+        assert e < 1e-4, (name, e)
+    with pytest.raises(rocm.ZsmpmError):
+        sim.pol.set_transfer_scheme(5)
+

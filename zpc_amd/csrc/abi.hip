@@ -56,6 +56,7 @@ struct zsmpm_policy_s {
   // P2G / G2P workgroup counts picked by ZSMPM_SUBSTEPS_AUTOTUNE (0: default)
   int32_t tuned_p2g = 0, tuned_g2p = 0;
   int32_t tuned_p2g_tile = 0, tuned_g2p_threads = 0;  // launch tiles (0: defaults)
+  int32_t transfer = 0;  // the substep drivers' transfers: ZSMPM_TRANSFER_APIC or ZSMPM_TRANSFER_CELL
   Workspace ws{};
   std::vector<void*> allocs;
   int64_t reserved_n = 0;
@@ -527,6 +528,8 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
   } else if (pol->closure_on || (chain && (chain->cached_stress || chain->compute_next))) {
     return ZSMPM_UNSUPPORTED_OPERATION;
   }
+  const bool cell = pol->transfer == ZSMPM_TRANSFER_CELL;
+  if (cell && (multi || (chain && chain->compute_next))) return ZSMPM_UNSUPPORTED_OPERATION;
   const float no_extf[3] = {0.f, 0.f, 0.f};
   const float* g3 = multi ? no_extf : extf;
   DeviceGuard g(pol->devid);
@@ -543,6 +546,13 @@ int32_t run_substep(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_p
   ws.grid_g2p = pol->tuned_g2p;
   ws.p2g_tile = pol->tuned_p2g_tile;
   ws.g2p_threads = pol->tuned_g2p_threads;
+  // the transfer scheme holds for this substep's launches only (the
+  // stage-wise entry points keep their own transfers)
+  struct CellScope {
+    Workspace& w;
+    CellScope(Workspace& w_, bool on) : w(w_) { w.cell = on; }
+    ~CellScope() { w.cell = false; }
+  } cell_scope(ws, cell);
   const int32_t lo[3] = {0, 0, 0}, hi[3] = {2, 2, 2};
   // The stress pass reads F and logJp and writes ws.contrib; the sparsity
   // stages read x only. It runs on the side stream beside them and P2G waits
@@ -650,6 +660,7 @@ int32_t capture_pair(zsmpm_policy_s* pol, const zsmpm_model* model, const zsmpm_
   key_put(key, &pol->tuned_g2p);
   key_put(key, &pol->tuned_p2g_tile);
   key_put(key, &pol->tuned_g2p_threads);
+  key_put(key, &pol->transfer);
   key_put(key, &pol->gcols);
   if (pol->gexec && key == pol->gkey) return ZSMPM_OK;
   destroy_graph(pol);
@@ -1805,6 +1816,12 @@ int32_t zsmpm_policy_set_launch_widths__rocm(zsmpm_policy pol, int32_t p2g_group
     return ZSMPM_INVALID_ARGUMENT;
   pol->tuned_p2g = p2g_groups;
   pol->tuned_g2p = g2p_groups;
+  return ZSMPM_OK;
+}
+
+int32_t zsmpm_policy_transfer_scheme__rocm(zsmpm_policy pol, int32_t scheme) {
+  if (!pol || (scheme != ZSMPM_TRANSFER_APIC && scheme != ZSMPM_TRANSFER_CELL)) return ZSMPM_INVALID_ARGUMENT;
+  pol->transfer = scheme;
   return ZSMPM_OK;
 }
 

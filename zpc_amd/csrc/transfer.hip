@@ -1435,7 +1435,8 @@ hipError_t check_launch_limits(int device) {
       reinterpret_cast<const void*>(&g2p_kernel<true, -1, true, false, 256>),
       reinterpret_cast<const void*>(&g2p2g_gather_kernel),
       reinterpret_cast<const void*>(&p2g_kernel<2, false>),
-      reinterpret_cast<const void*>(&p2g_kernel<2, true>)};
+      reinterpret_cast<const void*>(&p2g_kernel<2, true>),
+      reinterpret_cast<const void*>(&g2p_kernel<true, -1, false, true>)};
   for (int i = 0; i < (int)(sizeof(kernels) / sizeof(kernels[0])); ++i) {
     hipFuncAttributes attr;
     e = hipFuncGetAttributes(&attr, kernels[i]);
@@ -1482,7 +1483,10 @@ hipError_t p2g_scatter(const ParticlesView& ps, const TableView& t, const SortVi
   const bool multi = ss && ss->n > 0;
   const SceneSet sset = multi ? *ss : SceneSet{};
   const bool tile1 = ws.p2g_tile == 1;  // the folded launches take the tuned tile
-  if (fold && multi && tile1)
+  if (fold && ws.cell && !multi)  // the substep's P2C2G (one tile)
+    p2g_kernel<0, false, true><<<g, kP2GThreads, 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx,
+                                                         ws.err, q, sset, 1);
+  else if (fold && multi && tile1)
     p2g_kernel<0, true, false, 1><<<g, p2g_threads(1), 0, s>>>(ps, ws.contrib, t, sv, ws.nbr, grid, grid_cap, dt, dx,
                                                                ws.err, q, sset, 1);
   else if (fold && multi)
@@ -1573,7 +1577,15 @@ hipError_t g2p(const ParticlesView& ps, const ParticlesView* out, const TableVie
   NextStress ns{};
   if (next) ns = *next;
   const SceneSet sset = ss ? *ss : SceneSet{};
-  if (next && next->contrib) {
+  if (ws.cell) {  // the substep's G2C2P (no batch, no fused stress; one tile)
+    if ((next && next->contrib) || sset.n > 0) return hipErrorInvalidValue;
+    if (out)
+      g2p_kernel<true, -1, false, true><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx,
+                                                                  ns, ws.err, ws.counters, sset);
+    else
+      g2p_kernel<false, -1, false, true><<<g, kG2PThreads, 0, s>>>(ps, ps, t, sv, ws.nbr, grid, grid_cap, dt, dx,
+                                                                   ns, ws.err, ws.counters, sset);
+  } else if (next && next->contrib) {
     if (!out || sset.n > 0) return hipErrorInvalidValue;
     if (ns.m.kind == 0)
       g2p_kernel<true, 0><<<g, kG2PThreads, 0, s>>>(ps, *out, t, sv, ws.nbr, grid, grid_cap, dt, dx, ns, ws.err,

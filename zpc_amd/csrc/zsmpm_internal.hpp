@@ -181,6 +181,7 @@ struct Workspace {
   int32_t grid_g2p = 0;
   int32_t p2g_tile = 0;     // folded P2G launch tile: 0 four waves / 576-particle chunks, 1 two waves / 256
   int32_t g2p_threads = 0;  // sorted G2P threads per bucket: 256, else 128
+  bool cell = false;        // the substep's transfers are P2C2G / G2C2P (set by the substep driver only)
   float* lj_next[2] = {nullptr, nullptr};  // [n_cap] each: logJp caches of the fused multi-substep driver
   uint32_t* bitmap = nullptr;    // [bm_words] one bit per insertion order: the orders of new keys
   uint32_t* bm_prefix = nullptr; // [bm_words] exclusive prefix of the bit counts

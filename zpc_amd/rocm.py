@@ -130,6 +130,7 @@ class SparseGridDesc(C.Structure):
                 ("background", C.c_float)]
 
 
+TRANSFER_APIC, TRANSFER_CELL = 0, 1
 BHT_SEED = 2
 BHT_SERIAL = 1
 
@@ -198,6 +199,7 @@ def lib():
         "zsmpm_policy_closure__rocm": [vp, P(ClosureDesc)],
         "zsmpm_policy_launch_widths__rocm": [vp, P(C.c_int32), P(C.c_int32)],
         "zsmpm_policy_launch_tiles__rocm": [vp, P(C.c_int32), P(C.c_int32)],
+        "zsmpm_policy_transfer_scheme__rocm": [vp, i32],
         "zsmpm_policy_set_launch_tiles__rocm": [vp, i32, i32],
         "zsmpm_policy_set_launch_widths__rocm": [vp, C.c_int32, C.c_int32],
         "zsmpm_policy_colliders__rocm": [vp, P(ColliderDesc), i32],
@@ -331,6 +333,10 @@ class Policy:
     def set_launch_widths(self, p2g: int, g2p: int):
         """Fixes the P2G and G2P workgroup counts of later launches (0: default)."""
         check(self.L.zsmpm_policy_set_launch_widths__rocm(self.h, int(p2g), int(g2p)), "set_launch_widths")
+
+    def set_transfer_scheme(self, scheme: int):
+        """Transfers of later substeps: TRANSFER_APIC (P2G / G2P) or TRANSFER_CELL (P2C2G / G2C2P)."""
+        check(self.L.zsmpm_policy_transfer_scheme__rocm(self.h, int(scheme)), "transfer_scheme")
 
     def launch_tiles(self):
         """(P2G chunk, G2P threads per bucket) of the sorted substep's launches."""
