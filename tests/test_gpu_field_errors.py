@@ -16,8 +16,8 @@ floor any float32 implementation of the same algorithm reaches. Reported:
   channels (stage-level calls, rhs kept unfolded).
 Bounds: 1e-5 after one substep and for the grid, 1e-3 after 100 substeps, or
 (where float32 summation order decides) no worse than twice the float32
-oracle's own error. The table is printed and, when gpurun_out/ exists, written
-to gpurun_out/field_errors.md.
+oracle's own error. The table is printed and, when the environment names a
+report folder in ZSMPM_REPORT_DIR, written there as field_errors.md.
 """
 import os
 
@@ -28,8 +28,6 @@ from oracle import oracle as O
 from zpc_amd import scenes as S
 
 pytestmark = pytest.mark.gpu
-
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _err(a, ref, floor=0.0):
@@ -120,7 +118,7 @@ def test_zz_write_report():
     lines += [f"| {c} | {f} | {sc:.2e} | {e:.2e} | {r:.2e} |" for c, f, sc, e, r in ROWS]
     text = "\n".join(lines) + "\n"
     print(text, flush=True)
-    out = os.path.join(ROOT, "gpurun_out")
-    if os.path.isdir(out):
+    out = os.environ.get("ZSMPM_REPORT_DIR", "")
+    if out and os.path.isdir(out):
         with open(os.path.join(out, "field_errors.md"), "w") as fh:
             fh.write(text)
