@@ -272,7 +272,9 @@ int32_t zsmpm_g2c2p__rocm(zsmpm_policy pol, const zsmpm_particles* particles, co
                           const zsmpm_buckets* buckets, const zsmpm_grid* grid, float dt);
 /* One explicit substep in the reconstructed driver's stage order: table clear,
  * activation (blockLen 4, offset -2, displacement 0.5), enlargement {0,0,0}..{2,2,2},
- * bucket sort, grid clear, P2G, rhs fold + grid velocity + colliders, G2P.
+ * bucket sort, grid clear, P2G, rhs fold + grid velocity + colliders, G2P,
+ * on the structures MPMSimulator's builder sets up (grid channels m, v, rhs;
+ * the HashTable partition; maxVel; simulation/mpm/Simulator.cpp:44-130).
  * table, buckets and grid hold that substep's structures on return. */
 int32_t zsmpm_substep__rocm(zsmpm_policy pol, const zsmpm_model* model, const zsmpm_particles* particles,
                             const zsmpm_table* table, const zsmpm_buckets* buckets, const zsmpm_grid* grid,
