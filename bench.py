@@ -147,10 +147,28 @@ def x_by_id(torch, sim):
     return out
 
 
-def run_path(torch, rocm, sc, args, multi, graph, closure=None):
+def positions_by_id(torch, rocm, sc, args, closure, substeps, widths, tiles):
+    """Positions after `substeps` substeps of a second, untimed simulator that
+    carries particle ids (the timed one moves zpc's attributes only: m, x, v,
+    C, F, logJp), at the timed run's launch configuration; ids change nothing
+    else, so this is the timed run's trajectory, found again by particle."""
+    sim = rocm.Simulator(sc, reorder=not args.in_place, ids=True)
+    if closure is not None:
+        sim.pol.set_closure(closure[0], tol=closure[1], always_network=closure[2])
+    sim.pol.set_launch_widths(*widths)
+    sim.pol.set_launch_tiles(*tiles)
+    sim.substeps(substeps, graph=not args.in_place)
+    sim.sync()
+    x = x_by_id(torch, sim)
+    del sim
+    return x
+
+
+def run_path(torch, rocm, sc, args, multi, graph, closure=None, want_x=True):
     """Warm-up, a profiled eager pass and (graph) the timed replays of one
-    simulator; closure: (params or None) selects the learned closure."""
-    sim = rocm.Simulator(sc, reorder=not args.in_place)
+    simulator; closure: (params or None) selects the learned closure; want_x:
+    also the positions by particle id (for the comparisons between legs)."""
+    sim = rocm.Simulator(sc, reorder=not args.in_place, ids=False)
     if closure is not None:
         sim.pol.set_closure(closure[0], tol=closure[1], always_network=closure[2])
     tune = 0
@@ -206,12 +224,13 @@ def run_path(torch, rocm, sc, args, multi, graph, closure=None):
     b1 = sim.nblocks()
     substeps_run = tune + args.warmup + args.steps + (2 + args.steps if graph else 0)
     ctr = sim.pol.counters()
+    widths, tiles = sim.pol.launch_widths(), sim.pol.launch_tiles()
     out = {"stages": stages, "blocks": 0.5 * (b0 + b1), "ms": wall * 1e3 / args.steps, "eager_ms": eager_ms,
-           "launch_widths": sim.pol.launch_widths(),
-           "launch_tiles": sim.pol.launch_tiles(),
-           "fallbacks": int(ctr[4]) - fb0, "net_waves": int(ctr[9]) - nw0, "substeps": substeps_run,
-           "x": x_by_id(torch, sim)}
+           "launch_widths": widths, "launch_tiles": tiles,
+           "fallbacks": int(ctr[4]) - fb0, "net_waves": int(ctr[9]) - nw0, "substeps": substeps_run}
     del sim
+    if want_x:
+        out["x"] = positions_by_id(torch, rocm, sc, args, closure, substeps_run, widths, tiles)
     return out
 
 
@@ -753,13 +772,13 @@ def main() -> int:
     print(f"# device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     multi = not (args.in_place or args.single)
     graph = multi and not args.eager
-    plain = run_path(torch, rocm, sc, args, multi, graph)
+    plain = run_path(torch, rocm, sc, args, multi, graph, want_x=not args.no_surrogate)
     stages, blocks, ms_per_step, eager_ms = plain["stages"], plain["blocks"], plain["ms"], plain["eager_ms"]
     value = sc.n / (ms_per_step * 1e-3)
     surrogate = None
     if not args.no_surrogate and multi:
         surrogate = surrogate_leg(torch, rocm, sc, args, graph, plain)
-    x_plain = plain.pop("x")
+    x_plain = plain.pop("x", None)
     batched = None
     if not args.no_batch:
         del x_plain
@@ -825,6 +844,7 @@ def main() -> int:
             "path": "plain",
             "particle_order": "original (in place)" if args.in_place else
                               "bucket order, reordered inside G2P every substep",
+            "particle_attributes": "m, x, v, C, F, logJp (zpc's; no particle id)",
             "p2g_g2p_workgroups": list(plain["launch_widths"]),
             "p2g_chunk_g2p_threads": list(plain["launch_tiles"]),
             "driver": ("zsmpm_substeps_sorted__rocm with ZSMPM_SUBSTEPS_GRAPH (a captured two-substep "

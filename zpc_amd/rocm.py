@@ -701,16 +701,18 @@ class Simulator:
     stays resident."""
 
     def __init__(self, sc: S.Scene, device: int = 0, grid_capacity: Optional[int] = None,
-                 svd_fp64_gap: float = 0.0, reorder: bool = False):
+                 svd_fp64_gap: float = 0.0, reorder: bool = False, ids: Optional[bool] = None):
         self.sc = sc
         self.pol = Policy(device)
         T = int(sc.table_size)
         self.table = DeviceTable(T)
         cap = int(grid_capacity) if grid_capacity else max(T // 16, 64)
         self.grid = DeviceGrid(sc.dx, cap)
-        # reorder: particles are kept in bucket order (ping-pong buffers, ids carried)
+        # reorder: particles are kept in bucket order (ping-pong buffers); ids
+        # (by default with the reorder) are carried with them, so that a
+        # particle can be found again
         self.reorder = bool(reorder)
-        self.parts = DeviceParticles.from_scene(sc, ids=self.reorder)
+        self.parts = DeviceParticles.from_scene(sc, ids=self.reorder if ids is None else bool(ids))
         self.spare = self.parts.empty_like() if self.reorder else None
         self.buckets = DeviceBuckets(cap, sc.n)
         self.model = model_desc(sc.material, svd_fp64_gap)
