@@ -90,6 +90,31 @@ def test_activation_enlarge_exact(scene, request):
     _assert_table_equal(sim.table.host(), ctx.table())
 
 
+def test_enlarge_27_offsets_exact():
+    """An enlargement by all 27 neighbour offsets (lo -1, hi 2) with more active
+    blocks than 8 T / 27 (343 blocks in a table of 1,024 slots), so that the
+    candidates' insertion orders reach past 8 T while the table still holds
+    the result (729 blocks)."""
+    sc = S.perturb(S.cube_scene(cells=24), seed=11)
+    sc.table_size = 1024
+    rocm = _rocm()
+    sim = rocm.Simulator(sc)
+    ctx = O.Context()
+    sim.clear_table()
+    sim.activate()
+    sim.sync()
+    ctx.activate(sc.table_size, sc.x, sc.dx)
+    n0 = ctx.sizes()["cnt"]
+    print(f"27-offset enlargement: {n0} active blocks, T = {sc.table_size}", flush=True)
+    assert n0 * 27 > 8 * sc.table_size
+    _assert_table_equal(sim.table.host(), ctx.table())
+    sim.enlarge(lo=(-1, -1, -1), hi=(2, 2, 2))
+    sim.sync()
+    ctx.enlarge((-1, -1, -1), (2, 2, 2))
+    assert ctx.sizes()["cnt"] == 729
+    _assert_table_equal(sim.table.host(), ctx.table())
+
+
 def test_activation_shuffled_order_exact(cube):
     """First-occurrence numbering must follow particle order, not memory luck."""
     rng = np.random.default_rng(3)
